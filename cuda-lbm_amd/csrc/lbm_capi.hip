@@ -1,0 +1,574 @@
+// lbm_capi.hip — the extern "C" boundary of liblbm_hip.so (declared in include/lbm_hip.h).
+#include "lbm_ctx.hpp"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+using namespace lbm;
+
+namespace {
+
+constexpr uint8_t kSupported[] = {LBM_FLUID,
+                                  LBM_BOUNCE_BACK,
+                                  LBM_ZOU_HE_TOP,
+                                  LBM_ZOU_HE_LEFT,
+                                  LBM_ZG_OUTFLOW,
+                                  LBM_REGULARIZED_INLET_TOP,
+                                  LBM_REGULARIZED_BOUNCE_BACK,
+                                  LBM_REGULARIZED_BOUNCE_BACK_CORNER,
+                                  LBM_MOVING_WALL_BOUNCE_BACK};
+
+bool flag_supported(uint8_t f) {
+    for (uint8_t s : kSupported)
+        if (s == f) return true;
+    return false;
+}
+
+size_t storage_bytes(int storage) { return storage == LBM_F32 ? 4 : 2; }
+
+thread_local std::string g_create_err;
+
+}  // namespace
+
+int lbm_ctx::fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    err = buf;
+    return code;
+}
+
+int lbm_ctx::hip_fail(hipError_t e, const char* what) {
+    return fail(LBM_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+int lbm_ctx::activate() {
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    return LBM_OK;
+}
+
+int lbm_ctx::dev_alloc(void** p, size_t bytes, const char* what) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(LBM_E_NOMEM, "hipMalloc(%zu bytes) for %s failed: %s", bytes, what, hipGetErrorString(e));
+    }
+    return LBM_OK;
+}
+
+// Copy `bytes` from a host or device pointer to device memory `dst` on the stream.
+int lbm_ctx::upload(void* dst, const void* src, size_t bytes, int on_device) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync (upload)");
+    if (!on_device) {
+        e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    }
+    return LBM_OK;
+}
+
+int lbm_ctx::scratch(size_t bytes) {
+    if (bytes <= tmp_bytes) return LBM_OK;
+    if (tmp) {
+        (void)hipStreamSynchronize(stream);
+        (void)hipFree(tmp);
+        tmp = nullptr;
+        tmp_bytes = 0;
+    }
+    int rc = dev_alloc(&tmp, bytes, "staging buffer");
+    if (rc) return rc;
+    tmp_bytes = bytes;
+    return LBM_OK;
+}
+
+int lbm_ctx::ensure_partials(long long n) {
+    if (n <= partials_n) return LBM_OK;
+    if (partials) {
+        (void)hipStreamSynchronize(stream);
+        (void)hipFree(partials);
+        partials = nullptr;
+    }
+    int rc = dev_alloc(reinterpret_cast<void**>(&partials), size_t(n) * 3 * sizeof(double), "reduction partials");
+    if (rc) return rc;
+    partials_n = n;
+    return LBM_OK;
+}
+
+StepParams lbm_ctx::step_params(int src_idx, bool write_macros) const {
+    StepParams p{};
+    p.nx = cfg.nx;
+    p.ny = cfg.ny;
+    p.px = cfg.periodic_x;
+    p.py = cfg.periodic_y;
+    p.plane = plane;
+    p.src = buf[src_idx];
+    p.dst = buf[1 - src_idx];
+    p.flags = d_flags;
+    p.row_class = d_row_class;
+    p.omega = cfg.omega;
+    for (int k = 0; k < 9; k++) p.S[k] = cfg.S[k];
+    p.rho0 = cfg.rho0;
+    p.fx = cfg.force[0];
+    p.fy = cfg.force[1];
+    p.force_field = d_force_field;
+    p.ibm_force = ibm.active ? ibm.d_force_total : nullptr;
+    p.ibm_x0 = ibm.x0;
+    p.ibm_x1 = ibm.x1;
+    p.ibm_y0 = ibm.y0;
+    p.ibm_y1 = ibm.y1;
+    p.forced = (cfg.force[0] != 0.0f || cfg.force[1] != 0.0f || d_force_field || ibm.active) ? 1 : 0;
+    p.wall_ux = cfg.wall_u[0];
+    p.wall_uy = cfg.wall_u[1];
+    p.u_lid = cfg.u_lid;
+    p.quirks = cfg.quirks;
+    p.avg = (cfg.collision == LBM_CM && cfg.adapter == LBM_OPTIMAL_ADAPTER) ? d_avg : nullptr;
+    p.out_rho = write_macros ? d_rho : nullptr;
+    p.out_ux = write_macros ? d_ux : nullptr;
+    p.out_uy = write_macros ? d_uy : nullptr;
+    p.ntx = (cfg.nx + seg_width() - 1) / seg_width();
+    return p;
+}
+
+int lbm_ctx::seg_width() const { return cfg.storage == LBM_F32 ? 4 : 8; }
+
+int lbm_ctx::adapter_pass(int src_idx) {
+    StepParams p = step_params(src_idx, false);
+    long long nb = step_grid_blocks(p);
+    int rc = ensure_partials(nb);
+    if (rc) return rc;
+    hipError_t e = launch_step(cfg.collision, cfg.storage, 1, p, partials, stream);
+    if (e != hipSuccess) return hip_fail(e, "moments pass launch");
+    e = launch_finalize_avg(partials, nb, (long long)N, d_avg, d_avg64, stream);
+    if (e != hipSuccess) return hip_fail(e, "finalize launch");
+    avg_valid = true;
+    return LBM_OK;
+}
+
+// One step from buf[cur] into buf[1-cur].
+int lbm_ctx::one_step(bool write_macros) {
+    int rc;
+    if (ibm.active) {
+        rc = ibm_pass(cur);
+        if (rc) return rc;
+    }
+    bool adapt = cfg.collision == LBM_CM && cfg.adapter == LBM_OPTIMAL_ADAPTER;
+    bool lagged = adapt && cfg.adapter_mode == LBM_ADAPTER_LAGGED;
+    if (adapt && (!lagged || !avg_valid)) {
+        rc = adapter_pass(cur);
+        if (rc) return rc;
+    }
+    StepParams p = step_params(cur, write_macros);
+    int mode = lagged ? 2 : 0;
+    long long nb = step_grid_blocks(p);
+    if (mode == 2) {
+        rc = ensure_partials(nb);
+        if (rc) return rc;
+    }
+    hipError_t e = launch_step(cfg.collision, cfg.storage, mode, p, partials, stream);
+    if (e != hipSuccess) return hip_fail(e, "step launch");
+    if (mode == 2) {
+        e = launch_finalize_avg(partials, nb, (long long)N, d_avg, d_avg64, stream);
+        if (e != hipSuccess) return hip_fail(e, "finalize launch");
+    }
+    cur = 1 - cur;
+    timestep++;
+    if (write_macros) macros_valid = true;
+    return LBM_OK;
+}
+
+lbm_ctx::~lbm_ctx() {
+    (void)hipSetDevice(device);
+    if (own_stream) (void)hipStreamSynchronize(own_stream);
+    void* ptrs[] = {buf[0], buf[1], d_flags, d_row_class, d_force_field, d_rho, d_ux, d_uy, d_avg, d_avg64, partials, tmp};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    ibm_free();
+    adjoint_free();
+    if (own_stream) (void)hipStreamDestroy(own_stream);
+}
+
+// =====================================================================================
+extern "C" {
+
+const char* lbm_version(void) { return "lbm-mi355x 0.1 (gfx950)"; }
+
+// With a NULL context: the reason the calling thread's last lbm_create failed.
+const char* lbm_last_error(const lbm_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+
+int lbm_config_default(lbm_config* cfg, int32_t nx, int32_t ny) {
+    if (!cfg) return LBM_E_INVALID;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->nx = nx;
+    cfg->ny = ny;
+    cfg->collision = LBM_BGK;
+    cfg->adapter = LBM_NO_ADAPTER;
+    cfg->adapter_mode = LBM_ADAPTER_SAME_STEP;
+    float tau = 3.0f * (1.0f / 6.0f) + 0.5f;  // viscosity_to_tau(1/6), scenario.cuh:36-38
+    cfg->omega = 1.0f / tau;
+    const float S[9] = {0.0f, cfg->omega, cfg->omega, 0.0f, cfg->omega, 0.0f, cfg->omega, cfg->omega, cfg->omega};
+    for (int k = 0; k < 9; k++) cfg->S[k] = S[k];
+    cfg->rho0 = 1.0f;
+    cfg->storage = LBM_F32;
+    cfg->u_lid = 0.1f;  // ScenarioTrait::u_max (scenario.cuh:59)
+    return LBM_OK;
+}
+
+int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
+    if (!cfg || !out) return LBM_E_INVALID;
+    *out = nullptr;
+    auto* c = new lbm_ctx();
+    c->cfg = *cfg;
+    hipError_t e = hipGetDevice(&c->device);
+    if (e != hipSuccess) {
+        c->hip_fail(e, "hipGetDevice");
+        g_create_err = c->err;
+        delete c;
+        return LBM_E_HIP;
+    }
+    const lbm_config& k = *cfg;
+    auto bad = [&](const char* m) {
+        delete c;
+        g_create_err = m;
+        return LBM_E_INVALID;
+    };
+    if (k.nx < 3 || k.ny < 3) return bad("grid must be at least 3x3");
+    if ((long long)k.nx * k.ny > (1LL << 31) - 1) return bad("grid exceeds 2^31 nodes");
+    if (k.collision < LBM_BGK || k.collision > LBM_CM) return bad("unknown collision operator");
+    if (k.adapter != LBM_NO_ADAPTER && !(k.adapter == LBM_OPTIMAL_ADAPTER && k.collision == LBM_CM))
+        return bad("adapters apply to CM only");
+    if (k.adapter_mode != LBM_ADAPTER_SAME_STEP && k.adapter_mode != LBM_ADAPTER_LAGGED) return bad("bad adapter_mode");
+    if (k.storage < LBM_F32 || k.storage > LBM_BF16_SHIFTED) return bad("unknown storage");
+    // init.cuh:62-64: viscosity > 0 and tau > 0.5, i.e. 0 < omega < 2; u_max < 0.5.
+    if (k.collision == LBM_BGK && !(k.omega > 0.0f && k.omega < 2.0f)) return bad("BGK needs 0 < omega < 2 (tau > 0.5)");
+    for (int i = 0; i < 9; i++)
+        if (!std::isfinite(k.S[i])) return bad("relaxation rates must be finite");
+    if (!(std::fabs(k.u_lid) < 0.5f) || !(std::fabs(k.wall_u[0]) < 0.5f) || !(std::fabs(k.wall_u[1]) < 0.5f))
+        return bad("boundary velocities must stay below 0.5 (init.cuh:64)");
+    if (!(k.rho0 > 0.0f)) return bad("rho0 must be positive");
+    if (!std::isfinite(k.force[0]) || !std::isfinite(k.force[1])) return bad("force must be finite");
+
+    c->N = size_t(k.nx) * size_t(k.ny);
+    c->plane = (long long)((c->N + 63) / 64 * 64);
+    e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        c->hip_fail(e, "hipStreamCreate");
+        g_create_err = c->err;
+        delete c;
+        return LBM_E_HIP;
+    }
+    c->stream = c->own_stream;
+    size_t pb = size_t(c->plane) * 9 * storage_bytes(k.storage);
+    int rc = 0;
+    rc = rc ? rc : c->dev_alloc(&c->buf[0], pb, "populations A");
+    rc = rc ? rc : c->dev_alloc(&c->buf[1], pb, "populations B");
+    rc = rc ? rc : c->dev_alloc(reinterpret_cast<void**>(&c->d_rho), c->N * 4, "rho");
+    rc = rc ? rc : c->dev_alloc(reinterpret_cast<void**>(&c->d_ux), c->N * 4, "ux");
+    rc = rc ? rc : c->dev_alloc(reinterpret_cast<void**>(&c->d_uy), c->N * 4, "uy");
+    rc = rc ? rc : c->dev_alloc(reinterpret_cast<void**>(&c->d_avg), 4 * sizeof(float), "adapter averages");
+    rc = rc ? rc : c->dev_alloc(reinterpret_cast<void**>(&c->d_avg64), 4 * sizeof(double), "adapter averages");
+    if (rc) {
+        g_create_err = c->err;
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return LBM_OK;
+}
+
+int lbm_destroy(lbm_ctx* ctx) {
+    delete ctx;
+    return LBM_OK;
+}
+
+int lbm_set_stream(lbm_ctx* ctx, void* s) {
+    if (!ctx) return LBM_E_INVALID;
+    ctx->stream = s ? static_cast<hipStream_t>(s) : ctx->own_stream;
+    return LBM_OK;
+}
+
+int lbm_set_flags(lbm_ctx* ctx, const uint8_t* flags, int on_device) {
+    if (!ctx || !flags) return LBM_E_INVALID;
+    int rc = ctx->activate();
+    if (rc) return rc;
+    const int nx = ctx->cfg.nx, ny = ctx->cfg.ny;
+    std::vector<uint8_t> h(ctx->N);
+    if (on_device) {
+        hipError_t e = hipMemcpy(h.data(), flags, ctx->N, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return ctx->hip_fail(e, "hipMemcpy (flags)");
+    } else {
+        std::memcpy(h.data(), flags, ctx->N);
+    }
+    std::vector<uint8_t> rowc(size_t(ny), 0);
+    bool any = false;
+    for (int y = 0; y < ny; y++) {
+        for (int x = 0; x < nx; x++) {
+            uint8_t f = h[size_t(y) * nx + x];
+            if (!flag_supported(f))
+                return ctx->fail(LBM_E_UNSUPPORTED, "BC flag %d at node (%d,%d) is not supported", int(f), x, y);
+            if (f == LBM_FLUID) continue;
+            any = true;
+            if (f == LBM_ZG_OUTFLOW) {
+                // The interior neighbour's unknown populations must all stream from inside
+                // the domain: a stale slot of another node would be raced by its writer.
+                int n0 = 0, n1 = 0, ix = x, iy = y;
+                if (x == 0) { n0 = 1; ix = 1; }
+                else if (x == nx - 1) { n0 = -1; ix = nx - 2; }
+                else if (y == 0) { n1 = 1; iy = 1; }
+                else if (y == ny - 1) { n1 = -1; iy = ny - 2; }
+                else continue;
+                for (int i = 0; i < 9; i++) {
+                    if (CX[i] * n0 + CY[i] * n1 <= 0) continue;
+                    int xs = ix - CX[i], ys = iy - CY[i];
+                    bool okx = (xs >= 0 && xs < nx) || ctx->cfg.periodic_x;
+                    bool oky = (ys >= 0 && ys < ny) || ctx->cfg.periodic_y;
+                    if (!okx || !oky)
+                        return ctx->fail(LBM_E_UNSUPPORTED,
+                                         "ZG_OUTFLOW at (%d,%d): its interior neighbour has an unstreamed population", x, y);
+                }
+            }
+            if (f == LBM_REGULARIZED_BOUNCE_BACK_CORNER && (ctx->cfg.quirks & LBM_QUIRK_STALE_REST))
+                return ctx->fail(LBM_E_UNSUPPORTED, "REGULARIZED_BOUNCE_BACK_CORNER with LBM_QUIRK_STALE_REST");
+        }
+    }
+    if (ctx->d_flags) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(ctx->d_flags);
+        (void)hipFree(ctx->d_row_class);
+        ctx->d_flags = nullptr;
+        ctx->d_row_class = nullptr;
+    }
+    if (!any) return LBM_OK;
+    row_classes(h.data(), nx, ny, rowc.data());
+    rc = ctx->dev_alloc(reinterpret_cast<void**>(&ctx->d_flags), ctx->N, "flags");
+    if (rc) return rc;
+    rc = ctx->dev_alloc(reinterpret_cast<void**>(&ctx->d_row_class), size_t(ny), "row classes");
+    if (rc) return rc;
+    rc = ctx->upload(ctx->d_flags, h.data(), ctx->N, 0);
+    if (rc) return rc;
+    return ctx->upload(ctx->d_row_class, rowc.data(), size_t(ny), 0);
+}
+
+int lbm_init_from_macros(lbm_ctx* ctx, const float* rho, const float* u, int on_device) {
+    if (!ctx || !rho || !u) return LBM_E_INVALID;
+    int rc = ctx->activate();
+    if (rc) return rc;
+    const float *drho = rho, *du = u;
+    if (!on_device) {
+        rc = ctx->scratch(ctx->N * 12);
+        if (rc) return rc;
+        float* t = static_cast<float*>(ctx->tmp);
+        rc = ctx->upload(t, rho, ctx->N * 4, 0);
+        if (rc) return rc;
+        rc = ctx->upload(t + ctx->N, u, ctx->N * 8, 0);
+        if (rc) return rc;
+        drho = t;
+        du = t + ctx->N;
+    }
+    InitParams ip{drho, du, ctx->buf[0], ctx->buf[1], ctx->plane, (long long)ctx->N, ctx->cfg.rho0};
+    hipError_t e = launch_init(ctx->cfg.storage, ip, ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "init launch");
+    e = hipMemcpyAsync(ctx->d_rho, drho, ctx->N * 4, hipMemcpyDeviceToDevice, ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "rho copy");
+    e = launch_aos_to_soa(du, (long long)ctx->N, ctx->d_ux, ctx->d_uy, ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "u split");
+    if (!on_device) {
+        e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return ctx->hip_fail(e, "hipStreamSynchronize");
+    }
+    ctx->cur = 0;
+    ctx->timestep = 0;
+    ctx->initialized = true;
+    ctx->macros_valid = true;
+    ctx->avg_valid = false;
+    ctx->init_from_macros = true;
+    return LBM_OK;
+}
+
+int lbm_set_force_field(lbm_ctx* ctx, const float* force, int on_device) {
+    if (!ctx) return LBM_E_INVALID;
+    int rc = ctx->activate();
+    if (rc) return rc;
+    if (!force) {
+        if (ctx->d_force_field) {
+            (void)hipStreamSynchronize(ctx->stream);
+            (void)hipFree(ctx->d_force_field);
+            ctx->d_force_field = nullptr;
+        }
+        return LBM_OK;
+    }
+    if (!ctx->d_force_field) {
+        rc = ctx->dev_alloc(reinterpret_cast<void**>(&ctx->d_force_field), ctx->N * 8, "force field");
+        if (rc) return rc;
+    }
+    return ctx->upload(ctx->d_force_field, force, ctx->N * 8, on_device);
+}
+
+int lbm_step(lbm_ctx* ctx, int32_t nsteps) {
+    if (!ctx || nsteps < 0) return LBM_E_INVALID;
+    if (!ctx->initialized) return ctx->fail(LBM_E_STATE, "lbm_step before lbm_init_from_macros / lbm_set_pops");
+    int rc = ctx->activate();
+    if (rc) return rc;
+    for (int i = 0; i < nsteps; i++) {
+        rc = ctx->one_step(i == nsteps - 1);
+        if (rc) return rc;
+    }
+    return LBM_OK;
+}
+
+int lbm_sync(lbm_ctx* ctx) {
+    if (!ctx) return LBM_E_INVALID;
+    int rc = ctx->activate();
+    if (rc) return rc;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "hipStreamSynchronize");
+    return LBM_OK;
+}
+
+int lbm_get_timestep(const lbm_ctx* ctx, int64_t* t) {
+    if (!ctx || !t) return LBM_E_INVALID;
+    *t = ctx->timestep;
+    return LBM_OK;
+}
+
+int lbm_get_macros(lbm_ctx* ctx, float* rho, float* u, int on_device) {
+    if (!ctx) return LBM_E_INVALID;
+    if (!ctx->macros_valid) return ctx->fail(LBM_E_STATE, "no macroscopic fields stored yet");
+    int rc = ctx->activate();
+    if (rc) return rc;
+    hipError_t e;
+    if (on_device) {
+        if (rho) {
+            e = hipMemcpyAsync(rho, ctx->d_rho, ctx->N * 4, hipMemcpyDeviceToDevice, ctx->stream);
+            if (e != hipSuccess) return ctx->hip_fail(e, "rho copy");
+        }
+        if (u) {
+            e = launch_macros_to_aos(ctx->d_ux, ctx->d_uy, (long long)ctx->N, u, ctx->stream);
+            if (e != hipSuccess) return ctx->hip_fail(e, "u interleave");
+        }
+        return LBM_OK;
+    }
+    if (rho) {
+        e = hipMemcpyAsync(rho, ctx->d_rho, ctx->N * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e != hipSuccess) return ctx->hip_fail(e, "rho copy");
+    }
+    if (u) {
+        rc = ctx->scratch(ctx->N * 8);
+        if (rc) return rc;
+        e = launch_macros_to_aos(ctx->d_ux, ctx->d_uy, (long long)ctx->N, static_cast<float*>(ctx->tmp), ctx->stream);
+        if (e != hipSuccess) return ctx->hip_fail(e, "u interleave");
+        e = hipMemcpyAsync(u, ctx->tmp, ctx->N * 8, hipMemcpyDeviceToHost, ctx->stream);
+        if (e != hipSuccess) return ctx->hip_fail(e, "u copy");
+    }
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "hipStreamSynchronize");
+    return LBM_OK;
+}
+
+int lbm_get_pops(lbm_ctx* ctx, float* f, int on_device) {
+    if (!ctx || !f) return LBM_E_INVALID;
+    if (!ctx->initialized) return ctx->fail(LBM_E_STATE, "no state yet");
+    int rc = ctx->activate();
+    if (rc) return rc;
+    float* out = f;
+    if (!on_device) {
+        rc = ctx->scratch(ctx->N * 36);
+        if (rc) return rc;
+        out = static_cast<float*>(ctx->tmp);
+    }
+    hipError_t e = launch_pops_to_f32(ctx->cfg.storage, ctx->buf[ctx->cur], ctx->plane, (long long)ctx->N, ctx->cfg.rho0, out, ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "pops convert");
+    if (!on_device) {
+        e = hipMemcpyAsync(f, out, ctx->N * 36, hipMemcpyDeviceToHost, ctx->stream);
+        if (e != hipSuccess) return ctx->hip_fail(e, "pops copy");
+        e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return ctx->hip_fail(e, "hipStreamSynchronize");
+    }
+    return LBM_OK;
+}
+
+int lbm_set_pops(lbm_ctx* ctx, const float* f, int on_device) {
+    if (!ctx || !f) return LBM_E_INVALID;
+    int rc = ctx->activate();
+    if (rc) return rc;
+    const float* in = f;
+    if (!on_device) {
+        rc = ctx->scratch(ctx->N * 36);
+        if (rc) return rc;
+        rc = ctx->upload(ctx->tmp, f, ctx->N * 36, 0);
+        if (rc) return rc;
+        in = static_cast<const float*>(ctx->tmp);
+    }
+    // Both buffers get the state (the stale slots of the first step read it, as after init).
+    hipError_t e = launch_pops_from_f32(ctx->cfg.storage, ctx->buf[0], ctx->buf[1], ctx->plane, (long long)ctx->N, ctx->cfg.rho0, in, ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "pops convert");
+    if (!on_device) {
+        e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return ctx->hip_fail(e, "hipStreamSynchronize");
+    }
+    ctx->cur = 0;
+    ctx->initialized = true;
+    ctx->macros_valid = false;
+    ctx->avg_valid = false;
+    ctx->init_from_macros = false;
+    return LBM_OK;
+}
+
+int lbm_reduce(lbm_ctx* ctx, int32_t kind, const float* ref, double* out) {
+    if (!ctx || !out) return LBM_E_INVALID;
+    if (!ctx->macros_valid) return ctx->fail(LBM_E_STATE, "no macroscopic fields stored yet");
+    int rc = ctx->activate();
+    if (rc) return rc;
+    hipError_t e;
+    const long long N = (long long)ctx->N;
+    if (kind == LBM_REDUCE_MOMENT_AVG) {
+        if (!ctx->avg_valid) return ctx->fail(LBM_E_STATE, "no adapter averages computed yet");
+        e = hipMemcpyAsync(out, ctx->d_avg64, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+        if (e != hipSuccess) return ctx->hip_fail(e, "avg copy");
+        e = hipStreamSynchronize(ctx->stream);
+        return e == hipSuccess ? LBM_OK : ctx->hip_fail(e, "hipStreamSynchronize");
+    }
+    if (kind == LBM_REDUCE_ROW_MEAN_UX) {
+        rc = ctx->scratch(size_t(ctx->cfg.ny) * sizeof(double));
+        if (rc) return rc;
+        e = launch_row_mean_ux(ctx->d_ux, ctx->cfg.nx, ctx->cfg.ny, static_cast<double*>(ctx->tmp), ctx->stream);
+        if (e != hipSuccess) return ctx->hip_fail(e, "row mean launch");
+        e = hipMemcpyAsync(out, ctx->tmp, size_t(ctx->cfg.ny) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+        if (e != hipSuccess) return ctx->hip_fail(e, "row mean copy");
+        e = hipStreamSynchronize(ctx->stream);
+        return e == hipSuccess ? LBM_OK : ctx->hip_fail(e, "hipStreamSynchronize");
+    }
+    if (kind != LBM_REDUCE_MASS && kind != LBM_REDUCE_MOMENTUM && kind != LBM_REDUCE_L2_VS_REF)
+        return ctx->fail(LBM_E_INVALID, "unknown reduction kind %d", int(kind));
+    if (kind == LBM_REDUCE_L2_VS_REF && !ref) return ctx->fail(LBM_E_INVALID, "L2_VS_REF needs a reference field");
+    long long np = reduce_partials_needed(N);
+    size_t need = size_t(np) * 3 * sizeof(double) + 4 * sizeof(double) + (ref ? size_t(N) * 8 : 0);
+    rc = ctx->scratch(need);
+    if (rc) return rc;
+    double* part = static_cast<double*>(ctx->tmp);
+    double* o3 = part + 3 * np;
+    float* dref = nullptr;
+    if (kind == LBM_REDUCE_L2_VS_REF) {
+        dref = reinterpret_cast<float*>(o3 + 4);
+        rc = ctx->upload(dref, ref, size_t(N) * 8, 0);
+        if (rc) return rc;
+    }
+    e = launch_reduce(kind, ctx->d_rho, ctx->d_ux, ctx->d_uy, dref, N, part, o3, ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "reduce launch");
+    double h[3];
+    e = hipMemcpyAsync(h, o3, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "reduce copy");
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "hipStreamSynchronize");
+    out[0] = h[0];
+    if (kind != LBM_REDUCE_MASS) out[1] = h[1];
+    return LBM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,96 @@
+// lbm_ctx.hpp — the context behind the opaque `lbm_ctx*` of the C ABI.
+//
+// One context = one solver instance: the two population buffers (pull scheme, A/B), the
+// boundary flags, the macroscopic outputs of the last step and the optional IBM and
+// adjoint state, all device memory of the GPU that was current at lbm_create, and one
+// stream. Mirrors what `LBM<dim>` owns in the reference (lbm.cuh:36-43, 85-90) without
+// process-global symbols.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/lbm_hip.h"
+#include "lbm_kernels.hpp"
+
+// Immersed-boundary state (IBMManager<2>, IBMManager.cuh:31-51), rebuilt for a band of
+// Eulerian nodes: the union of the markers' stencils.
+struct IbmState {
+    bool active = false;
+    int n_markers = 0, n_band = 0, iters = 3;
+    int x0 = 0, x1 = 0, y0 = 0, y1 = 0;     // box holding the band (exclusive ends)
+    float* d_force_total = nullptr;        // AoS [N][2]: total node force inside the box
+    // device tables
+    float* d_xy = nullptr;                 // [n][2]
+    float* d_ds = nullptr;                 // [n]
+    int* d_band_node = nullptr;            // [n_band] global node index
+    int* d_mk_ptr = nullptr;               // [n+1] marker -> stencil entries
+    int* d_mk_band = nullptr;              // band index per entry
+    float* d_mk_w = nullptr;               // kernel weight per entry
+    int* d_bn_ptr = nullptr;               // [n_band+1] band node -> (marker, weight) entries
+    int* d_bn_mk = nullptr;
+    float* d_bn_w = nullptr;
+    int* d_box_map = nullptr;              // band index or -1 per cell of the box
+    float* d_lag_force = nullptr;          // [n][2] summed over the iterations of the last step
+    double* d_force_sum = nullptr;         // [2]
+    size_t shmem = 0;
+    std::vector<float> h_ds;               // arc lengths (host copy, for the force sum)
+};
+
+// Discrete-adjoint state: checkpoints of the forward populations.
+struct AdjointState {
+    int nsteps = 0, every = 0;
+    std::vector<float*> checkpoints;       // FP32 SoA [9][plane] each
+    float* d_seg = nullptr;                // states of one segment, [every][9][plane]
+    float* d_mu[2] = {nullptr, nullptr};   // adjoint of the pre-collision state
+    float* d_grad_force = nullptr;         // AoS [N][2]
+    float* d_u0 = nullptr;                 // initial velocity AoS (for dL/du0)
+    float* d_rho_init = nullptr;
+    bool ready = false;
+};
+
+struct lbm_ctx {
+    lbm_config cfg{};
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    size_t N = 0;
+    long long plane = 0;
+    void* buf[2] = {nullptr, nullptr};
+    int cur = 0;                       // buffer holding the latest post-collision state
+    uint8_t* d_flags = nullptr;
+    uint8_t* d_row_class = nullptr;
+    float* d_force_field = nullptr;
+    float *d_rho = nullptr, *d_ux = nullptr, *d_uy = nullptr;
+    float* d_avg = nullptr;
+    double* d_avg64 = nullptr;
+    double* partials = nullptr;
+    long long partials_n = 0;
+    void* tmp = nullptr;
+    size_t tmp_bytes = 0;
+    bool initialized = false, macros_valid = false, avg_valid = false, init_from_macros = false;
+    int64_t timestep = 0;
+    IbmState ibm;
+    AdjointState adj;
+    std::string err;
+
+    ~lbm_ctx();
+    int fail(int code, const char* fmt, ...);
+    int hip_fail(hipError_t e, const char* what);
+    int activate();
+    int dev_alloc(void** p, size_t bytes, const char* what);
+    int upload(void* dst, const void* src, size_t bytes, int on_device);
+    int scratch(size_t bytes);
+    int ensure_partials(long long n);
+    int seg_width() const;
+    lbm::StepParams step_params(int src_idx, bool write_macros) const;
+    int adapter_pass(int src_idx);
+    int one_step(bool write_macros);
+    // lbm_ibm.hip
+    int ibm_pass(int src_idx);
+    void ibm_free();
+    // lbm_adjoint.hip
+    void adjoint_free();
+};

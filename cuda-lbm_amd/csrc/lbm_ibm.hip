@@ -1,0 +1,381 @@
+// lbm_ibm.hip — immersed-boundary multi-direct forcing (IBMManager<2>::multi_direct,
+// IBMManager.cuh:222-252) for static markers.
+//
+// The reference runs five whole-domain kernels per iteration (memset, interpolate,
+// Lagrangian force, atomic spread, correct, accumulate) plus a D2D copy of u: ~180 B per
+// node per step for a force that is non-zero on a few thousand nodes. Here the Eulerian
+// side is restricted to the *band* — the union of the markers' kernel stencils — whose
+// post-stream macroscopics, corrected velocities and forces live in LDS of a single
+// workgroup for all iterations. Spreading is a gather by band node over a precomputed
+// (marker, weight) list in marker order, so the sum is deterministic and equals the
+// reference's scatter order. The result is the total node force on the band's bounding
+// box, which the fused step reads there.
+#include "lbm_ctx.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <vector>
+
+using namespace lbm;
+
+namespace {
+
+constexpr int kIbmBlock = 1024;
+
+struct IbmParams {
+    NodeArgs na;
+    int storage;
+    int n_markers, n_band, iters, clamp;
+    int x0, y0, bw, bh;
+    float fx, fy;                 // uniform force
+    const float* force_field;     // AoS or null
+    const int* band_node;
+    const int* mk_ptr;
+    const int* mk_band;
+    const float* mk_w;
+    const int* bn_ptr;
+    const int* bn_mk;
+    const float* bn_w;
+    const int* box_map;           // band index or -1 per box cell
+    float* force_total;           // AoS [N][2]
+    float* lag_force;             // AoS [n][2], summed over iterations
+};
+
+template <int STORE>
+__global__ __launch_bounds__(kIbmBlock) void ibm_kernel(IbmParams p) {
+    extern __shared__ float lds[];
+    const int nb = p.n_band, nm = p.n_markers;
+    float* s_rho = lds;
+    float* s_ux = s_rho + nb;
+    float* s_uy = s_ux + nb;
+    float* s_fx = s_uy + nb;   // accumulated force: F_ext + sum of iterations
+    float* s_fy = s_fx + nb;
+    float* s_lx = s_fy + nb;   // Lagrangian force of the current iteration
+    float* s_ly = s_lx + nm;
+    float* s_sx = s_ly + nm;   // summed over iterations
+    float* s_sy = s_sx + nm;
+    const int nx = p.na.nx;
+
+    // post-stream macroscopics of the band (uncorrected_macroscopics_kernel<2>,
+    // macroscopics.cu:5-38) and the external force (reset_forces, macroscopics.cuh:13-48)
+    for (int b = threadIdx.x; b < nb; b += kIbmBlock) {
+        int node = p.band_node[b];
+        int y = node / nx, x = node - y * nx;
+        Pop9 g = gather_node<STORE>(p.na, x, y);
+        float r = 0.f, jx = 0.f, jy = 0.f;
+#pragma unroll
+        for (int i = 0; i < Q; i++) {
+            r += g.v[i];
+            jx += g.v[i] * float(CX[i]);
+            jy += g.v[i] * float(CY[i]);
+        }
+        float inv = 1.0f / r;
+        s_rho[b] = r;
+        s_ux[b] = jx * inv;
+        s_uy[b] = jy * inv;
+        float fx = p.fx, fy = p.fy;
+        if (p.force_field) {
+            fx = p.force_field[2LL * node];
+            fy = p.force_field[2LL * node + 1];
+        }
+        s_fx[b] = fx;
+        s_fy[b] = fy;
+    }
+    for (int l = threadIdx.x; l < nm; l += kIbmBlock) s_sx[l] = s_sy[l] = 0.f;
+    __syncthreads();
+
+    for (int it = 0; it < p.iters; it++) {
+        // interpolate_velocities_kernel<2> + compute_lagrangian_kernel (IBM_impl.cu:5-51,
+        // IBM_impl.cuh:7-26): u_L, rho_L over the stencil, F_L = 2 rho_L (0 - u_L)
+        for (int l = threadIdx.x; l < nm; l += kIbmBlock) {
+            float ux = 0.f, uy = 0.f, r = 0.f;
+            for (int e = p.mk_ptr[l]; e < p.mk_ptr[l + 1]; e++) {
+                float k = p.mk_w[e];
+                int b = p.mk_band[e];
+                r += k * s_rho[b];
+                ux += k * s_ux[b];
+                uy += k * s_uy[b];
+            }
+            float lx = 2.0f * r * (0.0f - ux), ly = 2.0f * r * (0.0f - uy);
+            if (p.clamp) {
+                lx = lx > 1e-8f ? lx : 0.0f;
+                ly = ly > 1e-8f ? ly : 0.0f;
+            }
+            s_lx[l] = lx;
+            s_ly[l] = ly;
+            s_sx[l] += lx;
+            s_sy[l] += ly;
+        }
+        __syncthreads();
+        // spread_forces_kernel<2> (IBM_impl.cu:120-154) as a gather, then
+        // correct_velocities_kernel and accumulate_forces_kernel (IBM_impl.cuh:28-68)
+        for (int b = threadIdx.x; b < nb; b += kIbmBlock) {
+            float fx = 0.f, fy = 0.f;
+            for (int e = p.bn_ptr[b]; e < p.bn_ptr[b + 1]; e++) {
+                float w = p.bn_w[e];
+                int l = p.bn_mk[e];
+                fx += w * s_lx[l];
+                fy += w * s_ly[l];
+            }
+            float cx = s_ux[b] + fx / (2.0f * s_rho[b]);
+            float cy = s_uy[b] + fy / (2.0f * s_rho[b]);
+            if (p.clamp) {
+                cx = cx > 1e-8f ? cx : 0.0f;
+                cy = cy > 1e-8f ? cy : 0.0f;
+            }
+            s_ux[b] = cx;
+            s_uy[b] = cy;
+            s_fx[b] += fx;
+            s_fy[b] += fy;
+        }
+        __syncthreads();
+    }
+
+    // total force on the band's box: band nodes from LDS, the rest the external force
+    const int ncell = p.bw * p.bh;
+    for (int c = threadIdx.x; c < ncell; c += kIbmBlock) {
+        int yy = p.y0 + c / p.bw, xx = p.x0 + c % p.bw;
+        long long node = (long long)yy * nx + xx;
+        int b = p.box_map[c];
+        float fx, fy;
+        if (b >= 0) {
+            fx = s_fx[b];
+            fy = s_fy[b];
+        } else {
+            fx = p.fx;
+            fy = p.fy;
+            if (p.force_field) {
+                fx = p.force_field[2 * node];
+                fy = p.force_field[2 * node + 1];
+            }
+        }
+        p.force_total[2 * node] = fx;
+        p.force_total[2 * node + 1] = fy;
+    }
+    for (int l = threadIdx.x; l < nm; l += kIbmBlock) {
+        p.lag_force[2 * l] = s_sx[l];
+        p.lag_force[2 * l + 1] = s_sy[l];
+    }
+}
+
+// delta4 (IBMUtils.cuh:23-33), FP32 as in the reference.
+float delta4f(float r) {
+    float rabs = std::fabs(r);
+    if (rabs < 1.0f) return 0.125f * (3.0f - 2.0f * rabs + std::sqrt(1.0f + 4.0f * rabs - 4.0f * r * r));
+    if (rabs < 2.0f) return 0.125f * (5.0f - 2.0f * rabs - std::sqrt(-7.0f + 12.0f * rabs - 4.0f * r * r));
+    return 0.0f;
+}
+
+template <class T>
+int put(lbm_ctx* c, T** dst, const std::vector<T>& v, const char* what) {
+    size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
+    int rc = c->dev_alloc(reinterpret_cast<void**>(dst), bytes, what);
+    if (rc) return rc;
+    if (v.empty()) return LBM_OK;
+    return c->upload(*dst, v.data(), v.size() * sizeof(T), 0);
+}
+
+}  // namespace
+
+void lbm_ctx::ibm_free() {
+    void* ptrs[] = {ibm.d_force_total, ibm.d_xy, ibm.d_ds, ibm.d_band_node, ibm.d_mk_ptr, ibm.d_mk_band, ibm.d_mk_w,
+                    ibm.d_bn_ptr, ibm.d_bn_mk, ibm.d_bn_w, ibm.d_lag_force, ibm.d_force_sum, ibm.d_box_map};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    ibm = IbmState();
+}
+
+int lbm_ctx::ibm_pass(int src_idx) {
+    StepParams sp = step_params(src_idx, false);
+    IbmParams p{};
+    p.na = node_args(sp);
+    p.storage = cfg.storage;
+    p.n_markers = ibm.n_markers;
+    p.n_band = ibm.n_band;
+    p.iters = ibm.iters;
+    p.clamp = (cfg.quirks & LBM_QUIRK_IBM_CLAMP) ? 1 : 0;
+    p.x0 = ibm.x0;
+    p.y0 = ibm.y0;
+    p.bw = ibm.x1 - ibm.x0;
+    p.bh = ibm.y1 - ibm.y0;
+    p.fx = cfg.force[0];
+    p.fy = cfg.force[1];
+    p.force_field = d_force_field;
+    p.band_node = ibm.d_band_node;
+    p.mk_ptr = ibm.d_mk_ptr;
+    p.mk_band = ibm.d_mk_band;
+    p.mk_w = ibm.d_mk_w;
+    p.bn_ptr = ibm.d_bn_ptr;
+    p.bn_mk = ibm.d_bn_mk;
+    p.bn_w = ibm.d_bn_w;
+    p.box_map = ibm.d_box_map;
+    p.force_total = ibm.d_force_total;
+    p.lag_force = ibm.d_lag_force;
+    size_t sh = ibm.shmem;
+    if (cfg.storage == LBM_F16_SHIFTED)
+        hipLaunchKernelGGL(ibm_kernel<LBM_F16_SHIFTED>, dim3(1), dim3(kIbmBlock), sh, stream, p);
+    else if (cfg.storage == LBM_BF16_SHIFTED)
+        hipLaunchKernelGGL(ibm_kernel<LBM_BF16_SHIFTED>, dim3(1), dim3(kIbmBlock), sh, stream, p);
+    else
+        hipLaunchKernelGGL(ibm_kernel<LBM_F32>, dim3(1), dim3(kIbmBlock), sh, stream, p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "IBM launch");
+    return LBM_OK;
+}
+
+extern "C" {
+
+int lbm_ibm_set_markers(lbm_ctx* ctx, const float* xy, const float* ds, int32_t n, int32_t iters) {
+    if (!ctx || n < 0 || (n > 0 && !xy) || iters < 1) return LBM_E_INVALID;
+    int rc = ctx->activate();
+    if (rc) return rc;
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->ibm_free();
+    if (n == 0) return LBM_OK;
+    const int nx = ctx->cfg.nx, ny = ctx->cfg.ny;
+    const bool two = (ctx->cfg.quirks & LBM_QUIRK_IBM_2X2) != 0;
+    const bool no_ds = (ctx->cfg.quirks & LBM_QUIRK_IBM_NO_DS) != 0;
+    const int W = two ? 2 : 4, off = two ? 0 : -1;
+    for (int l = 0; l < n; l++)
+        if (!std::isfinite(xy[2 * l]) || !std::isfinite(xy[2 * l + 1]) || (ds && !(ds[l] > 0.0f)))
+            return ctx->fail(LBM_E_INVALID, "marker %d: coordinates must be finite and ds positive", l);
+    // stencils in the reference's loop order (x offset outer, y inner), out-of-domain skipped
+    struct Ent { int node; float k; };
+    std::vector<std::vector<Ent>> st(static_cast<size_t>(n));
+    std::map<int, int> band;
+    for (int l = 0; l < n; l++) {
+        float px = xy[2 * l], py = xy[2 * l + 1];
+        int gx = int(std::floor(px)), gy = int(std::floor(py));
+        for (int i = 0; i < W; i++)
+            for (int j = 0; j < W; j++) {
+                int xx = gx + off + i, yy = gy + off + j;
+                if (xx >= nx || xx < 0 || yy >= ny || yy < 0) continue;
+                float k = delta4f(px - float(xx)) * delta4f(py - float(yy));
+                int node = yy * nx + xx;
+                st[l].push_back({node, k});
+                band.emplace(node, 0);
+            }
+    }
+    if (band.empty()) return ctx->fail(LBM_E_INVALID, "no marker stencil touches the grid");
+    std::vector<int> band_node;
+    int xmin = nx, xmax = -1, ymin = ny, ymax = -1;
+    for (auto& kv : band) {
+        kv.second = int(band_node.size());
+        band_node.push_back(kv.first);
+        int y = kv.first / nx, x = kv.first % nx;
+        xmin = std::min(xmin, x);
+        xmax = std::max(xmax, x);
+        ymin = std::min(ymin, y);
+        ymax = std::max(ymax, y);
+    }
+    const int nb = int(band_node.size());
+    std::vector<int> mk_ptr(size_t(n) + 1, 0), mk_band;
+    std::vector<float> mk_w;
+    std::vector<std::vector<std::pair<int, float>>> per_node(static_cast<size_t>(nb));
+    for (int l = 0; l < n; l++) {
+        float d = (ds && !no_ds) ? ds[l] : 1.0f;
+        for (const Ent& e : st[l]) {
+            int b = band[e.node];
+            mk_band.push_back(b);
+            mk_w.push_back(e.k);
+            per_node[size_t(b)].push_back({l, e.k * d});
+        }
+        mk_ptr[l + 1] = int(mk_band.size());
+    }
+    std::vector<int> bn_ptr(size_t(nb) + 1, 0), bn_mk;
+    std::vector<float> bn_w;
+    for (int b = 0; b < nb; b++) {
+        for (auto& pr : per_node[size_t(b)]) {
+            bn_mk.push_back(pr.first);
+            bn_w.push_back(pr.second);
+        }
+        bn_ptr[b + 1] = int(bn_mk.size());
+    }
+    const int bw = xmax - xmin + 1, bh = ymax - ymin + 1;
+    std::vector<int> box_map(size_t(bw) * bh, -1);
+    for (int b = 0; b < nb; b++) {
+        int y = band_node[b] / nx, x = band_node[b] % nx;
+        box_map[size_t(y - ymin) * bw + (x - xmin)] = b;
+    }
+    // Every index the kernel forms comes from these tables: check them all here.
+    for (int v : band_node)
+        if (v < 0 || size_t(v) >= ctx->N) return ctx->fail(LBM_E_INVALID, "IBM band node out of range");
+    for (int v : mk_band)
+        if (v < 0 || v >= nb) return ctx->fail(LBM_E_INVALID, "IBM stencil entry out of range");
+    for (int v : bn_mk)
+        if (v < 0 || v >= n) return ctx->fail(LBM_E_INVALID, "IBM spread entry out of range");
+    if (mk_ptr.back() != int(mk_band.size()) || bn_ptr.back() != int(bn_mk.size()))
+        return ctx->fail(LBM_E_INVALID, "IBM table sizes inconsistent");
+    for (int v : box_map)
+        if (v < -1 || v >= nb) return ctx->fail(LBM_E_INVALID, "IBM box map out of range");
+    if (xmin < 0 || ymin < 0 || xmax >= nx || ymax >= ny) return ctx->fail(LBM_E_INVALID, "IBM box outside the grid");
+    size_t shmem = (size_t(5) * nb + size_t(4) * n) * sizeof(float);
+    int maxsh = 0;
+    (void)hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
+    if (shmem > size_t(maxsh))
+        return ctx->fail(LBM_E_UNSUPPORTED, "IBM band of %d nodes and %d markers needs %zu B of LDS (max %d)", nb, n,
+                         shmem, maxsh);
+    IbmState& I = ctx->ibm;
+    std::vector<float> xyv(xy, xy + 2 * size_t(n)), dsv(size_t(n), 1.0f);
+    if (ds)
+        for (int l = 0; l < n; l++) dsv[size_t(l)] = ds[l];
+    rc = put(ctx, &I.d_xy, xyv, "IBM markers");
+    rc = rc ? rc : put(ctx, &I.d_ds, dsv, "IBM ds");
+    rc = rc ? rc : put(ctx, &I.d_band_node, band_node, "IBM band");
+    rc = rc ? rc : put(ctx, &I.d_mk_ptr, mk_ptr, "IBM stencils");
+    rc = rc ? rc : put(ctx, &I.d_mk_band, mk_band, "IBM stencils");
+    rc = rc ? rc : put(ctx, &I.d_mk_w, mk_w, "IBM weights");
+    rc = rc ? rc : put(ctx, &I.d_bn_ptr, bn_ptr, "IBM spread lists");
+    rc = rc ? rc : put(ctx, &I.d_bn_mk, bn_mk, "IBM spread lists");
+    rc = rc ? rc : put(ctx, &I.d_bn_w, bn_w, "IBM spread weights");
+    rc = rc ? rc : put(ctx, &I.d_box_map, box_map, "IBM box map");
+    rc = rc ? rc : ctx->dev_alloc(reinterpret_cast<void**>(&I.d_force_total), ctx->N * 8, "IBM force field");
+    rc = rc ? rc : ctx->dev_alloc(reinterpret_cast<void**>(&I.d_lag_force), size_t(n) * 8, "IBM Lagrangian forces");
+    if (rc) {
+        ctx->ibm_free();
+        return rc;
+    }
+    (void)hipMemsetAsync(I.d_lag_force, 0, size_t(n) * 8, ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
+    I.n_markers = n;
+    I.n_band = nb;
+    I.iters = iters;
+    I.x0 = xmin;
+    I.x1 = xmax + 1;
+    I.y0 = ymin;
+    I.y1 = ymax + 1;
+    I.shmem = shmem;
+    I.h_ds = dsv;
+    I.active = true;
+    return LBM_OK;
+}
+
+int lbm_ibm_get_forces(lbm_ctx* ctx, float* lag_force, double* force_sum) {
+    if (!ctx) return LBM_E_INVALID;
+    if (!ctx->ibm.active) return ctx->fail(LBM_E_STATE, "no IBM markers set");
+    int rc = ctx->activate();
+    if (rc) return rc;
+    const int n = ctx->ibm.n_markers;
+    std::vector<float> h(2 * size_t(n));
+    hipError_t e = hipMemcpyAsync(h.data(), ctx->ibm.d_lag_force, 2 * size_t(n) * sizeof(float), hipMemcpyDeviceToHost,
+                                  ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "IBM force copy");
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "hipStreamSynchronize");
+    if (lag_force) std::copy(h.begin(), h.end(), lag_force);
+    if (force_sum) {
+        const bool no_ds = (ctx->cfg.quirks & LBM_QUIRK_IBM_NO_DS) != 0;
+        double tx = 0.0, ty = 0.0;
+        for (int l = 0; l < n; l++) {
+            double d = no_ds ? 1.0 : double(ctx->ibm.h_ds[size_t(l)]);
+            tx += double(h[2 * size_t(l)]) * d;
+            ty += double(h[2 * size_t(l) + 1]) * d;
+        }
+        force_sum[0] = tx;
+        force_sum[1] = ty;
+    }
+    return LBM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,284 @@
+// lbm_kernels.hip — gfx950 kernels of the D2Q9 step and their launchers.
+#include "lbm_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace lbm {
+
+constexpr int kBlock = 256;   // 4 wave64 per workgroup
+constexpr int kXcds = 8;      // MI355X: 8 XCDs, workgroups dealt round-robin
+
+// Workgroup i runs on XCD i % 8. Give each XCD a contiguous run of logical workgroups so
+// the cache lines two neighbouring row segments share stay in one XCD's L2.
+__device__ __forceinline__ long long xcd_logical_block(long long nb_padded) {
+    long long i = blockIdx.x;
+    return (i % kXcds) * (nb_padded / kXcds) + i / kXcds;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Deterministic block sum of three doubles; result valid in thread 0.
+__device__ __forceinline__ void block_sum3(double& a, double& b, double& c) {
+    __shared__ double part[3][kBlock / 64];
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        part[0][wv] = a;
+        part[1][wv] = b;
+        part[2][wv] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a = b = c = 0.0;
+        for (int w = 0; w < kBlock / 64; w++) {
+            a += part[0][w];
+            b += part[1][w];
+            c += part[2][w];
+        }
+    }
+}
+
+template <int COLL, int STORE, int MODE>
+__global__ __launch_bounds__(kBlock) void fused_step_kernel(StepParams p, double* partials, long long nb_padded) {
+    long long lb = xcd_logical_block(nb_padded);
+    MomAcc acc{0.0, 0.0, 0.0};
+    step_segment<COLL, STORE, MODE>(p, lb * kBlock + threadIdx.x, acc);
+    if constexpr (MODE != 0) {
+        block_sum3(acc.rho, acc.jmag, acc.pimag);
+        if (threadIdx.x == 0) {
+            partials[3 * lb + 0] = acc.rho;
+            partials[3 * lb + 1] = acc.jmag;
+            partials[3 * lb + 2] = acc.pimag;
+        }
+    }
+}
+
+// Sum `n` partial triples in a fixed order and store the means (sum / N) as floats, the
+// MomentInfo the adapter reads (lbm.cuh:25-31, macroscopics.cuh:116-118).
+__global__ __launch_bounds__(kBlock) void finalize_avg_kernel(const double* partials, long long n, double inv_nodes,
+                                                            float* avg, double* avg64) {
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (long long i = threadIdx.x; i < n; i += kBlock) {
+        a += partials[3 * i];
+        b += partials[3 * i + 1];
+        c += partials[3 * i + 2];
+    }
+    block_sum3(a, b, c);
+    if (threadIdx.x == 0) {
+        avg[0] = float(a * inv_nodes);
+        avg[1] = float(b * inv_nodes);
+        avg[2] = float(c * inv_nodes);
+        if (avg64) {
+            avg64[0] = a * inv_nodes;
+            avg64[1] = b * inv_nodes;
+            avg64[2] = c * inv_nodes;
+        }
+    }
+}
+
+// init_kernel + init_node (init.cuh:10-43): f = f_back = f_eq(rho, u).
+template <int STORE>
+__global__ __launch_bounds__(kBlock) void init_kernel(InitParams p) {
+    using St = Storage<STORE>;
+    using T = typename St::T;
+    long long n = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (n >= p.N) return;
+    float r = p.rho[n], ux = p.u[2 * n], uy = p.u[2 * n + 1];
+    float usq15 = 1.5f * (ux * ux + uy * uy);
+    T* a = static_cast<T*>(p.buf0);
+    T* b = static_cast<T*>(p.buf1);
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        float fe = feq(q, r, ux, uy, usq15);
+        St::store1(a + q * p.plane + n, q, p.rho0, fe);
+        St::store1(b + q * p.plane + n, q, p.rho0, fe);
+    }
+}
+
+// FP32 SoA <-> storage.
+template <int STORE>
+__global__ __launch_bounds__(kBlock) void pops_to_f32_kernel(const void* buf, long long plane, long long N, float rho0, float* out) {
+    using St = Storage<STORE>;
+    long long n = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (n >= N) return;
+    const typename St::T* b = static_cast<const typename St::T*>(buf);
+#pragma unroll
+    for (int q = 0; q < Q; q++) out[q * N + n] = St::load1(b + q * plane + n, q, rho0);
+}
+template <int STORE>
+__global__ __launch_bounds__(kBlock) void pops_from_f32_kernel(void* buf0, void* buf1, long long plane, long long N, float rho0, const float* in) {
+    using St = Storage<STORE>;
+    long long n = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (n >= N) return;
+    typename St::T* a = static_cast<typename St::T*>(buf0);
+    typename St::T* b = static_cast<typename St::T*>(buf1);
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        float v = in[q * N + n];
+        St::store1(a + q * plane + n, q, rho0, v);
+        if (b) St::store1(b + q * plane + n, q, rho0, v);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void macros_to_aos_kernel(const float* ux, const float* uy, long long N, float* u) {
+    long long n = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (n >= N) return;
+    u[2 * n] = ux[n];
+    u[2 * n + 1] = uy[n];
+}
+
+__global__ __launch_bounds__(kBlock) void aos_to_soa_kernel(const float* u, long long N, float* ux, float* uy) {
+    long long n = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (n >= N) return;
+    ux[n] = u[2 * n];
+    uy[n] = u[2 * n + 1];
+}
+
+// Validation reductions over the stored macroscopic fields, FP64, fixed order:
+// each workgroup sums a contiguous range, then one workgroup sums the partials.
+__global__ __launch_bounds__(kBlock) void reduce_fields_kernel(int kind, const float* rho, const float* ux, const float* uy,
+                                                             const float* ref, long long N, long long chunk,
+                                                             double* partials) {
+    long long beg = (long long)blockIdx.x * chunk, end = beg + chunk < N ? beg + chunk : N;
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (long long n = beg + threadIdx.x; n < end; n += kBlock) {
+        double r = rho[n], vx = ux[n], vy = uy[n];
+        if (kind == LBM_REDUCE_MASS) {
+            a += r;
+        } else if (kind == LBM_REDUCE_MOMENTUM) {
+            a += r * vx;
+            b += r * vy;
+        } else {  // L2 vs reference
+            double ex = vx - double(ref[2 * n]), ey = vy - double(ref[2 * n + 1]);
+            a += ex * ex + ey * ey;
+            b += double(ref[2 * n]) * double(ref[2 * n]) + double(ref[2 * n + 1]) * double(ref[2 * n + 1]);
+        }
+    }
+    block_sum3(a, b, c);
+    if (threadIdx.x == 0) {
+        partials[3 * blockIdx.x] = a;
+        partials[3 * blockIdx.x + 1] = b;
+        partials[3 * blockIdx.x + 2] = c;
+    }
+}
+__global__ __launch_bounds__(kBlock) void sum_partials_kernel(const double* partials, long long n, double* out) {
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (long long i = threadIdx.x; i < n; i += kBlock) {
+        a += partials[3 * i];
+        b += partials[3 * i + 1];
+        c += partials[3 * i + 2];
+    }
+    block_sum3(a, b, c);
+    if (threadIdx.x == 0) {
+        out[0] = a;
+        out[1] = b;
+        out[2] = c;
+    }
+}
+// Row means of u_x (the Poiseuille validator's x-average, poiseuilleScenario.cuh:62-74):
+// one workgroup per row, FP64.
+__global__ __launch_bounds__(kBlock) void row_mean_ux_kernel(const float* ux, int nx, double* out) {
+    int y = blockIdx.x;
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int x = threadIdx.x; x < nx; x += kBlock) a += double(ux[(long long)y * nx + x]);
+    block_sum3(a, b, c);
+    if (threadIdx.x == 0) out[y] = a / double(nx);
+}
+
+// ---- launchers ----------------------------------------------------------------------------
+
+long long step_grid_blocks(const StepParams& p) {
+    long long nseg = (long long)p.ntx * p.ny;
+    long long nb = (nseg + kBlock - 1) / kBlock;
+    return (nb + kXcds - 1) / kXcds * kXcds;
+}
+
+template <int COLL, int STORE, int MODE>
+static hipError_t launch_step_t(const StepParams& p, double* partials, hipStream_t s) {
+    long long nb = step_grid_blocks(p);
+    hipLaunchKernelGGL((fused_step_kernel<COLL, STORE, MODE>), dim3(unsigned(nb)), dim3(kBlock), 0, s, p, partials, nb);
+    return hipGetLastError();
+}
+
+template <int STORE>
+static hipError_t launch_step_s(int coll, int mode, const StepParams& p, double* partials, hipStream_t s) {
+    if (mode == 1) return launch_step_t<LBM_BGK, STORE, 1>(p, partials, s);
+    if (mode == 2) return launch_step_t<LBM_CM, STORE, 2>(p, partials, s);
+    if (coll == LBM_BGK) return launch_step_t<LBM_BGK, STORE, 0>(p, partials, s);
+    if (coll == LBM_MRT) return launch_step_t<LBM_MRT, STORE, 0>(p, partials, s);
+    return launch_step_t<LBM_CM, STORE, 0>(p, partials, s);
+}
+
+hipError_t launch_step(int coll, int storage, int mode, const StepParams& p, double* partials, hipStream_t s) {
+    if (storage == LBM_F16_SHIFTED) return launch_step_s<LBM_F16_SHIFTED>(coll, mode, p, partials, s);
+    if (storage == LBM_BF16_SHIFTED) return launch_step_s<LBM_BF16_SHIFTED>(coll, mode, p, partials, s);
+    return launch_step_s<LBM_F32>(coll, mode, p, partials, s);
+}
+
+hipError_t launch_finalize_avg(const double* partials, long long n, long long N, float* avg, double* avg64, hipStream_t s) {
+    hipLaunchKernelGGL(finalize_avg_kernel, dim3(1), dim3(kBlock), 0, s, partials, n, 1.0 / double(N), avg, avg64);
+    return hipGetLastError();
+}
+
+static unsigned blocks_for(long long n) { return unsigned((n + kBlock - 1) / kBlock); }
+
+hipError_t launch_init(int storage, const InitParams& p, hipStream_t s) {
+    unsigned nb = blocks_for(p.N);
+    if (storage == LBM_F16_SHIFTED) hipLaunchKernelGGL(init_kernel<LBM_F16_SHIFTED>, dim3(nb), dim3(kBlock), 0, s, p);
+    else if (storage == LBM_BF16_SHIFTED) hipLaunchKernelGGL(init_kernel<LBM_BF16_SHIFTED>, dim3(nb), dim3(kBlock), 0, s, p);
+    else hipLaunchKernelGGL(init_kernel<LBM_F32>, dim3(nb), dim3(kBlock), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_pops_to_f32(int storage, const void* buf, long long plane, long long N, float rho0, float* out, hipStream_t s) {
+    unsigned nb = blocks_for(N);
+    if (storage == LBM_F16_SHIFTED) hipLaunchKernelGGL(pops_to_f32_kernel<LBM_F16_SHIFTED>, dim3(nb), dim3(kBlock), 0, s, buf, plane, N, rho0, out);
+    else if (storage == LBM_BF16_SHIFTED) hipLaunchKernelGGL(pops_to_f32_kernel<LBM_BF16_SHIFTED>, dim3(nb), dim3(kBlock), 0, s, buf, plane, N, rho0, out);
+    else hipLaunchKernelGGL(pops_to_f32_kernel<LBM_F32>, dim3(nb), dim3(kBlock), 0, s, buf, plane, N, rho0, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_pops_from_f32(int storage, void* b0, void* b1, long long plane, long long N, float rho0, const float* in, hipStream_t s) {
+    unsigned nb = blocks_for(N);
+    if (storage == LBM_F16_SHIFTED) hipLaunchKernelGGL(pops_from_f32_kernel<LBM_F16_SHIFTED>, dim3(nb), dim3(kBlock), 0, s, b0, b1, plane, N, rho0, in);
+    else if (storage == LBM_BF16_SHIFTED) hipLaunchKernelGGL(pops_from_f32_kernel<LBM_BF16_SHIFTED>, dim3(nb), dim3(kBlock), 0, s, b0, b1, plane, N, rho0, in);
+    else hipLaunchKernelGGL(pops_from_f32_kernel<LBM_F32>, dim3(nb), dim3(kBlock), 0, s, b0, b1, plane, N, rho0, in);
+    return hipGetLastError();
+}
+
+hipError_t launch_macros_to_aos(const float* ux, const float* uy, long long N, float* u, hipStream_t s) {
+    hipLaunchKernelGGL(macros_to_aos_kernel, dim3(blocks_for(N)), dim3(kBlock), 0, s, ux, uy, N, u);
+    return hipGetLastError();
+}
+
+hipError_t launch_aos_to_soa(const float* u, long long N, float* ux, float* uy, hipStream_t s) {
+    hipLaunchKernelGGL(aos_to_soa_kernel, dim3(blocks_for(N)), dim3(kBlock), 0, s, u, N, ux, uy);
+    return hipGetLastError();
+}
+
+long long reduce_partials_needed(long long N) {
+    long long chunk = 64LL * kBlock;
+    return (N + chunk - 1) / chunk;
+}
+
+hipError_t launch_reduce(int kind, const float* rho, const float* ux, const float* uy, const float* ref, long long N,
+                         double* partials, double* out3, hipStream_t s) {
+    long long chunk = 64LL * kBlock;
+    long long nb = reduce_partials_needed(N);
+    hipLaunchKernelGGL(reduce_fields_kernel, dim3(unsigned(nb)), dim3(kBlock), 0, s, kind, rho, ux, uy, ref, N, chunk, partials);
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kBlock), 0, s, partials, nb, out3);
+    return hipGetLastError();
+}
+
+hipError_t launch_row_mean_ux(const float* ux, int nx, int ny, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(row_mean_ux_kernel, dim3(unsigned(ny)), dim3(kBlock), 0, s, ux, nx, out);
+    return hipGetLastError();
+}
+
+}  // namespace lbm

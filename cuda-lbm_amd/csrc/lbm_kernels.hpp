@@ -1,0 +1,33 @@
+// lbm_kernels.hpp — launchers of the D2Q9 kernels (internal to liblbm_hip.so).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lbm_step.hpp"
+
+namespace lbm {
+
+struct InitParams {
+    const float* rho;  // [N]
+    const float* u;    // AoS [N][2]
+    void* buf0;
+    void* buf1;
+    long long plane, N;
+    float rho0;
+};
+
+long long step_grid_blocks(const StepParams& p);
+// mode: 0 step, 1 moments pass, 2 step + moments (CM only).
+hipError_t launch_step(int coll, int storage, int mode, const StepParams& p, double* partials, hipStream_t s);
+hipError_t launch_finalize_avg(const double* partials, long long n, long long N, float* avg, double* avg64, hipStream_t s);
+hipError_t launch_init(int storage, const InitParams& p, hipStream_t s);
+hipError_t launch_pops_to_f32(int storage, const void* buf, long long plane, long long N, float rho0, float* out, hipStream_t s);
+hipError_t launch_pops_from_f32(int storage, void* b0, void* b1, long long plane, long long N, float rho0, const float* in, hipStream_t s);
+hipError_t launch_macros_to_aos(const float* ux, const float* uy, long long N, float* u, hipStream_t s);
+hipError_t launch_aos_to_soa(const float* u, long long N, float* ux, float* uy, hipStream_t s);
+long long reduce_partials_needed(long long N);
+hipError_t launch_reduce(int kind, const float* rho, const float* ux, const float* uy, const float* ref, long long N,
+                         double* partials, double* out3, hipStream_t s);
+hipError_t launch_row_mean_ux(const float* ux, int nx, int ny, double* out, hipStream_t s);
+
+}  // namespace lbm

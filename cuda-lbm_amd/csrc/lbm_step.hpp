@@ -1,0 +1,417 @@
+// lbm_step.hpp — per-thread body of the fused D2Q9 pull-scheme step.
+//
+// One thread owns a segment of V consecutive nodes of one row (V = 4 for FP32 storage,
+// 8 for FP16/BF16, i.e. 16 bytes per population plane). For each of the nine planes it
+// pulls the segment's post-stream values from the upstream row/column of the source
+// buffer — one possibly misaligned 16-byte load, which gfx950 serves as a single
+// global_load_dwordx4 — collides in FP32 registers and writes 16 aligned bytes per plane
+// to the destination buffer. That replaces the reference's stream, boundaries,
+// macroscopics, correction, equilibrium and collide kernels (main.cu:98-114) and their
+// f_eq buffer with one read and one write of each population.
+//
+// Segments touching a non-periodic edge, a boundary flag or a ragged row end take the
+// general path: a per-node gather that applies the reference's streaming rule exactly —
+// a population whose upstream node lies outside a non-periodic domain is not streamed and
+// keeps the value the destination slot already holds (the push kernel of streaming.cu:20-25
+// leaves that slot untouched) — followed by the boundary functor of the node's flag.
+//
+// The body is __host__ __device__: lbm_kernels.hip launches it on the GPU, and the CPU
+// tests run the same function over the same grid in a host emulation.
+#pragma once
+
+#include "lbm_node.hpp"
+
+#include <string.h>
+
+namespace lbm {
+
+// Kernel arguments (passed by value).
+struct StepParams {
+    int nx, ny;
+    int px, py;
+    long long plane;             // elements between population planes
+    const void* src;             // post-collision populations of step t-1
+    void* dst;                   // receives step t; holds step t-2 (stale slots)
+    const uint8_t* flags;        // per-node BC flag, or null (all FLUID)
+    const uint8_t* row_class;    // per row: 0 all FLUID, 1 non-FLUID only at x = 0 / nx-1, 2 other
+    float omega;
+    float S[9];
+    float rho0;
+    float fx, fy;                // uniform force
+    int forced;                  // any non-zero force in play
+    const float* force_field;    // AoS [N][2] or null
+    const float* ibm_force;      // AoS [N][2] total force inside the IBM box, or null
+    int ibm_x0, ibm_x1, ibm_y0, ibm_y1;
+    float wall_ux, wall_uy, u_lid;
+    unsigned quirks;
+    const float* avg;            // adapter averages (3) or null
+    float* out_rho;              // macroscopic outputs (SoA) or null
+    float* out_ux;
+    float* out_uy;
+    int ntx;                     // segments per row
+};
+
+// Row classes for the fast-path test: 0 all FLUID, 1 non-FLUID only at x = 0 or x = nx-1,
+// 2 anything else (the segment's flags are then read).
+inline void row_classes(const uint8_t* flags, int nx, int ny, uint8_t* out) {
+    for (int y = 0; y < ny; y++) {
+        uint8_t c = 0;
+        for (int x = 0; x < nx; x++) {
+            if (flags[(long long)y * nx + x] == LBM_FLUID) continue;
+            uint8_t k = (x == 0 || x == nx - 1) ? 1 : 2;
+            if (k > c) c = k;
+        }
+        out[y] = c;
+    }
+}
+
+// ---- storage traits -------------------------------------------------------------------
+
+LBM_HD float half_bits_to_float(uint16_t h) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    _Float16 v;
+    __builtin_memcpy(&v, &h, 2);
+    return float(v);
+#else
+    uint32_t sign = uint32_t(h & 0x8000u) << 16, ex = (h >> 10) & 0x1fu, mant = h & 0x3ffu, u;
+    if (ex == 0) {
+        if (mant == 0) {
+            u = sign;
+        } else {
+            int e = -1;
+            do { mant <<= 1; e++; } while (!(mant & 0x400u));
+            u = sign | (uint32_t(127 - 15 - e) << 23) | ((mant & 0x3ffu) << 13);
+        }
+    } else if (ex == 31) {
+        u = sign | 0x7f800000u | (mant << 13);
+    } else {
+        u = sign | ((ex - 15 + 127) << 23) | (mant << 13);
+    }
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+#endif
+}
+
+// Round to nearest even (v_cvt_f16_f32 on the GPU).
+LBM_HD uint16_t float_to_half_bits(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    _Float16 v = _Float16(x);
+    uint16_t h;
+    __builtin_memcpy(&h, &v, 2);
+    return h;
+#else
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    uint32_t sign = (u >> 16) & 0x8000u, ex = (u >> 23) & 0xffu, mant = u & 0x7fffffu;
+    if (ex == 0xffu) return uint16_t(sign | 0x7c00u | (mant ? 0x200u : 0u));
+    int e = int(ex) - 127 + 15;
+    if (e >= 31) return uint16_t(sign | 0x7c00u);
+    if (e <= 0) {
+        if (e < -10) return uint16_t(sign);
+        mant |= 0x800000u;
+        int shift = 14 - e;
+        uint32_t hm = mant >> shift, rem = mant & ((1u << shift) - 1u), halfway = 1u << (shift - 1);
+        if (rem > halfway || (rem == halfway && (hm & 1u))) hm++;
+        return uint16_t(sign | hm);
+    }
+    uint32_t hm = mant >> 13, rem = mant & 0x1fffu, h = sign | (uint32_t(e) << 10) | hm;
+    if (rem > 0x1000u || (rem == 0x1000u && (hm & 1u))) h++;
+    return uint16_t(h);
+#endif
+}
+
+LBM_HD float bf16_bits_to_float(uint16_t h) {
+    uint32_t u = uint32_t(h) << 16;
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+}
+// Round to nearest even; finite inputs only (populations of a running solver).
+LBM_HD uint16_t float_to_bf16_bits(float x) {
+    uint32_t u;
+    __builtin_memcpy(&u, &x, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+
+template <int STORE>
+struct Storage;
+
+template <>
+struct Storage<LBM_F32> {
+    using T = float;
+    static constexpr int V = 4;
+    LBM_HD static void load(const T* p, int, float, float* out) {
+        float v[4];
+        __builtin_memcpy(v, p, 16);
+#pragma unroll
+        for (int j = 0; j < 4; j++) out[j] = v[j];
+    }
+    LBM_HD static void store(T* p, int, float, const float* v) { __builtin_memcpy(p, v, 16); }
+    LBM_HD static float load1(const T* p, int, float) { return *p; }
+    LBM_HD static void store1(T* p, int, float, float v) { *p = v; }
+};
+
+template <int STORE>
+struct Shifted16 {
+    using T = uint16_t;
+    static constexpr int V = 8;
+    LBM_HD static float dec(uint16_t h) { return STORE == LBM_F16_SHIFTED ? half_bits_to_float(h) : bf16_bits_to_float(h); }
+    LBM_HD static uint16_t enc(float v) { return STORE == LBM_F16_SHIFTED ? float_to_half_bits(v) : float_to_bf16_bits(v); }
+    LBM_HD static void load(const T* p, int q, float rho0, float* out) {
+        uint16_t h[8];
+        __builtin_memcpy(h, p, 16);
+        float w0 = wgt(q) * rho0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) out[j] = dec(h[j]) + w0;
+    }
+    LBM_HD static void store(T* p, int q, float rho0, const float* v) {
+        float w0 = wgt(q) * rho0;
+        uint16_t h[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) h[j] = enc(v[j] - w0);
+        __builtin_memcpy(p, h, 16);
+    }
+    LBM_HD static float load1(const T* p, int q, float rho0) { return dec(*p) + wgt(q) * rho0; }
+    LBM_HD static void store1(T* p, int q, float rho0, float v) { *p = enc(v - wgt(q) * rho0); }
+};
+template <>
+struct Storage<LBM_F16_SHIFTED> : Shifted16<LBM_F16_SHIFTED> {};
+template <>
+struct Storage<LBM_BF16_SHIFTED> : Shifted16<LBM_BF16_SHIFTED> {};
+
+// Moment partial sums for the OptimalAdapter averages (FP64).
+struct MomAcc {
+    double rho, jmag, pimag;
+};
+
+// ---- general-path gather ----------------------------------------------------------------
+
+// What the per-node general path needs, passed by value (kernel arguments must not be
+// addressed through a generic pointer).
+struct NodeArgs {
+    const void* src;
+    const void* dst;
+    const uint8_t* flags;
+    long long plane;
+    int nx, ny, px, py;
+    float rho0, u_lid, wall_ux, wall_uy;
+    unsigned quirks;
+};
+
+LBM_HD NodeArgs node_args(const StepParams& p) {
+    NodeArgs a;
+    a.src = p.src;
+    a.dst = p.dst;
+    a.flags = p.flags;
+    a.plane = p.plane;
+    a.nx = p.nx;
+    a.ny = p.ny;
+    a.px = p.px;
+    a.py = p.py;
+    a.rho0 = p.rho0;
+    a.u_lid = p.u_lid;
+    a.wall_ux = p.wall_ux;
+    a.wall_uy = p.wall_uy;
+    a.quirks = p.quirks;
+    return a;
+}
+
+struct Pop9 {
+    float v[Q];
+};
+
+// Post-stream value of population q at node (x, y).
+template <int STORE>
+LBM_HD float gather(const NodeArgs& p, int q, int x, int y) {
+    using St = Storage<STORE>;
+    const typename St::T* src = static_cast<const typename St::T*>(p.src);
+    const typename St::T* dst = static_cast<const typename St::T*>(p.dst);
+    long long node = (long long)y * p.nx + x;
+    if (q == 0 && (p.quirks & LBM_QUIRK_STALE_REST)) return St::load1(dst + node, 0, p.rho0);
+    int xs = x - CX[q], ys = y - CY[q];
+    bool out = false;
+    if (xs < 0 || xs >= p.nx) {
+        if (p.px) xs = xs < 0 ? xs + p.nx : xs - p.nx;
+        else out = true;
+    }
+    if (ys < 0 || ys >= p.ny) {
+        if (p.py) ys = ys < 0 ? ys + p.ny : ys - p.ny;
+        else out = true;
+    }
+    if (out) return St::load1(dst + q * p.plane + node, q, p.rho0);
+    return St::load1(src + q * p.plane + (long long)ys * p.nx + xs, q, p.rho0);
+}
+
+// General path for one node: gather, then the boundary functor of its flag.
+template <int STORE>
+__host__ __device__ __noinline__ Pop9 gather_node(NodeArgs p, int x, int y) {
+    Pop9 r;
+    float* g = r.v;
+#pragma unroll
+    for (int q = 0; q < Q; q++) g[q] = gather<STORE>(p, q, x, y);
+    uint8_t fl = p.flags ? p.flags[(long long)y * p.nx + x] : uint8_t(LBM_FLUID);
+    if (fl == LBM_FLUID) return r;
+    Geom G{p.nx, p.ny, p.px, p.py};
+    switch (fl) {
+        case LBM_BOUNCE_BACK: bc_bounce_back(g, x, y, G, false, 0.f, 0.f, p.rho0); break;
+        case LBM_MOVING_WALL_BOUNCE_BACK: bc_bounce_back(g, x, y, G, true, p.wall_ux, p.wall_uy, p.rho0); break;
+        case LBM_ZOU_HE_TOP: bc_zou_he_top(g, p.u_lid); break;
+        case LBM_ZOU_HE_LEFT: bc_zou_he_left(g, p.u_lid, (p.quirks & LBM_QUIRK_ZOUHE_RHO) != 0); break;
+        case LBM_REGULARIZED_INLET_TOP: bc_regularized_inlet_top(g, p.u_lid); break;
+        case LBM_REGULARIZED_BOUNCE_BACK: bc_regularized_bounce_back(g, x, y, G); break;
+        case LBM_ZG_OUTFLOW: {
+            // zeroGradientOutflow.cuh:9-59: unknowns (c.n > 0) from the interior neighbour.
+            int n0 = 0, n1 = 0, ix = x, iy = y;
+            if (x == 0) { n0 = 1; ix = 1; }
+            else if (x == p.nx - 1) { n0 = -1; ix = p.nx - 2; }
+            else if (y == 0) { n1 = 1; iy = 1; }
+            else if (y == p.ny - 1) { n1 = -1; iy = p.ny - 2; }
+            else break;
+#pragma unroll
+            for (int i = 0; i < Q; i++)
+                if (CX[i] * n0 + CY[i] * n1 > 0) g[i] = gather<STORE>(p, i, ix, iy);
+            break;
+        }
+        case LBM_REGULARIZED_BOUNCE_BACK_CORNER: {
+            int dx = x == 0 ? x + 1 : x - 1, dy = y == 0 ? y + 1 : y - 1;
+            dx = dx < 1 ? 1 : (dx > p.nx - 2 ? p.nx - 2 : dx);
+            dy = dy < 1 ? 1 : (dy > p.ny - 2 ? p.ny - 2 : dy);
+            float rd = 0.f;
+#pragma unroll
+            for (int i = 0; i < Q; i++) rd += gather<STORE>(p, i, dx, dy);
+            bc_regularized_corner(g, x, y, G, rd);
+            break;
+        }
+        default: break;
+    }
+    return r;
+}
+
+// Node force: uniform, else the force field, else (inside the IBM box) the IBM total.
+LBM_HD void node_force(const StepParams& p, int x, int y, long long node, float& fx, float& fy) {
+    fx = p.fx;
+    fy = p.fy;
+    if (p.force_field) {
+        fx = p.force_field[2 * node];
+        fy = p.force_field[2 * node + 1];
+    }
+    if (p.ibm_force && x >= p.ibm_x0 && x < p.ibm_x1 && y >= p.ibm_y0 && y < p.ibm_y1) {
+        fx = p.ibm_force[2 * node];
+        fy = p.ibm_force[2 * node + 1];
+    }
+}
+
+// Collide one node in place. MODE: 0 step, 1 moments only, 2 step + moments.
+template <int COLL, int MODE>
+LBM_HD void node_update(const StepParams& p, float* g, float fx, float fy, Macro& m, MomAcc& acc) {
+    m = macroscopics(g, fx, fy);
+    if (MODE != 0) {
+        acc.rho += double(m.rho);
+        acc.jmag += double(m.rho * sqrtf(m.ux * m.ux + m.uy * m.uy));
+        acc.pimag += double(pi_magnitude(g));
+    }
+    if (MODE == 1) return;
+    if (COLL == LBM_BGK) collide_bgk(g, m, fx, fy, p.omega, p.forced != 0);
+    else if (COLL == LBM_MRT) collide_mrt(g, m, fx, fy, p.S, (p.quirks & LBM_QUIRK_MRT_FORCE_ROWS) != 0);
+    else collide_cm(g, m, fx, fy, p.S, p.avg);
+}
+
+// The thread body. `seg` indexes segments row-major; out-of-range segments do nothing.
+template <int COLL, int STORE, int MODE>
+LBM_HD void step_segment(const StepParams& p, long long seg, MomAcc& acc) {
+    using St = Storage<STORE>;
+    using T = typename St::T;
+    constexpr int V = St::V;
+    const long long nseg = (long long)p.ntx * p.ny;
+    if (seg >= nseg) return;
+    const int y = int(seg / p.ntx);
+    const int x0 = int(seg - (long long)y * p.ntx) * V;
+    const long long row0 = (long long)y * p.nx;
+    const T* src = static_cast<const T*>(p.src);
+    T* dst = static_cast<T*>(p.dst);
+
+    bool fast = x0 >= 1 && x0 + V <= p.nx - 1;
+    if (fast && !p.py) fast = y >= 1 && y <= p.ny - 2;
+    if (fast && p.row_class) {
+        uint8_t rc = p.row_class[y];
+        if (rc == 2) {
+#pragma unroll
+            for (int j = 0; j < V; j++) fast = fast && p.flags[row0 + x0 + j] == LBM_FLUID;
+        }
+    }
+    if (fast && (p.quirks & LBM_QUIRK_STALE_REST)) fast = false;
+
+    float pop[Q][V];
+    int nvalid = V;
+    if (fast) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            int ys = y - CY[q];
+            if (p.py) ys = ys < 0 ? ys + p.ny : (ys >= p.ny ? ys - p.ny : ys);
+            St::load(src + q * p.plane + (long long)ys * p.nx + (x0 - CX[q]), q, p.rho0, pop[q]);
+        }
+    } else {
+        nvalid = p.nx - x0 < V ? p.nx - x0 : V;
+        const NodeArgs na = node_args(p);
+#pragma unroll
+        for (int j = 0; j < V; j++) {
+            if (j < nvalid) {
+                Pop9 r = gather_node<STORE>(na, x0 + j, y);
+#pragma unroll
+                for (int q = 0; q < Q; q++) pop[q][j] = r.v[q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < Q; q++) pop[q][j] = 0.f;
+            }
+        }
+    }
+
+    float orho[V], oux[V], ouy[V];
+#pragma unroll
+    for (int j = 0; j < V; j++) {
+        if (j < nvalid) {
+            float g[Q];
+#pragma unroll
+            for (int q = 0; q < Q; q++) g[q] = pop[q][j];
+            float fx, fy;
+            node_force(p, x0 + j, y, row0 + x0 + j, fx, fy);
+            Macro m;
+            node_update<COLL, MODE>(p, g, fx, fy, m, acc);
+#pragma unroll
+            for (int q = 0; q < Q; q++) pop[q][j] = g[q];
+            orho[j] = m.rho;
+            oux[j] = m.ux;
+            ouy[j] = m.uy;
+        } else {
+            orho[j] = oux[j] = ouy[j] = 0.f;
+        }
+    }
+    if (MODE == 1) return;
+
+    if (nvalid == V) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) St::store(dst + q * p.plane + row0 + x0, q, p.rho0, pop[q]);
+        if (p.out_rho) {
+            __builtin_memcpy(p.out_rho + row0 + x0, orho, sizeof(orho));
+            __builtin_memcpy(p.out_ux + row0 + x0, oux, sizeof(oux));
+            __builtin_memcpy(p.out_uy + row0 + x0, ouy, sizeof(ouy));
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < V; j++) {
+            if (j < nvalid) {
+#pragma unroll
+                for (int q = 0; q < Q; q++) St::store1(dst + q * p.plane + row0 + x0 + j, q, p.rho0, pop[q][j]);
+                if (p.out_rho) {
+                    p.out_rho[row0 + x0 + j] = orho[j];
+                    p.out_ux[row0 + x0 + j] = oux[j];
+                    p.out_uy[row0 + x0 + j] = ouy[j];
+                }
+            }
+        }
+    }
+}
+
+}  // namespace lbm

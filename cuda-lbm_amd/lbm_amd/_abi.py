@@ -1,0 +1,135 @@
+"""ctypes view of include/lbm_hip.h and the loader of liblbm_hip.so.
+
+The library is the product: if it is missing this module raises, it never falls back to
+anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "liblbm_hip.so"
+
+# status codes
+LBM_OK = 0
+LBM_E_INVALID = -1
+LBM_E_UNSUPPORTED = -2
+LBM_E_HIP = -3
+LBM_E_NOMEM = -4
+LBM_E_STATE = -5
+
+# collision operators / adapters / storage
+LBM_BGK, LBM_MRT, LBM_CM = 0, 1, 2
+LBM_NO_ADAPTER, LBM_OPTIMAL_ADAPTER = 0, 1
+LBM_ADAPTER_SAME_STEP, LBM_ADAPTER_LAGGED = 0, 1
+LBM_F32, LBM_F16_SHIFTED, LBM_BF16_SHIFTED = 0, 1, 2
+
+# boundary flags (src/core/lbm_constants.cuh:377-397 order, + MOVING_WALL_BOUNCE_BACK)
+FLUID = 0
+BOUNCE_BACK = 1
+ZOU_HE_TOP = 2
+ZOU_HE_LEFT = 3
+ZOU_HE_TOP_LEFT_TOP_INFLOW = 4
+ZOU_HE_TOP_RIGHT_TOP_INFLOW = 5
+CYLINDER = 6
+ZG_OUTFLOW = 7
+PRESSURE_OUTLET = 8
+REGULARIZED_INLET_TOP = 9
+REGULARIZED_INLET_LEFT = 10
+REGULARIZED_BOUNCE_BACK = 11
+REGULARIZED_BOUNCE_BACK_CORNER = 12
+EXTRAPOLATED_CORNER_EDGE = 13
+CORNER_EDGE_BOUNCE_BACK = 14
+PRESSURE_INLET_LEFT = 15
+GUO_VELOCITY_INLET = 16
+GUO_PRESSURE_OUTLET = 17
+REGULARIZED_OUTLET = 18
+MOVING_WALL_BOUNCE_BACK = 19
+
+# quirks
+LBM_QUIRK_STALE_REST = 1 << 0
+LBM_QUIRK_ZOUHE_RHO = 1 << 1
+LBM_QUIRK_MRT_FORCE_ROWS = 1 << 2
+LBM_QUIRK_IBM_2X2 = 1 << 3
+LBM_QUIRK_IBM_CLAMP = 1 << 4
+LBM_QUIRK_IBM_NO_DS = 1 << 5
+
+# reductions
+LBM_REDUCE_MASS = 0
+LBM_REDUCE_MOMENTUM = 1
+LBM_REDUCE_L2_VS_REF = 2
+LBM_REDUCE_ROW_MEAN_UX = 3
+LBM_REDUCE_MOMENT_AVG = 4
+
+
+class LbmConfig(ctypes.Structure):
+    """struct lbm_config (include/lbm_hip.h)."""
+
+    _fields_ = [
+        ("nx", ctypes.c_int32),
+        ("ny", ctypes.c_int32),
+        ("collision", ctypes.c_int32),
+        ("adapter", ctypes.c_int32),
+        ("adapter_mode", ctypes.c_int32),
+        ("omega", ctypes.c_float),
+        ("S", ctypes.c_float * 9),
+        ("rho0", ctypes.c_float),
+        ("storage", ctypes.c_int32),
+        ("periodic_x", ctypes.c_int32),
+        ("periodic_y", ctypes.c_int32),
+        ("force", ctypes.c_float * 2),
+        ("wall_u", ctypes.c_float * 2),
+        ("u_lid", ctypes.c_float),
+        ("quirks", ctypes.c_uint32),
+        ("reserved", ctypes.c_int32 * 8),
+    ]
+
+
+# (name, restype, argtypes) of every entry point include/lbm_hip.h declares.
+_P = ctypes.c_void_p
+_F = ctypes.POINTER(ctypes.c_float)
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.c_int32
+PROTOTYPES = [
+    ("lbm_config_default", _I, [ctypes.POINTER(LbmConfig), _I, _I]),
+    ("lbm_create", _I, [ctypes.POINTER(LbmConfig), ctypes.POINTER(_P)]),
+    ("lbm_destroy", _I, [_P]),
+    ("lbm_set_stream", _I, [_P, _P]),
+    ("lbm_set_flags", _I, [_P, _P, _I]),
+    ("lbm_init_from_macros", _I, [_P, _P, _P, _I]),
+    ("lbm_set_force_field", _I, [_P, _P, _I]),
+    ("lbm_step", _I, [_P, _I]),
+    ("lbm_sync", _I, [_P]),
+    ("lbm_get_timestep", _I, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    ("lbm_get_macros", _I, [_P, _P, _P, _I]),
+    ("lbm_get_pops", _I, [_P, _P, _I]),
+    ("lbm_set_pops", _I, [_P, _P, _I]),
+    ("lbm_reduce", _I, [_P, _I, _P, _D]),
+    ("lbm_ibm_set_markers", _I, [_P, _P, _P, _I, _I]),
+    ("lbm_ibm_get_forces", _I, [_P, _P, _D]),
+    ("lbm_adjoint_forward", _I, [_P, _I, _I]),
+    ("lbm_adjoint_backward", _I, [_P, _P, _P, _P, _I]),
+    ("lbm_last_error", ctypes.c_char_p, [_P]),
+    ("lbm_version", ctypes.c_char_p, []),
+]
+
+_lib = None
+
+
+def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
+    """Load liblbm_hip.so (built by `make -C cuda-lbm_amd`) and declare its prototypes."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"liblbm_hip.so not found at {p}; build it with `make -C cuda-lbm_amd`")
+    lib = ctypes.CDLL(str(p))
+    for name, res, args in PROTOTYPES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib

@@ -1,0 +1,223 @@
+// lbm_emu.cpp — host emulation of liblbm_hip.so's step sequence (test infrastructure).
+//
+// Runs the product's own per-thread body (cuda-lbm_amd/csrc/lbm_step.hpp, a __host__
+// __device__ function) over the same segment grid on the CPU, with population planes
+// packed back to back (plane = N, no padding) so that any out-of-range index the kernel
+// would form lands outside an exact-size allocation. Built two ways:
+//   liblbm_emu.so  — driven from the CPU tests (parity of the kernel logic vs the oracle);
+//   lbm_emu_asan   — AddressSanitizer executable running a matrix of small cases (bounds
+//                    check of every access the kernel body makes, before any GPU run).
+#include "../../cuda-lbm_amd/csrc/lbm_step.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+using namespace lbm;
+
+namespace {
+
+template <int STORE>
+using ElemT = typename Storage<STORE>::T;
+
+template <int COLL, int STORE>
+int run_t(const lbm_config& c, const uint8_t* flags, const float* rho, const float* u, const float* force,
+          int nsteps, float* f_out, float* rho_out, float* u_out, double* avg_out) {
+    using St = Storage<STORE>;
+    using T = ElemT<STORE>;
+    const int nx = c.nx, ny = c.ny;
+    const long long N = (long long)nx * ny;
+    T* buf[2];
+    buf[0] = new T[9 * N];
+    buf[1] = new T[9 * N];
+    for (long long n = 0; n < N; n++) {
+        float r = rho[n], ux = u[2 * n], uy = u[2 * n + 1];
+        float usq15 = 1.5f * (ux * ux + uy * uy);
+        for (int q = 0; q < Q; q++) {
+            float fe = feq(q, r, ux, uy, usq15);
+            St::store1(buf[0] + q * N + n, q, c.rho0, fe);
+            St::store1(buf[1] + q * N + n, q, c.rho0, fe);
+        }
+    }
+    uint8_t* fl = nullptr;
+    uint8_t* rc = nullptr;
+    bool any = false;
+    if (flags) {
+        for (long long n = 0; n < N; n++) any = any || flags[n] != LBM_FLUID;
+    }
+    if (any) {
+        fl = new uint8_t[N];
+        rc = new uint8_t[ny];
+        std::memcpy(fl, flags, N);
+        row_classes(fl, nx, ny, rc);
+    }
+    float* ff = nullptr;
+    if (force) {
+        ff = new float[2 * N];
+        std::memcpy(ff, force, 2 * N * sizeof(float));
+    }
+    float* out_r = new float[N];
+    float* out_x = new float[N];
+    float* out_y = new float[N];
+    float avg[3] = {0.f, 0.f, 0.f};
+    double avg64[3] = {0, 0, 0};
+    const bool adapt = c.collision == LBM_CM && c.adapter == LBM_OPTIMAL_ADAPTER;
+    int cur = 0;
+    for (int s = 0; s < nsteps; s++) {
+        StepParams p{};
+        p.nx = nx;
+        p.ny = ny;
+        p.px = c.periodic_x;
+        p.py = c.periodic_y;
+        p.plane = N;
+        p.src = buf[cur];
+        p.dst = buf[1 - cur];
+        p.flags = fl;
+        p.row_class = rc;
+        p.omega = c.omega;
+        for (int k = 0; k < 9; k++) p.S[k] = c.S[k];
+        p.rho0 = c.rho0;
+        p.fx = c.force[0];
+        p.fy = c.force[1];
+        p.force_field = ff;
+        p.forced = (c.force[0] != 0.f || c.force[1] != 0.f || ff) ? 1 : 0;
+        p.wall_ux = c.wall_u[0];
+        p.wall_uy = c.wall_u[1];
+        p.u_lid = c.u_lid;
+        p.quirks = c.quirks;
+        p.avg = adapt ? avg : nullptr;
+        bool last = s == nsteps - 1;
+        p.out_rho = last ? out_r : nullptr;
+        p.out_ux = last ? out_x : nullptr;
+        p.out_uy = last ? out_y : nullptr;
+        p.ntx = (nx + St::V - 1) / St::V;
+        const long long nseg = (long long)p.ntx * ny;
+        if (adapt) {
+            MomAcc tot{0, 0, 0};
+            for (long long sg = 0; sg < nseg; sg++) step_segment<LBM_BGK, STORE, 1>(p, sg, tot);
+            avg64[0] = tot.rho / double(N);
+            avg64[1] = tot.jmag / double(N);
+            avg64[2] = tot.pimag / double(N);
+            for (int k = 0; k < 3; k++) avg[k] = float(avg64[k]);
+        }
+        MomAcc dummy{0, 0, 0};
+        for (long long sg = 0; sg < nseg; sg++) step_segment<COLL, STORE, 0>(p, sg, dummy);
+        cur = 1 - cur;
+    }
+    if (f_out)
+        for (int q = 0; q < Q; q++)
+            for (long long n = 0; n < N; n++) f_out[q * N + n] = St::load1(buf[cur] + q * N + n, q, c.rho0);
+    if (nsteps > 0) {
+        for (long long n = 0; n < N; n++) {
+            if (rho_out) rho_out[n] = out_r[n];
+            if (u_out) {
+                u_out[2 * n] = out_x[n];
+                u_out[2 * n + 1] = out_y[n];
+            }
+        }
+    }
+    if (avg_out)
+        for (int k = 0; k < 3; k++) avg_out[k] = avg64[k];
+    delete[] buf[0];
+    delete[] buf[1];
+    delete[] fl;
+    delete[] rc;
+    delete[] ff;
+    delete[] out_r;
+    delete[] out_x;
+    delete[] out_y;
+    return 0;
+}
+
+template <int STORE>
+int run_s(const lbm_config& c, const uint8_t* flags, const float* rho, const float* u, const float* force, int n,
+          float* f, float* r, float* uu, double* a) {
+    if (c.collision == LBM_BGK) return run_t<LBM_BGK, STORE>(c, flags, rho, u, force, n, f, r, uu, a);
+    if (c.collision == LBM_MRT) return run_t<LBM_MRT, STORE>(c, flags, rho, u, force, n, f, r, uu, a);
+    return run_t<LBM_CM, STORE>(c, flags, rho, u, force, n, f, r, uu, a);
+}
+
+}  // namespace
+
+extern "C" int emu_run(const lbm_config* c, const uint8_t* flags, const float* rho, const float* u, const float* force,
+                       int nsteps, float* f_out, float* rho_out, float* u_out, double* avg_out) {
+    if (c->storage == LBM_F16_SHIFTED) return run_s<LBM_F16_SHIFTED>(*c, flags, rho, u, force, nsteps, f_out, rho_out, u_out, avg_out);
+    if (c->storage == LBM_BF16_SHIFTED) return run_s<LBM_BF16_SHIFTED>(*c, flags, rho, u, force, nsteps, f_out, rho_out, u_out, avg_out);
+    return run_s<LBM_F32>(*c, flags, rho, u, force, nsteps, f_out, rho_out, u_out, avg_out);
+}
+
+#ifdef EMU_MAIN
+// Bounds matrix: every collision x storage x boundary layout on small and ragged grids.
+int main() {
+    const int sizes[][2] = {{3, 3}, {5, 4}, {8, 8}, {13, 7}, {16, 9}, {33, 17}, {64, 5}};
+    int cases = 0;
+    for (auto& sz : sizes) {
+        for (int layout = 0; layout < 5; layout++) {
+            for (int coll = 0; coll < 3; coll++) {
+                for (int storage = 0; storage < 3; storage++) {
+                    for (int adapt = 0; adapt < (coll == LBM_CM ? 2 : 1); adapt++) {
+                        lbm_config c;
+                        std::memset(&c, 0, sizeof(c));
+                        c.nx = sz[0];
+                        c.ny = sz[1];
+                        c.collision = coll;
+                        c.adapter = adapt;
+                        c.omega = 1.2f;
+                        for (int k = 0; k < 9; k++) c.S[k] = 1.1f;
+                        c.rho0 = 1.0f;
+                        c.storage = storage;
+                        c.u_lid = 0.05f;
+                        c.wall_u[0] = 0.04f;
+                        c.force[0] = 1e-5f;
+                        const long long N = (long long)c.nx * c.ny;
+                        std::vector<uint8_t> fl(N, LBM_FLUID);
+                        for (int y = 0; y < c.ny; y++)
+                            for (int x = 0; x < c.nx; x++) {
+                                uint8_t& f = fl[(long long)y * c.nx + x];
+                                bool xe = x == 0 || x == c.nx - 1, ye = y == 0 || y == c.ny - 1;
+                                switch (layout) {
+                                    case 0: c.periodic_x = c.periodic_y = 1; break;  // all fluid, periodic
+                                    case 1:  // channel: periodic x, BB walls
+                                        c.periodic_x = 1;
+                                        if (ye) f = LBM_BOUNCE_BACK;
+                                        break;
+                                    case 2:  // cavity: moving lid + BB
+                                        if (y == c.ny - 1 && !xe) f = LBM_MOVING_WALL_BOUNCE_BACK;
+                                        else if (xe || ye) f = LBM_BOUNCE_BACK;
+                                        break;
+                                    case 3:  // regularized cavity
+                                        if (xe && ye) f = LBM_REGULARIZED_BOUNCE_BACK_CORNER;
+                                        else if (y == c.ny - 1) f = LBM_REGULARIZED_INLET_TOP;
+                                        else if (xe || ye) f = LBM_REGULARIZED_BOUNCE_BACK;
+                                        break;
+                                    case 4:  // inflow / outflow channel
+                                        if (ye) f = LBM_BOUNCE_BACK;
+                                        else if (x == 0) f = LBM_ZOU_HE_LEFT;
+                                        else if (x == c.nx - 1) f = LBM_ZG_OUTFLOW;
+                                        else if (x == c.nx / 2 && y == c.ny / 2) f = LBM_BOUNCE_BACK;
+                                        break;
+                                }
+                            }
+                        std::vector<float> rho(N), u(2 * N), F(2 * N);
+                        for (long long n = 0; n < N; n++) {
+                            rho[n] = 1.0f + 0.001f * float(n % 7);
+                            u[2 * n] = 0.01f * float(n % 5) - 0.02f;
+                            u[2 * n + 1] = 0.01f * float(n % 3) - 0.01f;
+                            F[2 * n] = 1e-6f * float(n % 4);
+                            F[2 * n + 1] = -1e-6f * float(n % 3);
+                        }
+                        std::vector<float> f(9 * N), r(N), uu(2 * N);
+                        std::vector<double> a(3);
+                        emu_run(&c, fl.data(), rho.data(), u.data(), (cases & 1) ? F.data() : nullptr, 3, f.data(),
+                                r.data(), uu.data(), a.data());
+                        cases++;
+                    }
+                }
+            }
+        }
+    }
+    std::printf("emu bounds matrix: %d cases ok\n", cases);
+    return 0;
+}
+#endif

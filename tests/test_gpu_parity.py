@@ -1,0 +1,110 @@
+"""GPU parity: liblbm_hip.so on the MI355X against the oracle, through the C ABI.
+
+Bar: FP32 populations within 1e-5 relative of the FP32 oracle after N steps (the
+tolerance the build targets); reduced storages within one storage quantum."""
+import numpy as np
+import pytest
+
+from helpers import layout_case, max_rel, run_emu, run_gpu, run_oracle
+from lbm_amd import _abi as A
+from lbm_amd import scenarios as sc
+
+pytestmark = pytest.mark.gpu
+
+LAYOUTS = ["periodic", "channel", "cavity_mw", "cavity_reg", "inout", "zouhe_top", "open"]
+OPS = [(A.LBM_BGK, 0), (A.LBM_MRT, 0), (A.LBM_CM, 0), (A.LBM_CM, 1)]
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("op", OPS, ids=["bgk", "mrt", "cm", "cm_optimal"])
+def test_gpu_matches_oracle_fp32(gpu_available, layout, op):
+    coll, adapter = op
+    print(f"case layout={layout} op={op} 37x21 x 12 steps", flush=True)
+    case = layout_case(layout, 37, 21, coll, adapter=adapter)
+    fg, rg, ug, s = run_gpu(case, 12)
+    fo, ro, uo, _ = run_oracle(case, 12, 32)
+    ef, er, eu = max_rel(fg, fo), max_rel(rg, ro), float(np.max(np.abs(ug - uo)))
+    assert ef <= 1e-5, ef
+    assert er <= 1e-5, er
+    assert eu <= 1e-6, eu
+    s.close()
+
+
+@pytest.mark.parametrize("storage", [A.LBM_F16_SHIFTED, A.LBM_BF16_SHIFTED])
+@pytest.mark.parametrize("coll", [A.LBM_BGK, A.LBM_MRT, A.LBM_CM])
+def test_gpu_reduced_storage(gpu_available, storage, coll):
+    print(f"case storage={storage} coll={coll}", flush=True)
+    case = layout_case("cavity_mw", 40, 24, coll, storage=storage)
+    fg, *_ = run_gpu(case, 10)
+    fo, *_ = run_oracle(case, 10, 32)
+    quantum = 2.0 ** -11 if storage == A.LBM_F16_SHIFTED else 2.0 ** -8
+    # FP32 rounding differences (FMA contraction on the GPU) occasionally flip the rounding
+    # of a 16-bit deviation (|d| < 0.05): bound = a few storage quanta.
+    err = float(np.max(np.abs(fg - fo)))
+    assert err <= 4 * 0.05 * quantum, err
+
+
+@pytest.mark.parametrize("quirk", [A.LBM_QUIRK_STALE_REST, A.LBM_QUIRK_ZOUHE_RHO, A.LBM_QUIRK_MRT_FORCE_ROWS])
+def test_gpu_quirk_modes(gpu_available, quirk):
+    coll = A.LBM_MRT if quirk == A.LBM_QUIRK_MRT_FORCE_ROWS else A.LBM_BGK
+    case = layout_case("inout", 33, 17, coll, quirks=quirk)
+    fg, *_ = run_gpu(case, 8)
+    fo, *_ = run_oracle(case, 8, 32)
+    assert max_rel(fg, fo) <= 1e-5
+
+
+@pytest.mark.parametrize("shape", [(3, 3), (5, 4), (9, 5), (31, 6), (67, 3), (130, 66)])
+def test_gpu_ragged_and_tiny_grids(gpu_available, shape):
+    for layout in ["periodic", "cavity_mw", "open", "inout"]:
+        print(f"case {layout} {shape}", flush=True)
+        case = layout_case(layout, *shape, A.LBM_MRT)
+        fg, *_ = run_gpu(case, 6)
+        fo, *_ = run_oracle(case, 6, 32)
+        assert max_rel(fg, fo) <= 1e-5, (layout, shape)
+
+
+def test_gpu_taylor_green_128_against_oracle_and_analytic(gpu_available):
+    """BASELINE config 1 (TG 128^2, BGK, nu=0.01, u0=0.04) for 1000 steps."""
+    case = sc.taylor_green(128, nu=0.01, u0=0.04)
+    fg, rg, ug, s = run_gpu(case, 1000)
+    fo, ro, uo, _ = run_oracle(case, 1000, 32)
+    assert max_rel(fg, fo) <= 1e-5
+    e_gpu = sc.tg_error(ug, 0.01, 0.04, 1000)
+    _, _, u64, _ = run_oracle(case, 1000, 64)
+    e_64 = sc.tg_error(u64, 0.01, 0.04, 1000)
+    assert abs(e_gpu - e_64) <= 1e-3 * max(e_64, 1.0)
+    assert e_gpu < 1.0  # percent; analytic decay exp(-2 nu k^2 t)
+
+
+def test_gpu_mass_conservation_periodic(gpu_available):
+    case = layout_case("periodic", 256, 128, A.LBM_MRT, forced=True)
+    from lbm_amd import Solver
+
+    s = Solver(case.config)
+    case.apply(s)
+    s.step(1)
+    m0 = s.reduce(A.LBM_REDUCE_MASS)[0]
+    s.step(200)
+    m1 = s.reduce(A.LBM_REDUCE_MASS)[0]
+    assert abs(m1 - m0) / m0 < 1e-6
+    # momentum grows by exactly F per node per step (Guo forcing)
+    p0 = s.reduce(A.LBM_REDUCE_MOMENTUM)
+    s.step(100)
+    p1 = s.reduce(A.LBM_REDUCE_MOMENTUM)
+    N = case.config.N
+    assert abs((p1[0] - p0[0]) - 100 * N * 2e-5) < 1e-3 * 100 * N * 2e-5
+    s.close()
+
+
+def test_gpu_errors_are_reported(gpu_available):
+    from lbm_amd import Config, LbmError, Solver
+
+    with pytest.raises(LbmError):
+        Solver(Config(nx=2, ny=8))
+    s = Solver(Config(nx=8, ny=8))
+    with pytest.raises(LbmError):
+        s.step(1)  # not initialised
+    flags = np.full((8, 8), A.PRESSURE_OUTLET, np.uint8)
+    with pytest.raises(LbmError):
+        s.set_flags(flags)
+    s.close()
