@@ -90,6 +90,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-extras", action="store_true", help="only the headline workload")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
+                    help="also run one plain device copy of MB megabytes (for scaling profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
@@ -147,6 +149,14 @@ def main():
         grid["speedup_f16_vs_f32"] = round(grid["f16"]["mlups"] / grid["f32"]["mlups"], 3)
         grid["workload"] = "periodic BGK 8192x8192 (2 x 2.4 GB FP32 population buffers)"
         out["hbm_grid"] = grid
+    if args.calibrate_copy:
+        n = args.calibrate_copy * 1000000 // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = {"read": 4 * n, "write": 4 * n}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

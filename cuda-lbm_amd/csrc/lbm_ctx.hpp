@@ -48,6 +48,7 @@ struct AdjointState {
     float* d_grad_force = nullptr;         // AoS [N][2]
     float* d_u0 = nullptr;                 // initial velocity AoS (for dL/du0)
     float* d_rho_init = nullptr;
+    bool has_u0 = false;                   // forward started from lbm_init_from_macros
     bool ready = false;
 };
 

@@ -63,21 +63,42 @@ LBM_HD float feq(int q, float rho, float ux, float uy, float usq15) {
     return wgt(q) * rho * (1.0f + 3.0f * cu + 4.5f * cu * cu - usq15);
 }
 
-// BGK<2>::apply with Guo forcing (BGK.cuh:13-51).
+// BGK<2>::apply with Guo forcing (BGK.cuh:13-51). Directions come in opposite pairs
+// (q, opp q): the equilibrium and the Guo term split into a part even in c (shared by the
+// pair) and a part odd in c (added to one, subtracted from the other), so each pair costs
+// one evaluation.
 LBM_HD void collide_bgk(float* g, const Macro& m, float fx, float fy, float omega, bool forced) {
-    float usq15 = 1.5f * (m.ux * m.ux + m.uy * m.uy);
-    float gf = 1.0f - 0.5f * omega;
+    const float ux = m.ux, uy = m.uy, rho = m.rho;
+    const float base = 1.0f - 1.5f * (ux * ux + uy * uy);
+    const float om1 = 1.0f - omega;
+    const float gf = 1.0f - 0.5f * omega;
+    const float uf3 = 3.0f * (ux * fx + uy * fy);
+    {  // rest population
+        float fe = wgt(0) * rho * base;
+        float v = om1 * g[0] + omega * fe;
+        if (forced) v += wgt(0) * gf * (-uf3);
+        g[0] = v;
+    }
+    constexpr int PQ[4] = {1, 2, 5, 6};  // pairs (1,3), (2,4), (5,7), (6,8)
 #pragma unroll
-    for (int q = 0; q < Q; q++) {
-        float fe = feq(q, m.rho, m.ux, m.uy, usq15);
-        float v = g[q] - omega * (g[q] - fe);
+    for (int k = 0; k < 4; k++) {
+        const int q = PQ[k], o = OPP[q];
+        const float cx = float(CX[q]), cy = float(CY[q]);
+        const float cu = cx * ux + cy * uy;
+        const float wr = wgt(q) * rho;
+        const float sym = wr * (base + 4.5f * cu * cu);
+        const float asym = 3.0f * wr * cu;
+        float vq = om1 * g[q] + omega * (sym + asym);
+        float vo = om1 * g[o] + omega * (sym - asym);
         if (forced) {
-            float cx = float(CX[q]), cy = float(CY[q]);
-            float cu = cx * m.ux + cy * m.uy;
-            float ft = wgt(q) * gf * ((3.0f * (cx - m.ux) + 9.0f * cu * cx) * fx + (3.0f * (cy - m.uy) + 9.0f * cu * cy) * fy);
-            v += ft;
+            const float cf = cx * fx + cy * fy;
+            const float fs = wgt(q) * gf * (9.0f * cu * cf - uf3);
+            const float fa = wgt(q) * gf * 3.0f * cf;
+            vq += fs + fa;
+            vo += fs - fa;
         }
-        g[q] = v;
+        g[q] = vq;
+        g[o] = vo;
     }
 }
 

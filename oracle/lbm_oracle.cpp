@@ -4,6 +4,7 @@
 // cpu_baseline leg as the checker. Values cross this boundary as double; the solver runs
 // in float (precision 32) or double (precision 64).
 #include "lbm_oracle.hpp"
+#include "lbm_oracle_adjoint.hpp"
 
 #include <memory>
 #include <omp.h>
@@ -158,6 +159,28 @@ int oracle_get_force(void* p, double* F) {
             F[2 * n + 1] = double(s.force[s.N + n]);
         }
     });
+}
+
+// FP64 adjoint from the current (freshly initialised) state over T steps: dL/dF and dL/du0.
+int oracle_adjoint(void* p, int T, const double* dLdu, double* gradF, double* gradU0) {
+    auto* h = static_cast<Handle*>(p);
+    if (h->prec != 64) {
+        h->err = "the oracle adjoint runs in FP64";
+        return -1;
+    }
+    try {
+        adjoint(*h->s64, T, dLdu, gradF, gradU0);
+    } catch (const std::exception& e) {
+        h->err = e.what();
+        return -1;
+    }
+    return 0;
+}
+
+// 9x11 Jacobian of the node update (seeds g[0..8], Fx, Fy), row-major.
+int oracle_node_jacobian(const lbm_config* c, const double* g, const double* F, double* J) {
+    node_jacobian(*c, g, F[0], F[1], J);
+    return 0;
 }
 
 const char* oracle_last_error(void* p) { return static_cast<Handle*>(p)->err.c_str(); }

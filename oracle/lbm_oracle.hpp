@@ -23,6 +23,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace oracle {
@@ -143,7 +144,8 @@ inline float bf16_to_f32(uint16_t h) {
 // pow() and double literals promote the bracket to double (quirk Q15).
 template <class R>
 inline void equilibrium(R rho, R ux, R uy, R* feq, bool literal) {
-    if (literal) {
+    if constexpr (std::is_floating_point<R>::value) {
+      if (literal) {
         float cs = 1.0f / std::sqrt(3.0f);
         float cs2 = cs * cs;
         float cs4 = cs2 * cs2;
@@ -154,6 +156,7 @@ inline void equilibrium(R rho, R ux, R uy, R* feq, bool literal) {
             feq[q] = R(float(weight<float>(q) * float(rho) * br));
         }
         return;
+      }
     }
     R usq = ux * ux + uy * uy;
     for (int q = 0; q < Q; q++) {

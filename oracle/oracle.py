@@ -47,6 +47,8 @@ def lib():
             "oracle_ibm_get_forces": (ctypes.c_int, [_P, _D, _D]),
             "oracle_get_force": (ctypes.c_int, [_P, _D]),
             "oracle_last_error": (ctypes.c_char_p, [_P]),
+            "oracle_adjoint": (ctypes.c_int, [_P, ctypes.c_int, _D, _D, _D]),
+            "oracle_node_jacobian": (ctypes.c_int, [_P, _D, _D, _D]),
             "oracle_num_threads": (ctypes.c_int, []),
             "oracle_set_num_threads": (None, [ctypes.c_int]),
             "oracle_tables": (None, [_D, _P, _P, _P, _D, _D]),
@@ -190,6 +192,14 @@ class Oracle:
         self._ck(self.L.oracle_get_force(self.h, _dp(F)))
         return F.reshape(self.ny, self.nx, 2)
 
+    def adjoint(self, T, dLdu):
+        """FP64 adjoint over T steps from the freshly initialised state: (dL/dF, dL/du0)."""
+        g = np.ascontiguousarray(dLdu, np.float64).reshape(-1)
+        gF = np.zeros(2 * self.N)
+        gu = np.zeros(2 * self.N)
+        self._ck(self.L.oracle_adjoint(self.h, int(T), _dp(g), _dp(gF), _dp(gu)))
+        return gF.reshape(self.ny, self.nx, 2), gu.reshape(self.ny, self.nx, 2)
+
     def ibm_set_markers(self, xy, ds=None, iters=3):
         a = np.ascontiguousarray(xy, np.float64).reshape(-1)
         n = a.size // 2
@@ -202,6 +212,14 @@ class Oracle:
         tot = np.zeros(2)
         self._ck(self.L.oracle_ibm_get_forces(self.h, _dp(lag), _dp(tot)))
         return lag.reshape(self._n, 2), tot
+
+
+def node_jacobian(cfg_c, g, F=(0.0, 0.0)):
+    J = np.zeros(99)
+    gg = np.asarray(g, np.float64)
+    FF = np.asarray(F, np.float64)
+    lib().oracle_node_jacobian(ctypes.byref(cfg_c), _dp(gg), _dp(FF), _dp(J))
+    return J.reshape(9, 11)
 
 
 def num_threads():

@@ -8,6 +8,7 @@
 //   lbm_emu_asan   — AddressSanitizer executable running a matrix of small cases (bounds
 //                    check of every access the kernel body makes, before any GPU run).
 #include "../../cuda-lbm_amd/csrc/lbm_step.hpp"
+#include "../../cuda-lbm_amd/csrc/lbm_adjoint.hpp"
 
 #include <cstdio>
 #include <cstdlib>
@@ -145,6 +146,23 @@ extern "C" int emu_run(const lbm_config* c, const uint8_t* flags, const float* r
     if (c->storage == LBM_F16_SHIFTED) return run_s<LBM_F16_SHIFTED>(*c, flags, rho, u, force, nsteps, f_out, rho_out, u_out, avg_out);
     if (c->storage == LBM_BF16_SHIFTED) return run_s<LBM_BF16_SHIFTED>(*c, flags, rho, u, force, nsteps, f_out, rho_out, u_out, avg_out);
     return run_s<LBM_F32>(*c, flags, rho, u, force, nsteps, f_out, rho_out, u_out, avg_out);
+}
+
+// The product's node adjoints (lbm_adjoint.hpp) on the host: mode 0 BGK, 1 MRT, 2 snapshot
+// seed (lam = dL/du in lam[0..1]), 3 initial-state feq (g[0] = rho0, F = u0).
+extern "C" int emu_adjoint_node(int mode, const lbm_config* c, const float* g, const float* lam, const float* F,
+                                float* mu, float* gF) {
+    AdjOut o{};
+    if (mode == 0) o = adjoint_bgk(g, lam, F[0], F[1], c->omega);
+    else if (mode == 1) o = adjoint_mrt(g, lam, F[0], F[1], c->S);
+    else if (mode == 2) o = adjoint_snapshot(g, F[0], F[1], lam[0], lam[1]);
+    else {
+        adjoint_init_u(lam, g[0], F[0], F[1], o.gfx, o.gfy);
+    }
+    for (int q = 0; q < Q; q++) mu[q] = o.mu[q];
+    gF[0] = o.gfx;
+    gF[1] = o.gfy;
+    return 0;
 }
 
 #ifdef EMU_MAIN

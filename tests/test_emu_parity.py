@@ -41,11 +41,15 @@ def test_emulated_quirks_match_oracle(quirk):
 
 
 @pytest.mark.parametrize("storage", [A.LBM_F16_SHIFTED, A.LBM_BF16_SHIFTED])
-def test_emulated_reduced_storage_bgk_bit_exact(storage):
-    case = layout_case("cavity_mw", 24, 16, A.LBM_BGK, storage=storage)
+@pytest.mark.parametrize("coll", [A.LBM_BGK, A.LBM_MRT, A.LBM_CM])
+def test_emulated_reduced_storage(storage, coll):
+    case = layout_case("cavity_mw", 24, 16, coll, storage=storage)
     fe, *_ = run_emu(case, 10)
     fo, *_ = run_oracle(case, 10, 32)
-    assert np.array_equal(fe, fo.astype(np.float32))
+    # FP32 rounding differences flip the rounding of a few 16-bit deviations (|d| < 0.05)
+    quantum = 2.0 ** -11 if storage == A.LBM_F16_SHIFTED else 2.0 ** -8
+    err = float(np.max(np.abs(fe - fo)))
+    assert err <= 4 * 0.05 * quantum, err
 
 
 @pytest.mark.parametrize("sizes", [(3, 3), (5, 4), (9, 5), (31, 6), (64, 3)])

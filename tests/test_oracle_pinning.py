@@ -1,0 +1,414 @@
+"""Pinning the CPU oracle (oracle/lbm_oracle.hpp) without executing the reference.
+
+The reference's D2Q9 path cannot be compiled here (no nvcc, and it is broken as shipped),
+so every part of the oracle is checked against something it does not restate: the
+lattice's exact moment identities, closed-form special cases (equilibria, fixed points,
+pure streaming as a permutation), operator equivalences that hold only when every term is
+right (MRT with equal rates == BGK, including Guo forcing), independent constructions
+(the central-moment matrix built from its definition), and flow invariants (mass,
+momentum gain F per step, exact wall velocities of the regularized boundaries).
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from lbm_amd import _abi as A
+from lbm_amd import scenarios as sc
+from lbm_amd.solver import Config
+
+W, CX, CY, OPP, M, MINV = orc.tables()
+C = np.stack([CX, CY], axis=1).astype(np.float64)
+CS2 = 1.0 / 3.0
+
+
+def cfg(**kw):
+    base = dict(nx=8, ny=8, omega=1.3, S=(0.0, 1.1, 1.2, 0.0, 1.4, 0.0, 1.4, 1.3, 1.3))
+    base.update(kw)
+    return Config(**base).to_c()
+
+
+# ---- lattice ---------------------------------------------------------------------------
+
+def test_lattice_tables():
+    assert abs(W.sum() - 1.0) < 1e-15
+    assert np.allclose((W[:, None] * C).sum(0), 0.0, atol=1e-15)
+    second = np.einsum("q,qa,qb->ab", W, C, C)
+    assert np.allclose(second, CS2 * np.eye(2), atol=1e-15)
+    for q in range(9):
+        assert np.array_equal(C[OPP[q]], -C[q])
+    assert np.allclose(M @ MINV, np.eye(9), atol=1e-14)
+    # M rows are the Lallemand-Luo polynomials of the velocities
+    cx, cy = C[:, 0], C[:, 1]
+    c2 = cx ** 2 + cy ** 2
+    rows = [np.ones(9), -4 + 3 * c2, 4 - 21 / 2 * c2 + 9 / 2 * c2 ** 2, cx, (-5 + 3 * c2) * cx, cy,
+            (-5 + 3 * c2) * cy, cx ** 2 - cy ** 2, cx * cy]
+    assert np.allclose(M, np.array(rows), atol=1e-12)
+
+
+# ---- equilibrium -----------------------------------------------------------------------
+
+@pytest.mark.parametrize("rho,ux,uy", [(1.0, 0.0, 0.0), (1.03, 0.05, -0.02), (0.97, -0.1, 0.08)])
+def test_equilibrium_moments(rho, ux, uy):
+    f = orc.equilibrium(rho, ux, uy)
+    u = np.array([ux, uy])
+    assert abs(f.sum() - rho) < 1e-14
+    assert np.allclose(f @ C, rho * u, atol=1e-14)
+    assert np.allclose(np.einsum("q,qa,qb->ab", f, C, C), rho * (CS2 * np.eye(2) + np.outer(u, u)), atol=1e-14)
+    g = orc.equilibrium(rho, -ux, -uy)
+    assert np.allclose(g[OPP], f, atol=1e-15)
+    lit = orc.equilibrium(rho, ux, uy, prec=32, literal=True)  # equilibrium.cu as written (Q15)
+    assert np.max(np.abs(lit - f)) < 1e-6
+
+
+# ---- collision operators ---------------------------------------------------------------
+
+def random_pops(rng, rho=1.0, u=(0.03, -0.02)):
+    f = orc.equilibrium(rho, *u)
+    return f * (1 + 0.05 * rng.uniform(-1, 1, 9))
+
+
+def guo_source(u, F):
+    """Guo et al. 2002 forcing term without the (1 - omega/2) factor."""
+    cu = C @ u
+    return W * (3 * ((C - u) @ F) + 9 * cu * (C @ F))
+
+
+def macros(g, F):
+    r = g.sum()
+    j = g @ C
+    return r, j / r + 0.5 * np.asarray(F) / r
+
+
+@pytest.mark.parametrize("F", [(0.0, 0.0), (2e-4, -1e-4)])
+def test_bgk_closed_forms(F):
+    rng = np.random.default_rng(1)
+    g = random_pops(rng)
+    r, u = macros(g, F)
+    c = cfg(collision=A.LBM_BGK, omega=1.0)
+    out = orc.collide_node(c, g, F)
+    expect = orc.equilibrium(r, *u) + 0.5 * guo_source(u, np.array(F))
+    assert np.allclose(out, expect, atol=1e-15)
+    c = cfg(collision=A.LBM_BGK, omega=1.7)
+    out = orc.collide_node(c, g, F)
+    assert abs(out.sum() - g.sum()) < 1e-14
+    assert np.allclose(out @ C, g @ C + np.array(F), atol=1e-14)  # momentum gains F
+
+
+@pytest.mark.parametrize("F", [(0.0, 0.0), (2e-4, -1e-4)])
+def test_mrt_with_equal_rates_is_bgk(F):
+    rng = np.random.default_rng(2)
+    g = random_pops(rng)
+    om = 1.37
+    bgk = orc.collide_node(cfg(collision=A.LBM_BGK, omega=om), g, F)
+    mrt = orc.collide_node(cfg(collision=A.LBM_MRT, S=(om,) * 9), g, F)
+    assert np.allclose(mrt, bgk, atol=1e-14)
+    if F != (0.0, 0.0):
+        lit = orc.collide_node(cfg(collision=A.LBM_MRT, S=(om,) * 9, quirks=A.LBM_QUIRK_MRT_FORCE_ROWS), g, F)
+        assert np.max(np.abs(lit - bgk)) > 1e-6  # MRT.cu:17-20 as written is not Guo forcing
+
+
+def test_mrt_forcing_moments_are_guo_moments():
+    rng = np.random.default_rng(3)
+    u = rng.uniform(-0.1, 0.1, 2)
+    F = rng.uniform(-1e-3, 1e-3, 2)
+    Fm = M @ guo_source(u, F)
+    fu = F @ u
+    expect = [0, 6 * fu, -6 * fu, F[0], -F[0], F[1], -F[1], 2 * (F[0] * u[0] - F[1] * u[1]),
+              F[0] * u[1] + F[1] * u[0]]
+    assert np.allclose(Fm, expect, atol=1e-16)
+    # and the oracle's MRT applies exactly these: with S=0 the collision is f + M^-1 F_m
+    g = random_pops(rng)
+    r, uu = macros(g, F)
+    out = orc.collide_node(cfg(collision=A.LBM_MRT, S=(0.0,) * 9), g, F)
+    assert np.allclose(out, g + MINV @ (M @ guo_source(uu, F)), atol=1e-15)
+
+
+def cm_T(ux, uy):
+    """Central-moment matrix from the definitions of CM.cuh:53-68 (rows k0..k8)."""
+    dx, dy = CX - ux, CY - uy
+    return np.array([np.ones(9), dx, dy, dx ** 2 + dy ** 2, dx ** 2 - dy ** 2, dx * dy, dx ** 2 * dy, dx * dy ** 2,
+                     dx ** 2 * dy ** 2])
+
+
+@pytest.mark.parametrize("ux,uy", [(0.0, 0.0), (0.05, -0.03), (-0.12, 0.09)])
+def test_cm_inverse_matches_definition(ux, uy):
+    Ti = orc.cm_matrix_inverse(ux, uy)
+    assert np.allclose(cm_T(ux, uy) @ Ti, np.eye(9), atol=1e-13)
+
+
+def test_cm_unit_rates_give_factorised_equilibrium():
+    rng = np.random.default_rng(4)
+    g = random_pops(rng, 1.02, (0.06, -0.04))
+    r, u = macros(g, (0.0, 0.0))
+    out = orc.collide_node(cfg(collision=A.LBM_CM, S=(1.0,) * 9), g, (0.0, 0.0))
+
+    def phi(c, v):  # 1-D populations with central moments (1, 0, cs^2) about v
+        return {1: (CS2 + v * v + v) / 2, -1: (CS2 + v * v - v) / 2, 0: 1 - CS2 - v * v}[c]
+
+    expect = np.array([r * phi(CX[q], u[0]) * phi(CY[q], u[1]) for q in range(9)])
+    assert np.allclose(out, expect, atol=1e-14)
+
+
+def test_cm_forcing_conserves_and_adds_momentum():
+    rng = np.random.default_rng(5)
+    g = random_pops(rng)
+    F = (3e-4, -2e-4)
+    out = orc.collide_node(cfg(collision=A.LBM_CM, S=(0, 0, 0, 1.0, 1.3, 1.3, 1.0, 1.0, 1.0)), g, F)
+    assert abs(out.sum() - g.sum()) < 1e-14
+    assert np.allclose(out @ C, g @ C + np.array(F), atol=1e-14)
+
+
+def test_optimal_adapter_values():
+    avg = [1.0, 0.05, 0.4]
+    t = 0.0003 * 1.0 - 0.00775 * (0.03 / 0.05) + 0.00016 * (0.38 / 0.4) + 0.0087
+    assert abs(orc.adapter_rate(1.0, 0.03, 0.38, avg) - 1 / (3 * t + 0.5)) < 1e-15
+    # tau* <= 0 -> 0.005 (adapters.cuh:98)
+    assert abs(orc.adapter_rate(1.0, 10.0, 0.38, avg) - 1 / (3 * 0.005 + 0.5)) < 1e-15
+    # NaN (zero averages) -> 0.005
+    assert abs(orc.adapter_rate(0.0, 0.0, 0.0, [0.0, 0.0, 0.0]) - 1 / (3 * 0.005 + 0.5)) < 1e-15
+    # clamp at 1.5 (adapters.cuh:99)
+    assert abs(orc.adapter_rate(1e4, 0.0, 0.0, [1.0, 1.0, 1.0]) - 1 / (3 * 1.5 + 0.5)) < 1e-15
+
+
+# ---- streaming and boundaries ----------------------------------------------------------
+
+def run(c, flags=None, rho=None, u=None, steps=1, precision=64, F=None, f=None):
+    o = orc.Oracle(c, precision=precision)
+    if flags is not None:
+        o.set_flags(flags)
+    if F is not None:
+        o.set_force_field(F)
+    N = c.nx * c.ny
+    if f is not None:
+        o.set_pops(f)
+    else:
+        o.init_from_macros(np.ones(N) if rho is None else rho, np.zeros(2 * N) if u is None else u)
+    o.step(steps)
+    return o
+
+
+def test_periodic_streaming_is_a_permutation():
+    rng = np.random.default_rng(6)
+    nx, ny, n = 11, 7, 5
+    f0 = rng.uniform(0.01, 0.5, (9, ny, nx))
+    c = cfg(nx=nx, ny=ny, collision=A.LBM_BGK, omega=0.0, periodic_x=True, periodic_y=True)
+    o = run(c, f=f0, steps=n)
+    f = o.get_pops()
+    for q in range(9):
+        assert np.array_equal(f[q], np.roll(f0[q], (n * CY[q], n * CX[q]), axis=(0, 1)))
+
+
+def test_rest_population_quirk_streams_a_stale_copy():
+    rng = np.random.default_rng(7)
+    f0 = rng.uniform(0.01, 0.5, (9, 5, 6))
+    c = cfg(nx=6, ny=5, collision=A.LBM_BGK, omega=0.0, periodic_x=True, periodic_y=True,
+            quirks=A.LBM_QUIRK_STALE_REST)
+    o = orc.Oracle(c, precision=64)
+    f_back = f0.copy()
+    f_back[0] = 0.123
+    o.set_pops(f0, f_back)
+    o.step(1)
+    assert np.all(o.get_pops()[0] == 0.123)  # streaming.cu:9 starts at i = 1
+
+
+def test_bounce_back_reflects_arrivals():
+    rng = np.random.default_rng(8)
+    nx, ny = 6, 5
+    f0 = rng.uniform(0.01, 0.5, (9, ny, nx))
+    flags = np.zeros((ny, nx), np.uint8)
+    flags[0] = flags[-1] = A.BOUNCE_BACK
+    c = cfg(nx=nx, ny=ny, collision=A.LBM_BGK, omega=0.0, periodic_x=True)
+    o = run(c, flags=flags, f=f0)
+    f = o.get_pops()
+    for x in range(nx):
+        for i in range(1, 9):
+            if CY[i] < 0:  # arrives at the bottom wall from y = 1
+                arrived = f0[i, 1, (x - CX[i]) % nx]
+                assert f[OPP[i], 0, x] == arrived
+                assert f[i, 0, x] == arrived
+    # (this on-node variant does not conserve mass exactly: the wall node's outgoing
+    # populations leave the domain while copies of the arrivals replace its unknowns)
+
+
+def test_moving_wall_couette_profile_is_linear():
+    ny, nx = 12, 4
+    flags = np.zeros((ny, nx), np.uint8)
+    flags[0] = A.BOUNCE_BACK
+    flags[-1] = A.MOVING_WALL_BOUNCE_BACK
+    c = cfg(nx=nx, ny=ny, collision=A.LBM_BGK, omega=1.0, periodic_x=True, wall_u=(0.05, 0.0))
+    o = run(c, flags=flags, steps=6000)
+    _, u = o.get_macros()
+    prof = u[:, 0, 0]
+    y = np.arange(ny)
+    p = np.polyfit(y, prof, 1)
+    assert np.max(np.abs(np.polyval(p, y) - prof)) < 1e-9
+    assert p[0] > 0 and abs(u[:, :, 1]).max() < 1e-12
+    # the lid's momentum term has the sign of the wall velocity
+    c2 = cfg(nx=nx, ny=ny, collision=A.LBM_BGK, omega=1.0, periodic_x=True, wall_u=(-0.05, 0.0))
+    _, u2 = run(c2, flags=flags, steps=6000).get_macros()
+    assert np.allclose(u2[:, 0, 0], -prof, atol=1e-12)
+
+
+def test_regularized_walls_impose_their_velocity_exactly():
+    n = 9
+    case = sc.lid_driven_cavity(n, u_lid=0.08, Re=10.0, collision=A.LBM_BGK, lid="regularized")
+    o = orc.Oracle(case.config.to_c(), precision=64)
+    case.apply_oracle(o)
+    o.step(30)
+    _, u = o.get_macros()  # post-boundary velocities, no force
+    assert np.allclose(u[-1, 1:-1, 0], 0.08, atol=1e-14) and np.allclose(u[-1, 1:-1, 1], 0.0, atol=1e-14)
+    assert np.allclose(u[0, 1:-1], 0.0, atol=1e-14)
+    assert np.allclose(u[1:-1, 0], 0.0, atol=1e-14) and np.allclose(u[1:-1, -1], 0.0, atol=1e-14)
+    for yy, xx in ((0, 0), (0, n - 1), (n - 1, 0), (n - 1, n - 1)):
+        assert np.allclose(u[yy, xx], 0.0, atol=1e-14)
+
+
+@pytest.mark.parametrize("literal", [False, True])
+def test_uniform_flow_through_zou_he_inlet_and_zg_outlet_is_steady(literal):
+    nx, ny, u0 = 10, 6, float(np.float32(0.04))  # the config carries float32
+    flags = np.zeros((ny, nx), np.uint8)
+    flags[:, 0] = A.ZOU_HE_LEFT
+    flags[:, -1] = A.ZG_OUTFLOW
+    c = cfg(nx=nx, ny=ny, collision=A.LBM_BGK, omega=1.2, periodic_y=True, u_lid=u0,
+            quirks=A.LBM_QUIRK_ZOUHE_RHO if literal else 0)
+    u = np.zeros((ny, nx, 2))
+    u[..., 0] = u0
+    o = run(c, flags=flags, u=u.reshape(-1), steps=50)
+    feq = orc.equilibrium(1.0, u0, 0.0)
+    err = np.max(np.abs(o.get_pops() - feq[:, None, None]))
+    if literal:
+        assert err > 1e-4  # zouHeInflow.cuh:15-16 as written mis-measures the inlet density
+    else:
+        assert err < 1e-14
+
+
+def test_zou_he_top_keeps_uniform_flow():
+    nx, ny, u0 = 6, 7, float(np.float32(0.05))
+    flags = np.zeros((ny, nx), np.uint8)
+    flags[-1] = A.ZOU_HE_TOP
+    c = cfg(nx=nx, ny=ny, collision=A.LBM_BGK, omega=1.1, periodic_x=True, periodic_y=True, u_lid=u0)
+    u = np.zeros((ny, nx, 2))
+    u[..., 0] = u0
+    o = run(c, flags=flags, u=u.reshape(-1), steps=30)
+    feq = orc.equilibrium(1.0, u0, 0.0)
+    assert np.max(np.abs(o.get_pops() - feq[:, None, None])) < 1e-14
+
+
+def test_guo_force_adds_exactly_F_per_node_and_step():
+    rng = np.random.default_rng(9)
+    nx, ny = 12, 10
+    c = cfg(nx=nx, ny=ny, collision=A.LBM_MRT, periodic_x=True, periodic_y=True, force=(1e-4, -3e-5))
+    rho = 1 + 0.01 * rng.uniform(-1, 1, nx * ny)
+    u = 0.03 * rng.uniform(-1, 1, 2 * nx * ny)
+    o = run(c, rho=rho, u=u, steps=1)
+    p0 = o.get_pops()
+    o.step(25)
+    p1 = o.get_pops()
+    assert abs(p1.sum() - p0.sum()) < 1e-11
+    dj = np.einsum("qyx,qa->a", p1 - p0, C)
+    assert np.allclose(dj, 25 * nx * ny * np.array([1e-4, -3e-5]), rtol=1e-9)
+
+
+# ---- immersed boundary -----------------------------------------------------------------
+
+@pytest.mark.parametrize("r", [0.0, 0.13, 0.5, 0.77, 0.999])
+def test_delta4_moment_conditions(r):
+    js = np.arange(-3, 4)
+    phi = np.array([orc.delta4(r - j) for j in js])
+    assert abs(phi.sum() - 1.0) < 1e-14
+    assert abs(((r - js) * phi).sum()) < 1e-14
+    assert abs(phi[js % 2 == 0].sum() - 0.5) < 1e-14
+    assert abs((phi ** 2).sum() - 3.0 / 8.0) < 1e-14  # Peskin's 4-point sum-of-squares condition
+
+
+def test_ibm_markers_slow_the_flow_and_feel_drag():
+    case = sc.ibm_cylinder(nx=64, ny=32, u_in=0.04, D=8.0, n_markers=26, Re=10.0, cx=20.0)
+    o = orc.Oracle(case.config.to_c(), precision=64)
+    case.apply_oracle(o)
+    o.step(300)
+    lag, tot = o.ibm_get_forces()
+    _, u = o.get_macros()
+    assert tot[0] < 0  # the fluid is pushed against the flow
+    assert abs(u[16, 20, 0]) < 0.25 * 0.04  # no-slip body: the flow inside is nearly stopped
+    # the 2x2 stencil of IBM_impl.cu:19-20 changes the result
+    c2 = sc.ibm_cylinder(nx=64, ny=32, u_in=0.04, D=8.0, n_markers=26, Re=10.0, cx=20.0)
+    c2.config.quirks = A.LBM_QUIRK_IBM_2X2
+    o2 = orc.Oracle(c2.config.to_c(), precision=64)
+    c2.apply_oracle(o2)
+    o2.step(300)
+    _, u2 = o2.get_macros()
+    assert np.max(np.abs(u2 - u)) > 1e-4
+
+
+@pytest.mark.parametrize("r,s", [(0.1, 0.2), (0.5, 0.5), (0.9, 0.3)])
+def test_two_by_two_stencil_weights_do_not_sum_to_one(r, s):
+    # IBM_impl.cu:19-20 sums the 4-point kernel over floor(p)+{0,1} only (quirk Q11)
+    w = sum(orc.delta4(r - i) * orc.delta4(s - j) for i in (0, 1) for j in (0, 1))
+    full = sum(orc.delta4(r - i) * orc.delta4(s - j) for i in range(-1, 3) for j in range(-1, 3))
+    assert 0.5 < w < 0.8 and abs(full - 1.0) < 1e-14
+
+
+# ---- validators ------------------------------------------------------------------------
+
+def test_taylor_green_decays_at_the_analytic_rate():
+    nu, u0, n, t = 0.02, 0.03, 48, 600
+    case = sc.taylor_green(n, nu=nu, u0=u0)
+    o = orc.Oracle(case.config.to_c(), precision=64)
+    case.apply_oracle(o)
+    o.step(t)
+    _, u = o.get_macros()
+    err = sc.tg_error(u, nu, u0, t)
+    assert err < 1.0  # percent
+    k2 = 2 * (2 * math.pi / n) ** 2
+    e0 = np.sum(case.u.astype(np.float64) ** 2)
+    ratio = np.sum(u ** 2) / e0
+    assert abs(ratio / math.exp(-2 * nu * k2 * t) - 1) < 2e-2
+
+
+def test_poiseuille_profile_is_a_parabola():
+    ny, nu, umax = 17, 0.1, 0.02
+    case = sc.poiseuille(nx=4, ny=ny, nu=nu, u_max=umax, collision=A.LBM_BGK, adapter=0)
+    o = orc.Oracle(case.config.to_c(), precision=64)
+    case.apply_oracle(o)
+    o.step(8000)
+    _, u = o.get_macros()
+    prof = u[:, 0, 0]
+    y = np.arange(ny)
+    p = np.polyfit(y, prof, 2)
+    assert np.max(np.abs(np.polyval(p, y) - prof)) < 1e-9 * umax
+    assert p[0] < 0
+    F = case.extra["F"]
+    assert abs(p[0] + F / (2 * nu)) < 1e-6 * F / nu  # curvature -F/(2 nu) of the steady solution
+
+
+def test_ghia_error_formula_is_zero_on_the_table():
+    n = 129
+    u = np.zeros((n, n, 2))
+    for i, yv in enumerate(sc.GHIA_Y):
+        y = int(math.floor(yv * (n - 1) + 0.5))
+        u[y, n // 2, 0] = sc.GHIA_U[1000][i] * 0.1
+        x = int(math.floor(sc.GHIA_X[i] * (n - 1) + 0.5))
+        u[n // 2, x, 1] = sc.GHIA_V[1000][i] * 0.1
+    assert sc.ghia_error(u, 0.1, 1000) < 1e-12
+    assert sc.ghia_closest(999) == 1000 and sc.ghia_closest(4000) == 3200
+
+
+# ---- storage rounding ------------------------------------------------------------------
+
+def test_half_and_bfloat16_rounding_match_numpy_and_torch():
+    import torch
+
+    rng = np.random.default_rng(10)
+    x = np.concatenate([rng.normal(0, 1e-2, 20000), rng.normal(0, 1e-5, 5000), rng.normal(0, 10, 5000),
+                        np.array([6.1e-5, 6.0e-8, 3.0e-8, 2.98e-8, 65504.0, 65520.0, -0.0, 1.0])]).astype(np.float32)
+    L = orc.lib()
+    h = np.array([L.oracle_f32_to_f16(float(v)) for v in x], np.uint16)
+    assert np.array_equal(h, x.astype(np.float16).view(np.uint16))
+    back = np.array([L.oracle_f16_to_f32(int(v)) for v in h], np.float32)
+    assert np.array_equal(back, x.astype(np.float16).astype(np.float32))
+    b = np.array([L.oracle_f32_to_bf16(float(v)) for v in x], np.uint16)
+    tb = torch.from_numpy(x).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    assert np.array_equal(b, tb)
