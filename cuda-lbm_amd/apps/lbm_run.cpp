@@ -1,0 +1,44 @@
+// lbm_run — C++ driver for a compile-time scenario (the role of src/main.cu): allocate,
+// init, run the fused steps through liblbm_hip.so and print the scenario's validation.
+// Build with -DRUN_TAYLOR_GREEN or -DRUN_LID_DRIVEN and -DNX=.. -DNY=.. .
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+
+#include "core/lbm.cuh"
+
+#if defined(RUN_TAYLOR_GREEN)
+#include "taylorGreenPeriodic.cuh"
+using Scenario = TaylorGreenPeriodic;
+#elif defined(RUN_LID_DRIVEN)
+#include "lidDrivenMovingWall.cuh"
+using Scenario = LidDrivenMovingWall;
+#else
+#error "choose a scenario: -DRUN_TAYLOR_GREEN or -DRUN_LID_DRIVEN"
+#endif
+
+int main(int argc, char** argv) {
+    const int steps = argc > 1 ? std::atoi(argv[1]) : 1000;
+    try {
+        LBM<dimensions> lbm;
+        lbm.allocate<Scenario>();
+        lbm.init<Scenario>();
+        auto t0 = std::chrono::steady_clock::now();
+        lbm.step<Scenario>(steps);
+        lbm.update_macroscopics();
+        double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::printf("scenario %s grid %dx%d steps %d wall %.3f s (%.1f MLUPS incl. launch)\n", Scenario::name(), NX, NY,
+                    steps, s, double(NX) * NY * steps / s / 1e6);
+        if constexpr (Scenario::has_analytical_solution) {
+            std::printf("validation error %.6f %%\n", lbm.compute_error<Scenario>());
+        } else {
+            std::printf("centre u_x(y):");
+            for (int y = 0; y < NY; y += NY / 8) std::printf(" %.5f", lbm.h_u[2 * (y * NX + NX / 2)]);
+            std::printf("\n");
+        }
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "lbm_run: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

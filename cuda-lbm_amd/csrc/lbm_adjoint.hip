@@ -222,7 +222,7 @@ int lbm_adjoint_backward(lbm_ctx* ctx, const float* dL_du, float* dL_dforce, flo
             StepParams sp = ctx->step_params(ctx->cur, false);
             sp.src = seg + size_t(i - 1) * pf;
             sp.dst = seg + size_t(i) * pf;
-            e = launch_step(ctx->cfg.collision, LBM_F32, 0, sp, nullptr, ctx->stream);
+            e = launch_step(ctx->cfg.collision, LBM_F32, 0, sp, step_policy(sp, LBM_F32), nullptr, ctx->stream);
             if (e != hipSuccess) return ctx->hip_fail(e, "segment recompute");
         }
         for (int t = t0 + len; t >= t0 + 1; t--) {

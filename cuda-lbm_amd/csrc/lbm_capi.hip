@@ -136,14 +136,17 @@ StepParams lbm_ctx::step_params(int src_idx, bool write_macros) const {
     return p;
 }
 
-int lbm_ctx::seg_width() const { return cfg.storage == LBM_F32 ? 4 : 8; }
+int lbm_ctx::seg_width() const {
+    return cfg.storage == LBM_F32 ? Storage<LBM_F32>::V : Storage<LBM_F16_SHIFTED>::V;
+}
 
 int lbm_ctx::adapter_pass(int src_idx) {
     StepParams p = step_params(src_idx, false);
-    long long nb = step_grid_blocks(p);
+    LaunchPolicy lp = step_policy(p, cfg.storage);
+    long long nb = step_grid_blocks(p, lp);
     int rc = ensure_partials(nb);
     if (rc) return rc;
-    hipError_t e = launch_step(cfg.collision, cfg.storage, 1, p, partials, stream);
+    hipError_t e = launch_step(cfg.collision, cfg.storage, 1, p, lp, partials, stream);
     if (e != hipSuccess) return hip_fail(e, "moments pass launch");
     e = launch_finalize_avg(partials, nb, (long long)N, d_avg, d_avg64, stream);
     if (e != hipSuccess) return hip_fail(e, "finalize launch");
@@ -165,13 +168,14 @@ int lbm_ctx::one_step(bool write_macros) {
         if (rc) return rc;
     }
     StepParams p = step_params(cur, write_macros);
+    LaunchPolicy lp = step_policy(p, cfg.storage);
     int mode = lagged ? 2 : 0;
-    long long nb = step_grid_blocks(p);
+    long long nb = step_grid_blocks(p, lp);
     if (mode == 2) {
         rc = ensure_partials(nb);
         if (rc) return rc;
     }
-    hipError_t e = launch_step(cfg.collision, cfg.storage, mode, p, partials, stream);
+    hipError_t e = launch_step(cfg.collision, cfg.storage, mode, p, lp, partials, stream);
     if (e != hipSuccess) return hip_fail(e, "step launch");
     if (mode == 2) {
         e = launch_finalize_avg(partials, nb, (long long)N, d_avg, d_avg64, stream);

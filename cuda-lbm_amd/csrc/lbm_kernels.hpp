@@ -16,9 +16,18 @@ struct InitParams {
     float rho0;
 };
 
-long long step_grid_blocks(const StepParams& p);
+// Launch shape of the fused step: workgroup size, XCD-contiguous workgroup order and
+// non-temporal stores, chosen by whether the two population buffers fit in the Infinity Cache.
+struct LaunchPolicy {
+    int block = 256;
+    int remap = 0;
+    int nt = 0;
+};
+LaunchPolicy step_policy(const StepParams& p, int storage);
+long long step_grid_blocks(const StepParams& p, const LaunchPolicy& lp);
 // mode: 0 step, 1 moments pass, 2 step + moments (CM only).
-hipError_t launch_step(int coll, int storage, int mode, const StepParams& p, double* partials, hipStream_t s);
+hipError_t launch_step(int coll, int storage, int mode, const StepParams& p, const LaunchPolicy& lp, double* partials,
+                       hipStream_t s);
 hipError_t launch_finalize_avg(const double* partials, long long n, long long N, float* avg, double* avg64, hipStream_t s);
 hipError_t launch_init(int storage, const InitParams& p, hipStream_t s);
 hipError_t launch_pops_to_f32(int storage, const void* buf, long long plane, long long N, float rho0, float* out, hipStream_t s);

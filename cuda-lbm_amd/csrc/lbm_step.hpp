@@ -49,6 +49,7 @@ struct StepParams {
     float* out_ux;
     float* out_uy;
     int ntx;                     // segments per row
+    int nt;                      // non-temporal population stores
 };
 
 // Row classes for the fast-path test: 0 all FLUID, 1 non-FLUID only at x = 0 or x = nx-1,
@@ -138,40 +139,103 @@ LBM_HD uint16_t float_to_bf16_bits(float x) {
 template <int STORE>
 struct Storage;
 
+// Stores of one 16-byte vector: plain, or non-temporal (streaming past the caches; used
+// when the population buffers do not fit in the 256 MB Infinity Cache).
+LBM_HD void store16(void* p, const uint32_t* d, bool nt) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (nt) {
+        uint32_t* q = static_cast<uint32_t*>(p);
+#pragma unroll
+        for (int k = 0; k < 4; k++) __builtin_nontemporal_store(d[k], q + k);
+        return;
+    }
+#endif
+    (void)nt;
+    __builtin_memcpy(p, d, 16);
+}
+
+// FP32: four nodes per 16-byte vector. The x-shifted pull is one dword-aligned 16-byte
+// load (a single global_load_dwordx4 on gfx950).
 template <>
 struct Storage<LBM_F32> {
     using T = float;
     static constexpr int V = 4;
-    LBM_HD static void load(const T* p, int, float, float* out) {
+    static constexpr int kRightHalo = 1;  // elements read past the segment on the right
+    struct Vec {
         float v[4];
-        __builtin_memcpy(v, p, 16);
-#pragma unroll
-        for (int j = 0; j < 4; j++) out[j] = v[j];
+    };
+    LBM_HD static Vec load_shifted(const T* row, int x0, int cx) {
+        Vec r;
+        __builtin_memcpy(r.v, row + (x0 - cx), 16);
+        return r;
     }
-    LBM_HD static void store(T* p, int, float, const float* v) { __builtin_memcpy(p, v, 16); }
+    LBM_HD static float get(const Vec& r, int j, int, float) { return r.v[j]; }
+    LBM_HD static void put(Vec& r, int j, int, float, float v) { r.v[j] = v; }
+    LBM_HD static void store_vec(T* p, const Vec& r, bool nt) {
+        uint32_t d[4];
+        __builtin_memcpy(d, r.v, 16);
+        store16(p, d, nt);
+    }
     LBM_HD static float load1(const T* p, int, float) { return *p; }
     LBM_HD static void store1(T* p, int, float, float v) { *p = v; }
 };
 
+// 16-bit shifted storage: LBM_V16 nodes (4 or 8) per 8- or 16-byte vector. A pull shifted
+// by one node is rebuilt from the aligned vector and the neighbouring dword with a 16-bit
+// funnel shift (v_alignbit_b32), keeping every load dword-aligned. 4 nodes per thread is the
+// default: at 8 the MRT and CM bodies need over 200 VGPRs and the 8192^2 step runs ~2x slower
+// (MRT FP16 1721 us vs 868 us measured on MI355X).
+#ifndef LBM_V16
+#define LBM_V16 4
+#endif
 template <int STORE>
 struct Shifted16 {
     using T = uint16_t;
-    static constexpr int V = 8;
+    static constexpr int V = LBM_V16;
+    static constexpr int W = V / 2;       // dwords per vector
+    static constexpr int kRightHalo = 2;
+    struct Vec {
+        uint32_t d[W];
+    };
     LBM_HD static float dec(uint16_t h) { return STORE == LBM_F16_SHIFTED ? half_bits_to_float(h) : bf16_bits_to_float(h); }
     LBM_HD static uint16_t enc(float v) { return STORE == LBM_F16_SHIFTED ? float_to_half_bits(v) : float_to_bf16_bits(v); }
-    LBM_HD static void load(const T* p, int q, float rho0, float* out) {
-        uint16_t h[8];
-        __builtin_memcpy(h, p, 16);
-        float w0 = wgt(q) * rho0;
+    LBM_HD static uint32_t funnel16(uint32_t hi, uint32_t lo) { return uint32_t(((uint64_t(hi) << 32) | lo) >> 16); }
+    LBM_HD static Vec load_shifted(const T* row, int x0, int cx) {
+        Vec a;
+        __builtin_memcpy(a.d, row + x0, 4 * W);
+        if (cx == 0) return a;
+        Vec r;
+        if (cx > 0) {  // nodes x0-1 .. x0+V-2
+            uint32_t pv;
+            __builtin_memcpy(&pv, row + x0 - 2, 4);
+            r.d[0] = funnel16(a.d[0], pv);
 #pragma unroll
-        for (int j = 0; j < 8; j++) out[j] = dec(h[j]) + w0;
+            for (int k = 1; k < W; k++) r.d[k] = funnel16(a.d[k], a.d[k - 1]);
+        } else {  // nodes x0+1 .. x0+V
+            uint32_t nv;
+            __builtin_memcpy(&nv, row + x0 + V, 4);
+#pragma unroll
+            for (int k = 0; k < W - 1; k++) r.d[k] = funnel16(a.d[k + 1], a.d[k]);
+            r.d[W - 1] = funnel16(nv, a.d[W - 1]);
+        }
+        return r;
     }
-    LBM_HD static void store(T* p, int q, float rho0, const float* v) {
-        float w0 = wgt(q) * rho0;
-        uint16_t h[8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) h[j] = enc(v[j] - w0);
-        __builtin_memcpy(p, h, 16);
+    LBM_HD static float get(const Vec& r, int j, int q, float rho0) {
+        uint32_t w = r.d[j >> 1];
+        uint16_t h = uint16_t((j & 1) ? (w >> 16) : (w & 0xffffu));
+        return dec(h) + wgt(q) * rho0;
+    }
+    LBM_HD static void put(Vec& r, int j, int q, float rho0, float v) {
+        uint32_t h = enc(v - wgt(q) * rho0);
+        uint32_t& w = r.d[j >> 1];
+        w = (j & 1) ? ((w & 0xffffu) | (h << 16)) : ((w & 0xffff0000u) | h);
+    }
+    LBM_HD static void store_vec(T* p, const Vec& r, bool nt) {
+        if constexpr (W == 4) {
+            store16(p, r.d, nt);
+        } else {
+            __builtin_memcpy(p, r.d, 4 * W);
+        }
     }
     LBM_HD static float load1(const T* p, int q, float rho0) { return dec(*p) + wgt(q) * rho0; }
     LBM_HD static void store1(T* p, int q, float rho0, float v) { *p = enc(v - wgt(q) * rho0); }
@@ -323,6 +387,7 @@ template <int COLL, int STORE, int MODE>
 LBM_HD void step_segment(const StepParams& p, long long seg, MomAcc& acc) {
     using St = Storage<STORE>;
     using T = typename St::T;
+    using Vec = typename St::Vec;
     constexpr int V = St::V;
     const long long nseg = (long long)p.ntx * p.ny;
     if (seg >= nseg) return;
@@ -332,7 +397,7 @@ LBM_HD void step_segment(const StepParams& p, long long seg, MomAcc& acc) {
     const T* src = static_cast<const T*>(p.src);
     T* dst = static_cast<T*>(p.dst);
 
-    bool fast = x0 >= 1 && x0 + V <= p.nx - 1;
+    bool fast = x0 >= 1 && x0 + V + St::kRightHalo <= p.nx;
     if (fast && !p.py) fast = y >= 1 && y <= p.ny - 2;
     if (fast && p.row_class) {
         uint8_t rc = p.row_class[y];
@@ -343,35 +408,19 @@ LBM_HD void step_segment(const StepParams& p, long long seg, MomAcc& acc) {
     }
     if (fast && (p.quirks & LBM_QUIRK_STALE_REST)) fast = false;
 
-    float pop[Q][V];
-    int nvalid = V;
     if (fast) {
+        float pop[Q][V];
 #pragma unroll
         for (int q = 0; q < Q; q++) {
             int ys = y - CY[q];
             if (p.py) ys = ys < 0 ? ys + p.ny : (ys >= p.ny ? ys - p.ny : ys);
-            St::load(src + q * p.plane + (long long)ys * p.nx + (x0 - CX[q]), q, p.rho0, pop[q]);
+            Vec raw = St::load_shifted(src + q * p.plane + (long long)ys * p.nx, x0, CX[q]);
+#pragma unroll
+            for (int j = 0; j < V; j++) pop[q][j] = St::get(raw, j, q, p.rho0);
         }
-    } else {
-        nvalid = p.nx - x0 < V ? p.nx - x0 : V;
-        const NodeArgs na = node_args(p);
+        float orho[V], oux[V], ouy[V];
 #pragma unroll
         for (int j = 0; j < V; j++) {
-            if (j < nvalid) {
-                Pop9 r = gather_node<STORE>(na, x0 + j, y);
-#pragma unroll
-                for (int q = 0; q < Q; q++) pop[q][j] = r.v[q];
-            } else {
-#pragma unroll
-                for (int q = 0; q < Q; q++) pop[q][j] = 0.f;
-            }
-        }
-    }
-
-    float orho[V], oux[V], ouy[V];
-#pragma unroll
-    for (int j = 0; j < V; j++) {
-        if (j < nvalid) {
             float g[Q];
 #pragma unroll
             for (int q = 0; q < Q; q++) g[q] = pop[q][j];
@@ -384,32 +433,40 @@ LBM_HD void step_segment(const StepParams& p, long long seg, MomAcc& acc) {
             orho[j] = m.rho;
             oux[j] = m.ux;
             ouy[j] = m.uy;
-        } else {
-            orho[j] = oux[j] = ouy[j] = 0.f;
         }
-    }
-    if (MODE == 1) return;
-
-    if (nvalid == V) {
+        if (MODE == 1) return;
 #pragma unroll
-        for (int q = 0; q < Q; q++) St::store(dst + q * p.plane + row0 + x0, q, p.rho0, pop[q]);
+        for (int q = 0; q < Q; q++) {
+            Vec raw;
+#pragma unroll
+            for (int j = 0; j < V; j++) St::put(raw, j, q, p.rho0, pop[q][j]);
+            St::store_vec(dst + q * p.plane + row0 + x0, raw, p.nt != 0);
+        }
         if (p.out_rho) {
             __builtin_memcpy(p.out_rho + row0 + x0, orho, sizeof(orho));
             __builtin_memcpy(p.out_ux + row0 + x0, oux, sizeof(oux));
             __builtin_memcpy(p.out_uy + row0 + x0, ouy, sizeof(ouy));
         }
-    } else {
+        return;
+    }
+
+    // general path, one node at a time
+    const int nvalid = p.nx - x0 < V ? p.nx - x0 : V;
+    const NodeArgs na = node_args(p);
+    for (int j = 0; j < nvalid; j++) {
+        const int x = x0 + j;
+        Pop9 r = gather_node<STORE>(na, x, y);
+        float fx, fy;
+        node_force(p, x, y, row0 + x, fx, fy);
+        Macro m;
+        node_update<COLL, MODE>(p, r.v, fx, fy, m, acc);
+        if (MODE == 1) continue;
 #pragma unroll
-        for (int j = 0; j < V; j++) {
-            if (j < nvalid) {
-#pragma unroll
-                for (int q = 0; q < Q; q++) St::store1(dst + q * p.plane + row0 + x0 + j, q, p.rho0, pop[q][j]);
-                if (p.out_rho) {
-                    p.out_rho[row0 + x0 + j] = orho[j];
-                    p.out_ux[row0 + x0 + j] = oux[j];
-                    p.out_uy[row0 + x0 + j] = ouy[j];
-                }
-            }
+        for (int q = 0; q < Q; q++) St::store1(dst + q * p.plane + row0 + x, q, p.rho0, r.v[q]);
+        if (p.out_rho) {
+            p.out_rho[row0 + x] = m.rho;
+            p.out_ux[row0 + x] = m.ux;
+            p.out_uy[row0 + x] = m.uy;
         }
     }
 }

@@ -64,7 +64,7 @@ def test_gpu_ragged_and_tiny_grids(gpu_available, shape):
 
 
 def test_gpu_taylor_green_128_against_oracle_and_analytic(gpu_available):
-    """BASELINE config 1 (TG 128^2, BGK, nu=0.01, u0=0.04) for 1000 steps."""
+    """Taylor-Green 128^2, BGK, nu=0.01, u0=0.04, for 1000 steps."""
     case = sc.taylor_green(128, nu=0.01, u0=0.04)
     fg, rg, ug, s = run_gpu(case, 1000)
     fo, ro, uo, _ = run_oracle(case, 1000, 32)
