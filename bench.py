@@ -4,8 +4,10 @@
 Workload (the configuration the metric is quoted on): lid-driven cavity, D2Q9 MRT,
 1024x1024, moving-wall lid u = 0.1, Re = 1000, FP32 populations. A "step" is one pass of
 the fused collide-stream kernel over the grid. Also reported on the same line: the same
-cavity with FP16 shifted storage, the 8192^2 periodic BGK grid that lives in HBM (the
-256 MB Infinity Cache holds the 1024^2 cavity), and the CPU oracle timed on the host.
+cavity with FP16 shifted storage (and its velocity difference from FP32), the 8192^2
+periodic BGK grid that lives in HBM (the 256 MB Infinity Cache holds the 1024^2 cavity),
+the IBM cylinder, forward + adjoint of the 512^2 inverse problem, the Taylor-Green error
+per storage format, and the CPU oracle timed on the host.
 
 Prints one JSON line.
 """
@@ -63,6 +65,111 @@ def run_case(case, steps, warmup, torch):
                "GB_s": round(gbs, 1), "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_node": B}
 
 
+def run_adjoint(torch, steps=2000, every=100):
+    """Forward + adjoint of the inverse-problem grid (BGK 512^2 periodic, nu = 0.02, FP32,
+    checkpoint every 100 steps). Times plain forward steps, the checkpointing forward and
+    the backward (segment recompute + adjoint steps), all with HIP events on one stream."""
+    from lbm_amd import Solver
+    from lbm_amd import scenarios as sc
+
+    case = sc.inverse_problem(512, steps=steps)
+    N = case.config.N
+    stream = torch.cuda.Stream()
+    s = Solver(case.config)
+    s.set_stream(stream.cuda_stream)
+    case.apply(s)
+    dev = torch.device("cuda")
+    F = torch.as_tensor(case.force_field, device=dev).contiguous()
+    w = torch.randn(N * 2, device=dev, generator=torch.Generator(device=dev).manual_seed(7))
+    gF = torch.empty(N * 2, device=dev)
+    gU = torch.empty(N * 2, device=dev)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(stream):
+            a.record(stream)
+            fn()
+            b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / 1000.0
+
+    s.step(50)  # warm-up
+    t_plain = timed(lambda: s.step(steps))
+
+    def fwd():
+        with torch.cuda.stream(stream):
+            s.set_force_field(F)
+            s.init_from_macros(torch.as_tensor(case.rho, device=dev), torch.as_tensor(case.u, device=dev))
+            s.adjoint_forward(steps, every)
+
+    fwd()
+    s.adjoint_backward(w, gF, gU)  # warm-up (allocates the segment buffers)
+    t_fwd = timed(fwd)
+    t_bwd = timed(lambda: s.adjoint_backward(w, gF, gU))
+    s.close()
+    ok = bool(torch.isfinite(gF).all())
+    return {"workload": "inverse problem BGK 512x512 periodic nu=0.02 FP32, 2000 steps, checkpoint every 100",
+            "forward_mlups": round(N * steps / t_plain / 1e6, 1),
+            "checkpointing_forward_mlups": round(N * steps / t_fwd / 1e6, 1),
+            "adjoint_mlups": round(N * steps / t_bwd / 1e6, 1),
+            "adjoint_vs_forward_step_cost": round(t_bwd / t_plain, 3),
+            "backward_includes": "segment recompute (1 forward step per adjoint step) + adjoint step",
+            "gradient_finite": ok}
+
+
+def run_accuracy(torch):
+    """Taylor-Green 128^2 (BGK, nu = 0.01, u0 = 0.04, 1000 steps) L2 error against the analytic
+    decay for each storage format, the reference's TaylorGreenScenario::compute_error."""
+    from lbm_amd import Solver
+    from lbm_amd import _abi as A
+    from lbm_amd import scenarios as sc
+
+    out = {}
+    for name, st in (("f32", A.LBM_F32), ("f16", A.LBM_F16_SHIFTED), ("bf16", A.LBM_BF16_SHIFTED)):
+        case = sc.taylor_green(128, nu=0.01, u0=0.04, storage=st)
+        s = Solver(case.config)
+        case.apply(s)
+        s.step(1000)
+        _, u = s.get_macros()
+        s.close()
+        out[name] = round(sc.tg_error(u, 0.01, 0.04, 1000), 5)
+    out["unit"] = "% relative L2 vs analytic decay, Taylor-Green 128^2 BGK 1000 steps"
+    return out
+
+
+def rel_l2(a, b):
+    import numpy as np
+
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
+
+
+def profiled_traffic(kernel_tag):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary for this kernel and
+    workload (FETCH_SIZE and WRITE_SIZE, each scaled by the copy-kernel calibration)."""
+    p = ROOT / "profiles" / f"pmc_{kernel_tag}.txt"
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def copy_calibration(torch, mb):
+    """One plain device copy of `mb` MB, whose profiler byte counters calibrate the step's."""
+    n = mb * 1000000 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    return {"read": 4 * n, "write": 4 * n}
+
+
 def cpu_baseline(sample_steps=120):
     """The FP32 oracle (OpenMP over rows) on the host cores: a bounded sample of the
     same cavity (1024^2 MRT, `sample_steps` steps)."""
@@ -90,6 +197,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-extras", action="store_true", help="only the headline workload")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["hbm_grid", "adjoint"], default=None,
+                    help="run one extra section alone (for profiling its kernels); prints that section")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="also run one plain device copy of MB megabytes (for scaling profiler byte counters)")
     args = ap.parse_args()
@@ -107,9 +216,23 @@ def main():
         print(json.dumps({"error": "no GPU"}))
         sys.exit(1)
 
+    if args.only == "hbm_grid":
+        res = {}
+        for name, st in (("f32", A.LBM_F32), ("f16", A.LBM_F16_SHIFTED)):
+            s, res[name] = run_case(sc.bandwidth_grid(8192, storage=st), max(20, args.steps // 20), 10, torch)
+            s.close()
+        if args.calibrate_copy:
+            res["copy_calibration_bytes"] = copy_calibration(torch, args.calibrate_copy)
+        print(json.dumps({"hbm_grid": res}))
+        return
+    if args.only == "adjoint":
+        print(json.dumps({"adjoint": run_adjoint(torch)}))
+        return
+
     # headline: LDC 1024^2 MRT FP32
     case = sc.lid_driven_cavity(1024, storage=A.LBM_F32)
     s, head = run_case(case, args.steps, args.warmup, torch)
+    _, u32 = s.get_macros()
     s.close()
     out = {
         "metric": METRIC,
@@ -128,18 +251,23 @@ def main():
                                "populations, fluid at rest at t=0",
                    "nodes": case.config.N, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": head["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": head["frac_hbm_peak"], "traffic": None,
+                     "frac": head["frac_hbm_peak"], "traffic": profiled_traffic("fused_step_ldc1024_mrt_f32"),
                      "kernel": "fused_step_kernel<MRT,F32>",
+                     "kernel_avg_us": round(1000.0 * head["ms_per_step"], 3),
+                     "algorithmic_bytes_per_launch": head["bytes_per_node"] * case.config.N,
                      "algorithmic_bytes_per_node": head["bytes_per_node"],
-                     "note": "72 B/node algorithmic; the 75 MB working set is resident in the 256 MB "
-                             "Infinity Cache, see hbm_grid for the HBM-bound figure"},
+                     "note": "one fused-kernel launch per step, timed with HIP events on its stream; 72 B/node "
+                             "algorithmic; the 75 MB working set stays in the 256 MB Infinity Cache, see "
+                             "hbm_grid for the HBM-bound grid"},
     }
     if not args.no_extras:
         c16 = sc.lid_driven_cavity(1024, storage=A.LBM_F16_SHIFTED)
         s, r16 = run_case(c16, args.steps, args.warmup, torch)
+        _, u16 = s.get_macros()
         s.close()
         out["fp16_storage"] = dict(r16, workload="same cavity, FP16 shifted storage",
-                                   speedup_vs_fp32=round(r16["mlups"] / head["mlups"], 3))
+                                   speedup_vs_fp32=round(r16["mlups"] / head["mlups"], 3),
+                                   u_rel_l2_vs_fp32=rel_l2(u16, u32))
         grid = {}
         for name, st in (("f32", A.LBM_F32), ("f16", A.LBM_F16_SHIFTED)):
             g = sc.bandwidth_grid(8192, storage=st)
@@ -149,14 +277,15 @@ def main():
         grid["speedup_f16_vs_f32"] = round(grid["f16"]["mlups"] / grid["f32"]["mlups"], 3)
         grid["workload"] = "periodic BGK 8192x8192 (2 x 2.4 GB FP32 population buffers)"
         out["hbm_grid"] = grid
+        cyl = sc.ibm_cylinder()
+        s, rc = run_case(cyl, 200, 20, torch)
+        s.close()
+        out["ibm_cylinder"] = dict(rc, workload="IBM cylinder 2048x512 CM, 201 markers, 3 IBM iterations per step "
+                                                "(IBM kernel + fused step per step)")
+        out["adjoint"] = run_adjoint(torch)
+        out["accuracy"] = run_accuracy(torch)
     if args.calibrate_copy:
-        n = args.calibrate_copy * 1000000 // 4
-        a = torch.ones(n, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        out["copy_calibration_bytes"] = {"read": 4 * n, "write": 4 * n}
+        out["copy_calibration_bytes"] = copy_calibration(torch, args.calibrate_copy)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

@@ -127,26 +127,31 @@ int lbm_adjoint_forward(lbm_ctx* ctx, int32_t nsteps, int32_t every) {
                          "the adjoint covers BGK/MRT, FP32 storage, fully periodic FLUID-only domains without IBM or quirks");
     int rc = ctx->activate();
     if (rc) return rc;
-    (void)hipStreamSynchronize(ctx->stream);
-    ctx->adjoint_free();
     AdjointState& A = ctx->adj;
     const size_t pb = size_t(ctx->plane) * 9 * sizeof(float);
     const int ncp = (nsteps + every - 1) / every;
-    A.checkpoints.assign(size_t(ncp), nullptr);
-    for (int i = 0; i < ncp; i++) {
-        rc = ctx->dev_alloc(reinterpret_cast<void**>(&A.checkpoints[size_t(i)]), pb, "adjoint checkpoint");
-        if (rc) {
-            ctx->adjoint_free();
-            return rc;
+    A.ready = false;
+    if (A.checkpoints.size() != size_t(ncp)) {  // same count: the buffers are reused as they are
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->adjoint_free();
+        A.checkpoints.assign(size_t(ncp), nullptr);
+        for (int i = 0; i < ncp; i++) {
+            rc = ctx->dev_alloc(reinterpret_cast<void**>(&A.checkpoints[size_t(i)]), pb, "adjoint checkpoint");
+            if (rc) {
+                ctx->adjoint_free();
+                return rc;
+            }
         }
     }
     const bool from_init = ctx->timestep == 0 && ctx->init_from_macros && ctx->macros_valid;
     if (from_init) {
-        rc = ctx->dev_alloc(reinterpret_cast<void**>(&A.d_rho_init), ctx->N * 4, "adjoint rho0");
-        rc = rc ? rc : ctx->dev_alloc(reinterpret_cast<void**>(&A.d_u0), ctx->N * 8, "adjoint u0");
-        if (rc) {
-            ctx->adjoint_free();
-            return rc;
+        if (!A.d_u0) {
+            rc = ctx->dev_alloc(reinterpret_cast<void**>(&A.d_rho_init), ctx->N * 4, "adjoint rho0");
+            rc = rc ? rc : ctx->dev_alloc(reinterpret_cast<void**>(&A.d_u0), ctx->N * 8, "adjoint u0");
+            if (rc) {
+                ctx->adjoint_free();
+                return rc;
+            }
         }
         (void)hipMemcpyAsync(A.d_rho_init, ctx->d_rho, ctx->N * 4, hipMemcpyDeviceToDevice, ctx->stream);
         (void)hipMemcpyAsync(A.d_u0, ctx->d_ux, ctx->N * 4, hipMemcpyDeviceToDevice, ctx->stream);
@@ -178,9 +183,18 @@ int lbm_adjoint_backward(lbm_ctx* ctx, const float* dL_du, float* dL_dforce, flo
     const size_t N = ctx->N;
     const size_t pb = size_t(ctx->plane) * 9 * sizeof(float);
     const int K = A.every, T = A.nsteps;
+    if (A.d_seg && A.seg_states != K) {  // checkpoint spacing changed since the last backward
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(A.d_seg);
+        A.d_seg = nullptr;
+    }
     if (!A.d_seg) {
         rc = ctx->dev_alloc(reinterpret_cast<void**>(&A.d_seg), pb * size_t(K), "adjoint segment states");
-        rc = rc ? rc : ctx->dev_alloc(reinterpret_cast<void**>(&A.d_mu[0]), pb, "adjoint mu");
+        if (rc) return rc;
+        A.seg_states = K;
+    }
+    if (!A.d_mu[0]) {
+        rc = ctx->dev_alloc(reinterpret_cast<void**>(&A.d_mu[0]), pb, "adjoint mu");
         rc = rc ? rc : ctx->dev_alloc(reinterpret_cast<void**>(&A.d_mu[1]), pb, "adjoint mu");
         rc = rc ? rc : ctx->dev_alloc(reinterpret_cast<void**>(&A.d_grad_force), N * 8, "adjoint dL/dF");
         if (rc) return rc;
@@ -222,7 +236,7 @@ int lbm_adjoint_backward(lbm_ctx* ctx, const float* dL_du, float* dL_dforce, flo
             StepParams sp = ctx->step_params(ctx->cur, false);
             sp.src = seg + size_t(i - 1) * pf;
             sp.dst = seg + size_t(i) * pf;
-            e = launch_step(ctx->cfg.collision, LBM_F32, 0, sp, step_policy(sp, LBM_F32), nullptr, ctx->stream);
+            e = launch_step(ctx->cfg.collision, LBM_F32, 0, sp, step_policy(ctx->plane, LBM_F32), nullptr, ctx->stream);
             if (e != hipSuccess) return ctx->hip_fail(e, "segment recompute");
         }
         for (int t = t0 + len; t >= t0 + 1; t--) {

@@ -132,17 +132,12 @@ StepParams lbm_ctx::step_params(int src_idx, bool write_macros) const {
     p.out_rho = write_macros ? d_rho : nullptr;
     p.out_ux = write_macros ? d_ux : nullptr;
     p.out_uy = write_macros ? d_uy : nullptr;
-    p.ntx = (cfg.nx + seg_width() - 1) / seg_width();
-    return p;
-}
-
-int lbm_ctx::seg_width() const {
-    return cfg.storage == LBM_F32 ? Storage<LBM_F32>::V : Storage<LBM_F16_SHIFTED>::V;
+    return p;  // ntx and nt are set by launch_step from the launch policy
 }
 
 int lbm_ctx::adapter_pass(int src_idx) {
     StepParams p = step_params(src_idx, false);
-    LaunchPolicy lp = step_policy(p, cfg.storage);
+    LaunchPolicy lp = step_policy(plane, cfg.storage);
     long long nb = step_grid_blocks(p, lp);
     int rc = ensure_partials(nb);
     if (rc) return rc;
@@ -168,7 +163,7 @@ int lbm_ctx::one_step(bool write_macros) {
         if (rc) return rc;
     }
     StepParams p = step_params(cur, write_macros);
-    LaunchPolicy lp = step_policy(p, cfg.storage);
+    LaunchPolicy lp = step_policy(plane, cfg.storage);
     int mode = lagged ? 2 : 0;
     long long nb = step_grid_blocks(p, lp);
     if (mode == 2) {
@@ -190,6 +185,7 @@ int lbm_ctx::one_step(bool write_macros) {
 lbm_ctx::~lbm_ctx() {
     (void)hipSetDevice(device);
     if (own_stream) (void)hipStreamSynchronize(own_stream);
+    if (stream != own_stream) (void)hipStreamSynchronize(stream);
     void* ptrs[] = {buf[0], buf[1], d_flags, d_row_class, d_force_field, d_rho, d_ux, d_uy, d_avg, d_avg64, partials, tmp};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -293,7 +289,10 @@ int lbm_destroy(lbm_ctx* ctx) {
 
 int lbm_set_stream(lbm_ctx* ctx, void* s) {
     if (!ctx) return LBM_E_INVALID;
-    ctx->stream = s ? static_cast<hipStream_t>(s) : ctx->own_stream;
+    // NULL is the HIP null stream (PyTorch's default stream reports handle 0), not a request
+    // for the context's own stream: mapping it to the own non-blocking stream would let the
+    // library race with the caller's work.
+    ctx->stream = static_cast<hipStream_t>(s);
     return LBM_OK;
 }
 

@@ -43,10 +43,11 @@ struct IbmState {
 struct AdjointState {
     int nsteps = 0, every = 0;
     std::vector<float*> checkpoints;       // FP32 SoA [9][plane] each
-    float* d_seg = nullptr;                // states of one segment, [every][9][plane]
+    float* d_seg = nullptr;                // states of one segment, [seg_states][9][plane]
+    int seg_states = 0;
     float* d_mu[2] = {nullptr, nullptr};   // adjoint of the pre-collision state
     float* d_grad_force = nullptr;         // AoS [N][2]
-    float* d_u0 = nullptr;                 // initial velocity AoS (for dL/du0)
+    float* d_u0 = nullptr;                 // initial velocity SoA [2][N] (for dL/du0)
     float* d_rho_init = nullptr;
     bool has_u0 = false;                   // forward started from lbm_init_from_macros
     bool ready = false;
@@ -85,7 +86,6 @@ struct lbm_ctx {
     int upload(void* dst, const void* src, size_t bytes, int on_device);
     int scratch(size_t bytes);
     int ensure_partials(long long n);
-    int seg_width() const;
     lbm::StepParams step_params(int src_idx, bool write_macros) const;
     int adapter_pass(int src_idx);
     int one_step(bool write_macros);

@@ -9,10 +9,9 @@ constexpr int kBlock = 256;      // workgroup of the auxiliary kernels (4 wave64
 constexpr int kMaxStepBlock = 512;
 constexpr int kXcds = 8;         // MI355X: 8 XCDs, workgroups dealt round-robin
 
-// Workgroup i runs on XCD i % 8. For grids larger than the Infinity Cache, give each XCD a
-// contiguous run of logical workgroups so the cache lines two neighbouring row segments
-// share stay in one XCD's L2 (measured: 25% faster on an 8192^2 grid, slower on a
-// cache-resident 1024^2 grid, where the identity map is used).
+// Workgroup i runs on XCD i % 8. With remap, each XCD gets a contiguous run of logical
+// workgroups, so the cache lines that neighbouring row segments share stay in one XCD's L2
+// (measured: 25% faster on an 8192^2 grid).
 __device__ __forceinline__ long long logical_block(long long nb_padded, int remap) {
     long long i = blockIdx.x;
     if (!remap) return i;
@@ -48,12 +47,12 @@ __device__ __forceinline__ void block_sum3(double& a, double& b, double& c) {
     }
 }
 
-template <int COLL, int STORE, int MODE>
+template <int COLL, int STORE, int MODE, int SEG>
 __global__ __launch_bounds__(kMaxStepBlock) void fused_step_kernel(StepParams p, double* partials, long long nb_padded,
                                                                    int remap) {
     long long lb = logical_block(nb_padded, remap);
     MomAcc acc{0.0, 0.0, 0.0};
-    step_segment<COLL, STORE, MODE>(p, lb * blockDim.x + threadIdx.x, acc);
+    step_segment<COLL, STORE, MODE, SEG>(p, lb * blockDim.x + threadIdx.x, acc);
     if constexpr (MODE != 0) {
         block_sum3(acc.rho, acc.jmag, acc.pimag);
         if (threadIdx.x == 0) {
@@ -90,7 +89,7 @@ __global__ __launch_bounds__(kBlock) void finalize_avg_kernel(const double* part
 // init_kernel + init_node (init.cuh:10-43): f = f_back = f_eq(rho, u).
 template <int STORE>
 __global__ __launch_bounds__(kBlock) void init_kernel(InitParams p) {
-    using St = Storage<STORE>;
+    using St = Elem<STORE>;
     using T = typename St::T;
     long long n = (long long)blockIdx.x * kBlock + threadIdx.x;
     if (n >= p.N) return;
@@ -109,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void init_kernel(InitParams p) {
 // FP32 SoA <-> storage.
 template <int STORE>
 __global__ __launch_bounds__(kBlock) void pops_to_f32_kernel(const void* buf, long long plane, long long N, float rho0, float* out) {
-    using St = Storage<STORE>;
+    using St = Elem<STORE>;
     long long n = (long long)blockIdx.x * kBlock + threadIdx.x;
     if (n >= N) return;
     const typename St::T* b = static_cast<const typename St::T*>(buf);
@@ -118,7 +117,7 @@ __global__ __launch_bounds__(kBlock) void pops_to_f32_kernel(const void* buf, lo
 }
 template <int STORE>
 __global__ __launch_bounds__(kBlock) void pops_from_f32_kernel(void* buf0, void* buf1, long long plane, long long N, float rho0, const float* in) {
-    using St = Storage<STORE>;
+    using St = Elem<STORE>;
     long long n = (long long)blockIdx.x * kBlock + threadIdx.x;
     if (n >= N) return;
     typename St::T* a = static_cast<typename St::T*>(buf0);
@@ -198,41 +197,69 @@ __global__ __launch_bounds__(kBlock) void row_mean_ux_kernel(const float* ux, in
 
 // ---- launchers ----------------------------------------------------------------------------
 
-LaunchPolicy step_policy(const StepParams& p, int storage) {
-    const double bytes = 2.0 * 9.0 * double(p.plane) * (storage == LBM_F32 ? 4.0 : 2.0);
+// Measured on MI355X (BGK, periodic, back-to-back launches; us per step):
+//   FP32 1024^2: 1 node/thread, 64-thread groups, remap 18.1 (4 nodes, 256, identity 26.8)
+//   FP32 2048^2: 1 node, 128, remap, nt 68.9 (4 nodes, 256, remap, nt 79.4)
+//   FP32 8192^2: 4 nodes, 128, remap, nt 944 (1 node: 998)
+//   FP16 1024^2: 2 nodes, 256, remap 16.9 (4 nodes: 21.4)
+//   FP16 2048^2: 2 nodes, 64, remap 44.2;  FP16 8192^2: 2 nodes, 128, remap, nt 611 (4 nodes: 675)
+// Small grids are latency-bound and want many wavefronts (narrow segments); only the FP32
+// grids far beyond the Infinity Cache prefer 16-byte vectors per thread.
+LaunchPolicy step_policy(long long plane, int storage) {
+    const bool f32 = storage == LBM_F32;
+    const double bytes = 2.0 * 9.0 * double(plane) * (f32 ? 4.0 : 2.0);
     const bool resident = bytes <= 192.0e6;  // fits the 256 MB Infinity Cache with room to spare
     LaunchPolicy lp;
-    lp.remap = resident ? 0 : 1;
-    lp.nt = (!resident && storage == LBM_F32) ? 1 : 0;
-    // HBM-resident sweep at 8192^2, BGK (MI355X): FP32 256/remap/nt 947 us (128: 954 us);
-    // FP16 64/remap 664 us (512: 764 us).
-    lp.block = resident ? 256 : (storage == LBM_F32 ? 256 : 64);
+    lp.remap = 1;
+    if (resident) {
+        lp.block = f32 ? 64 : 256;
+        lp.nt = 0;
+        lp.seg = f32 ? kSegNarrow<LBM_F32> : kSegNarrow<LBM_F16_SHIFTED>;
+    } else if (f32 && plane > (1LL << 23)) {  // beyond 8.4M nodes (1.2 GB of FP32 populations)
+        lp.block = 128;
+        lp.nt = 1;
+        lp.seg = kSegWide;
+    } else {
+        lp.block = 128;
+        lp.nt = 1;
+        lp.seg = f32 ? kSegNarrow<LBM_F32> : kSegNarrow<LBM_F16_SHIFTED>;
+    }
     return lp;
 }
 
 long long step_grid_blocks(const StepParams& p, const LaunchPolicy& lp) {
-    long long nseg = (long long)p.ntx * p.ny;
+    long long ntx = (p.nx + lp.seg - 1) / lp.seg;
+    long long nseg = ntx * p.ny;
     long long nb = (nseg + lp.block - 1) / lp.block;
     return lp.remap ? (nb + kXcds - 1) / kXcds * kXcds : nb;
 }
 
-template <int COLL, int STORE, int MODE>
+template <int COLL, int STORE, int MODE, int SEG>
 static hipError_t launch_step_t(StepParams p, const LaunchPolicy& lp, double* partials, hipStream_t s) {
     long long nb = step_grid_blocks(p, lp);
+    p.ntx = (p.nx + SEG - 1) / SEG;
     p.nt = lp.nt;
-    hipLaunchKernelGGL((fused_step_kernel<COLL, STORE, MODE>), dim3(unsigned(nb)), dim3(lp.block), 0, s, p, partials, nb,
-                       lp.remap);
+    hipLaunchKernelGGL((fused_step_kernel<COLL, STORE, MODE, SEG>), dim3(unsigned(nb)), dim3(lp.block), 0, s, p, partials,
+                       nb, lp.remap);
     return hipGetLastError();
+}
+
+template <int STORE, int SEG>
+static hipError_t launch_step_w(int coll, int mode, const StepParams& p, const LaunchPolicy& lp, double* partials,
+                                hipStream_t s) {
+    if (mode == 1) return launch_step_t<LBM_BGK, STORE, 1, SEG>(p, lp, partials, s);
+    if (mode == 2) return launch_step_t<LBM_CM, STORE, 2, SEG>(p, lp, partials, s);
+    if (coll == LBM_BGK) return launch_step_t<LBM_BGK, STORE, 0, SEG>(p, lp, partials, s);
+    if (coll == LBM_MRT) return launch_step_t<LBM_MRT, STORE, 0, SEG>(p, lp, partials, s);
+    return launch_step_t<LBM_CM, STORE, 0, SEG>(p, lp, partials, s);
 }
 
 template <int STORE>
 static hipError_t launch_step_s(int coll, int mode, const StepParams& p, const LaunchPolicy& lp, double* partials,
                                 hipStream_t s) {
-    if (mode == 1) return launch_step_t<LBM_BGK, STORE, 1>(p, lp, partials, s);
-    if (mode == 2) return launch_step_t<LBM_CM, STORE, 2>(p, lp, partials, s);
-    if (coll == LBM_BGK) return launch_step_t<LBM_BGK, STORE, 0>(p, lp, partials, s);
-    if (coll == LBM_MRT) return launch_step_t<LBM_MRT, STORE, 0>(p, lp, partials, s);
-    return launch_step_t<LBM_CM, STORE, 0>(p, lp, partials, s);
+    if (lp.seg == kSegWide) return launch_step_w<STORE, kSegWide>(coll, mode, p, lp, partials, s);
+    if (lp.seg == kSegNarrow<STORE>) return launch_step_w<STORE, kSegNarrow<STORE>>(coll, mode, p, lp, partials, s);
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_step(int coll, int storage, int mode, const StepParams& p, const LaunchPolicy& lp, double* partials,

@@ -16,16 +16,18 @@ struct InitParams {
     float rho0;
 };
 
-// Launch shape of the fused step: workgroup size, XCD-contiguous workgroup order and
-// non-temporal stores, chosen by whether the two population buffers fit in the Infinity Cache.
+// Launch shape of the fused step: workgroup size, XCD-contiguous workgroup order,
+// non-temporal stores and nodes per thread, chosen by whether the two population buffers
+// fit in the Infinity Cache.
 struct LaunchPolicy {
     int block = 256;
     int remap = 0;
     int nt = 0;
+    int seg = kSegWide;
 };
-LaunchPolicy step_policy(const StepParams& p, int storage);
+LaunchPolicy step_policy(long long plane, int storage);
 long long step_grid_blocks(const StepParams& p, const LaunchPolicy& lp);
-// mode: 0 step, 1 moments pass, 2 step + moments (CM only).
+// mode: 0 step, 1 moments pass, 2 step + moments (CM only). Sets p.ntx and p.nt from lp.
 hipError_t launch_step(int coll, int storage, int mode, const StepParams& p, const LaunchPolicy& lp, double* partials,
                        hipStream_t s);
 hipError_t launch_finalize_avg(const double* partials, long long n, long long N, float* avg, double* avg64, hipStream_t s);

@@ -136,7 +136,11 @@ LBM_HD uint16_t float_to_bf16_bits(float x) {
     return uint16_t(u >> 16);
 }
 
-template <int STORE>
+// Population storage of one plane row, seen as segments of V consecutive nodes (one
+// segment per thread). STORE picks the element format, V the segment width: narrow
+// segments give a small grid enough wavefronts to hide latency, wide ones move more bytes
+// per instruction on HBM-resident grids.
+template <int STORE, int V>
 struct Storage;
 
 // Stores of one 16-byte vector: plain, or non-temporal (streaming past the caches; used
@@ -154,44 +158,61 @@ LBM_HD void store16(void* p, const uint32_t* d, bool nt) {
     __builtin_memcpy(p, d, 16);
 }
 
-// FP32: four nodes per 16-byte vector. The x-shifted pull is one dword-aligned 16-byte
-// load (a single global_load_dwordx4 on gfx950).
-template <>
-struct Storage<LBM_F32> {
+// Stores of W dwords (W = 1, 2 or 4), plain or non-temporal.
+template <int W>
+LBM_HD void store_dwords(void* p, const uint32_t* d, bool nt) {
+    if constexpr (W == 4) {
+        store16(p, d, nt);
+    } else {
+#if defined(__HIP_DEVICE_COMPILE__)
+        if (nt) {
+#pragma unroll
+            for (int k = 0; k < W; k++) __builtin_nontemporal_store(d[k], static_cast<uint32_t*>(p) + k);
+            return;
+        }
+#endif
+        (void)nt;
+        __builtin_memcpy(p, d, 4 * W);
+    }
+}
+
+// FP32: V nodes (1, 2 or 4) per vector. The x-shifted pull is one dword-aligned vector
+// load (a single global_load_dwordx4 on gfx950 for 4 nodes).
+template <int V_>
+struct Storage<LBM_F32, V_> {
     using T = float;
-    static constexpr int V = 4;
+    static constexpr int V = V_;
+    static_assert(V == 1 || V == 2 || V == 4, "FP32 segments hold 1, 2 or 4 nodes");
     static constexpr int kRightHalo = 1;  // elements read past the segment on the right
     struct Vec {
-        float v[4];
+        float v[V];
     };
     LBM_HD static Vec load_shifted(const T* row, int x0, int cx) {
         Vec r;
-        __builtin_memcpy(r.v, row + (x0 - cx), 16);
+        __builtin_memcpy(r.v, row + (x0 - cx), 4 * V);
         return r;
     }
     LBM_HD static float get(const Vec& r, int j, int, float) { return r.v[j]; }
     LBM_HD static void put(Vec& r, int j, int, float, float v) { r.v[j] = v; }
     LBM_HD static void store_vec(T* p, const Vec& r, bool nt) {
-        uint32_t d[4];
-        __builtin_memcpy(d, r.v, 16);
-        store16(p, d, nt);
+        uint32_t d[V];
+        __builtin_memcpy(d, r.v, 4 * V);
+        store_dwords<V>(p, d, nt);
     }
     LBM_HD static float load1(const T* p, int, float) { return *p; }
     LBM_HD static void store1(T* p, int, float, float v) { *p = v; }
 };
 
-// 16-bit shifted storage: LBM_V16 nodes (4 or 8) per 8- or 16-byte vector. A pull shifted
+// 16-bit shifted storage: V nodes (2, 4 or 8) per 4-, 8- or 16-byte vector. A pull shifted
 // by one node is rebuilt from the aligned vector and the neighbouring dword with a 16-bit
-// funnel shift (v_alignbit_b32), keeping every load dword-aligned. 4 nodes per thread is the
-// default: at 8 the MRT and CM bodies need over 200 VGPRs and the 8192^2 step runs ~2x slower
-// (MRT FP16 1721 us vs 868 us measured on MI355X).
-#ifndef LBM_V16
-#define LBM_V16 4
-#endif
-template <int STORE>
+// funnel shift (v_alignbit_b32), keeping every load dword-aligned. At 8 nodes the MRT and CM
+// bodies need over 200 VGPRs and the 8192^2 step runs ~2x slower than at 4 (MRT FP16
+// 1721 us vs 868 us on MI355X), so 8 is not instantiated.
+template <int STORE, int V_>
 struct Shifted16 {
     using T = uint16_t;
-    static constexpr int V = LBM_V16;
+    static constexpr int V = V_;
+    static_assert(V == 2 || V == 4 || V == 8, "16-bit segments hold 2, 4 or 8 nodes");
     static constexpr int W = V / 2;       // dwords per vector
     static constexpr int kRightHalo = 2;
     struct Vec {
@@ -230,20 +251,23 @@ struct Shifted16 {
         uint32_t& w = r.d[j >> 1];
         w = (j & 1) ? ((w & 0xffffu) | (h << 16)) : ((w & 0xffff0000u) | h);
     }
-    LBM_HD static void store_vec(T* p, const Vec& r, bool nt) {
-        if constexpr (W == 4) {
-            store16(p, r.d, nt);
-        } else {
-            __builtin_memcpy(p, r.d, 4 * W);
-        }
-    }
+    LBM_HD static void store_vec(T* p, const Vec& r, bool nt) { store_dwords<W>(p, r.d, nt); }
     LBM_HD static float load1(const T* p, int q, float rho0) { return dec(*p) + wgt(q) * rho0; }
     LBM_HD static void store1(T* p, int q, float rho0, float v) { *p = enc(v - wgt(q) * rho0); }
 };
-template <>
-struct Storage<LBM_F16_SHIFTED> : Shifted16<LBM_F16_SHIFTED> {};
-template <>
-struct Storage<LBM_BF16_SHIFTED> : Shifted16<LBM_BF16_SHIFTED> {};
+template <int V>
+struct Storage<LBM_F16_SHIFTED, V> : Shifted16<LBM_F16_SHIFTED, V> {};
+template <int V>
+struct Storage<LBM_BF16_SHIFTED, V> : Shifted16<LBM_BF16_SHIFTED, V> {};
+
+// Segment widths instantiated: wide (4 nodes, for HBM-resident grids) and narrow (FP32 1,
+// 16-bit 2 nodes, for grids that stay in the Infinity Cache).
+constexpr int kSegWide = 4;
+template <int STORE>
+constexpr int kSegNarrow = STORE == LBM_F32 ? 1 : 2;
+// Single-element access (element type, load1/store1) does not depend on the width.
+template <int STORE>
+using Elem = Storage<STORE, kSegNarrow<STORE>>;
 
 // Moment partial sums for the OptimalAdapter averages (FP64).
 struct MomAcc {
@@ -289,7 +313,7 @@ struct Pop9 {
 // Post-stream value of population q at node (x, y).
 template <int STORE>
 LBM_HD float gather(const NodeArgs& p, int q, int x, int y) {
-    using St = Storage<STORE>;
+    using St = Elem<STORE>;
     const typename St::T* src = static_cast<const typename St::T*>(p.src);
     const typename St::T* dst = static_cast<const typename St::T*>(p.dst);
     long long node = (long long)y * p.nx + x;
@@ -383,9 +407,9 @@ LBM_HD void node_update(const StepParams& p, float* g, float fx, float fy, Macro
 }
 
 // The thread body. `seg` indexes segments row-major; out-of-range segments do nothing.
-template <int COLL, int STORE, int MODE>
+template <int COLL, int STORE, int MODE, int SEG>
 LBM_HD void step_segment(const StepParams& p, long long seg, MomAcc& acc) {
-    using St = Storage<STORE>;
+    using St = Storage<STORE, SEG>;
     using T = typename St::T;
     using Vec = typename St::Vec;
     constexpr int V = St::V;

@@ -120,8 +120,9 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out);
 /* Replaces LBM<2>::free / ~LBM (lbm.cuh:127-140, 379-381). NULL is accepted. */
 int lbm_destroy(lbm_ctx* ctx);
 
-/* Run the context's work on `hip_stream` (a hipStream_t) instead of its own stream,
- * e.g. PyTorch's current stream. NULL restores the context's own stream. */
+/* Run the context's work on `hip_stream` (a hipStream_t) instead of the non-blocking stream
+ * the context creates for itself, e.g. PyTorch's current stream. NULL selects the HIP null
+ * stream (PyTorch's default stream reports handle 0). */
 int lbm_set_stream(lbm_ctx* ctx, void* hip_stream);
 
 /* Replaces setup_boundary_flags (boundaries.cuh:170-197): one BC flag (enum lbm_bc_flag)

@@ -152,6 +152,33 @@ int oracle_ibm_get_forces(void* p, double* lag, double* total) {
     });
 }
 
+// The three IBM operators of one iteration on given fields (FP64), for pinning them one by
+// one: interpolation (IBM_impl.cu:5-51) of rho and u (SoA [2][N]) to the markers -> u_lag
+// (SoA [2][n]), rho_lag [n]; the Lagrangian force (IBM_impl.cuh:7-26) of those -> F_lag_out
+// (SoA [2][n]); spreading (IBM_impl.cu:120-154) of F_lag_in (SoA [2][n]) -> F_eul (SoA [2][N]).
+int oracle_ibm_operators(const lbm_config* cfg, const double* rho, const double* u, const double* xy,
+                         const double* ds, int n, const double* F_lag_in, double* u_lag, double* rho_lag,
+                         double* F_lag_out, double* F_eul) {
+    try {
+        Solver<double> s(*cfg);
+        s.set_markers(xy, ds, n, 1);
+        std::copy(rho, rho + s.N, s.rho.begin());
+        std::copy(u, u + 2 * s.N, s.eul_u_prev.begin());
+        s.ibm_interpolate();
+        s.ibm_lagrangian();
+        std::copy(s.lag_u.begin(), s.lag_u.end(), u_lag);
+        std::copy(s.lag_rho.begin(), s.lag_rho.end(), rho_lag);
+        std::copy(s.lag_force.begin(), s.lag_force.end(), F_lag_out);
+        std::copy(F_lag_in, F_lag_in + 2 * n, s.lag_force.begin());
+        std::fill(s.eul_force_iter.begin(), s.eul_force_iter.end(), 0.0);
+        s.ibm_spread();
+        std::copy(s.eul_force_iter.begin(), s.eul_force_iter.end(), F_eul);
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
 int oracle_get_force(void* p, double* F) {
     return dispatch(static_cast<Handle*>(p), [&](auto& s) {
         for (size_t n = 0; n < s.N; n++) {

@@ -12,7 +12,13 @@
 // Every function cites the reference lines it restates. Reference quirks are opt-in
 // (lbm_config.quirks); the default is the corrected behaviour.
 //
-// Parity-pinning status (what checks each part) is listed in tests/test_oracle_*.py.
+// Parity pinning: every function is checked by tests/test_oracle_pinning.py or
+// tests/test_adjoint_cpu.py against something it does not restate (moment identities,
+// closed forms, fixed points, operator equivalences, adjointness, finite differences).
+// Parity unpinned: the exact FP32 rounding sequence of the reference's pow/double-literal
+// equilibrium (equilibrium.cu:12-37), the float-atomic summation orders of
+// macroscopics.cuh:116-118 and IBM_impl.cu:150-151 (nondeterministic in the reference),
+// and device libm (sqrtf). All three act at FP32 rounding level.
 #pragma once
 
 #include "../include/lbm_hip.h"

@@ -46,6 +46,7 @@ def lib():
             "oracle_ibm_set_markers": (ctypes.c_int, [_P, _D, _D, ctypes.c_int, ctypes.c_int]),
             "oracle_ibm_get_forces": (ctypes.c_int, [_P, _D, _D]),
             "oracle_get_force": (ctypes.c_int, [_P, _D]),
+            "oracle_ibm_operators": (ctypes.c_int, [_P, _D, _D, _D, _D, ctypes.c_int, _D, _D, _D, _D, _D]),
             "oracle_last_error": (ctypes.c_char_p, [_P]),
             "oracle_adjoint": (ctypes.c_int, [_P, ctypes.c_int, _D, _D, _D]),
             "oracle_node_jacobian": (ctypes.c_int, [_P, _D, _D, _D]),
@@ -101,6 +102,25 @@ def cm_matrix_inverse(ux, uy):
 
 def delta4(r):
     return lib().oracle_delta4(float(r))
+
+
+def ibm_operators(cfg_c, rho, u, xy, ds, F_lag):
+    """One IBM iteration's operators on given fields (FP64): returns (u_lag [2, n],
+    rho_lag [n], F_lag_out [2, n] = Lagrangian force of u_lag, F_eul [2, ny, nx] = spread of
+    F_lag [2, n]). rho [ny, nx], u [2, ny, nx], xy [n, 2]."""
+    n = len(xy)
+    N = cfg_c.nx * cfg_c.ny
+    r = np.ascontiguousarray(rho, np.float64).reshape(-1)
+    uu = np.ascontiguousarray(u, np.float64).reshape(-1)
+    p = np.ascontiguousarray(xy, np.float64).reshape(-1)
+    d = np.ascontiguousarray(ds, np.float64).reshape(-1)
+    fl = np.ascontiguousarray(F_lag, np.float64).reshape(-1)
+    ul, rl, flo, fe = np.zeros(2 * n), np.zeros(n), np.zeros(2 * n), np.zeros(2 * N)
+    rc = lib().oracle_ibm_operators(ctypes.byref(cfg_c), _dp(r), _dp(uu), _dp(p), _dp(d), n, _dp(fl), _dp(ul),
+                                    _dp(rl), _dp(flo), _dp(fe))
+    if rc:
+        raise RuntimeError("oracle_ibm_operators failed")
+    return ul.reshape(2, n), rl, flo.reshape(2, n), fe.reshape(2, cfg_c.ny, cfg_c.nx)
 
 
 def adapter_rate(rho, j, pi, avg):

@@ -19,14 +19,11 @@ using namespace lbm;
 
 namespace {
 
-template <int STORE>
-using ElemT = typename Storage<STORE>::T;
-
-template <int COLL, int STORE>
+template <int COLL, int STORE, int SEG>
 int run_t(const lbm_config& c, const uint8_t* flags, const float* rho, const float* u, const float* force,
           int nsteps, float* f_out, float* rho_out, float* u_out, double* avg_out) {
-    using St = Storage<STORE>;
-    using T = ElemT<STORE>;
+    using St = Elem<STORE>;
+    using T = typename St::T;
     const int nx = c.nx, ny = c.ny;
     const long long N = (long long)nx * ny;
     T* buf[2];
@@ -92,18 +89,18 @@ int run_t(const lbm_config& c, const uint8_t* flags, const float* rho, const flo
         p.out_rho = last ? out_r : nullptr;
         p.out_ux = last ? out_x : nullptr;
         p.out_uy = last ? out_y : nullptr;
-        p.ntx = (nx + St::V - 1) / St::V;
+        p.ntx = (nx + SEG - 1) / SEG;
         const long long nseg = (long long)p.ntx * ny;
         if (adapt) {
             MomAcc tot{0, 0, 0};
-            for (long long sg = 0; sg < nseg; sg++) step_segment<LBM_BGK, STORE, 1>(p, sg, tot);
+            for (long long sg = 0; sg < nseg; sg++) step_segment<LBM_BGK, STORE, 1, SEG>(p, sg, tot);
             avg64[0] = tot.rho / double(N);
             avg64[1] = tot.jmag / double(N);
             avg64[2] = tot.pimag / double(N);
             for (int k = 0; k < 3; k++) avg[k] = float(avg64[k]);
         }
         MomAcc dummy{0, 0, 0};
-        for (long long sg = 0; sg < nseg; sg++) step_segment<COLL, STORE, 0>(p, sg, dummy);
+        for (long long sg = 0; sg < nseg; sg++) step_segment<COLL, STORE, 0, SEG>(p, sg, dummy);
         cur = 1 - cur;
     }
     if (f_out)
@@ -131,21 +128,34 @@ int run_t(const lbm_config& c, const uint8_t* flags, const float* rho, const flo
     return 0;
 }
 
-template <int STORE>
-int run_s(const lbm_config& c, const uint8_t* flags, const float* rho, const float* u, const float* force, int n,
+template <int STORE, int SEG>
+int run_w(const lbm_config& c, const uint8_t* flags, const float* rho, const float* u, const float* force, int n,
           float* f, float* r, float* uu, double* a) {
-    if (c.collision == LBM_BGK) return run_t<LBM_BGK, STORE>(c, flags, rho, u, force, n, f, r, uu, a);
-    if (c.collision == LBM_MRT) return run_t<LBM_MRT, STORE>(c, flags, rho, u, force, n, f, r, uu, a);
-    return run_t<LBM_CM, STORE>(c, flags, rho, u, force, n, f, r, uu, a);
+    if (c.collision == LBM_BGK) return run_t<LBM_BGK, STORE, SEG>(c, flags, rho, u, force, n, f, r, uu, a);
+    if (c.collision == LBM_MRT) return run_t<LBM_MRT, STORE, SEG>(c, flags, rho, u, force, n, f, r, uu, a);
+    return run_t<LBM_CM, STORE, SEG>(c, flags, rho, u, force, n, f, r, uu, a);
+}
+
+template <int STORE>
+int run_s(const lbm_config& c, int seg, const uint8_t* flags, const float* rho, const float* u, const float* force,
+          int n, float* f, float* r, float* uu, double* a) {
+    if (seg == kSegWide) return run_w<STORE, kSegWide>(c, flags, rho, u, force, n, f, r, uu, a);
+    if (seg == kSegNarrow<STORE>) return run_w<STORE, kSegNarrow<STORE>>(c, flags, rho, u, force, n, f, r, uu, a);
+    return -1;
 }
 
 }  // namespace
 
-extern "C" int emu_run(const lbm_config* c, const uint8_t* flags, const float* rho, const float* u, const float* force,
-                       int nsteps, float* f_out, float* rho_out, float* u_out, double* avg_out) {
-    if (c->storage == LBM_F16_SHIFTED) return run_s<LBM_F16_SHIFTED>(*c, flags, rho, u, force, nsteps, f_out, rho_out, u_out, avg_out);
-    if (c->storage == LBM_BF16_SHIFTED) return run_s<LBM_BF16_SHIFTED>(*c, flags, rho, u, force, nsteps, f_out, rho_out, u_out, avg_out);
-    return run_s<LBM_F32>(*c, flags, rho, u, force, nsteps, f_out, rho_out, u_out, avg_out);
+// seg: nodes per thread, the wide (4) or the storage's narrow width (FP32 1, 16-bit 2);
+// 0 selects the wide one.
+extern "C" int emu_run(const lbm_config* c, int seg, const uint8_t* flags, const float* rho, const float* u,
+                       const float* force, int nsteps, float* f_out, float* rho_out, float* u_out, double* avg_out) {
+    if (seg == 0) seg = kSegWide;
+    if (c->storage == LBM_F16_SHIFTED)
+        return run_s<LBM_F16_SHIFTED>(*c, seg, flags, rho, u, force, nsteps, f_out, rho_out, u_out, avg_out);
+    if (c->storage == LBM_BF16_SHIFTED)
+        return run_s<LBM_BF16_SHIFTED>(*c, seg, flags, rho, u, force, nsteps, f_out, rho_out, u_out, avg_out);
+    return run_s<LBM_F32>(*c, seg, flags, rho, u, force, nsteps, f_out, rho_out, u_out, avg_out);
 }
 
 // The product's node adjoints (lbm_adjoint.hpp) on the host: mode 0 BGK, 1 MRT, 2 snapshot
@@ -227,9 +237,15 @@ int main() {
                         }
                         std::vector<float> f(9 * N), r(N), uu(2 * N);
                         std::vector<double> a(3);
-                        emu_run(&c, fl.data(), rho.data(), u.data(), (cases & 1) ? F.data() : nullptr, 3, f.data(),
-                                r.data(), uu.data(), a.data());
-                        cases++;
+                        const int segs[2] = {kSegWide, storage == LBM_F32 ? 1 : 2};
+                        for (int seg : segs) {
+                            if (emu_run(&c, seg, fl.data(), rho.data(), u.data(), (cases & 1) ? F.data() : nullptr, 3,
+                                        f.data(), r.data(), uu.data(), a.data()) != 0) {
+                                std::printf("emu_run rejected seg %d\n", seg);
+                                return 1;
+                            }
+                            cases++;
+                        }
                     }
                 }
             }

@@ -14,14 +14,18 @@ from lbm_amd import _abi as A
 
 LAYOUTS = ["periodic", "channel", "cavity_mw", "cavity_reg", "inout", "zouhe_top", "open"]
 OPS = [(A.LBM_BGK, 0), (A.LBM_MRT, 0), (A.LBM_CM, 0), (A.LBM_CM, 1)]
+# nodes per thread: the wide segments of HBM-resident grids and the narrow ones of
+# cache-resident grids (FP32 1 node, 16-bit 2 nodes)
+WIDE, NARROW32, NARROW16 = 4, 1, 2
 
 
+@pytest.mark.parametrize("seg", [WIDE, NARROW32])
 @pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("op", OPS, ids=["bgk", "mrt", "cm", "cm_optimal"])
-def test_emulated_kernel_matches_oracle_fp32(layout, op):
+def test_emulated_kernel_matches_oracle_fp32(layout, op, seg):
     coll, adapter = op
     case = layout_case(layout, 21, 13, coll, adapter=adapter)
-    fe, re, ue, _ = run_emu(case, 12)
+    fe, re, ue, _ = run_emu(case, 12, seg)
     fo, ro, uo, _ = run_oracle(case, 12, 32)
     assert max_rel(fe, fo) <= 1e-5
     assert max_rel(re, ro) <= 1e-5
@@ -40,11 +44,12 @@ def test_emulated_quirks_match_oracle(quirk):
     assert max_rel(fb, fo) > 1e-4  # the quirk changes the result
 
 
+@pytest.mark.parametrize("seg", [WIDE, NARROW16])
 @pytest.mark.parametrize("storage", [A.LBM_F16_SHIFTED, A.LBM_BF16_SHIFTED])
 @pytest.mark.parametrize("coll", [A.LBM_BGK, A.LBM_MRT, A.LBM_CM])
-def test_emulated_reduced_storage(storage, coll):
+def test_emulated_reduced_storage(storage, coll, seg):
     case = layout_case("cavity_mw", 24, 16, coll, storage=storage)
-    fe, *_ = run_emu(case, 10)
+    fe, *_ = run_emu(case, 10, seg)
     fo, *_ = run_oracle(case, 10, 32)
     # FP32 rounding differences flip the rounding of a few 16-bit deviations (|d| < 0.05)
     quantum = 2.0 ** -11 if storage == A.LBM_F16_SHIFTED else 2.0 ** -8
@@ -56,9 +61,15 @@ def test_emulated_reduced_storage(storage, coll):
 def test_emulated_ragged_and_tiny_grids(sizes):
     for layout in ["periodic", "cavity_mw", "open"]:
         case = layout_case(layout, *sizes, A.LBM_BGK)
-        fe, *_ = run_emu(case, 5)
         fo, *_ = run_oracle(case, 5, 32)
-        assert max_rel(fe, fo) <= 1e-5, (layout, sizes)
+        for seg in (WIDE, NARROW32):
+            fe, *_ = run_emu(case, 5, seg)
+            assert max_rel(fe, fo) <= 1e-5, (layout, sizes, seg)
+        c16 = layout_case(layout, *sizes, A.LBM_BGK, storage=A.LBM_F16_SHIFTED)
+        f16o, *_ = run_oracle(c16, 5, 32)
+        for seg in (WIDE, NARROW16):
+            fe, *_ = run_emu(c16, 5, seg)
+            assert float(np.max(np.abs(fe - f16o))) <= 4 * 0.05 * 2.0 ** -11, (layout, sizes, seg)
 
 
 def test_asan_bounds_matrix():

@@ -33,9 +33,14 @@ def test_gpu_ibm_matches_oracle(gpu_available, collision, quirks):
     scale = float(np.max(np.abs(lo)))
     el = float(np.max(np.abs(lg - lo)))
     assert el <= 1e-4 * scale + 1e-7, el
-    # the lift (y) total is a near-cancelling sum: compare both components on the drag scale
     et = float(np.max(np.abs(tg - to)))
-    assert et <= 1e-4 * max(abs(to[0]), 1e-6) + 1e-7, (tg, to)
+    if quirks & A.LBM_QUIRK_IBM_CLAMP:
+        # the clamp zeroes every force <= 1e-8 (all of the drag), so the totals are sums of
+        # values at the threshold: bound them by the per-marker bound summed over markers
+        assert et <= len(lo) * (1e-4 * scale + 1e-7), (tg, to)
+    else:
+        # the lift (y) total is a near-cancelling sum: compare both components on the drag scale
+        assert et <= 1e-4 * max(abs(to[0]), 1e-6) + 1e-7, (tg, to)
     s.close()
 
 

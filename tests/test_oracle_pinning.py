@@ -343,6 +343,49 @@ def test_ibm_markers_slow_the_flow_and_feel_drag():
     assert np.max(np.abs(u2 - u)) > 1e-4
 
 
+def _ibm_fields(nx=16, ny=12, n=7, seed=11):
+    rng = np.random.default_rng(seed)
+    xy = np.stack([rng.uniform(3.0, nx - 4.0, n), rng.uniform(3.0, ny - 4.0, n)], axis=1)
+    ds = rng.uniform(0.5, 1.5, n)
+    return xy, ds
+
+
+def test_ibm_interpolation_is_exact_for_linear_fields():
+    # The 4x4 Peskin stencil reproduces constants and linear functions (sum phi = 1,
+    # sum phi r = 0), so interpolation of a + b x + c y is exact away from the edges; a
+    # wrong node offset, a swapped axis or a missing weight breaks it.
+    nx, ny = 16, 12
+    xy, ds = _ibm_fields(nx, ny)
+    y, x = np.mgrid[0:ny, 0:nx].astype(np.float64)
+    rho = 1.0 + 0.01 * x - 0.02 * y
+    u = np.stack([0.03 + 0.002 * x + 0.001 * y, -0.01 + 0.003 * x - 0.004 * y])
+    c = cfg(nx=nx, ny=ny)
+    ul, rl, flo, _ = orc.ibm_operators(c, rho, u, xy, ds, np.zeros((2, len(xy))))
+    px, py = xy[:, 0], xy[:, 1]
+    assert np.allclose(rl, 1.0 + 0.01 * px - 0.02 * py, atol=1e-14)
+    assert np.allclose(ul[0], 0.03 + 0.002 * px + 0.001 * py, atol=1e-14)
+    assert np.allclose(ul[1], -0.01 + 0.003 * px - 0.004 * py, atol=1e-14)
+    # direct forcing toward a resting body: F = 2 rho_L (0 - u_L) (IBM_impl.cuh:15-20)
+    assert np.allclose(flo, -2.0 * rl * ul, atol=1e-15)
+
+
+def test_ibm_spreading_is_the_ds_weighted_adjoint_of_interpolation():
+    # <I u, F ds> = <u, S F> for any u and F: spreading uses the same stencil weights as
+    # interpolation, times the marker arc length ds (the Q13 correction).
+    nx, ny = 16, 12
+    xy, ds = _ibm_fields(nx, ny)
+    rng = np.random.default_rng(5)
+    u = rng.normal(size=(2, ny, nx))
+    F = rng.normal(size=(2, len(xy)))
+    c = cfg(nx=nx, ny=ny)
+    ul, _, _, fe = orc.ibm_operators(c, np.ones((ny, nx)), u, xy, ds, F)
+    lhs = float(np.sum(ul * F * ds[None, :]))
+    rhs = float(np.sum(u * fe))
+    assert abs(lhs - rhs) < 1e-12 * max(1.0, abs(rhs))
+    # total spread force = sum_l F_l ds_l (the stencil weights sum to one)
+    assert np.allclose(fe.sum(axis=(1, 2)), (F * ds[None, :]).sum(axis=1), atol=1e-12)
+
+
 @pytest.mark.parametrize("r,s", [(0.1, 0.2), (0.5, 0.5), (0.9, 0.3)])
 def test_two_by_two_stencil_weights_do_not_sum_to_one(r, s):
     # IBM_impl.cu:19-20 sums the 4-point kernel over floor(p)+{0,1} only (quirk Q11)
