@@ -149,9 +149,51 @@ int lbm_ctx::adapter_pass(int src_idx) {
     return LBM_OK;
 }
 
+// After a two-step launch, put the edge nodes' previous state back into buf[1-cur], where
+// one-step launches (and the IBM gather) read unstreamed slots.
+int lbm_ctx::restore_edges() {
+    if (!edge_in_store) return LBM_OK;
+    hipError_t e = launch_edge_copy(cfg.storage, buf[1 - cur], d_edge[edge_cur], plane, cfg.nx, cfg.ny, cfg.periodic_x,
+                                    cfg.periodic_y, 0, stream);
+    if (e != hipSuccess) return hip_fail(e, "edge restore launch");
+    edge_in_store = false;
+    return LBM_OK;
+}
+
+bool lbm_ctx::two_step_ok() const {
+    const bool adapt = cfg.collision == LBM_CM && cfg.adapter == LBM_OPTIMAL_ADAPTER;
+    return cfg.steps_per_launch != 1 && !adapt && !ibm.active && !(cfg.quirks & LBM_QUIRK_STALE_REST) && flags_local;
+}
+
+// Two steps from buf[cur] into buf[1-cur] in one launch (the same arithmetic as two
+// one-step launches).
+int lbm_ctx::two_step(bool write_macros) {
+    hipError_t e;
+    if (n_edge > 0 && !edge_in_store) {  // state t-1 at the edge nodes: buf[1-cur] -> store
+        e = launch_edge_copy(cfg.storage, buf[1 - cur], d_edge[edge_cur], plane, cfg.nx, cfg.ny, cfg.periodic_x,
+                             cfg.periodic_y, 1, stream);
+        if (e != hipSuccess) return hip_fail(e, "edge save launch");
+    }
+    StepParams p = step_params(cur, write_macros);
+    p.edge_prev = d_edge[edge_cur];
+    p.edge_next = d_edge[1 - edge_cur];
+    p.n_edge = n_edge;
+    e = launch_step2(cfg.collision, cfg.storage, p, step2_policy(plane, cfg.storage), stream);
+    if (e != hipSuccess) return hip_fail(e, "two-step launch");
+    if (n_edge > 0) {
+        edge_cur = 1 - edge_cur;
+        edge_in_store = true;
+    }
+    cur = 1 - cur;
+    timestep += 2;
+    if (write_macros) macros_valid = true;
+    return LBM_OK;
+}
+
 // One step from buf[cur] into buf[1-cur].
 int lbm_ctx::one_step(bool write_macros) {
-    int rc;
+    int rc = restore_edges();
+    if (rc) return rc;
     if (ibm.active) {
         rc = ibm_pass(cur);
         if (rc) return rc;
@@ -186,7 +228,8 @@ lbm_ctx::~lbm_ctx() {
     (void)hipSetDevice(device);
     if (own_stream) (void)hipStreamSynchronize(own_stream);
     if (stream != own_stream) (void)hipStreamSynchronize(stream);
-    void* ptrs[] = {buf[0], buf[1], d_flags, d_row_class, d_force_field, d_rho, d_ux, d_uy, d_avg, d_avg64, partials, tmp};
+    void* ptrs[] = {buf[0],  buf[1], d_flags, d_row_class, d_force_field, d_rho,     d_ux,
+                    d_uy,    d_avg,  d_avg64, partials,    tmp,           d_edge[0], d_edge[1]};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     ibm_free();
@@ -273,6 +316,12 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
     rc = rc ? rc : c->dev_alloc(reinterpret_cast<void**>(&c->d_uy), c->N * 4, "uy");
     rc = rc ? rc : c->dev_alloc(reinterpret_cast<void**>(&c->d_avg), 4 * sizeof(float), "adapter averages");
     rc = rc ? rc : c->dev_alloc(reinterpret_cast<void**>(&c->d_avg64), 4 * sizeof(double), "adapter averages");
+    c->n_edge = edge_count(k.nx, k.ny, k.periodic_x, k.periodic_y);
+    if (c->n_edge > 0) {
+        const size_t eb = size_t(c->n_edge) * 9 * storage_bytes(k.storage);
+        rc = rc ? rc : c->dev_alloc(&c->d_edge[0], eb, "edge store");
+        rc = rc ? rc : c->dev_alloc(&c->d_edge[1], eb, "edge store");
+    }
     if (rc) {
         g_create_err = c->err;
         delete c;
@@ -309,7 +358,7 @@ int lbm_set_flags(lbm_ctx* ctx, const uint8_t* flags, int on_device) {
         std::memcpy(h.data(), flags, ctx->N);
     }
     std::vector<uint8_t> rowc(size_t(ny), 0);
-    bool any = false;
+    bool any = false, local = true;
     for (int y = 0; y < ny; y++) {
         for (int x = 0; x < nx; x++) {
             uint8_t f = h[size_t(y) * nx + x];
@@ -317,6 +366,7 @@ int lbm_set_flags(lbm_ctx* ctx, const uint8_t* flags, int on_device) {
                 return ctx->fail(LBM_E_UNSUPPORTED, "BC flag %d at node (%d,%d) is not supported", int(f), x, y);
             if (f == LBM_FLUID) continue;
             any = true;
+            if (f == LBM_ZG_OUTFLOW || f == LBM_REGULARIZED_BOUNCE_BACK_CORNER) local = false;
             if (f == LBM_ZG_OUTFLOW) {
                 // The interior neighbour's unknown populations must all stream from inside
                 // the domain: a stale slot of another node would be raced by its writer.
@@ -347,6 +397,7 @@ int lbm_set_flags(lbm_ctx* ctx, const uint8_t* flags, int on_device) {
         ctx->d_flags = nullptr;
         ctx->d_row_class = nullptr;
     }
+    ctx->flags_local = local;
     if (!any) return LBM_OK;
     row_classes(h.data(), nx, ny, rowc.data());
     rc = ctx->dev_alloc(reinterpret_cast<void**>(&ctx->d_flags), ctx->N, "flags");
@@ -386,6 +437,7 @@ int lbm_init_from_macros(lbm_ctx* ctx, const float* rho, const float* u, int on_
         if (e != hipSuccess) return ctx->hip_fail(e, "hipStreamSynchronize");
     }
     ctx->cur = 0;
+    ctx->edge_in_store = false;
     ctx->timestep = 0;
     ctx->initialized = true;
     ctx->macros_valid = true;
@@ -418,7 +470,13 @@ int lbm_step(lbm_ctx* ctx, int32_t nsteps) {
     if (!ctx->initialized) return ctx->fail(LBM_E_STATE, "lbm_step before lbm_init_from_macros / lbm_set_pops");
     int rc = ctx->activate();
     if (rc) return rc;
-    for (int i = 0; i < nsteps; i++) {
+    int i = 0;
+    if (ctx->two_step_ok())
+        for (; i + 2 <= nsteps; i += 2) {
+            rc = ctx->two_step(i + 2 == nsteps);
+            if (rc) return rc;
+        }
+    for (; i < nsteps; i++) {
         rc = ctx->one_step(i == nsteps - 1);
         if (rc) return rc;
     }
@@ -516,6 +574,7 @@ int lbm_set_pops(lbm_ctx* ctx, const float* f, int on_device) {
         if (e != hipSuccess) return ctx->hip_fail(e, "hipStreamSynchronize");
     }
     ctx->cur = 0;
+    ctx->edge_in_store = false;
     ctx->initialized = true;
     ctx->macros_valid = false;
     ctx->avg_valid = false;

@@ -62,6 +62,14 @@ struct lbm_ctx {
     long long plane = 0;
     void* buf[2] = {nullptr, nullptr};
     int cur = 0;                       // buffer holding the latest post-collision state
+    // Two-step launches: edge stores of the non-periodic edge nodes ([9][n_edge] each).
+    // edge_in_store: the state one step before buf[cur] at those nodes lives in
+    // d_edge[edge_cur] rather than in buf[1-cur] (true after a two-step launch).
+    void* d_edge[2] = {nullptr, nullptr};
+    int n_edge = 0;
+    int edge_cur = 0;
+    bool edge_in_store = false;
+    bool flags_local = true;           // no flag needs a neighbour's populations
     uint8_t* d_flags = nullptr;
     uint8_t* d_row_class = nullptr;
     float* d_force_field = nullptr;
@@ -89,6 +97,9 @@ struct lbm_ctx {
     lbm::StepParams step_params(int src_idx, bool write_macros) const;
     int adapter_pass(int src_idx);
     int one_step(bool write_macros);
+    bool two_step_ok() const;
+    int two_step(bool write_macros);
+    int restore_edges();
     // lbm_ibm.hip
     int ibm_pass(int src_idx);
     void ibm_free();

@@ -63,6 +63,44 @@ __global__ __launch_bounds__(kMaxStepBlock) void fused_step_kernel(StepParams p,
     }
 }
 
+// Two time steps per launch; see step2_phase_a / step2_phase_b.
+template <int COLL, int STORE, int TY>
+__global__ __launch_bounds__(kT2X * TY) void fused_step2_kernel(StepParams p, long long nb_padded, int remap,
+                                                                 int tiles_x, long long n_tiles) {
+    using T = typename Elem<STORE>::T;
+    __shared__ T tile[Q * Tile2<TY>::NH];
+    const long long lb = logical_block(nb_padded, remap);
+    if (lb >= n_tiles) return;  // whole workgroup: remap padding
+    const int by = int(lb / tiles_x), bx = int(lb - (long long)by * tiles_x);
+    const int x0 = bx * kT2X, y0 = by * TY;
+    for (int h = threadIdx.x; h < Tile2<TY>::NH; h += kT2X * TY) step2_phase_a<COLL, STORE, TY>(p, tile, h, x0, y0);
+    __syncthreads();
+    step2_phase_b<COLL, STORE, TY>(p, tile, threadIdx.x, x0, y0);
+}
+
+// Copy the populations of the non-periodic edge nodes between a population buffer and an
+// edge store ([9][n_edge], edge_slot order). to_store = 1: buffer -> store (every slot);
+// 0: store -> buffer (each node from its first slot only).
+template <int STORE>
+__global__ __launch_bounds__(kBlock) void edge_copy_kernel(void* buf, void* store, long long plane, int nx, int ny, int px,
+                                                         int py, int n_edge, int to_store) {
+    using T = typename Elem<STORE>::T;
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= n_edge) return;
+    int x, y;
+    edge_node(e, nx, ny, py, x, y);
+    const long long node = (long long)y * nx + x;
+    T* b = static_cast<T*>(buf);
+    T* st = static_cast<T*>(store);
+    if (to_store) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) st[(long long)q * n_edge + e] = b[q * plane + node];
+    } else if (edge_slot(x, y, nx, ny, px, py) == e) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) b[q * plane + node] = st[(long long)q * n_edge + e];
+    }
+}
+
 // Sum `n` partial triples in a fixed order and store the means (sum / N) as floats, the
 // MomentInfo the adapter reads (lbm.cuh:25-31, macroscopics.cuh:116-118).
 __global__ __launch_bounds__(kBlock) void finalize_avg_kernel(const double* partials, long long n, double inv_nodes,
@@ -268,6 +306,66 @@ hipError_t launch_step(int coll, int storage, int mode, const StepParams& p, con
     if (storage == LBM_F16_SHIFTED) return launch_step_s<LBM_F16_SHIFTED>(coll, mode, p, lp, partials, s);
     if (storage == LBM_BF16_SHIFTED) return launch_step_s<LBM_BF16_SHIFTED>(coll, mode, p, lp, partials, s);
     return launch_step_s<LBM_F32>(coll, mode, p, lp, partials, s);
+}
+
+long long step2_grid_blocks(const StepParams& p, const Step2Policy& lp) {
+    const long long tiles = (long long)((p.nx + kT2X - 1) / kT2X) * ((p.ny + lp.ty - 1) / lp.ty);
+    return lp.remap ? (tiles + kXcds - 1) / kXcds * kXcds : tiles;
+}
+
+template <int COLL, int STORE, int TY>
+static hipError_t launch_step2_t(const StepParams& p, const Step2Policy& lp, hipStream_t s) {
+    const int tiles_x = (p.nx + kT2X - 1) / kT2X;
+    const long long n_tiles = (long long)tiles_x * ((p.ny + TY - 1) / TY);
+    const long long nb = step2_grid_blocks(p, lp);
+    hipLaunchKernelGGL((fused_step2_kernel<COLL, STORE, TY>), dim3(unsigned(nb)), dim3(kT2X * TY), 0, s, p, nb, lp.remap,
+                       tiles_x, n_tiles);
+    return hipGetLastError();
+}
+
+template <int STORE, int TY>
+static hipError_t launch_step2_c(int coll, const StepParams& p, const Step2Policy& lp, hipStream_t s) {
+    if (coll == LBM_BGK) return launch_step2_t<LBM_BGK, STORE, TY>(p, lp, s);
+    if (coll == LBM_MRT) return launch_step2_t<LBM_MRT, STORE, TY>(p, lp, s);
+    return launch_step2_t<LBM_CM, STORE, TY>(p, lp, s);
+}
+
+template <int STORE>
+static hipError_t launch_step2_s(int coll, const StepParams& p, const Step2Policy& lp, hipStream_t s) {
+    if (lp.ty == 4) return launch_step2_c<STORE, 4>(coll, p, lp, s);
+    if (lp.ty == 8) return launch_step2_c<STORE, 8>(coll, p, lp, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_step2(int coll, int storage, const StepParams& p, const Step2Policy& lp, hipStream_t s) {
+    if (p.n_edge != edge_count(p.nx, p.ny, p.px, p.py)) return hipErrorInvalidValue;
+    if (p.n_edge > 0 && (!p.edge_prev || !p.edge_next)) return hipErrorInvalidValue;
+    if (storage == LBM_F16_SHIFTED) return launch_step2_s<LBM_F16_SHIFTED>(coll, p, lp, s);
+    if (storage == LBM_BF16_SHIFTED) return launch_step2_s<LBM_BF16_SHIFTED>(coll, p, lp, s);
+    return launch_step2_s<LBM_F32>(coll, p, lp, s);
+}
+
+Step2Policy step2_policy(long long plane, int storage) {
+    (void)plane;
+    (void)storage;
+    Step2Policy lp;
+    lp.ty = 4;
+    lp.remap = 1;
+    return lp;
+}
+
+hipError_t launch_edge_copy(int storage, void* buf, void* store, long long plane, int nx, int ny, int px, int py,
+                            int to_store, hipStream_t s) {
+    const int n_edge = edge_count(nx, ny, px, py);
+    if (n_edge == 0) return hipSuccess;
+    const unsigned nb = unsigned((n_edge + kBlock - 1) / kBlock);
+    if (storage == LBM_F16_SHIFTED)
+        hipLaunchKernelGGL(edge_copy_kernel<LBM_F16_SHIFTED>, dim3(nb), dim3(kBlock), 0, s, buf, store, plane, nx, ny, px, py, n_edge, to_store);
+    else if (storage == LBM_BF16_SHIFTED)
+        hipLaunchKernelGGL(edge_copy_kernel<LBM_BF16_SHIFTED>, dim3(nb), dim3(kBlock), 0, s, buf, store, plane, nx, ny, px, py, n_edge, to_store);
+    else
+        hipLaunchKernelGGL(edge_copy_kernel<LBM_F32>, dim3(nb), dim3(kBlock), 0, s, buf, store, plane, nx, ny, px, py, n_edge, to_store);
+    return hipGetLastError();
 }
 
 hipError_t launch_finalize_avg(const double* partials, long long n, long long N, float* avg, double* avg64, hipStream_t s) {

@@ -30,6 +30,18 @@ long long step_grid_blocks(const StepParams& p, const LaunchPolicy& lp);
 // mode: 0 step, 1 moments pass, 2 step + moments (CM only). Sets p.ntx and p.nt from lp.
 hipError_t launch_step(int coll, int storage, int mode, const StepParams& p, const LaunchPolicy& lp, double* partials,
                        hipStream_t s);
+// Two steps per launch (temporal blocking): tile height (4 or 8 rows of kT2X nodes) and
+// XCD remap. The caller sets p.edge_prev / edge_next / n_edge when a non-periodic axis exists.
+struct Step2Policy {
+    int ty = 4;
+    int remap = 1;
+};
+Step2Policy step2_policy(long long plane, int storage);
+long long step2_grid_blocks(const StepParams& p, const Step2Policy& lp);
+hipError_t launch_step2(int coll, int storage, const StepParams& p, const Step2Policy& lp, hipStream_t s);
+// to_store = 1: edge nodes of `buf` -> store; 0: store -> edge nodes of `buf`.
+hipError_t launch_edge_copy(int storage, void* buf, void* store, long long plane, int nx, int ny, int px, int py,
+                            int to_store, hipStream_t s);
 hipError_t launch_finalize_avg(const double* partials, long long n, long long N, float* avg, double* avg64, hipStream_t s);
 hipError_t launch_init(int storage, const InitParams& p, hipStream_t s);
 hipError_t launch_pops_to_f32(int storage, const void* buf, long long plane, long long N, float rho0, float* out, hipStream_t s);

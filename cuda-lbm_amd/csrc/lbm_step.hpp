@@ -50,6 +50,11 @@ struct StepParams {
     float* out_uy;
     int ntx;                     // segments per row
     int nt;                      // non-temporal population stores
+    // two-step launches: state t-1 (read) and t+1 (written) at the non-periodic edge nodes,
+    // each [9][n_edge] in the storage format (edge_slot order)
+    const void* edge_prev;
+    void* edge_next;
+    int n_edge;
 };
 
 // Row classes for the fast-path test: 0 all FLUID, 1 non-FLUID only at x = 0 or x = nx-1,
@@ -278,20 +283,58 @@ struct MomAcc {
 
 // What the per-node general path needs, passed by value (kernel arguments must not be
 // addressed through a generic pointer).
+// `stale` is where a pull from outside a non-periodic domain reads (the slot the
+// reference's push stream leaves untouched): the node's own slot in the buffer being
+// written (stale_edge = 0, index = node, stride = plane), or, inside a two-step launch,
+// the edge store (stale_edge = 1, index = edge_slot, stride = n_edge).
 struct NodeArgs {
     const void* src;
     const void* dst;
+    const void* stale;
     const uint8_t* flags;
-    long long plane;
-    int nx, ny, px, py;
+    long long plane, stale_stride;
+    int nx, ny, px, py, stale_edge;
     float rho0, u_lid, wall_ux, wall_uy;
     unsigned quirks;
 };
+
+// Slot of a node on a non-periodic domain edge in the edge store: rows y = 0 and NY-1
+// first (when y is not periodic), then columns x = 0 and NX-1 (when x is not periodic).
+// A corner has a row slot and a column slot; readers use the first, writers fill both.
+// Returns -1 for nodes off the non-periodic edges.
+LBM_HD int edge_slot(int x, int y, int nx, int ny, int px, int py) {
+    if (!py) {
+        if (y == 0) return x;
+        if (y == ny - 1) return nx + x;
+    }
+    const int base = py ? 0 : 2 * nx;
+    if (!px) {
+        if (x == 0) return base + y;
+        if (x == nx - 1) return base + ny + y;
+    }
+    return -1;
+}
+LBM_HD int edge_count(int nx, int ny, int px, int py) { return (py ? 0 : 2 * nx) + (px ? 0 : 2 * ny); }
+// The node of edge slot e (inverse of edge_slot; a corner is reached from both its slots).
+LBM_HD void edge_node(int e, int nx, int ny, int py, int& x, int& y) {
+    const int rows = py ? 0 : 2 * nx;
+    if (e < rows) {
+        x = e < nx ? e : e - nx;
+        y = e < nx ? 0 : ny - 1;
+    } else {
+        const int c = e - rows;
+        x = c < ny ? 0 : nx - 1;
+        y = c < ny ? c : c - ny;
+    }
+}
 
 LBM_HD NodeArgs node_args(const StepParams& p) {
     NodeArgs a;
     a.src = p.src;
     a.dst = p.dst;
+    a.stale = p.dst;
+    a.stale_stride = p.plane;
+    a.stale_edge = 0;
     a.flags = p.flags;
     a.plane = p.plane;
     a.nx = p.nx;
@@ -328,8 +371,28 @@ LBM_HD float gather(const NodeArgs& p, int q, int x, int y) {
         if (p.py) ys = ys < 0 ? ys + p.ny : ys - p.ny;
         else out = true;
     }
-    if (out) return St::load1(dst + q * p.plane + node, q, p.rho0);
+    if (out) {
+        const typename St::T* st = static_cast<const typename St::T*>(p.stale);
+        long long idx = p.stale_edge ? (long long)edge_slot(x, y, p.nx, p.ny, p.px, p.py) : node;
+        return St::load1(st + q * p.stale_stride + idx, q, p.rho0);
+    }
     return St::load1(src + q * p.plane + (long long)ys * p.nx + xs, q, p.rho0);
+}
+
+// Boundary functors that need only the node's own post-stream populations. Returns false
+// for the flags that also read a neighbour (ZG outflow, regularized corner).
+LBM_HD bool apply_local_bc(float* g, uint8_t fl, int x, int y, const NodeArgs& p) {
+    Geom G{p.nx, p.ny, p.px, p.py};
+    switch (fl) {
+        case LBM_FLUID: return true;
+        case LBM_BOUNCE_BACK: bc_bounce_back(g, x, y, G, false, 0.f, 0.f, p.rho0); return true;
+        case LBM_MOVING_WALL_BOUNCE_BACK: bc_bounce_back(g, x, y, G, true, p.wall_ux, p.wall_uy, p.rho0); return true;
+        case LBM_ZOU_HE_TOP: bc_zou_he_top(g, p.u_lid); return true;
+        case LBM_ZOU_HE_LEFT: bc_zou_he_left(g, p.u_lid, (p.quirks & LBM_QUIRK_ZOUHE_RHO) != 0); return true;
+        case LBM_REGULARIZED_INLET_TOP: bc_regularized_inlet_top(g, p.u_lid); return true;
+        case LBM_REGULARIZED_BOUNCE_BACK: bc_regularized_bounce_back(g, x, y, G); return true;
+        default: return false;
+    }
 }
 
 // General path for one node: gather, then the boundary functor of its flag.
@@ -341,14 +404,9 @@ __host__ __device__ __noinline__ Pop9 gather_node(NodeArgs p, int x, int y) {
     for (int q = 0; q < Q; q++) g[q] = gather<STORE>(p, q, x, y);
     uint8_t fl = p.flags ? p.flags[(long long)y * p.nx + x] : uint8_t(LBM_FLUID);
     if (fl == LBM_FLUID) return r;
+    if (apply_local_bc(g, fl, x, y, p)) return r;
     Geom G{p.nx, p.ny, p.px, p.py};
     switch (fl) {
-        case LBM_BOUNCE_BACK: bc_bounce_back(g, x, y, G, false, 0.f, 0.f, p.rho0); break;
-        case LBM_MOVING_WALL_BOUNCE_BACK: bc_bounce_back(g, x, y, G, true, p.wall_ux, p.wall_uy, p.rho0); break;
-        case LBM_ZOU_HE_TOP: bc_zou_he_top(g, p.u_lid); break;
-        case LBM_ZOU_HE_LEFT: bc_zou_he_left(g, p.u_lid, (p.quirks & LBM_QUIRK_ZOUHE_RHO) != 0); break;
-        case LBM_REGULARIZED_INLET_TOP: bc_regularized_inlet_top(g, p.u_lid); break;
-        case LBM_REGULARIZED_BOUNCE_BACK: bc_regularized_bounce_back(g, x, y, G); break;
         case LBM_ZG_OUTFLOW: {
             // zeroGradientOutflow.cuh:9-59: unknowns (c.n > 0) from the interior neighbour.
             int n0 = 0, n1 = 0, ix = x, iy = y;
@@ -492,6 +550,140 @@ LBM_HD void step_segment(const StepParams& p, long long seg, MomAcc& acc) {
             p.out_ux[row0 + x] = m.ux;
             p.out_uy[row0 + x] = m.uy;
         }
+    }
+}
+
+// ---- two steps per launch (temporal blocking) --------------------------------------------
+//
+// A workgroup owns a tile of kT2X x TY nodes. Phase A computes the post-collision state t+1
+// of the tile plus a one-node halo from the state t in HBM and keeps it in LDS in the
+// storage format (so every value is what a one-step launch would have stored); phase B
+// computes state t+2 of the tile from LDS and writes it to HBM. Per step, the populations
+// cross HBM once every two steps instead of every step; the halo costs (kT2X+2)(TY+2) /
+// (kT2X TY) phase-A node updates per node.
+//
+// Unstreamed slots: a one-step launch reads them from the buffer it writes (state t-1 at
+// step t+1, state t at step t+2). Here phase B reads state t from the source buffer, and
+// phase A reads state t-1 from a small edge store, because the destination buffer is being
+// overwritten by other workgroups during the launch; phase A also saves state t+1 of its own
+// edge nodes into the next edge store. Boundary functors that read a neighbour's
+// populations (ZG outflow, regularized corner) and the stale-rest quirk need the one-step
+// launch.
+constexpr int kT2X = 64;  // tile width: one wavefront per tile row
+
+template <int TY>
+struct Tile2 {
+    static constexpr int HX = kT2X + 2, HY = TY + 2, NH = HX * HY;
+};
+
+template <int COLL, int STORE, int TY>
+LBM_HD void step2_phase_a(const StepParams& p, typename Elem<STORE>::T* tile, int h, int x0, int y0) {
+    using St = Elem<STORE>;
+    using T = typename St::T;
+    constexpr int HX = Tile2<TY>::HX, NH = Tile2<TY>::NH;
+    const int hx = h % HX, hy = h / HX;
+    int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
+    const bool own = hx >= 1 && hx <= kT2X && hy >= 1 && hy <= TY && gx < p.nx && gy < p.ny;
+    if (gx < 0 || gx >= p.nx) {
+        if (!p.px) return;
+        gx = ((gx % p.nx) + p.nx) % p.nx;
+    }
+    if (gy < 0 || gy >= p.ny) {
+        if (!p.py) return;
+        gy = ((gy % p.ny) + p.ny) % p.ny;
+    }
+    const uint32_t node = uint32_t(gy) * uint32_t(p.nx) + uint32_t(gx);
+    const uint8_t fl = p.flags ? p.flags[node] : uint8_t(LBM_FLUID);
+    const bool ex = !p.px && (gx == 0 || gx == p.nx - 1), ey = !p.py && (gy == 0 || gy == p.ny - 1);
+    // Pull sources with the periodic wrap; a source outside a non-periodic domain reads the
+    // node's unstreamed slot, state t-1, from the edge store. Inline (no call, no scratch):
+    // every wave of a boundary tile has some lanes on the domain edge.
+    // Plane bases are wave-uniform (scalar registers); per-lane offsets are 32-bit (N < 2^31).
+    const T* src = static_cast<const T*>(p.src);
+    const int xm = gx == 0 ? (p.px ? p.nx - 1 : -1) : gx - 1, xp = gx == p.nx - 1 ? (p.px ? 0 : -1) : gx + 1;
+    const int ym = gy == 0 ? (p.py ? p.ny - 1 : -1) : gy - 1, yp = gy == p.ny - 1 ? (p.py ? 0 : -1) : gy + 1;
+    const uint32_t rm = uint32_t(ym) * uint32_t(p.nx), r0 = uint32_t(gy) * uint32_t(p.nx), rp = uint32_t(yp) * uint32_t(p.nx);
+    float g[Q];
+    if (!ex && !ey) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const uint32_t xs = uint32_t(CX[q] > 0 ? xm : (CX[q] < 0 ? xp : gx));
+            const uint32_t row = CY[q] > 0 ? rm : (CY[q] < 0 ? rp : r0);
+            g[q] = St::load1(src + q * p.plane + (row + xs), q, p.rho0);
+        }
+    } else {
+        const T* prev = static_cast<const T*>(p.edge_prev);
+        const uint32_t eslot = uint32_t(edge_slot(gx, gy, p.nx, p.ny, p.px, p.py));
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int xs = CX[q] > 0 ? xm : (CX[q] < 0 ? xp : gx);
+            const int ys = CY[q] > 0 ? ym : (CY[q] < 0 ? yp : gy);
+            const uint32_t row = CY[q] > 0 ? rm : (CY[q] < 0 ? rp : r0);
+            g[q] = (xs < 0 || ys < 0) ? St::load1(prev + q * p.n_edge + eslot, q, p.rho0)
+                                      : St::load1(src + q * p.plane + (row + uint32_t(xs)), q, p.rho0);
+        }
+    }
+    if (fl != LBM_FLUID) apply_local_bc(g, fl, gx, gy, node_args(p));
+    float fx, fy;
+    node_force(p, gx, gy, node, fx, fy);
+    Macro m;
+    MomAcc acc{0.0, 0.0, 0.0};
+    node_update<COLL, 0>(p, g, fx, fy, m, acc);
+#pragma unroll
+    for (int q = 0; q < Q; q++) St::store1(tile + q * NH + h, q, p.rho0, g[q]);
+    if (own && (ex || ey)) {
+        T* en = static_cast<T*>(p.edge_next);
+        const int base = p.py ? 0 : 2 * p.nx;
+        int slot[2] = {-1, -1};
+        if (ey) slot[0] = gy == 0 ? gx : p.nx + gx;
+        if (ex) slot[1] = base + (gx == 0 ? gy : p.ny + gy);
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            if (slot[k] < 0) continue;
+#pragma unroll
+            for (int q = 0; q < Q; q++) St::store1(en + (long long)q * p.n_edge + slot[k], q, p.rho0, g[q]);
+        }
+    }
+}
+
+template <int COLL, int STORE, int TY>
+LBM_HD void step2_phase_b(const StepParams& p, const typename Elem<STORE>::T* tile, int tid, int x0, int y0) {
+    using St = Elem<STORE>;
+    using T = typename St::T;
+    constexpr int HX = Tile2<TY>::HX, NH = Tile2<TY>::NH;
+    const int tx = tid % kT2X, ty = tid / kT2X;
+    const int x = x0 + tx, y = y0 + ty;
+    if (x >= p.nx || y >= p.ny) return;
+    const uint32_t node = uint32_t(y) * uint32_t(p.nx) + uint32_t(x);
+    const T* src = static_cast<const T*>(p.src);
+    const bool ex = !p.px && (x == 0 || x == p.nx - 1), ey = !p.py && (y == 0 || y == p.ny - 1);
+    float g[Q];
+    if (!ex && !ey) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) g[q] = St::load1(tile + q * NH + (ty + 1 - CY[q]) * HX + (tx + 1 - CX[q]), q, p.rho0);
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int xs = x - CX[q], ys = y - CY[q];
+            const bool out = (!p.px && (xs < 0 || xs >= p.nx)) || (!p.py && (ys < 0 || ys >= p.ny));
+            g[q] = out ? St::load1(src + q * p.plane + node, q, p.rho0)
+                       : St::load1(tile + q * NH + (ty + 1 - CY[q]) * HX + (tx + 1 - CX[q]), q, p.rho0);
+        }
+    }
+    const uint8_t fl = p.flags ? p.flags[node] : uint8_t(LBM_FLUID);
+    if (fl != LBM_FLUID) apply_local_bc(g, fl, x, y, node_args(p));
+    float fx, fy;
+    node_force(p, x, y, node, fx, fy);
+    Macro m;
+    MomAcc acc{0.0, 0.0, 0.0};
+    node_update<COLL, 0>(p, g, fx, fy, m, acc);
+    T* dst = static_cast<T*>(p.dst);
+#pragma unroll
+    for (int q = 0; q < Q; q++) St::store1(dst + q * p.plane + node, q, p.rho0, g[q]);
+    if (p.out_rho) {
+        p.out_rho[node] = m.rho;
+        p.out_ux[node] = m.ux;
+        p.out_uy[node] = m.uy;
     }
 }
 

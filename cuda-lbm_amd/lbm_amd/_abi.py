@@ -82,7 +82,8 @@ class LbmConfig(ctypes.Structure):
         ("wall_u", ctypes.c_float * 2),
         ("u_lid", ctypes.c_float),
         ("quirks", ctypes.c_uint32),
-        ("reserved", ctypes.c_int32 * 8),
+        ("steps_per_launch", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 7),
     ]
 
 

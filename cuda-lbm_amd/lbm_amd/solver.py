@@ -41,6 +41,7 @@ class Config:
     wall_u: tuple = (0.0, 0.0)
     u_lid: float = 0.1
     quirks: int = 0
+    steps_per_launch: int = 0  # 0: two steps per launch where supported, 1: one
 
     def to_c(self) -> A.LbmConfig:
         c = A.LbmConfig()
@@ -56,6 +57,7 @@ class Config:
         c.wall_u[0], c.wall_u[1] = self.wall_u
         c.u_lid = self.u_lid
         c.quirks = self.quirks
+        c.steps_per_launch = self.steps_per_launch
         return c
 
     @property

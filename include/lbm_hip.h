@@ -105,7 +105,10 @@ typedef struct lbm_config {
     float    wall_u[2];      /* velocity of LBM_MOVING_WALL_BOUNCE_BACK nodes               */
     float    u_lid;          /* Scenario::u_max as used by REGULARIZED_INLET_TOP / ZOU_HE_*  */
     uint32_t quirks;         /* LBM_QUIRK_* bitmask                                         */
-    int32_t  reserved[8];
+    int32_t  steps_per_launch; /* 0: two time steps per launch wherever supported (the same
+                                  algorithm; agrees with 1 to FP32 rounding), 1: one step per
+                                  launch                                                       */
+    int32_t  reserved[7];
 } lbm_config;
 
 typedef struct lbm_ctx lbm_ctx;

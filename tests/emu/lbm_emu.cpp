@@ -10,6 +10,7 @@
 #include "../../cuda-lbm_amd/csrc/lbm_step.hpp"
 #include "../../cuda-lbm_amd/csrc/lbm_adjoint.hpp"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -19,9 +20,39 @@ using namespace lbm;
 
 namespace {
 
+// A value no kernel stores: NaN in the storage format (poisons unwritten tile entries).
+template <class T>
+T poison() {
+    if constexpr (sizeof(T) == 4) {
+        float f;
+        uint32_t b = 0x7fc00000u;
+        std::memcpy(&f, &b, 4);
+        return T(f);
+    } else {
+        return T(0x7e00);  // FP16 NaN; a BF16 NaN as well (exponent all ones, mantissa != 0)
+    }
+}
+
+// One two-step launch: every tile's phase A into a fresh poisoned tile, then its phase B.
+// reverse = 1 runs the tiles in reverse order (results must not depend on tile order).
+template <int COLL, int STORE, int TY>
+void launch2_emu(const StepParams& p, int reverse) {
+    using T = typename Elem<STORE>::T;
+    const int tiles_x = (p.nx + kT2X - 1) / kT2X, tiles_y = (p.ny + TY - 1) / TY;
+    const long long nt = (long long)tiles_x * tiles_y;
+    std::vector<T> tile(size_t(Q) * Tile2<TY>::NH);
+    for (long long k = 0; k < nt; k++) {
+        const long long lb = reverse ? nt - 1 - k : k;
+        const int by = int(lb / tiles_x), bx = int(lb - (long long)by * tiles_x);
+        std::fill(tile.begin(), tile.end(), poison<T>());
+        for (int h = 0; h < Tile2<TY>::NH; h++) step2_phase_a<COLL, STORE, TY>(p, tile.data(), h, bx * kT2X, by * TY);
+        for (int t = 0; t < kT2X * TY; t++) step2_phase_b<COLL, STORE, TY>(p, tile.data(), t, bx * kT2X, by * TY);
+    }
+}
+
 template <int COLL, int STORE, int SEG>
 int run_t(const lbm_config& c, const uint8_t* flags, const float* rho, const float* u, const float* force,
-          int nsteps, float* f_out, float* rho_out, float* u_out, double* avg_out) {
+          int nsteps, float* f_out, float* rho_out, float* u_out, double* avg_out, int two_ty, int reverse) {
     using St = Elem<STORE>;
     using T = typename St::T;
     const int nx = c.nx, ny = c.ny;
@@ -61,8 +92,34 @@ int run_t(const lbm_config& c, const uint8_t* flags, const float* rho, const flo
     float avg[3] = {0.f, 0.f, 0.f};
     double avg64[3] = {0, 0, 0};
     const bool adapt = c.collision == LBM_CM && c.adapter == LBM_OPTIMAL_ADAPTER;
+    bool local = true;
+    if (fl)
+        for (long long n = 0; n < N; n++)
+            local = local && fl[n] != LBM_ZG_OUTFLOW && fl[n] != LBM_REGULARIZED_BOUNCE_BACK_CORNER;
+    const bool two = two_ty != 0 && !adapt && !(c.quirks & LBM_QUIRK_STALE_REST) && local;
+    const int n_edge = edge_count(nx, ny, c.periodic_x, c.periodic_y);
+    std::vector<T> edge[2] = {std::vector<T>(size_t(9) * n_edge), std::vector<T>(size_t(9) * n_edge)};
+    int edge_cur = 0;
+    bool in_store = false;
     int cur = 0;
-    for (int s = 0; s < nsteps; s++) {
+    for (int s = 0; s < nsteps;) {
+        const bool pair = two && s + 2 <= nsteps;
+        if (pair && n_edge > 0 && !in_store) {  // state t-1 at the edge nodes -> store
+            for (int e = 0; e < n_edge; e++) {
+                int x, y;
+                edge_node(e, nx, ny, c.periodic_y, x, y);
+                for (int q = 0; q < Q; q++) edge[edge_cur][size_t(q) * n_edge + e] = buf[1 - cur][q * N + (long long)y * nx + x];
+            }
+        }
+        if (!pair && in_store) {  // back into the buffer one-step launches read
+            for (int e = 0; e < n_edge; e++) {
+                int x, y;
+                edge_node(e, nx, ny, c.periodic_y, x, y);
+                if (edge_slot(x, y, nx, ny, c.periodic_x, c.periodic_y) != e) continue;
+                for (int q = 0; q < Q; q++) buf[1 - cur][q * N + (long long)y * nx + x] = edge[edge_cur][size_t(q) * n_edge + e];
+            }
+            in_store = false;
+        }
         StepParams p{};
         p.nx = nx;
         p.ny = ny;
@@ -85,10 +142,24 @@ int run_t(const lbm_config& c, const uint8_t* flags, const float* rho, const flo
         p.u_lid = c.u_lid;
         p.quirks = c.quirks;
         p.avg = adapt ? avg : nullptr;
-        bool last = s == nsteps - 1;
+        bool last = s + (pair ? 2 : 1) == nsteps;
         p.out_rho = last ? out_r : nullptr;
         p.out_ux = last ? out_x : nullptr;
         p.out_uy = last ? out_y : nullptr;
+        if (pair) {
+            p.edge_prev = n_edge ? edge[edge_cur].data() : nullptr;
+            p.edge_next = n_edge ? edge[1 - edge_cur].data() : nullptr;
+            p.n_edge = n_edge;
+            if (two_ty == 8) launch2_emu<COLL, STORE, 8>(p, reverse);
+            else launch2_emu<COLL, STORE, 4>(p, reverse);
+            if (n_edge > 0) {
+                edge_cur = 1 - edge_cur;
+                in_store = true;
+            }
+            cur = 1 - cur;
+            s += 2;
+            continue;
+        }
         p.ntx = (nx + SEG - 1) / SEG;
         const long long nseg = (long long)p.ntx * ny;
         if (adapt) {
@@ -102,6 +173,7 @@ int run_t(const lbm_config& c, const uint8_t* flags, const float* rho, const flo
         MomAcc dummy{0, 0, 0};
         for (long long sg = 0; sg < nseg; sg++) step_segment<COLL, STORE, 0, SEG>(p, sg, dummy);
         cur = 1 - cur;
+        s += 1;
     }
     if (f_out)
         for (int q = 0; q < Q; q++)
@@ -130,32 +202,39 @@ int run_t(const lbm_config& c, const uint8_t* flags, const float* rho, const flo
 
 template <int STORE, int SEG>
 int run_w(const lbm_config& c, const uint8_t* flags, const float* rho, const float* u, const float* force, int n,
-          float* f, float* r, float* uu, double* a) {
-    if (c.collision == LBM_BGK) return run_t<LBM_BGK, STORE, SEG>(c, flags, rho, u, force, n, f, r, uu, a);
-    if (c.collision == LBM_MRT) return run_t<LBM_MRT, STORE, SEG>(c, flags, rho, u, force, n, f, r, uu, a);
-    return run_t<LBM_CM, STORE, SEG>(c, flags, rho, u, force, n, f, r, uu, a);
+          float* f, float* r, float* uu, double* a, int two_ty, int rev) {
+    if (c.collision == LBM_BGK) return run_t<LBM_BGK, STORE, SEG>(c, flags, rho, u, force, n, f, r, uu, a, two_ty, rev);
+    if (c.collision == LBM_MRT) return run_t<LBM_MRT, STORE, SEG>(c, flags, rho, u, force, n, f, r, uu, a, two_ty, rev);
+    return run_t<LBM_CM, STORE, SEG>(c, flags, rho, u, force, n, f, r, uu, a, two_ty, rev);
 }
 
 template <int STORE>
 int run_s(const lbm_config& c, int seg, const uint8_t* flags, const float* rho, const float* u, const float* force,
-          int n, float* f, float* r, float* uu, double* a) {
-    if (seg == kSegWide) return run_w<STORE, kSegWide>(c, flags, rho, u, force, n, f, r, uu, a);
-    if (seg == kSegNarrow<STORE>) return run_w<STORE, kSegNarrow<STORE>>(c, flags, rho, u, force, n, f, r, uu, a);
+          int n, float* f, float* r, float* uu, double* a, int two_ty, int rev) {
+    if (seg == kSegWide) return run_w<STORE, kSegWide>(c, flags, rho, u, force, n, f, r, uu, a, two_ty, rev);
+    if (seg == kSegNarrow<STORE>)
+        return run_w<STORE, kSegNarrow<STORE>>(c, flags, rho, u, force, n, f, r, uu, a, two_ty, rev);
     return -1;
 }
 
 }  // namespace
 
-// seg: nodes per thread, the wide (4) or the storage's narrow width (FP32 1, 16-bit 2);
-// 0 selects the wide one.
-extern "C" int emu_run(const lbm_config* c, int seg, const uint8_t* flags, const float* rho, const float* u,
-                       const float* force, int nsteps, float* f_out, float* rho_out, float* u_out, double* avg_out) {
+// seg: nodes per thread of one-step launches, the wide (4) or the storage's narrow width
+// (FP32 1, 16-bit 2); 0 selects the wide one. two_ty: 0 one step per launch; 4 or 8 two-step
+// launches with that tile height where the configuration allows them (as the library does).
+// reverse: run two-step tiles in reverse order.
+extern "C" int emu_run(const lbm_config* c, int seg, int two_ty, int reverse, const uint8_t* flags, const float* rho,
+                       const float* u, const float* force, int nsteps, float* f_out, float* rho_out, float* u_out,
+                       double* avg_out) {
     if (seg == 0) seg = kSegWide;
+    if (two_ty != 0 && two_ty != 4 && two_ty != 8) return -1;
     if (c->storage == LBM_F16_SHIFTED)
-        return run_s<LBM_F16_SHIFTED>(*c, seg, flags, rho, u, force, nsteps, f_out, rho_out, u_out, avg_out);
+        return run_s<LBM_F16_SHIFTED>(*c, seg, flags, rho, u, force, nsteps, f_out, rho_out, u_out, avg_out, two_ty,
+                                      reverse);
     if (c->storage == LBM_BF16_SHIFTED)
-        return run_s<LBM_BF16_SHIFTED>(*c, seg, flags, rho, u, force, nsteps, f_out, rho_out, u_out, avg_out);
-    return run_s<LBM_F32>(*c, seg, flags, rho, u, force, nsteps, f_out, rho_out, u_out, avg_out);
+        return run_s<LBM_BF16_SHIFTED>(*c, seg, flags, rho, u, force, nsteps, f_out, rho_out, u_out, avg_out, two_ty,
+                                       reverse);
+    return run_s<LBM_F32>(*c, seg, flags, rho, u, force, nsteps, f_out, rho_out, u_out, avg_out, two_ty, reverse);
 }
 
 // The product's node adjoints (lbm_adjoint.hpp) on the host: mode 0 BGK, 1 MRT, 2 snapshot
@@ -178,7 +257,7 @@ extern "C" int emu_adjoint_node(int mode, const lbm_config* c, const float* g, c
 #ifdef EMU_MAIN
 // Bounds matrix: every collision x storage x boundary layout on small and ragged grids.
 int main() {
-    const int sizes[][2] = {{3, 3}, {5, 4}, {8, 8}, {13, 7}, {16, 9}, {33, 17}, {64, 5}};
+    const int sizes[][2] = {{3, 3}, {5, 4}, {8, 8}, {13, 7}, {16, 9}, {33, 17}, {64, 5}, {70, 11}, {130, 9}};
     int cases = 0;
     for (auto& sz : sizes) {
         for (int layout = 0; layout < 5; layout++) {
@@ -239,9 +318,17 @@ int main() {
                         std::vector<double> a(3);
                         const int segs[2] = {kSegWide, storage == LBM_F32 ? 1 : 2};
                         for (int seg : segs) {
-                            if (emu_run(&c, seg, fl.data(), rho.data(), u.data(), (cases & 1) ? F.data() : nullptr, 3,
-                                        f.data(), r.data(), uu.data(), a.data()) != 0) {
+                            if (emu_run(&c, seg, 0, 0, fl.data(), rho.data(), u.data(), (cases & 1) ? F.data() : nullptr,
+                                        3, f.data(), r.data(), uu.data(), a.data()) != 0) {
                                 std::printf("emu_run rejected seg %d\n", seg);
+                                return 1;
+                            }
+                            cases++;
+                        }
+                        for (int ty : {4, 8}) {  // two-step launches (5 steps: two pairs and a single)
+                            if (emu_run(&c, 0, ty, cases & 1, fl.data(), rho.data(), u.data(),
+                                        (cases & 2) ? F.data() : nullptr, 5, f.data(), r.data(), uu.data(), a.data()) != 0) {
+                                std::printf("emu_run rejected two-step ty %d\n", ty);
                                 return 1;
                             }
                             cases++;

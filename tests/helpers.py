@@ -22,8 +22,8 @@ def emu_lib():
     if _emu is None:
         L = ctypes.CDLL(str(EMU))
         L.emu_run.restype = ctypes.c_int
-        L.emu_run.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 4 + [ctypes.c_int] + \
-            [ctypes.c_void_p] * 4
+        L.emu_run.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4 + \
+            [ctypes.c_int] + [ctypes.c_void_p] * 4
         _emu = L
     return _emu
 
@@ -32,9 +32,11 @@ def _p(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
 
 
-def run_emu(case: sc.Case, nsteps: int, seg: int = 0):
-    """Host emulation of the fused kernel (same body, same grid). seg: nodes per thread (4, or
-    the storage's narrow width: FP32 1, 16-bit 2; 0 = 4). Returns (f, rho, u, avg)."""
+def run_emu(case: sc.Case, nsteps: int, seg: int = 0, two_ty: int = 0, reverse: bool = False):
+    """Host emulation of the fused kernels (same bodies, same grids). seg: nodes per thread of
+    one-step launches (4, or the storage's narrow width: FP32 1, 16-bit 2; 0 = 4). two_ty: 0
+    one step per launch, 4 or 8 two-step launches with that tile height where the library
+    would use them; reverse runs their tiles in reverse order. Returns (f, rho, u, avg)."""
     c = case.config.to_c()
     N = case.config.N
     fl = None if case.flags is None else np.ascontiguousarray(case.flags, np.uint8).reshape(-1)
@@ -45,8 +47,8 @@ def run_emu(case: sc.Case, nsteps: int, seg: int = 0):
     r = np.zeros(N, np.float32)
     uu = np.zeros(2 * N, np.float32)
     avg = np.zeros(3, np.float64)
-    rc = emu_lib().emu_run(ctypes.byref(c), seg, _p(fl), _p(rho), _p(u), _p(F), nsteps, _p(f), _p(r), _p(uu),
-                           _p(avg))
+    rc = emu_lib().emu_run(ctypes.byref(c), seg, two_ty, int(reverse), _p(fl), _p(rho), _p(u), _p(F), nsteps, _p(f),
+                           _p(r), _p(uu), _p(avg))
     assert rc == 0
     ny, nx = case.config.ny, case.config.nx
     return f.reshape(9, ny, nx), r.reshape(ny, nx), uu.reshape(ny, nx, 2), avg

@@ -43,7 +43,8 @@ def test_python_prototypes_cover_the_header():
 def test_config_struct_layout_matches_header():
     from lbm_amd import _abi
 
-    # 5 int32 + omega + S[9] + rho0 + storage + 2 periodic + force[2] + wall_u[2] + u_lid + quirks + reserved[8]
+    # 5 int32 + omega + S[9] + rho0 + storage + 2 periodic + force[2] + wall_u[2] + u_lid + quirks
+    # + steps_per_launch + reserved[7]
     assert ctypes.sizeof(_abi.LbmConfig) == 4 * (5 + 1 + 9 + 1 + 1 + 2 + 2 + 2 + 1 + 1 + 8)
 
 

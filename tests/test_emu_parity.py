@@ -79,3 +79,33 @@ def test_asan_bounds_matrix():
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
     assert "cases ok" in r.stdout
+
+
+@pytest.mark.parametrize("ty", [4, 8])
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("op", OPS, ids=["bgk", "mrt", "cm", "cm_optimal"])
+def test_emulated_two_step_launches(layout, op, ty):
+    # Two steps per launch (phase A into an LDS tile, phase B from it) must give exactly the
+    # populations of one step per launch, whatever order the tiles run in; layouts or
+    # operators it does not cover (ZG outflow, regularized corner, the adapter) fall back to
+    # one-step launches inside the same run. 70x37 spans two tile columns, partial tiles and
+    # 7 steps end with a single step after three pairs.
+    coll, adapter = op
+    case = layout_case(layout, 70, 37, coll, adapter=adapter)
+    f1, r1, u1, _ = run_emu(case, 7, seg=1)
+    f2, r2, u2, _ = run_emu(case, 7, two_ty=ty)
+    f3, *_ = run_emu(case, 7, two_ty=ty, reverse=True)
+    assert np.array_equal(f1, f2) and np.array_equal(r1, r2) and np.array_equal(u1, u2)
+    assert np.array_equal(f2, f3)
+    fo, *_ = run_oracle(case, 7, 32)
+    assert max_rel(f2, fo) <= 1e-5
+
+
+@pytest.mark.parametrize("storage", [A.LBM_F16_SHIFTED, A.LBM_BF16_SHIFTED])
+@pytest.mark.parametrize("layout", ["periodic", "cavity_mw", "open"])
+def test_emulated_two_step_reduced_storage(storage, layout):
+    case = layout_case(layout, 70, 37, A.LBM_MRT, storage=storage)
+    f1, *_ = run_emu(case, 6, seg=2)
+    f2, *_ = run_emu(case, 6, two_ty=4)
+    f3, *_ = run_emu(case, 6, two_ty=8, reverse=True)
+    assert np.array_equal(f1, f2) and np.array_equal(f1, f3)
