@@ -498,6 +498,12 @@ int lbm_get_timestep(const lbm_ctx* ctx, int64_t* t) {
     return LBM_OK;
 }
 
+int lbm_get_steps_per_launch(const lbm_ctx* ctx, int32_t* steps) {
+    if (!ctx || !steps) return LBM_E_INVALID;
+    *steps = ctx->two_step_ok() ? 2 : 1;
+    return LBM_OK;
+}
+
 int lbm_get_macros(lbm_ctx* ctx, float* rho, float* u, int on_device) {
     if (!ctx) return LBM_E_INVALID;
     if (!ctx->macros_valid) return ctx->fail(LBM_E_STATE, "no macroscopic fields stored yet");

@@ -65,15 +65,15 @@ __global__ __launch_bounds__(kMaxStepBlock) void fused_step_kernel(StepParams p,
 
 // Two time steps per launch; see step2_phase_a / step2_phase_b.
 template <int COLL, int STORE, int TY>
-__global__ __launch_bounds__(kT2X * TY) void fused_step2_kernel(StepParams p, long long nb_padded, int remap,
-                                                                 int tiles_x, long long n_tiles) {
+__global__ __launch_bounds__(Tile2<TY>::THREADS) void fused_step2_kernel(StepParams p, long long nb_padded, int remap,
+                                                                         int tiles_x, long long n_tiles) {
     using T = typename Elem<STORE>::T;
     __shared__ T tile[Q * Tile2<TY>::NH];
     const long long lb = logical_block(nb_padded, remap);
     if (lb >= n_tiles) return;  // whole workgroup: remap padding
     const int by = int(lb / tiles_x), bx = int(lb - (long long)by * tiles_x);
-    const int x0 = bx * kT2X, y0 = by * TY;
-    for (int h = threadIdx.x; h < Tile2<TY>::NH; h += kT2X * TY) step2_phase_a<COLL, STORE, TY>(p, tile, h, x0, y0);
+    const int x0 = bx * kT2W, y0 = by * TY;
+    step2_phase_a<COLL, STORE, TY>(p, tile, threadIdx.x, x0, y0);
     __syncthreads();
     step2_phase_b<COLL, STORE, TY>(p, tile, threadIdx.x, x0, y0);
 }
@@ -309,17 +309,17 @@ hipError_t launch_step(int coll, int storage, int mode, const StepParams& p, con
 }
 
 long long step2_grid_blocks(const StepParams& p, const Step2Policy& lp) {
-    const long long tiles = (long long)((p.nx + kT2X - 1) / kT2X) * ((p.ny + lp.ty - 1) / lp.ty);
+    const long long tiles = (long long)((p.nx + kT2W - 1) / kT2W) * ((p.ny + lp.ty - 1) / lp.ty);
     return lp.remap ? (tiles + kXcds - 1) / kXcds * kXcds : tiles;
 }
 
 template <int COLL, int STORE, int TY>
 static hipError_t launch_step2_t(const StepParams& p, const Step2Policy& lp, hipStream_t s) {
-    const int tiles_x = (p.nx + kT2X - 1) / kT2X;
+    const int tiles_x = (p.nx + kT2W - 1) / kT2W;
     const long long n_tiles = (long long)tiles_x * ((p.ny + TY - 1) / TY);
     const long long nb = step2_grid_blocks(p, lp);
-    hipLaunchKernelGGL((fused_step2_kernel<COLL, STORE, TY>), dim3(unsigned(nb)), dim3(kT2X * TY), 0, s, p, nb, lp.remap,
-                       tiles_x, n_tiles);
+    hipLaunchKernelGGL((fused_step2_kernel<COLL, STORE, TY>), dim3(unsigned(nb)), dim3(Tile2<TY>::THREADS), 0, s, p, nb,
+                       lp.remap, tiles_x, n_tiles);
     return hipGetLastError();
 }
 
@@ -332,8 +332,8 @@ static hipError_t launch_step2_c(int coll, const StepParams& p, const Step2Polic
 
 template <int STORE>
 static hipError_t launch_step2_s(int coll, const StepParams& p, const Step2Policy& lp, hipStream_t s) {
-    if (lp.ty == 4) return launch_step2_c<STORE, 4>(coll, p, lp, s);
-    if (lp.ty == 8) return launch_step2_c<STORE, 8>(coll, p, lp, s);
+    if (lp.ty == 6) return launch_step2_c<STORE, 6>(coll, p, lp, s);
+    if (lp.ty == 14) return launch_step2_c<STORE, 14>(coll, p, lp, s);
     return hipErrorInvalidValue;
 }
 
@@ -349,7 +349,7 @@ Step2Policy step2_policy(long long plane, int storage) {
     (void)plane;
     (void)storage;
     Step2Policy lp;
-    lp.ty = 4;
+    lp.ty = 6;
     lp.remap = 1;
     return lp;
 }

@@ -555,35 +555,37 @@ LBM_HD void step_segment(const StepParams& p, long long seg, MomAcc& acc) {
 
 // ---- two steps per launch (temporal blocking) --------------------------------------------
 //
-// A workgroup owns a tile of kT2X x TY nodes. Phase A computes the post-collision state t+1
-// of the tile plus a one-node halo from the state t in HBM and keeps it in LDS in the
-// storage format (so every value is what a one-step launch would have stored); phase B
-// computes state t+2 of the tile from LDS and writes it to HBM. Per step, the populations
-// cross HBM once every two steps instead of every step; the halo costs (kT2X+2)(TY+2) /
-// (kT2X TY) phase-A node updates per node.
+// A workgroup owns a tile of kT2W x TY nodes and runs TY + 2 wavefronts: wavefront w, lane
+// l holds node (x0 - 1 + l, y0 - 1 + w), so the tile plus a one-node halo is exactly
+// 64 x (TY + 2) threads. Phase A computes every thread's post-collision state t+1 from the
+// state t in HBM and keeps it in LDS in the storage format (every value is what a one-step
+// launch would have stored); phase B computes state t+2 of the tile's inner kT2W x TY nodes
+// from LDS and writes it to HBM. The populations cross HBM once per two steps; the halo
+// costs 64 (TY + 2) / (kT2W TY) phase-A node updates per node.
 //
 // Unstreamed slots: a one-step launch reads them from the buffer it writes (state t-1 at
 // step t+1, state t at step t+2). Here phase B reads state t from the source buffer, and
 // phase A reads state t-1 from a small edge store, because the destination buffer is being
-// overwritten by other workgroups during the launch; phase A also saves state t+1 of its own
-// edge nodes into the next edge store. Boundary functors that read a neighbour's
+// overwritten by other workgroups during the launch; phase A also saves state t+1 of the
+// tile's own edge nodes into the next edge store. Boundary functors that read a neighbour's
 // populations (ZG outflow, regularized corner) and the stale-rest quirk need the one-step
 // launch.
-constexpr int kT2X = 64;  // tile width: one wavefront per tile row
+constexpr int kT2L = 64;          // lanes of a tile row (one wavefront)
+constexpr int kT2W = kT2L - 2;    // output columns per tile
 
 template <int TY>
 struct Tile2 {
-    static constexpr int HX = kT2X + 2, HY = TY + 2, NH = HX * HY;
+    static constexpr int ROWS = TY + 2, NH = kT2L * ROWS, THREADS = NH;
 };
 
 template <int COLL, int STORE, int TY>
-LBM_HD void step2_phase_a(const StepParams& p, typename Elem<STORE>::T* tile, int h, int x0, int y0) {
+LBM_HD void step2_phase_a(const StepParams& p, typename Elem<STORE>::T* tile, int t, int x0, int y0) {
     using St = Elem<STORE>;
     using T = typename St::T;
-    constexpr int HX = Tile2<TY>::HX, NH = Tile2<TY>::NH;
-    const int hx = h % HX, hy = h / HX;
-    int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
-    const bool own = hx >= 1 && hx <= kT2X && hy >= 1 && hy <= TY && gx < p.nx && gy < p.ny;
+    constexpr int NH = Tile2<TY>::NH;
+    const int l = t % kT2L, w = t / kT2L;
+    int gx = x0 - 1 + l, gy = y0 - 1 + w;
+    const bool own = l >= 1 && l <= kT2W && w >= 1 && w <= TY && gx < p.nx && gy < p.ny;
     if (gx < 0 || gx >= p.nx) {
         if (!p.px) return;
         gx = ((gx % p.nx) + p.nx) % p.nx;
@@ -596,9 +598,8 @@ LBM_HD void step2_phase_a(const StepParams& p, typename Elem<STORE>::T* tile, in
     const uint8_t fl = p.flags ? p.flags[node] : uint8_t(LBM_FLUID);
     const bool ex = !p.px && (gx == 0 || gx == p.nx - 1), ey = !p.py && (gy == 0 || gy == p.ny - 1);
     // Pull sources with the periodic wrap; a source outside a non-periodic domain reads the
-    // node's unstreamed slot, state t-1, from the edge store. Inline (no call, no scratch):
-    // every wave of a boundary tile has some lanes on the domain edge.
-    // Plane bases are wave-uniform (scalar registers); per-lane offsets are 32-bit (N < 2^31).
+    // node's unstreamed slot, state t-1, from the edge store. Plane bases are wave-uniform;
+    // per-lane offsets are 32-bit (N < 2^31).
     const T* src = static_cast<const T*>(p.src);
     const int xm = gx == 0 ? (p.px ? p.nx - 1 : -1) : gx - 1, xp = gx == p.nx - 1 ? (p.px ? 0 : -1) : gx + 1;
     const int ym = gy == 0 ? (p.py ? p.ny - 1 : -1) : gy - 1, yp = gy == p.ny - 1 ? (p.py ? 0 : -1) : gy + 1;
@@ -630,7 +631,7 @@ LBM_HD void step2_phase_a(const StepParams& p, typename Elem<STORE>::T* tile, in
     MomAcc acc{0.0, 0.0, 0.0};
     node_update<COLL, 0>(p, g, fx, fy, m, acc);
 #pragma unroll
-    for (int q = 0; q < Q; q++) St::store1(tile + q * NH + h, q, p.rho0, g[q]);
+    for (int q = 0; q < Q; q++) St::store1(tile + q * NH + t, q, p.rho0, g[q]);
     if (own && (ex || ey)) {
         T* en = static_cast<T*>(p.edge_next);
         const int base = p.py ? 0 : 2 * p.nx;
@@ -647,12 +648,13 @@ LBM_HD void step2_phase_a(const StepParams& p, typename Elem<STORE>::T* tile, in
 }
 
 template <int COLL, int STORE, int TY>
-LBM_HD void step2_phase_b(const StepParams& p, const typename Elem<STORE>::T* tile, int tid, int x0, int y0) {
+LBM_HD void step2_phase_b(const StepParams& p, const typename Elem<STORE>::T* tile, int t, int x0, int y0) {
     using St = Elem<STORE>;
     using T = typename St::T;
-    constexpr int HX = Tile2<TY>::HX, NH = Tile2<TY>::NH;
-    const int tx = tid % kT2X, ty = tid / kT2X;
-    const int x = x0 + tx, y = y0 + ty;
+    constexpr int NH = Tile2<TY>::NH;
+    const int l = t % kT2L, w = t / kT2L;
+    if (l < 1 || l > kT2W || w < 1 || w > TY) return;
+    const int x = x0 - 1 + l, y = y0 - 1 + w;
     if (x >= p.nx || y >= p.ny) return;
     const uint32_t node = uint32_t(y) * uint32_t(p.nx) + uint32_t(x);
     const T* src = static_cast<const T*>(p.src);
@@ -660,14 +662,14 @@ LBM_HD void step2_phase_b(const StepParams& p, const typename Elem<STORE>::T* ti
     float g[Q];
     if (!ex && !ey) {
 #pragma unroll
-        for (int q = 0; q < Q; q++) g[q] = St::load1(tile + q * NH + (ty + 1 - CY[q]) * HX + (tx + 1 - CX[q]), q, p.rho0);
+        for (int q = 0; q < Q; q++) g[q] = St::load1(tile + q * NH + t - CY[q] * kT2L - CX[q], q, p.rho0);
     } else {
 #pragma unroll
         for (int q = 0; q < Q; q++) {
             const int xs = x - CX[q], ys = y - CY[q];
             const bool out = (!p.px && (xs < 0 || xs >= p.nx)) || (!p.py && (ys < 0 || ys >= p.ny));
             g[q] = out ? St::load1(src + q * p.plane + node, q, p.rho0)
-                       : St::load1(tile + q * NH + (ty + 1 - CY[q]) * HX + (tx + 1 - CX[q]), q, p.rho0);
+                       : St::load1(tile + q * NH + t - CY[q] * kT2L - CX[q], q, p.rho0);
         }
     }
     const uint8_t fl = p.flags ? p.flags[node] : uint8_t(LBM_FLUID);

@@ -103,6 +103,7 @@ PROTOTYPES = [
     ("lbm_step", _I, [_P, _I]),
     ("lbm_sync", _I, [_P]),
     ("lbm_get_timestep", _I, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    ("lbm_get_steps_per_launch", _I, [_P, ctypes.POINTER(ctypes.c_int32)]),
     ("lbm_get_macros", _I, [_P, _P, _P, _I]),
     ("lbm_get_pops", _I, [_P, _P, _I]),
     ("lbm_set_pops", _I, [_P, _P, _I]),

@@ -155,6 +155,13 @@ class Solver:
         self._check(self.lib.lbm_get_timestep(self.h, ctypes.byref(t)))
         return t.value
 
+    @property
+    def steps_per_launch(self) -> int:
+        """Time steps per kernel launch lbm_step uses now (2 or 1)."""
+        n = ctypes.c_int32()
+        self._check(self.lib.lbm_get_steps_per_launch(self.h, ctypes.byref(n)))
+        return n.value
+
     def get_macros(self):
         """(rho [ny, nx], u [ny, nx, 2]) as float32 numpy arrays."""
         rho = np.empty(self.N, np.float32)

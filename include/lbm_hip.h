@@ -142,11 +142,17 @@ int lbm_set_force_field(lbm_ctx* ctx, const float* force, int on_device);
 
 /* One call = `nsteps` iterations of the reference's loop body (main.cu:96-114):
  * stream, boundaries, macroscopics, forces, IBM, Guo correction, equilibrium, collision,
- * fused into one pull-scheme kernel per step (plus the IBM / adapter passes when active).
- * Asynchronous on the context's stream. The last step also stores rho and u. */
+ * fused into one pull-scheme kernel per step (plus the IBM / adapter passes when active),
+ * or into one kernel per pair of steps where steps_per_launch allows it (an odd remainder
+ * runs as a single step). Asynchronous on the context's stream. The last step also stores
+ * rho and u. */
 int lbm_step(lbm_ctx* ctx, int32_t nsteps);
 int lbm_sync(lbm_ctx* ctx);
 int lbm_get_timestep(const lbm_ctx* ctx, int64_t* t);
+/* Time steps per kernel launch that lbm_step uses for the current configuration, flags and
+ * bodies: 2 (two-step launches) or 1. No counterpart in the reference, which launches eight
+ * kernels per step; for profiling and bandwidth accounting. */
+int lbm_get_steps_per_launch(const lbm_ctx* ctx, int32_t* steps);
 
 /* Replaces update_macroscopics (lbm.cuh:148-154): density [N] and Guo-corrected
  * velocity AoS [N][2] of the last step's post-stream state. Either pointer may be NULL. */

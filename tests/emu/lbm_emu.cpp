@@ -38,15 +38,15 @@ T poison() {
 template <int COLL, int STORE, int TY>
 void launch2_emu(const StepParams& p, int reverse) {
     using T = typename Elem<STORE>::T;
-    const int tiles_x = (p.nx + kT2X - 1) / kT2X, tiles_y = (p.ny + TY - 1) / TY;
+    const int tiles_x = (p.nx + kT2W - 1) / kT2W, tiles_y = (p.ny + TY - 1) / TY;
     const long long nt = (long long)tiles_x * tiles_y;
     std::vector<T> tile(size_t(Q) * Tile2<TY>::NH);
     for (long long k = 0; k < nt; k++) {
         const long long lb = reverse ? nt - 1 - k : k;
         const int by = int(lb / tiles_x), bx = int(lb - (long long)by * tiles_x);
         std::fill(tile.begin(), tile.end(), poison<T>());
-        for (int h = 0; h < Tile2<TY>::NH; h++) step2_phase_a<COLL, STORE, TY>(p, tile.data(), h, bx * kT2X, by * TY);
-        for (int t = 0; t < kT2X * TY; t++) step2_phase_b<COLL, STORE, TY>(p, tile.data(), t, bx * kT2X, by * TY);
+        for (int t = 0; t < Tile2<TY>::THREADS; t++) step2_phase_a<COLL, STORE, TY>(p, tile.data(), t, bx * kT2W, by * TY);
+        for (int t = 0; t < Tile2<TY>::THREADS; t++) step2_phase_b<COLL, STORE, TY>(p, tile.data(), t, bx * kT2W, by * TY);
     }
 }
 
@@ -150,8 +150,8 @@ int run_t(const lbm_config& c, const uint8_t* flags, const float* rho, const flo
             p.edge_prev = n_edge ? edge[edge_cur].data() : nullptr;
             p.edge_next = n_edge ? edge[1 - edge_cur].data() : nullptr;
             p.n_edge = n_edge;
-            if (two_ty == 8) launch2_emu<COLL, STORE, 8>(p, reverse);
-            else launch2_emu<COLL, STORE, 4>(p, reverse);
+            if (two_ty == 14) launch2_emu<COLL, STORE, 14>(p, reverse);
+            else launch2_emu<COLL, STORE, 6>(p, reverse);
             if (n_edge > 0) {
                 edge_cur = 1 - edge_cur;
                 in_store = true;
@@ -220,14 +220,14 @@ int run_s(const lbm_config& c, int seg, const uint8_t* flags, const float* rho, 
 }  // namespace
 
 // seg: nodes per thread of one-step launches, the wide (4) or the storage's narrow width
-// (FP32 1, 16-bit 2); 0 selects the wide one. two_ty: 0 one step per launch; 4 or 8 two-step
+// (FP32 1, 16-bit 2); 0 selects the wide one. two_ty: 0 one step per launch; 6 or 14: two-step
 // launches with that tile height where the configuration allows them (as the library does).
 // reverse: run two-step tiles in reverse order.
 extern "C" int emu_run(const lbm_config* c, int seg, int two_ty, int reverse, const uint8_t* flags, const float* rho,
                        const float* u, const float* force, int nsteps, float* f_out, float* rho_out, float* u_out,
                        double* avg_out) {
     if (seg == 0) seg = kSegWide;
-    if (two_ty != 0 && two_ty != 4 && two_ty != 8) return -1;
+    if (two_ty != 0 && two_ty != 6 && two_ty != 14) return -1;
     if (c->storage == LBM_F16_SHIFTED)
         return run_s<LBM_F16_SHIFTED>(*c, seg, flags, rho, u, force, nsteps, f_out, rho_out, u_out, avg_out, two_ty,
                                       reverse);
@@ -325,7 +325,7 @@ int main() {
                             }
                             cases++;
                         }
-                        for (int ty : {4, 8}) {  // two-step launches (5 steps: two pairs and a single)
+                        for (int ty : {6, 14}) {  // two-step launches (5 steps: two pairs and a single)
                             if (emu_run(&c, 0, ty, cases & 1, fl.data(), rho.data(), u.data(),
                                         (cases & 2) ? F.data() : nullptr, 5, f.data(), r.data(), uu.data(), a.data()) != 0) {
                                 std::printf("emu_run rejected two-step ty %d\n", ty);

@@ -35,7 +35,7 @@ def _p(a):
 def run_emu(case: sc.Case, nsteps: int, seg: int = 0, two_ty: int = 0, reverse: bool = False):
     """Host emulation of the fused kernels (same bodies, same grids). seg: nodes per thread of
     one-step launches (4, or the storage's narrow width: FP32 1, 16-bit 2; 0 = 4). two_ty: 0
-    one step per launch, 4 or 8 two-step launches with that tile height where the library
+    one step per launch, 6 or 14 two-step launches with that tile height where the library
     would use them; reverse runs their tiles in reverse order. Returns (f, rho, u, avg)."""
     c = case.config.to_c()
     N = case.config.N

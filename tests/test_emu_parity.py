@@ -81,7 +81,7 @@ def test_asan_bounds_matrix():
     assert "cases ok" in r.stdout
 
 
-@pytest.mark.parametrize("ty", [4, 8])
+@pytest.mark.parametrize("ty", [6, 14])
 @pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("op", OPS, ids=["bgk", "mrt", "cm", "cm_optimal"])
 def test_emulated_two_step_launches(layout, op, ty):
@@ -106,6 +106,6 @@ def test_emulated_two_step_launches(layout, op, ty):
 def test_emulated_two_step_reduced_storage(storage, layout):
     case = layout_case(layout, 70, 37, A.LBM_MRT, storage=storage)
     f1, *_ = run_emu(case, 6, seg=2)
-    f2, *_ = run_emu(case, 6, two_ty=4)
-    f3, *_ = run_emu(case, 6, two_ty=8, reverse=True)
+    f2, *_ = run_emu(case, 6, two_ty=6)
+    f3, *_ = run_emu(case, 6, two_ty=14, reverse=True)
     assert np.array_equal(f1, f2) and np.array_equal(f1, f3)
