@@ -38,13 +38,11 @@ struct Macro {
     float rho, ux, uy;
 };
 LBM_HD Macro macroscopics(const float* g, float fx, float fy) {
-    float r = 0.f, jx = 0.f, jy = 0.f;
-#pragma unroll
-    for (int i = 0; i < Q; i++) {
-        r += g[i];
-        jx += g[i] * float(CX[i]);
-        jy += g[i] * float(CY[i]);
-    }
+    // The reference's sums over i = 0..8 of g_i, g_i c_ix, g_i c_iy, in that order; the
+    // terms with c = 0 add an exact zero and are left out, those with c = -1 subtract.
+    const float r = (((((((g[0] + g[1]) + g[2]) + g[3]) + g[4]) + g[5]) + g[6]) + g[7]) + g[8];
+    const float jx = ((((g[1] - g[3]) + g[5]) - g[6]) - g[7]) + g[8];
+    const float jy = ((((g[2] - g[4]) + g[5]) + g[6]) - g[7]) - g[8];
     // one reciprocal per node; the reference divides three times (macroscopics.cu:29-30,
     // 108-109)
     float inv = recip(r);

@@ -436,17 +436,34 @@ __host__ __device__ __noinline__ Pop9 gather_node(NodeArgs p, int x, int y) {
 }
 
 // Node force: uniform, else the force field, else (inside the IBM box) the IBM total.
+// Written as loads into zero-initialised values followed by value selects: "x = uniform
+// field; if (c) x = load" is otherwise folded into one load through a selected address,
+// which pins the kernel argument block in scratch memory and turns the load into a flat load.
+struct ForceArgs {
+    float fx, fy;
+    const float* field;
+    const float* ibm;
+    int bx0, bx1, by0, by1;
+};
+LBM_HD ForceArgs force_args(const StepParams& p) {
+    return ForceArgs{p.fx, p.fy, p.force_field, p.ibm_force, p.ibm_x0, p.ibm_x1, p.ibm_y0, p.ibm_y1};
+}
+LBM_HD void node_force(ForceArgs a, int x, int y, long long node, float& fx, float& fy) {
+    const bool use_field = a.field != nullptr;
+    const bool use_ibm = a.ibm != nullptr && x >= a.bx0 && x < a.bx1 && y >= a.by0 && y < a.by1;
+    float vx = 0.0f, vy = 0.0f;
+    if (use_ibm) {
+        vx = a.ibm[2 * node];
+        vy = a.ibm[2 * node + 1];
+    } else if (use_field) {
+        vx = a.field[2 * node];
+        vy = a.field[2 * node + 1];
+    }
+    fx = (use_ibm || use_field) ? vx : a.fx;
+    fy = (use_ibm || use_field) ? vy : a.fy;
+}
 LBM_HD void node_force(const StepParams& p, int x, int y, long long node, float& fx, float& fy) {
-    fx = p.fx;
-    fy = p.fy;
-    if (p.force_field) {
-        fx = p.force_field[2 * node];
-        fy = p.force_field[2 * node + 1];
-    }
-    if (p.ibm_force && x >= p.ibm_x0 && x < p.ibm_x1 && y >= p.ibm_y0 && y < p.ibm_y1) {
-        fx = p.ibm_force[2 * node];
-        fy = p.ibm_force[2 * node + 1];
-    }
+    node_force(force_args(p), x, y, node, fx, fy);
 }
 
 // Collide one node in place. MODE: 0 step, 1 moments only, 2 step + moments.
