@@ -78,6 +78,45 @@ __global__ __launch_bounds__(Tile2<TY>::THREADS) void fused_step2_kernel(StepPar
     step2_phase_b<COLL, STORE, TY>(p, tile, threadIdx.x, x0, y0);
 }
 
+// Two time steps per launch, marching strips; see step2s_phase_a / step2s_phase_b.
+template <int COLL, int STORE, int W>
+__global__ __launch_bounds__(kT2L * W) void fused_step2s_kernel(StepParams p, long long nb_padded, int remap,
+                                                                 int strips_x, long long n_strips, int H) {
+    using T = typename Elem<STORE>::T;
+    constexpr int R = W + 2;  // ring rows
+    __shared__ T ring[Q * R * kT2L];
+    const long long lb = logical_block(nb_padded, remap);
+    if (lb >= n_strips) return;  // whole workgroup: remap padding
+    const int sy = int(lb / strips_x), sx = int(lb - (long long)sy * strips_x);
+    const int x0 = sx * kT2W, y0 = sy * H;
+    const int hs = p.ny - y0 < H ? p.ny - y0 : H;  // output rows of this strip
+    const int l = threadIdx.x & (kT2L - 1), w = threadIdx.x / kT2L;
+    const int K = (hs + 1) / W + 1;  // chunks: phase-A rows 0 .. hs + 1, phase-B rows 0 .. hs - 1
+    float g[Q];
+    int i = w, gx = 0, gy = 0;
+    bool va = i <= hs + 1 && strip_node(p, x0, y0, l, i, gx, gy);
+    if (va) step2_gather<STORE>(p, gx, gy, g);
+    for (int k = 0; k < K; k++) {
+        if (va) step2s_phase_a<COLL, STORE>(p, ring, R, i % R, l, i, x0, y0, hs, gx, gy, g);
+        __syncthreads();
+        // the next chunk's pull sources, loaded before this chunk's phase B
+        const int in = (k + 1) * W + w;
+        int nxg = 0, nyg = 0;
+        const bool vn = k + 1 < K && in <= hs + 1 && strip_node(p, x0, y0, l, in, nxg, nyg);
+        float gn[Q];
+        if (vn) step2_gather<STORE>(p, nxg, nyg, gn);
+        const int j = k * W - 2 + w;
+        if (j >= 0 && j < hs) step2s_phase_b<COLL, STORE>(p, ring, R, l, j, x0, y0);
+        __syncthreads();
+        i = in;
+        va = vn;
+        gx = nxg;
+        gy = nyg;
+#pragma unroll
+        for (int q = 0; q < Q; q++) g[q] = gn[q];
+    }
+}
+
 // Copy the populations of the non-periodic edge nodes between a population buffer and an
 // edge store ([9][n_edge], edge_slot order). to_store = 1: buffer -> store (every slot);
 // 0: store -> buffer (each node from its first slot only).
@@ -309,8 +348,26 @@ hipError_t launch_step(int coll, int storage, int mode, const StepParams& p, con
 }
 
 long long step2_grid_blocks(const StepParams& p, const Step2Policy& lp) {
-    const long long tiles = (long long)((p.nx + kT2W - 1) / kT2W) * ((p.ny + lp.ty - 1) / lp.ty);
+    const int rows = lp.strip ? lp.h : lp.ty;
+    const long long tiles = (long long)((p.nx + kT2W - 1) / kT2W) * ((p.ny + rows - 1) / rows);
     return lp.remap ? (tiles + kXcds - 1) / kXcds * kXcds : tiles;
+}
+
+template <int COLL, int STORE, int W>
+static hipError_t launch_step2s_t(const StepParams& p, const Step2Policy& lp, hipStream_t s) {
+    const int strips_x = (p.nx + kT2W - 1) / kT2W;
+    const long long n_strips = (long long)strips_x * ((p.ny + lp.h - 1) / lp.h);
+    const long long nb = step2_grid_blocks(p, lp);
+    hipLaunchKernelGGL((fused_step2s_kernel<COLL, STORE, W>), dim3(unsigned(nb)), dim3(kT2L * W), 0, s, p, nb, lp.remap,
+                       strips_x, n_strips, lp.h);
+    return hipGetLastError();
+}
+
+template <int STORE, int W>
+static hipError_t launch_step2s_c(int coll, const StepParams& p, const Step2Policy& lp, hipStream_t s) {
+    if (coll == LBM_BGK) return launch_step2s_t<LBM_BGK, STORE, W>(p, lp, s);
+    if (coll == LBM_MRT) return launch_step2s_t<LBM_MRT, STORE, W>(p, lp, s);
+    return launch_step2s_t<LBM_CM, STORE, W>(p, lp, s);
 }
 
 template <int COLL, int STORE, int TY>
@@ -332,6 +389,12 @@ static hipError_t launch_step2_c(int coll, const StepParams& p, const Step2Polic
 
 template <int STORE>
 static hipError_t launch_step2_s(int coll, const StepParams& p, const Step2Policy& lp, hipStream_t s) {
+    if (lp.strip) {
+        if (lp.h < 1) return hipErrorInvalidValue;
+        if (lp.w == 4) return launch_step2s_c<STORE, 4>(coll, p, lp, s);
+        if (lp.w == 8) return launch_step2s_c<STORE, 8>(coll, p, lp, s);
+        return hipErrorInvalidValue;
+    }
     if (lp.ty == 6) return launch_step2_c<STORE, 6>(coll, p, lp, s);
     if (lp.ty == 14) return launch_step2_c<STORE, 14>(coll, p, lp, s);
     return hipErrorInvalidValue;
@@ -345,12 +408,24 @@ hipError_t launch_step2(int coll, int storage, const StepParams& p, const Step2P
     return launch_step2_s<LBM_F32>(coll, p, lp, s);
 }
 
+// Measured on MI355X (us per step, two-step launches):
+//   1024^2 FP32 BGK: tiles of 6 rows 8.6; strips of 4 wavefronts x 16 rows 11.0
+//   8192^2 FP32 BGK: tiles 553; strips 4 x 32 539, 4 x 64 592-610 (MRT: tiles 646, 4 x 32 598)
+//   8192^2 FP16 BGK: tiles 529; strips 4 x 64 473
+// Cache-resident grids have too few rows per strip for the march to pay; HBM grids gain
+// from computing each phase-A row once and from the prefetch.
 Step2Policy step2_policy(long long plane, int storage) {
-    (void)plane;
-    (void)storage;
+    const double bytes = 2.0 * 9.0 * double(plane) * (storage == LBM_F32 ? 4.0 : 2.0);
     Step2Policy lp;
-    lp.ty = 6;
     lp.remap = 1;
+    if (bytes <= 192.0e6) {
+        lp.strip = 0;
+        lp.ty = 6;
+    } else {
+        lp.strip = 1;
+        lp.w = 4;
+        lp.h = storage == LBM_F32 ? 32 : 64;
+    }
     return lp;
 }
 

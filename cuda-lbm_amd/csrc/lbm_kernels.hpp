@@ -30,11 +30,16 @@ long long step_grid_blocks(const StepParams& p, const LaunchPolicy& lp);
 // mode: 0 step, 1 moments pass, 2 step + moments (CM only). Sets p.ntx and p.nt from lp.
 hipError_t launch_step(int coll, int storage, int mode, const StepParams& p, const LaunchPolicy& lp, double* partials,
                        hipStream_t s);
-// Two steps per launch (temporal blocking): tile height (6 or 14 rows of kT2W nodes) and
-// XCD remap. The caller sets p.edge_prev / edge_next / n_edge when a non-periodic axis exists.
+// Two steps per launch (temporal blocking), in one of two forms: tiles (ty = 6 or 14 rows of
+// kT2W nodes per workgroup) or marching strips (strip = 1: w = 4 or 8 wavefronts per
+// workgroup, h rows per strip). XCD remap of the workgroup order. The caller sets
+// p.edge_prev / edge_next / n_edge when a non-periodic axis exists.
 struct Step2Policy {
     int ty = 6;
     int remap = 1;
+    int strip = 0;
+    int w = 8;
+    int h = 64;
 };
 Step2Policy step2_policy(long long plane, int storage);
 long long step2_grid_blocks(const StepParams& p, const Step2Policy& lp);

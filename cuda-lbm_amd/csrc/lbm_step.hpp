@@ -706,4 +706,142 @@ LBM_HD void step2_phase_b(const StepParams& p, const typename Elem<STORE>::T* ti
     }
 }
 
+// ---- two steps per launch, marching strips ------------------------------------------------
+//
+// A workgroup of W wavefronts owns a strip of kT2W columns and H rows and marches up it in
+// chunks of W rows: iteration k computes phase-A rows kW .. kW+W-1 of the strip (strip row i
+// = grid row y0 - 1 + i, one row per wavefront, lane l = column x0 - 1 + l) into a ring of
+// W + 2 LDS rows, then phase-B rows kW - 2 .. kW + W - 3 (grid row y0 + j) from the three
+// ring rows each needs. A phase-A row is computed once per strip (the tile form recomputes
+// the two halo rows of every tile), and the pull sources of the next chunk are loaded into
+// registers before the current chunk's phase B, so their latency overlaps that compute.
+// Unstreamed slots are handled as in the tile form (edge store for phase A, source buffer
+// for phase B).
+
+// Pull sources of phase-A node (gx, gy) (already wrapped and inside the domain).
+template <int STORE>
+LBM_HD void step2_gather(const StepParams& p, int gx, int gy, float* g) {
+    using St = Elem<STORE>;
+    using T = typename St::T;
+    const bool ex = !p.px && (gx == 0 || gx == p.nx - 1), ey = !p.py && (gy == 0 || gy == p.ny - 1);
+    const T* src = static_cast<const T*>(p.src);
+    const int xm = gx == 0 ? (p.px ? p.nx - 1 : -1) : gx - 1, xp = gx == p.nx - 1 ? (p.px ? 0 : -1) : gx + 1;
+    const int ym = gy == 0 ? (p.py ? p.ny - 1 : -1) : gy - 1, yp = gy == p.ny - 1 ? (p.py ? 0 : -1) : gy + 1;
+    const uint32_t rm = uint32_t(ym) * uint32_t(p.nx), r0 = uint32_t(gy) * uint32_t(p.nx), rp = uint32_t(yp) * uint32_t(p.nx);
+    if (!ex && !ey) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const uint32_t xs = uint32_t(CX[q] > 0 ? xm : (CX[q] < 0 ? xp : gx));
+            const uint32_t row = CY[q] > 0 ? rm : (CY[q] < 0 ? rp : r0);
+            g[q] = St::load1(src + q * p.plane + (row + xs), q, p.rho0);
+        }
+    } else {
+        const T* prev = static_cast<const T*>(p.edge_prev);
+        const uint32_t eslot = uint32_t(edge_slot(gx, gy, p.nx, p.ny, p.px, p.py));
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int xs = CX[q] > 0 ? xm : (CX[q] < 0 ? xp : gx);
+            const int ys = CY[q] > 0 ? ym : (CY[q] < 0 ? yp : gy);
+            const uint32_t row = CY[q] > 0 ? rm : (CY[q] < 0 ? rp : r0);
+            g[q] = (xs < 0 || ys < 0) ? St::load1(prev + q * p.n_edge + eslot, q, p.rho0)
+                                      : St::load1(src + q * p.plane + (row + uint32_t(xs)), q, p.rho0);
+        }
+    }
+}
+
+// Grid coordinates of strip node (lane l, strip row i); false outside a non-periodic domain.
+LBM_HD bool strip_node(const StepParams& p, int x0, int y0, int l, int i, int& gx, int& gy) {
+    gx = x0 - 1 + l;
+    gy = y0 - 1 + i;
+    if (gx < 0 || gx >= p.nx) {
+        if (!p.px) return false;
+        gx = ((gx % p.nx) + p.nx) % p.nx;
+    }
+    if (gy < 0 || gy >= p.ny) {
+        if (!p.py) return false;
+        gy = ((gy % p.ny) + p.ny) % p.ny;
+    }
+    return true;
+}
+
+// Phase A of strip node (l, i) from its gathered sources g: collide, store into the ring
+// row `slot` and, for the strip's own edge nodes, into the next edge store.
+template <int COLL, int STORE>
+LBM_HD void step2s_phase_a(const StepParams& p, typename Elem<STORE>::T* ring, int ring_rows, int slot, int l, int i,
+                           int x0, int y0, int hs, int gx, int gy, float* g) {
+    using St = Elem<STORE>;
+    using T = typename St::T;
+    const uint32_t node = uint32_t(gy) * uint32_t(p.nx) + uint32_t(gx);
+    const uint8_t fl = p.flags ? p.flags[node] : uint8_t(LBM_FLUID);
+    if (fl != LBM_FLUID) apply_local_bc(g, fl, gx, gy, node_args(p));
+    float fx, fy;
+    node_force(p, gx, gy, node, fx, fy);
+    Macro m;
+    MomAcc acc{0.0, 0.0, 0.0};
+    node_update<COLL, 0>(p, g, fx, fy, m, acc);
+    const int stride = ring_rows * kT2L;
+#pragma unroll
+    for (int q = 0; q < Q; q++) St::store1(ring + q * stride + slot * kT2L + l, q, p.rho0, g[q]);
+    const bool ex = !p.px && (gx == 0 || gx == p.nx - 1), ey = !p.py && (gy == 0 || gy == p.ny - 1);
+    const bool own = l >= 1 && l <= kT2W && i >= 1 && i <= hs && x0 - 1 + l < p.nx;
+    if (own && (ex || ey)) {
+        T* en = static_cast<T*>(p.edge_next);
+        const int base = p.py ? 0 : 2 * p.nx;
+        int es[2] = {-1, -1};
+        if (ey) es[0] = gy == 0 ? gx : p.nx + gx;
+        if (ex) es[1] = base + (gx == 0 ? gy : p.ny + gy);
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            if (es[k] < 0) continue;
+#pragma unroll
+            for (int q = 0; q < Q; q++) St::store1(en + (long long)q * p.n_edge + es[k], q, p.rho0, g[q]);
+        }
+    }
+}
+
+// Phase B of strip row j (grid row y0 + j), lane l: pulls from ring rows j, j+1, j+2.
+template <int COLL, int STORE>
+LBM_HD void step2s_phase_b(const StepParams& p, const typename Elem<STORE>::T* ring, int ring_rows, int l, int j,
+                           int x0, int y0) {
+    using St = Elem<STORE>;
+    using T = typename St::T;
+    if (l < 1 || l > kT2W) return;
+    const int x = x0 - 1 + l, y = y0 + j;
+    if (x >= p.nx) return;
+    const uint32_t node = uint32_t(y) * uint32_t(p.nx) + uint32_t(x);
+    const T* src = static_cast<const T*>(p.src);
+    const int stride = ring_rows * kT2L;
+    // ring rows of strip rows j (below), j + 1 (own), j + 2 (above)
+    const int s_lo = j % ring_rows, s_mid = (j + 1) % ring_rows, s_hi = (j + 2) % ring_rows;
+    const bool ex = !p.px && (x == 0 || x == p.nx - 1), ey = !p.py && (y == 0 || y == p.ny - 1);
+    float g[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int srow = CY[q] > 0 ? s_lo : (CY[q] < 0 ? s_hi : s_mid);
+        const T* a = ring + q * stride + srow * kT2L + (l - CX[q]);
+        if (!ex && !ey) {
+            g[q] = St::load1(a, q, p.rho0);
+        } else {
+            const int xs = x - CX[q], ys = y - CY[q];
+            const bool out = (!p.px && (xs < 0 || xs >= p.nx)) || (!p.py && (ys < 0 || ys >= p.ny));
+            g[q] = out ? St::load1(src + q * p.plane + node, q, p.rho0) : St::load1(a, q, p.rho0);
+        }
+    }
+    const uint8_t fl = p.flags ? p.flags[node] : uint8_t(LBM_FLUID);
+    if (fl != LBM_FLUID) apply_local_bc(g, fl, x, y, node_args(p));
+    float fx, fy;
+    node_force(p, x, y, node, fx, fy);
+    Macro m;
+    MomAcc acc{0.0, 0.0, 0.0};
+    node_update<COLL, 0>(p, g, fx, fy, m, acc);
+    T* dst = static_cast<T*>(p.dst);
+#pragma unroll
+    for (int q = 0; q < Q; q++) St::store1(dst + q * p.plane + node, q, p.rho0, g[q]);
+    if (p.out_rho) {
+        p.out_rho[node] = m.rho;
+        p.out_ux[node] = m.ux;
+        p.out_uy[node] = m.uy;
+    }
+}
+
 }  // namespace lbm

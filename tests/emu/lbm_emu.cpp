@@ -50,6 +50,41 @@ void launch2_emu(const StepParams& p, int reverse) {
     }
 }
 
+// One marching-strip two-step launch (W wavefronts, H rows per strip), chunk by chunk: all
+// phase-A rows of a chunk, then its phase-B rows, on a poisoned ring.
+template <int COLL, int STORE, int W>
+void launch2s_emu(const StepParams& p, int H, int reverse) {
+    using T = typename Elem<STORE>::T;
+    constexpr int R = W + 2;
+    const int strips_x = (p.nx + kT2W - 1) / kT2W, strips_y = (p.ny + H - 1) / H;
+    const long long ns = (long long)strips_x * strips_y;
+    std::vector<T> ring(size_t(Q) * R * kT2L);
+    for (long long k0 = 0; k0 < ns; k0++) {
+        const long long lb = reverse ? ns - 1 - k0 : k0;
+        const int sy = int(lb / strips_x), sx = int(lb - (long long)sy * strips_x);
+        const int x0 = sx * kT2W, y0 = sy * H;
+        const int hs = p.ny - y0 < H ? p.ny - y0 : H;
+        const int K = (hs + 1) / W + 1;
+        std::fill(ring.begin(), ring.end(), poison<T>());
+        for (int k = 0; k < K; k++) {
+            for (int w = 0; w < W; w++)
+                for (int l = 0; l < kT2L; l++) {
+                    const int i = k * W + w;
+                    int gx, gy;
+                    if (i > hs + 1 || !strip_node(p, x0, y0, l, i, gx, gy)) continue;
+                    float g[Q];
+                    step2_gather<STORE>(p, gx, gy, g);
+                    step2s_phase_a<COLL, STORE>(p, ring.data(), R, i % R, l, i, x0, y0, hs, gx, gy, g);
+                }
+            for (int w = 0; w < W; w++)
+                for (int l = 0; l < kT2L; l++) {
+                    const int j = k * W - 2 + w;
+                    if (j >= 0 && j < hs) step2s_phase_b<COLL, STORE>(p, ring.data(), R, l, j, x0, y0);
+                }
+        }
+    }
+}
+
 template <int COLL, int STORE, int SEG>
 int run_t(const lbm_config& c, const uint8_t* flags, const float* rho, const float* u, const float* force,
           int nsteps, float* f_out, float* rho_out, float* u_out, double* avg_out, int two_ty, int reverse) {
@@ -150,8 +185,15 @@ int run_t(const lbm_config& c, const uint8_t* flags, const float* rho, const flo
             p.edge_prev = n_edge ? edge[edge_cur].data() : nullptr;
             p.edge_next = n_edge ? edge[1 - edge_cur].data() : nullptr;
             p.n_edge = n_edge;
-            if (two_ty == 14) launch2_emu<COLL, STORE, 14>(p, reverse);
-            else launch2_emu<COLL, STORE, 6>(p, reverse);
+            if (two_ty < 0) {  // marching strips: -(W * 1000 + H)
+                const int W = (-two_ty) / 1000, H = (-two_ty) % 1000;
+                if (W == 8) launch2s_emu<COLL, STORE, 8>(p, H, reverse);
+                else launch2s_emu<COLL, STORE, 4>(p, H, reverse);
+            } else if (two_ty == 14) {
+                launch2_emu<COLL, STORE, 14>(p, reverse);
+            } else {
+                launch2_emu<COLL, STORE, 6>(p, reverse);
+            }
             if (n_edge > 0) {
                 edge_cur = 1 - edge_cur;
                 in_store = true;
@@ -221,13 +263,19 @@ int run_s(const lbm_config& c, int seg, const uint8_t* flags, const float* rho, 
 
 // seg: nodes per thread of one-step launches, the wide (4) or the storage's narrow width
 // (FP32 1, 16-bit 2); 0 selects the wide one. two_ty: 0 one step per launch; 6 or 14: two-step
-// launches with that tile height where the configuration allows them (as the library does).
-// reverse: run two-step tiles in reverse order.
+// tile launches with that tile height, -(W * 1000 + H): two-step marching strips of W
+// wavefronts and H rows; both where the configuration allows them (as the library does).
+// reverse: run two-step tiles or strips in reverse order.
 extern "C" int emu_run(const lbm_config* c, int seg, int two_ty, int reverse, const uint8_t* flags, const float* rho,
                        const float* u, const float* force, int nsteps, float* f_out, float* rho_out, float* u_out,
                        double* avg_out) {
     if (seg == 0) seg = kSegWide;
-    if (two_ty != 0 && two_ty != 6 && two_ty != 14) return -1;
+    if (two_ty < 0) {
+        const int W = (-two_ty) / 1000, H = (-two_ty) % 1000;
+        if ((W != 4 && W != 8) || H < 1) return -1;
+    } else if (two_ty != 0 && two_ty != 6 && two_ty != 14) {
+        return -1;
+    }
     if (c->storage == LBM_F16_SHIFTED)
         return run_s<LBM_F16_SHIFTED>(*c, seg, flags, rho, u, force, nsteps, f_out, rho_out, u_out, avg_out, two_ty,
                                       reverse);
@@ -325,7 +373,7 @@ int main() {
                             }
                             cases++;
                         }
-                        for (int ty : {6, 14}) {  // two-step launches (5 steps: two pairs and a single)
+                        for (int ty : {6, 14, -4003, -8016}) {  // two-step tiles and strips (5 steps)
                             if (emu_run(&c, 0, ty, cases & 1, fl.data(), rho.data(), u.data(),
                                         (cases & 2) ? F.data() : nullptr, 5, f.data(), r.data(), uu.data(), a.data()) != 0) {
                                 std::printf("emu_run rejected two-step ty %d\n", ty);

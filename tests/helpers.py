@@ -35,8 +35,11 @@ def _p(a):
 def run_emu(case: sc.Case, nsteps: int, seg: int = 0, two_ty: int = 0, reverse: bool = False):
     """Host emulation of the fused kernels (same bodies, same grids). seg: nodes per thread of
     one-step launches (4, or the storage's narrow width: FP32 1, 16-bit 2; 0 = 4). two_ty: 0
-    one step per launch, 6 or 14 two-step launches with that tile height where the library
-    would use them; reverse runs their tiles in reverse order. Returns (f, rho, u, avg)."""
+    one step per launch, 6 or 14 two-step tile launches with that tile height, (W, H) two-step
+    marching strips of W wavefronts and H rows; both where the library would use them; reverse
+    runs tiles or strips in reverse order. Returns (f, rho, u, avg)."""
+    if isinstance(two_ty, tuple):
+        two_ty = -(two_ty[0] * 1000 + two_ty[1])
     c = case.config.to_c()
     N = case.config.N
     fl = None if case.flags is None else np.ascontiguousarray(case.flags, np.uint8).reshape(-1)

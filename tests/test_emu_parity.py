@@ -81,7 +81,7 @@ def test_asan_bounds_matrix():
     assert "cases ok" in r.stdout
 
 
-@pytest.mark.parametrize("ty", [6, 14])
+@pytest.mark.parametrize("ty", [6, 14, (4, 3), (4, 16), (8, 8), (8, 37)], ids=["tile6", "tile14", "strip4x3", "strip4x16", "strip8x8", "strip8x37"])
 @pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("op", OPS, ids=["bgk", "mrt", "cm", "cm_optimal"])
 def test_emulated_two_step_launches(layout, op, ty):
@@ -108,4 +108,5 @@ def test_emulated_two_step_reduced_storage(storage, layout):
     f1, *_ = run_emu(case, 6, seg=2)
     f2, *_ = run_emu(case, 6, two_ty=6)
     f3, *_ = run_emu(case, 6, two_ty=14, reverse=True)
-    assert np.array_equal(f1, f2) and np.array_equal(f1, f3)
+    f4, *_ = run_emu(case, 6, two_ty=(8, 16), reverse=True)
+    assert np.array_equal(f1, f2) and np.array_equal(f1, f3) and np.array_equal(f1, f4)

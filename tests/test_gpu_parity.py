@@ -162,3 +162,22 @@ def test_gpu_two_step_launches_match_one_step(gpu_available, layout, coll, stora
     else:
         assert float(np.max(np.abs(f2 - f1))) <= 4 * 0.05 * 2.0 ** -11
         assert float(np.max(np.abs(f2 - fo))) <= 4 * 0.05 * 2.0 ** -11
+
+
+@pytest.mark.parametrize("layout", ["periodic", "open", "channel"])
+def test_gpu_two_step_strips_on_hbm_sized_grid(gpu_available, layout):
+    # Above 192 MB of populations two-step launches march 62-column strips (phase-A rows in
+    # an LDS ring, next rows prefetched); 2600x2100 FP32 = 393 MB. 5 steps = two strip
+    # launches and a single step.
+    print(f"case strips {layout} 2600x2100", flush=True)
+    case = layout_case(layout, 2600, 2100, A.LBM_MRT)
+    case.config.steps_per_launch = 1
+    f1, _, u1, s1 = run_gpu(case, 5)
+    s1.close()
+    case.config.steps_per_launch = 0
+    f2, _, u2, s2 = run_gpu(case, 5)
+    assert s2.steps_per_launch == 2
+    s2.close()
+    assert max_rel(f2, f1) <= 1e-5 and float(np.max(np.abs(u2 - u1))) <= 1e-6
+    fo, *_ = run_oracle(case, 5, 32)
+    assert max_rel(f2, fo) <= 1e-5
