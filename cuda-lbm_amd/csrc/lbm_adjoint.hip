@@ -37,72 +37,113 @@ struct AdjParams {
     float* grad_u0;         // AoS
 };
 
-__device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
-
-__device__ __forceinline__ void gather_state(const AdjParams& p, int x, int y, float* g) {
-#pragma unroll
-    for (int q = 0; q < Q; q++)
-        g[q] = p.state[q * p.plane + (long long)wrap(y - CY[q], p.ny) * p.nx + wrap(x - CX[q], p.nx)];
+// Periodic neighbours of (x, y): columns x-1, x+1 and row offsets of rows y-1, y, y+1
+// (32-bit element offsets; N < 2^31).
+struct Nb {
+    uint32_t xm, x, xp, rm, r0, rp;
+};
+__device__ __forceinline__ Nb neighbours(int x, int y, int nx, int ny) {
+    Nb b;
+    b.x = uint32_t(x);
+    b.xm = uint32_t(x == 0 ? nx - 1 : x - 1);
+    b.xp = uint32_t(x == nx - 1 ? 0 : x + 1);
+    b.rm = uint32_t(y == 0 ? ny - 1 : y - 1) * uint32_t(nx);
+    b.r0 = uint32_t(y) * uint32_t(nx);
+    b.rp = uint32_t(y == ny - 1 ? 0 : y + 1) * uint32_t(nx);
+    return b;
+}
+// Offset of the node at (x + s cx_q, y + s cy_q): s = -1 upstream (pull), +1 downstream.
+template <int S>
+__device__ __forceinline__ uint32_t nb_offset(const Nb& b, int q) {
+    const int dx = S * CX[q], dy = S * CY[q];
+    return (dy < 0 ? b.rm : (dy > 0 ? b.rp : b.r0)) + (dx < 0 ? b.xm : (dx > 0 ? b.xp : b.x));
 }
 
-__device__ __forceinline__ void node_force(const AdjParams& p, long long n, float& fx, float& fy) {
-    fx = p.fx;
-    fy = p.fy;
-    if (p.force) {
-        fx = p.force[2 * n];
-        fy = p.force[2 * n + 1];
+__device__ __forceinline__ void gather_state(const AdjParams& p, const Nb& b, float* g) {
+#pragma unroll
+    for (int q = 0; q < Q; q++) g[q] = p.state[q * p.plane + nb_offset<-1>(b, q)];
+}
+
+// Uniform force, else the field (value selects: see node_force in lbm_step.hpp).
+__device__ __forceinline__ void node_force(const AdjParams& p, uint32_t n, float& fx, float& fy) {
+    const float* f = p.force;
+    float vx = 0.0f, vy = 0.0f;
+    if (f) {
+        vx = f[2 * size_t(n)];
+        vy = f[2 * size_t(n) + 1];
     }
+    fx = f ? vx : p.fx;
+    fy = f ? vy : p.fy;
+}
+
+// Workgroups cover rows in pieces of kAdjBlock nodes, ordered so that each of the 8 XCDs
+// runs a contiguous range of rows (the y +- 1 neighbours then share its L2); the grid is
+// padded to a multiple of 8 workgroups.
+constexpr int kAdjXcds = 8;
+__device__ __forceinline__ bool adj_node(const AdjParams& p, int& x, int& y, uint32_t& n) {
+    const int bpr = (p.nx + kAdjBlock - 1) / kAdjBlock;
+    const unsigned nb = gridDim.x;
+    const unsigned lb = (blockIdx.x % kAdjXcds) * (nb / kAdjXcds) + blockIdx.x / kAdjXcds;
+    y = int(lb / unsigned(bpr));
+    x = int(lb - unsigned(y) * unsigned(bpr)) * kAdjBlock + int(threadIdx.x);
+    if (y >= p.ny || x >= p.nx) return false;
+    n = uint32_t(y) * uint32_t(p.nx) + uint32_t(x);
+    return true;
 }
 
 __global__ __launch_bounds__(kAdjBlock) void adjoint_seed_kernel(AdjParams p) {
-    long long n = (long long)blockIdx.x * kAdjBlock + threadIdx.x;
-    if (n >= (long long)p.nx * p.ny) return;
-    int y = int(n / p.nx), x = int(n - (long long)y * p.nx);
+    int x, y;
+    uint32_t n;
+    if (!adj_node(p, x, y, n)) return;
+    const Nb b = neighbours(x, y, p.nx, p.ny);
     float g[Q];
-    gather_state(p, x, y, g);
+    gather_state(p, b, g);
     float fx, fy;
     node_force(p, n, fx, fy);
-    AdjOut o = adjoint_snapshot(g, fx, fy, p.dLdu[2 * n], p.dLdu[2 * n + 1]);
+    AdjOut o = adjoint_snapshot(g, fx, fy, p.dLdu[2 * size_t(n)], p.dLdu[2 * size_t(n) + 1]);
 #pragma unroll
     for (int q = 0; q < Q; q++) p.mu_cur[q * p.plane + n] = o.mu[q];
-    p.grad_force[2 * n] = o.gfx;
-    p.grad_force[2 * n + 1] = o.gfy;
+    p.grad_force[2 * size_t(n)] = o.gfx;
+    p.grad_force[2 * size_t(n) + 1] = o.gfy;
 }
 
 template <int COLL>
 __global__ __launch_bounds__(kAdjBlock) void adjoint_step_kernel(AdjParams p) {
-    long long n = (long long)blockIdx.x * kAdjBlock + threadIdx.x;
-    if (n >= (long long)p.nx * p.ny) return;
-    int y = int(n / p.nx), x = int(n - (long long)y * p.nx);
+    int x, y;
+    uint32_t n;
+    if (!adj_node(p, x, y, n)) return;
+    const Nb b = neighbours(x, y, p.nx, p.ny);
     float lam[Q], g[Q];
 #pragma unroll
-    for (int q = 0; q < Q; q++)
-        lam[q] = p.mu_next[q * p.plane + (long long)wrap(y + CY[q], p.ny) * p.nx + wrap(x + CX[q], p.nx)];
-    gather_state(p, x, y, g);
+    for (int q = 0; q < Q; q++) lam[q] = p.mu_next[q * p.plane + nb_offset<1>(b, q)];
+    gather_state(p, b, g);
     float fx, fy;
     node_force(p, n, fx, fy);
     AdjOut o = COLL == LBM_BGK ? adjoint_bgk(g, lam, fx, fy, p.omega) : adjoint_mrt(g, lam, fx, fy, p.S);
 #pragma unroll
     for (int q = 0; q < Q; q++) p.mu_cur[q * p.plane + n] = o.mu[q];
-    p.grad_force[2 * n] += o.gfx;
-    p.grad_force[2 * n + 1] += o.gfy;
+    p.grad_force[2 * size_t(n)] += o.gfx;
+    p.grad_force[2 * size_t(n) + 1] += o.gfy;
 }
 
 __global__ __launch_bounds__(kAdjBlock) void adjoint_u0_kernel(AdjParams p) {
-    long long n = (long long)blockIdx.x * kAdjBlock + threadIdx.x;
-    if (n >= (long long)p.nx * p.ny) return;
-    int y = int(n / p.nx), x = int(n - (long long)y * p.nx);
+    int x, y;
+    uint32_t n;
+    if (!adj_node(p, x, y, n)) return;
+    const Nb b = neighbours(x, y, p.nx, p.ny);
     float lam[Q];
 #pragma unroll
-    for (int q = 0; q < Q; q++)
-        lam[q] = p.mu_next[q * p.plane + (long long)wrap(y + CY[q], p.ny) * p.nx + wrap(x + CX[q], p.nx)];
+    for (int q = 0; q < Q; q++) lam[q] = p.mu_next[q * p.plane + nb_offset<1>(b, q)];
     float gx, gy;
     adjoint_init_u(lam, p.rho0[n], p.u0x[n], p.u0y[n], gx, gy);
-    p.grad_u0[2 * n] = gx;
-    p.grad_u0[2 * n + 1] = gy;
+    p.grad_u0[2 * size_t(n)] = gx;
+    p.grad_u0[2 * size_t(n) + 1] = gy;
 }
 
-unsigned node_blocks(size_t N) { return unsigned((N + kAdjBlock - 1) / kAdjBlock); }
+dim3 node_grid(int nx, int ny) {
+    const unsigned long long nb = (unsigned long long)((nx + kAdjBlock - 1) / kAdjBlock) * unsigned(ny);
+    return dim3(unsigned((nb + kAdjXcds - 1) / kAdjXcds * kAdjXcds));
+}
 
 }  // namespace
 
@@ -157,13 +198,21 @@ int lbm_adjoint_forward(lbm_ctx* ctx, int32_t nsteps, int32_t every) {
         (void)hipMemcpyAsync(A.d_u0, ctx->d_ux, ctx->N * 4, hipMemcpyDeviceToDevice, ctx->stream);
         (void)hipMemcpyAsync(A.d_u0 + ctx->N, ctx->d_uy, ctx->N * 4, hipMemcpyDeviceToDevice, ctx->stream);
     }
-    for (int t = 0; t < nsteps; t++) {
+    const bool pairs = ctx->two_step_ok();
+    for (int t = 0; t < nsteps;) {
         if (t % every == 0) {
             hipError_t e = hipMemcpyAsync(A.checkpoints[size_t(t / every)], ctx->buf[ctx->cur], pb, hipMemcpyDeviceToDevice,
                                           ctx->stream);
             if (e != hipSuccess) return ctx->hip_fail(e, "checkpoint copy");
         }
-        rc = ctx->one_step(t == nsteps - 1);
+        // a pair of steps per launch unless a checkpoint falls between them
+        if (pairs && t + 2 <= nsteps && (t + 1) % every != 0) {
+            rc = ctx->two_step(t + 2 == nsteps);
+            t += 2;
+        } else {
+            rc = ctx->one_step(t == nsteps - 1);
+            t += 1;
+        }
         if (rc) return rc;
     }
     A.nsteps = nsteps;
@@ -222,7 +271,7 @@ int lbm_adjoint_backward(lbm_ctx* ctx, const float* dL_du, float* dL_dforce, flo
     for (int k = 0; k < 9; k++) p.S[k] = ctx->cfg.S[k];
     p.grad_force = A.d_grad_force;
     p.dLdu = dldu;
-    const unsigned nb = node_blocks(N);
+    const dim3 nb = node_grid(ctx->cfg.nx, ctx->cfg.ny);
     int cur = 0;  // mu[cur] receives mu_t
     const int nseg = (T + K - 1) / K;
     float* seg = A.d_seg;
@@ -232,22 +281,33 @@ int lbm_adjoint_backward(lbm_ctx* ctx, const float* dL_du, float* dL_dforce, flo
         const int len = (T - t0) < K ? (T - t0) : K; // adjoint steps t0+1 .. t0+len
         hipError_t e = hipMemcpyAsync(seg, A.checkpoints[size_t(s)], pb, hipMemcpyDeviceToDevice, ctx->stream);
         if (e != hipSuccess) return ctx->hip_fail(e, "segment restore");
-        for (int i = 1; i < len; i++) {  // recompute f*_{t0+1} .. f*_{t0+len-1}
+        // recompute f*_{t0+1} .. f*_{t0+len-1}: pairs of steps per launch, the launch also
+        // storing its intermediate state; an odd remainder as one step
+        const bool pairs = ctx->cfg.steps_per_launch != 1;
+        for (int i = 1; i < len;) {
             StepParams sp = ctx->step_params(ctx->cur, false);
             sp.src = seg + size_t(i - 1) * pf;
-            sp.dst = seg + size_t(i) * pf;
-            e = launch_step(ctx->cfg.collision, LBM_F32, 0, sp, step_policy(ctx->plane, LBM_F32), nullptr, ctx->stream);
+            if (pairs && i + 1 < len) {
+                sp.mid = seg + size_t(i) * pf;
+                sp.dst = seg + size_t(i + 1) * pf;
+                e = launch_step2(ctx->cfg.collision, LBM_F32, sp, step2_policy(ctx->plane, LBM_F32), ctx->stream);
+                i += 2;
+            } else {
+                sp.dst = seg + size_t(i) * pf;
+                e = launch_step(ctx->cfg.collision, LBM_F32, 0, sp, step_policy(ctx->plane, LBM_F32), nullptr, ctx->stream);
+                i += 1;
+            }
             if (e != hipSuccess) return ctx->hip_fail(e, "segment recompute");
         }
         for (int t = t0 + len; t >= t0 + 1; t--) {
             p.state = seg + size_t(t - 1 - t0) * pf;
             p.mu_cur = A.d_mu[cur];
             p.mu_next = A.d_mu[1 - cur];
-            if (t == T) hipLaunchKernelGGL(adjoint_seed_kernel, dim3(nb), dim3(kAdjBlock), 0, ctx->stream, p);
+            if (t == T) hipLaunchKernelGGL(adjoint_seed_kernel, nb, dim3(kAdjBlock), 0, ctx->stream, p);
             else if (ctx->cfg.collision == LBM_BGK)
-                hipLaunchKernelGGL(adjoint_step_kernel<LBM_BGK>, dim3(nb), dim3(kAdjBlock), 0, ctx->stream, p);
+                hipLaunchKernelGGL(adjoint_step_kernel<LBM_BGK>, nb, dim3(kAdjBlock), 0, ctx->stream, p);
             else
-                hipLaunchKernelGGL(adjoint_step_kernel<LBM_MRT>, dim3(nb), dim3(kAdjBlock), 0, ctx->stream, p);
+                hipLaunchKernelGGL(adjoint_step_kernel<LBM_MRT>, nb, dim3(kAdjBlock), 0, ctx->stream, p);
             e = hipGetLastError();
             if (e != hipSuccess) return ctx->hip_fail(e, "adjoint step launch");
             cur = 1 - cur;
@@ -260,7 +320,7 @@ int lbm_adjoint_backward(lbm_ctx* ctx, const float* dL_du, float* dL_dforce, flo
         p.u0x = A.d_u0;
         p.u0y = A.d_u0 + N;
         p.grad_u0 = gu0;
-        hipLaunchKernelGGL(adjoint_u0_kernel, dim3(nb), dim3(kAdjBlock), 0, ctx->stream, p);
+        hipLaunchKernelGGL(adjoint_u0_kernel, nb, dim3(kAdjBlock), 0, ctx->stream, p);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return ctx->hip_fail(e, "adjoint u0 launch");
     }

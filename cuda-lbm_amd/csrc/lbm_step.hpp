@@ -55,6 +55,7 @@ struct StepParams {
     const void* edge_prev;
     void* edge_next;
     int n_edge;
+    void* mid;                   // two-step launches: also receives state t+1, or null
 };
 
 // Row classes for the fast-path test: 0 all FLUID, 1 non-FLUID only at x = 0 or x = nx-1,
@@ -649,6 +650,11 @@ LBM_HD void step2_phase_a(const StepParams& p, typename Elem<STORE>::T* tile, in
     node_update<COLL, 0>(p, g, fx, fy, m, acc);
 #pragma unroll
     for (int q = 0; q < Q; q++) St::store1(tile + q * NH + t, q, p.rho0, g[q]);
+    if (own && p.mid) {
+        T* md = static_cast<T*>(p.mid);
+#pragma unroll
+        for (int q = 0; q < Q; q++) St::store1(md + q * p.plane + node, q, p.rho0, g[q]);
+    }
     if (own && (ex || ey)) {
         T* en = static_cast<T*>(p.edge_next);
         const int base = p.py ? 0 : 2 * p.nx;
@@ -784,6 +790,11 @@ LBM_HD void step2s_phase_a(const StepParams& p, typename Elem<STORE>::T* ring, i
     for (int q = 0; q < Q; q++) St::store1(ring + q * stride + slot * kT2L + l, q, p.rho0, g[q]);
     const bool ex = !p.px && (gx == 0 || gx == p.nx - 1), ey = !p.py && (gy == 0 || gy == p.ny - 1);
     const bool own = l >= 1 && l <= kT2W && i >= 1 && i <= hs && x0 - 1 + l < p.nx;
+    if (own && p.mid) {
+        T* md = static_cast<T*>(p.mid);
+#pragma unroll
+        for (int q = 0; q < Q; q++) St::store1(md + q * p.plane + node, q, p.rho0, g[q]);
+    }
     if (own && (ex || ey)) {
         T* en = static_cast<T*>(p.edge_next);
         const int base = p.py ? 0 : 2 * p.nx;

@@ -285,6 +285,43 @@ extern "C" int emu_run(const lbm_config* c, int seg, int two_ty, int reverse, co
     return run_s<LBM_F32>(*c, seg, flags, rho, u, force, nsteps, f_out, rho_out, u_out, avg_out, two_ty, reverse);
 }
 
+// One two-step launch from FP32 populations f_in ([9][N], fully periodic, all FLUID, FP32
+// storage) with the intermediate-state output on: f_mid receives state t+1, f_out state t+2.
+// form 0: tiles of 6 rows; 1: strips of 4 wavefronts x 8 rows.
+extern "C" int emu_two_step_mid(const lbm_config* c, int form, const float* f_in, float* f_mid, float* f_out) {
+    if (c->storage != LBM_F32 || !c->periodic_x || !c->periodic_y) return -1;
+    const long long N = (long long)c->nx * c->ny;
+    std::vector<float> a(f_in, f_in + 9 * N), b(size_t(9) * N, poison<float>()), m(size_t(9) * N, poison<float>());
+    StepParams p{};
+    p.nx = c->nx;
+    p.ny = c->ny;
+    p.px = 1;
+    p.py = 1;
+    p.plane = N;
+    p.src = a.data();
+    p.dst = b.data();
+    p.omega = c->omega;
+    for (int k = 0; k < 9; k++) p.S[k] = c->S[k];
+    p.rho0 = c->rho0;
+    p.fx = c->force[0];
+    p.fy = c->force[1];
+    p.forced = (c->force[0] != 0.f || c->force[1] != 0.f) ? 1 : 0;
+    p.quirks = c->quirks;
+    p.mid = m.data();
+    if (form == 0) {
+        if (c->collision == LBM_BGK) launch2_emu<LBM_BGK, LBM_F32, 6>(p, 0);
+        else if (c->collision == LBM_MRT) launch2_emu<LBM_MRT, LBM_F32, 6>(p, 0);
+        else launch2_emu<LBM_CM, LBM_F32, 6>(p, 0);
+    } else {
+        if (c->collision == LBM_BGK) launch2s_emu<LBM_BGK, LBM_F32, 4>(p, 8, 0);
+        else if (c->collision == LBM_MRT) launch2s_emu<LBM_MRT, LBM_F32, 4>(p, 8, 0);
+        else launch2s_emu<LBM_CM, LBM_F32, 4>(p, 8, 0);
+    }
+    std::memcpy(f_mid, m.data(), 9 * N * sizeof(float));
+    std::memcpy(f_out, b.data(), 9 * N * sizeof(float));
+    return 0;
+}
+
 // The product's node adjoints (lbm_adjoint.hpp) on the host: mode 0 BGK, 1 MRT, 2 snapshot
 // seed (lam = dL/du in lam[0..1]), 3 initial-state feq (g[0] = rho0, F = u0).
 extern "C" int emu_adjoint_node(int mode, const lbm_config* c, const float* g, const float* lam, const float* F,

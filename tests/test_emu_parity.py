@@ -110,3 +110,27 @@ def test_emulated_two_step_reduced_storage(storage, layout):
     f3, *_ = run_emu(case, 6, two_ty=14, reverse=True)
     f4, *_ = run_emu(case, 6, two_ty=(8, 16), reverse=True)
     assert np.array_equal(f1, f2) and np.array_equal(f1, f3) and np.array_equal(f1, f4)
+
+
+@pytest.mark.parametrize("form", [0, 1], ids=["tiles", "strips"])
+@pytest.mark.parametrize("coll", [A.LBM_BGK, A.LBM_MRT])
+def test_emulated_two_step_intermediate_state(form, coll):
+    # The adjoint's segment recompute runs two-step launches that also write state t+1:
+    # both outputs must equal one and two single steps.
+    import ctypes
+
+    from helpers import emu_lib
+
+    case = layout_case("periodic", 70, 37, coll)
+    f0, *_ = run_emu(case, 0)
+    f1, *_ = run_emu(case, 1, seg=1)
+    f2, *_ = run_emu(case, 2, seg=1)
+    L = emu_lib()
+    L.emu_two_step_mid.restype = ctypes.c_int
+    L.emu_two_step_mid.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 3
+    fi = np.ascontiguousarray(f0, np.float32)
+    fm = np.zeros_like(fi)
+    fo = np.zeros_like(fi)
+    c = case.config.to_c()
+    assert L.emu_two_step_mid(ctypes.byref(c), form, fi.ctypes.data, fm.ctypes.data, fo.ctypes.data) == 0
+    assert np.array_equal(fm, f1) and np.array_equal(fo, f2)
