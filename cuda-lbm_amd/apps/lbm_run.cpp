@@ -1,6 +1,8 @@
 // lbm_run — C++ driver for a compile-time scenario (the role of src/main.cu): allocate,
 // init, run the fused steps through liblbm_hip.so and print the scenario's validation.
 // Build with -DRUN_TAYLOR_GREEN or -DRUN_LID_DRIVEN and -DNX=.. -DNY=.. .
+// Usage: lbm_run [steps [output_dir]]; with an output directory the final fields are also
+// written as .bin dumps and a .vti file (save_macroscopics / save_vtk).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -19,6 +21,7 @@ using Scenario = LidDrivenMovingWall;
 
 int main(int argc, char** argv) {
     const int steps = argc > 1 ? std::atoi(argv[1]) : 1000;
+    const char* out_dir = argc > 2 ? argv[2] : nullptr;
     try {
         LBM<dimensions> lbm;
         lbm.allocate<Scenario>();
@@ -35,6 +38,12 @@ int main(int argc, char** argv) {
             std::printf("centre u_x(y):");
             for (int y = 0; y < NY; y += NY / 8) std::printf(" %.5f", lbm.h_u[2 * (y * NX + NX / 2)]);
             std::printf("\n");
+        }
+        if (out_dir) {
+            lbm.save_macroscopics(lbm.timestep, out_dir);
+            lbm.save_vtk(lbm.timestep, out_dir);
+            std::printf("wrote %s/density/density_%d.bin, velocity_%d.bin, lbm_%d.vti\n", out_dir, lbm.timestep,
+                        lbm.timestep, lbm.timestep);
         }
     } catch (const std::exception& e) {
         std::fprintf(stderr, "lbm_run: %s\n", e.what());

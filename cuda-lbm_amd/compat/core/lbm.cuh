@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <filesystem>
 #include <fstream>
 #include <sstream>
 #include <stdexcept>
@@ -189,10 +190,30 @@ public:
         }
     }
 
+    // save_macroscopics (lbm.cuh:173-204): raw float32 dumps
+    // <dir>/density/density_<t>.bin (N values) and <dir>/velocity/velocity_<t>.bin (AoS u,
+    // 2N values: the "aos" layout of the reference's graphics/visualize.py reader).
+    void save_macroscopics(int t, const std::string& dir = "output") {
+        update_macroscopics();
+        namespace fs = std::filesystem;
+        fs::create_directories(dir + "/density");
+        fs::create_directories(dir + "/velocity");
+        const size_t n = size_t(NX) * NY;
+        std::ostringstream rn, vn;
+        rn << dir << "/density/density_" << t << ".bin";
+        vn << dir << "/velocity/velocity_" << t << ".bin";
+        std::ofstream rf(rn.str(), std::ios::binary);
+        std::ofstream vf(vn.str(), std::ios::binary);
+        if (!rf || !vf) throw std::runtime_error("cannot open " + rn.str() + " / " + vn.str());
+        rf.write(reinterpret_cast<const char*>(h_rho.data()), std::streamsize(n * sizeof(float)));
+        vf.write(reinterpret_cast<const char*>(h_u.data()), std::streamsize(2 * n * sizeof(float)));
+    }
+
     // save_vtk (lbm.cuh:269-343): VTK ImageData with appended raw Float32 "Density" and
     // 3-component "Velocity".
     void save_vtk(int t, const std::string& dir = "output") {
         update_macroscopics();
+        std::filesystem::create_directories(dir);
         std::ostringstream name;
         name << dir << "/lbm_" << t << ".vti";
         std::ofstream out(name.str(), std::ios::binary);

@@ -19,7 +19,9 @@ constexpr uint8_t kSupported[] = {LBM_FLUID,
                                   LBM_REGULARIZED_INLET_TOP,
                                   LBM_REGULARIZED_BOUNCE_BACK,
                                   LBM_REGULARIZED_BOUNCE_BACK_CORNER,
-                                  LBM_MOVING_WALL_BOUNCE_BACK};
+                                  LBM_MOVING_WALL_BOUNCE_BACK,
+                                  LBM_CYLINDER,
+                                  LBM_PRESSURE_OUTLET};
 
 bool flag_supported(uint8_t f) {
     for (uint8_t s : kSupported)
@@ -366,7 +368,22 @@ int lbm_set_flags(lbm_ctx* ctx, const uint8_t* flags, int on_device) {
                 return ctx->fail(LBM_E_UNSUPPORTED, "BC flag %d at node (%d,%d) is not supported", int(f), x, y);
             if (f == LBM_FLUID) continue;
             any = true;
-            if (f == LBM_ZG_OUTFLOW || f == LBM_REGULARIZED_BOUNCE_BACK_CORNER) local = false;
+            if (flag_reads_neighbour(f)) local = false;
+            if (f == LBM_PRESSURE_OUTLET) {
+                // reads the post-stream populations of (x - 1, y); as for ZG_OUTFLOW, none of
+                // them may be an unstreamed slot (another thread writes that slot)
+                if (x == 0 && !ctx->cfg.periodic_x)
+                    return ctx->fail(LBM_E_UNSUPPORTED, "PRESSURE_OUTLET at (%d,%d) has no node to its left", x, y);
+                const int ix = x == 0 ? nx - 1 : x - 1;
+                for (int i = 0; i < 9; i++) {
+                    int xs = ix - CX[i], ys = y - CY[i];
+                    bool okx = (xs >= 0 && xs < nx) || ctx->cfg.periodic_x;
+                    bool oky = (ys >= 0 && ys < ny) || ctx->cfg.periodic_y;
+                    if (!okx || !oky)
+                        return ctx->fail(LBM_E_UNSUPPORTED,
+                                         "PRESSURE_OUTLET at (%d,%d): the node to its left has an unstreamed population", x, y);
+                }
+            }
             if (f == LBM_ZG_OUTFLOW) {
                 // The interior neighbour's unknown populations must all stream from inside
                 // the domain: a stale slot of another node would be raced by its writer.

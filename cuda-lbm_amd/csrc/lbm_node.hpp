@@ -298,6 +298,28 @@ LBM_HD void bc_bounce_back(float* g, int x, int y, const Geom& G, bool moving, f
     }
 }
 
+// CylinderBoundary::apply (cylinderBoundary.cuh:22-34): full-way bounce-back, every
+// post-stream population sent back along its opposite direction.
+LBM_HD void bc_cylinder(float* g) {
+    float s[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) s[q] = g[q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) g[OPP[q]] = s[q];
+}
+
+// PressureOutlet::apply (pressureOutlet.cuh:7-42): f = f_eq(rho = 1, u_i), u_i the velocity
+// of the interior node's post-stream populations gi (the node to the left).
+LBM_HD void bc_pressure_outlet(float* g, const float* gi) {
+    const float r = (((((((gi[0] + gi[1]) + gi[2]) + gi[3]) + gi[4]) + gi[5]) + gi[6]) + gi[7]) + gi[8];
+    const float jx = ((((gi[1] - gi[3]) + gi[5]) - gi[6]) - gi[7]) + gi[8];
+    const float jy = ((((gi[2] - gi[4]) + gi[5]) + gi[6]) - gi[7]) - gi[8];
+    const float ux = jx / r, uy = jy / r;
+    const float usq15 = 1.5f * (ux * ux + uy * uy);
+#pragma unroll
+    for (int q = 0; q < Q; q++) g[q] = feq(q, 1.0f, ux, uy, usq15);
+}
+
 // Regularisation of the non-equilibrium part from Pi^(1) (Latt et al. 2008) shared by the
 // regularized functors: g_q = feq_q + w_q/(2cs^4) Q_q:Pi.
 LBM_HD void regularize(float* g, const float* fe, float pxx, float pyy, float pxy) {

@@ -380,8 +380,14 @@ LBM_HD float gather(const NodeArgs& p, int q, int x, int y) {
     return St::load1(src + q * p.plane + (long long)ys * p.nx + xs, q, p.rho0);
 }
 
+// Flags whose functor also reads a neighbour's post-stream populations: their nodes take the
+// one-step launch's per-node path (two-step launches do not cover them).
+LBM_HD bool flag_reads_neighbour(uint8_t f) {
+    return f == LBM_ZG_OUTFLOW || f == LBM_REGULARIZED_BOUNCE_BACK_CORNER || f == LBM_PRESSURE_OUTLET;
+}
+
 // Boundary functors that need only the node's own post-stream populations. Returns false
-// for the flags that also read a neighbour (ZG outflow, regularized corner).
+// for the flags that also read a neighbour (flag_reads_neighbour).
 LBM_HD bool apply_local_bc(float* g, uint8_t fl, int x, int y, const NodeArgs& p) {
     Geom G{p.nx, p.ny, p.px, p.py};
     switch (fl) {
@@ -392,6 +398,7 @@ LBM_HD bool apply_local_bc(float* g, uint8_t fl, int x, int y, const NodeArgs& p
         case LBM_ZOU_HE_LEFT: bc_zou_he_left(g, p.u_lid, (p.quirks & LBM_QUIRK_ZOUHE_RHO) != 0); return true;
         case LBM_REGULARIZED_INLET_TOP: bc_regularized_inlet_top(g, p.u_lid); return true;
         case LBM_REGULARIZED_BOUNCE_BACK: bc_regularized_bounce_back(g, x, y, G); return true;
+        case LBM_CYLINDER: bc_cylinder(g); return true;
         default: return false;
     }
 }
@@ -419,6 +426,14 @@ __host__ __device__ __noinline__ Pop9 gather_node(NodeArgs p, int x, int y) {
 #pragma unroll
             for (int i = 0; i < Q; i++)
                 if (CX[i] * n0 + CY[i] * n1 > 0) g[i] = gather<STORE>(p, i, ix, iy);
+            break;
+        }
+        case LBM_PRESSURE_OUTLET: {
+            const int ix = x == 0 ? p.nx - 1 : x - 1;  // x = 0 only on periodic-x grids
+            float gi[Q];
+#pragma unroll
+            for (int i = 0; i < Q; i++) gi[i] = gather<STORE>(p, i, ix, y);
+            bc_pressure_outlet(g, gi);
             break;
         }
         case LBM_REGULARIZED_BOUNCE_BACK_CORNER: {

@@ -63,9 +63,9 @@ enum lbm_bc_flag {
     LBM_ZOU_HE_LEFT,                   /* zouHeInflow.cuh:9-32                         */
     LBM_ZOU_HE_TOP_LEFT_TOP_INFLOW,    /* not implemented in the reference either      */
     LBM_ZOU_HE_TOP_RIGHT_TOP_INFLOW,   /* not implemented in the reference either      */
-    LBM_CYLINDER,
+    LBM_CYLINDER,                      /* cylinderBoundary.cuh:22-34 (full-way BB)     */
     LBM_ZG_OUTFLOW,                    /* zeroGradientOutflow.cuh:9-59                 */
-    LBM_PRESSURE_OUTLET,
+    LBM_PRESSURE_OUTLET,               /* pressureOutlet.cuh:7-42                      */
     LBM_REGULARIZED_INLET_TOP,         /* regularizedInlet.cuh:15-69                   */
     LBM_REGULARIZED_INLET_LEFT,
     LBM_REGULARIZED_BOUNCE_BACK,       /* regularizedBounceBack.cuh:13-100             */

@@ -631,6 +631,39 @@ struct Solver {
         g[8] = g[6] - R(0.5) * d13 + (R(1) / R(2)) * r * ux - (R(1) / R(6)) * r * uy;
     }
 
+    // CylinderBoundary::apply (cylinderBoundary.cuh:22-34) with SoA indexing (quirk Q8
+    // fixed): every post-stream population is sent back along its opposite direction.
+    void bc_cylinder(R* g) const {
+        R saved[Q];
+        for (int j = 0; j < Q; j++) saved[j] = g[j];
+        for (int i = 0; i < Q; i++) g[OPP[i]] = saved[i];
+    }
+
+    // PressureOutlet::apply (pressureOutlet.cuh:7-42) with SoA indexing (quirk Q8 fixed):
+    // the velocity of the node to the left (x - 1, periodic wrap in x) from its post-stream
+    // populations, then f = f_eq(rho = 1, u) written as in that functor.
+    void bc_pressure_outlet(R* g, int x, int y, const std::vector<R>& snap) const {
+        int ix = x - 1;
+        if (ix < 0) ix += nx;
+        size_t in = size_t(y) * nx + ix;
+        R rho_i = 0;
+        for (int i = 0; i < Q; i++) rho_i += snap[i * N + in];
+        R ux = 0, uy = 0;
+        for (int i = 0; i < Q; i++) {
+            ux += snap[i * N + in] * R(CX[i]);
+            uy += snap[i * N + in] * R(CY[i]);
+        }
+        ux /= rho_i;
+        uy /= rho_i;
+        const R target_rho = R(1);
+        for (int i = 0; i < Q; i++) {
+            R cu = R(CX[i]) * ux + R(CY[i]) * uy;
+            R uu = ux * ux + uy * uy;
+            R cs2 = R(1) / R(3);
+            g[i] = weight<R>(i) * target_rho * (R(1) + cu / cs2 + (cu * cu) / (R(2) * cs2 * cs2) - uu / (R(2) * cs2));
+        }
+    }
+
     // boundaries_kernel_2D (boundaries.cuh:10-85): switch on the node's flag.
     void apply_boundaries() {
         std::vector<R> snap = f;
@@ -651,6 +684,8 @@ struct Solver {
                     case LBM_REGULARIZED_INLET_TOP: bc_regularized_inlet_top(g); break;
                     case LBM_REGULARIZED_BOUNCE_BACK: bc_regularized_bounce_back(g, x, y); break;
                     case LBM_REGULARIZED_BOUNCE_BACK_CORNER: bc_regularized_corner(g, x, y, snap); break;
+                    case LBM_CYLINDER: bc_cylinder(g); break;
+                    case LBM_PRESSURE_OUTLET: bc_pressure_outlet(g, x, y, snap); break;
                     default: break;
                 }
                 for (int q = 0; q < Q; q++) f[q * N + node] = g[q];

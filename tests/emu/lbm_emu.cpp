@@ -130,7 +130,7 @@ int run_t(const lbm_config& c, const uint8_t* flags, const float* rho, const flo
     bool local = true;
     if (fl)
         for (long long n = 0; n < N; n++)
-            local = local && fl[n] != LBM_ZG_OUTFLOW && fl[n] != LBM_REGULARIZED_BOUNCE_BACK_CORNER;
+            local = local && !flag_reads_neighbour(fl[n]);
     const bool two = two_ty != 0 && !adapt && !(c.quirks & LBM_QUIRK_STALE_REST) && local;
     const int n_edge = edge_count(nx, ny, c.periodic_x, c.periodic_y);
     std::vector<T> edge[2] = {std::vector<T>(size_t(9) * n_edge), std::vector<T>(size_t(9) * n_edge)};
@@ -345,7 +345,7 @@ int main() {
     const int sizes[][2] = {{3, 3}, {5, 4}, {8, 8}, {13, 7}, {16, 9}, {33, 17}, {64, 5}, {70, 11}, {130, 9}};
     int cases = 0;
     for (auto& sz : sizes) {
-        for (int layout = 0; layout < 5; layout++) {
+        for (int layout = 0; layout < 6; layout++) {
             for (int coll = 0; coll < 3; coll++) {
                 for (int storage = 0; storage < 3; storage++) {
                     for (int adapt = 0; adapt < (coll == LBM_CM ? 2 : 1); adapt++) {
@@ -388,6 +388,12 @@ int main() {
                                         else if (x == 0) f = LBM_ZOU_HE_LEFT;
                                         else if (x == c.nx - 1) f = LBM_ZG_OUTFLOW;
                                         else if (x == c.nx / 2 && y == c.ny / 2) f = LBM_BOUNCE_BACK;
+                                        break;
+                                    case 5:  // pressure outlet, full-way bounce-back block
+                                        if (ye) f = LBM_BOUNCE_BACK;
+                                        else if (x == 0) f = LBM_ZOU_HE_LEFT;
+                                        else if (x == c.nx - 1) f = LBM_PRESSURE_OUTLET;
+                                        else if (x == c.nx / 3 && y == c.ny / 2) f = LBM_CYLINDER;
                                         break;
                                 }
                             }

@@ -88,7 +88,8 @@ def layout_case(layout: str, nx: int, ny: int, collision: int, storage: int = A.
 
     layouts: periodic, channel (periodic x, bounce-back walls), cavity_mw (moving-wall lid),
     cavity_reg (the reference's regularized cavity), inout (Zou-He inlet, ZG outlet, walls,
-    one solid node), open (no flags, non-periodic: the reference's unstreamed edge slots).
+    one solid node), open (no flags, non-periodic: the reference's unstreamed edge slots),
+    cyl_outlet (Zou-He inlet, pressure outlet, walls, a block of full-way bounce-back nodes).
     """
     rng = np.random.default_rng(seed)
     om = 1.25
@@ -123,6 +124,13 @@ def layout_case(layout: str, nx: int, ny: int, collision: int, storage: int = A.
         flags[-1, 1:-1] = A.ZOU_HE_TOP
     elif layout == "open":
         pass
+    elif layout == "cyl_outlet":
+        # Zou-He inlet, pressure outlet, bounce-back walls, a full-way bounce-back block
+        flags[:, 0] = A.ZOU_HE_LEFT
+        flags[:, -1] = A.PRESSURE_OUTLET
+        flags[0, :] = flags[-1, :] = A.BOUNCE_BACK
+        cy, cx = ny // 2, nx // 3
+        flags[max(cy - 2, 1):cy + 2, max(cx - 2, 1):cx + 2] = A.CYLINDER
     else:
         raise ValueError(layout)
     cfg = Config(nx=nx, ny=ny, collision=collision, adapter=adapter, omega=om, S=S, storage=storage,

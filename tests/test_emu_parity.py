@@ -12,7 +12,7 @@ import pytest
 from helpers import layout_case, max_rel, run_emu, run_oracle
 from lbm_amd import _abi as A
 
-LAYOUTS = ["periodic", "channel", "cavity_mw", "cavity_reg", "inout", "zouhe_top", "open"]
+LAYOUTS = ["periodic", "channel", "cavity_mw", "cavity_reg", "inout", "zouhe_top", "open", "cyl_outlet"]
 OPS = [(A.LBM_BGK, 0), (A.LBM_MRT, 0), (A.LBM_CM, 0), (A.LBM_CM, 1)]
 # nodes per thread: the wide segments of HBM-resident grids and the narrow ones of
 # cache-resident grids (FP32 1 node, 16-bit 2 nodes)

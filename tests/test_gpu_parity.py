@@ -11,7 +11,7 @@ from lbm_amd import scenarios as sc
 
 pytestmark = pytest.mark.gpu
 
-LAYOUTS = ["periodic", "channel", "cavity_mw", "cavity_reg", "inout", "zouhe_top", "open"]
+LAYOUTS = ["periodic", "channel", "cavity_mw", "cavity_reg", "inout", "zouhe_top", "open", "cyl_outlet"]
 OPS = [(A.LBM_BGK, 0), (A.LBM_MRT, 0), (A.LBM_CM, 0), (A.LBM_CM, 1)]
 
 
@@ -130,7 +130,7 @@ def test_gpu_errors_are_reported(gpu_available):
     s = Solver(Config(nx=8, ny=8))
     with pytest.raises(LbmError):
         s.step(1)  # not initialised
-    flags = np.full((8, 8), A.PRESSURE_OUTLET, np.uint8)
+    flags = np.full((8, 8), A.GUO_PRESSURE_OUTLET, np.uint8)
     with pytest.raises(LbmError):
         s.set_flags(flags)
     s.close()
