@@ -186,9 +186,7 @@ LBM_HD float optimal_adapter_rate(float rho, float j_mag, float pi_mag, const fl
 // change dk, so f* = f + T^-1(u) dk. `avg` != nullptr selects OptimalAdapter for k6..k8.
 LBM_HD void collide_cm(float* g, const Macro& m, float Fx, float Fy, const float* S, const float* avg) {
     const float ux = m.ux, uy = m.uy;
-    // rows by c_y: (c_x = -1, 0, +1)
-    const float* row_m = nullptr;
-    (void)row_m;
+    // populations grouped by c_x, each ordered c_y = -1, 0, +1
     float fm[3] = {g[7], g[3], g[6]};  // c_x = -1 : c_y = -1, 0, +1
     float f0[3] = {g[4], g[0], g[2]};  // c_x =  0
     float fp[3] = {g[8], g[1], g[5]};  // c_x = +1
