@@ -73,9 +73,10 @@ __global__ __launch_bounds__(Tile2<TY>::THREADS) void fused_step2_kernel(StepPar
     if (lb >= n_tiles) return;  // whole workgroup: remap padding
     const int by = int(lb / tiles_x), bx = int(lb - (long long)by * tiles_x);
     const int x0 = bx * kT2W, y0 = by * TY;
-    step2_phase_a<COLL, STORE, TY>(p, tile, threadIdx.x, x0, y0);
+    const bool fast_x = clear_of_x_edges(p, x0);
+    step2_phase_a<COLL, STORE, TY>(p, tile, threadIdx.x, x0, y0, fast_x);
     __syncthreads();
-    step2_phase_b<COLL, STORE, TY>(p, tile, threadIdx.x, x0, y0);
+    step2_phase_b<COLL, STORE, TY>(p, tile, threadIdx.x, x0, y0, fast_x);
 }
 
 // Two time steps per launch, marching strips; see step2s_phase_a / step2s_phase_b.
@@ -92,10 +93,11 @@ __global__ __launch_bounds__(kT2L * W) void fused_step2s_kernel(StepParams p, lo
     const int hs = p.ny - y0 < H ? p.ny - y0 : H;  // output rows of this strip
     const int l = threadIdx.x & (kT2L - 1), w = threadIdx.x / kT2L;
     const int K = (hs + 1) / W + 1;  // chunks: phase-A rows 0 .. hs + 1, phase-B rows 0 .. hs - 1
+    const bool fast_x = clear_of_x_edges(p, x0);
     float g[Q];
     int i = w, gx = 0, gy = 0;
     bool va = i <= hs + 1 && strip_node(p, x0, y0, l, i, gx, gy);
-    if (va) step2_gather<STORE>(p, gx, gy, g);
+    if (va) step2_gather<STORE>(p, gx, gy, g, fast_x && clear_of_y_edges(p, gy));
     for (int k = 0; k < K; k++) {
         if (va) step2s_phase_a<COLL, STORE>(p, ring, R, i % R, l, i, x0, y0, hs, gx, gy, g);
         __syncthreads();
@@ -104,9 +106,9 @@ __global__ __launch_bounds__(kT2L * W) void fused_step2s_kernel(StepParams p, lo
         int nxg = 0, nyg = 0;
         const bool vn = k + 1 < K && in <= hs + 1 && strip_node(p, x0, y0, l, in, nxg, nyg);
         float gn[Q];
-        if (vn) step2_gather<STORE>(p, nxg, nyg, gn);
+        if (vn) step2_gather<STORE>(p, nxg, nyg, gn, fast_x && clear_of_y_edges(p, nyg));
         const int j = k * W - 2 + w;
-        if (j >= 0 && j < hs) step2s_phase_b<COLL, STORE>(p, ring, R, l, j, x0, y0);
+        if (j >= 0 && j < hs) step2s_phase_b<COLL, STORE>(p, ring, R, l, j, x0, y0, fast_x);
         __syncthreads();
         i = in;
         va = vn;

@@ -110,7 +110,8 @@ LBM_HD void collide_bgk(float* g, const Macro& m, float fx, float fy, float omeg
 // MRT<2>::apply (MRT.cu:26-76) in delta form f* = f + M^-1 dm, with the Lallemand-Luo
 // basis (lbm_constants.cuh:33-55) written out sparsely and m_eq in closed form.
 // Forcing moments per MRT.cu:4-24; `literal_rows` keeps its rows 4/5 swap (quirk Q10).
-LBM_HD void collide_mrt(float* g, const Macro& m, float fx, float fy, const float* S, bool literal_rows) {
+// `forced` (uniform) = 0 skips the forcing moments, which are then all zero.
+LBM_HD void collide_mrt(float* g, const Macro& m, float fx, float fy, const float* S, bool literal_rows, bool forced) {
     const float rho = m.rho, ux = m.ux, uy = m.uy;
     float a = g[1] + g[2] + g[3] + g[4];
     float d = g[5] + g[6] + g[7] + g[8];
@@ -136,15 +137,19 @@ LBM_HD void collide_mrt(float* g, const Macro& m, float fx, float fy, const floa
                     -rho * uy,
                     rho * (ux * ux - uy * uy),
                     rho * ux * uy};
-    float fu = fx * ux + fy * uy;
-    float F[Q] = {0.0f, 6.0f * fu, -6.0f * fu, fx, -fx, fy, -fy, 2.0f * (fx * ux - fy * uy), fx * uy + fy * ux};
-    if (literal_rows) {
-        F[4] = fy;
-        F[5] = -fx;
-    }
     float dm[Q];
 #pragma unroll
-    for (int k = 0; k < Q; k++) dm[k] = -S[k] * (mm[k] - meq[k]) + (1.0f - 0.5f * S[k]) * F[k];
+    for (int k = 0; k < Q; k++) dm[k] = -S[k] * (mm[k] - meq[k]);
+    if (forced) {
+        float fu = fx * ux + fy * uy;
+        float F[Q] = {0.0f, 6.0f * fu, -6.0f * fu, fx, -fx, fy, -fy, 2.0f * (fx * ux - fy * uy), fx * uy + fy * ux};
+        if (literal_rows) {
+            F[4] = fy;
+            F[5] = -fx;
+        }
+#pragma unroll
+        for (int k = 0; k < Q; k++) dm[k] += (1.0f - 0.5f * S[k]) * F[k];
+    }
     // M^-1 rows (lbm_constants.cuh:45-55)
     const float r9 = 1.0f / 9.0f, r18 = 1.0f / 18.0f, r36 = 1.0f / 36.0f, r6 = 1.0f / 6.0f, r12 = 1.0f / 12.0f;
     float c0 = dm[0] * r9;

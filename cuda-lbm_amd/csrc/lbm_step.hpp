@@ -493,7 +493,7 @@ LBM_HD void node_update(const StepParams& p, float* g, float fx, float fy, Macro
     }
     if (MODE == 1) return;
     if (COLL == LBM_BGK) collide_bgk(g, m, fx, fy, p.omega, p.forced != 0);
-    else if (COLL == LBM_MRT) collide_mrt(g, m, fx, fy, p.S, (p.quirks & LBM_QUIRK_MRT_FORCE_ROWS) != 0);
+    else if (COLL == LBM_MRT) collide_mrt(g, m, fx, fy, p.S, (p.quirks & LBM_QUIRK_MRT_FORCE_ROWS) != 0, p.forced != 0);
     else collide_cm(g, m, fx, fy, p.S, p.avg);
 }
 
@@ -611,8 +611,22 @@ struct Tile2 {
     static constexpr int ROWS = TY + 2, NH = kT2L * ROWS, THREADS = NH;
 };
 
+// True when the columns x0 - 1 .. x0 + kT2W of a tile or strip hold no node of a
+// non-periodic x edge (uniform over the workgroup).
+LBM_HD bool clear_of_x_edges(const StepParams& p, int x0) { return p.px || (x0 - 1 > 0 && x0 + kT2W < p.nx - 1); }
+// True when grid row y (in the domain) is not a non-periodic y edge (uniform over a
+// wavefront: one row per wavefront in both two-step forms).
+LBM_HD bool clear_of_y_edges(const StepParams& p, int y) { return p.py || (y > 0 && y < p.ny - 1); }
+
+// Pull sources of phase-A node (gx, gy) (already wrapped and inside the domain); defined
+// with the strip form below.
+template <int STORE>
+LBM_HD void step2_gather(const StepParams& p, int gx, int gy, float* g, bool fast);
+
+// Phase A of a tile, thread t: node (x0 - 1 + t % 64, y0 - 1 + t / 64) at state t+1 into
+// the tile. fast_x = clear_of_x_edges(p, x0).
 template <int COLL, int STORE, int TY>
-LBM_HD void step2_phase_a(const StepParams& p, typename Elem<STORE>::T* tile, int t, int x0, int y0) {
+LBM_HD void step2_phase_a(const StepParams& p, typename Elem<STORE>::T* tile, int t, int x0, int y0, bool fast_x) {
     using St = Elem<STORE>;
     using T = typename St::T;
     constexpr int NH = Tile2<TY>::NH;
@@ -630,33 +644,10 @@ LBM_HD void step2_phase_a(const StepParams& p, typename Elem<STORE>::T* tile, in
     const uint32_t node = uint32_t(gy) * uint32_t(p.nx) + uint32_t(gx);
     const uint8_t fl = p.flags ? p.flags[node] : uint8_t(LBM_FLUID);
     const bool ex = !p.px && (gx == 0 || gx == p.nx - 1), ey = !p.py && (gy == 0 || gy == p.ny - 1);
-    // Pull sources with the periodic wrap; a source outside a non-periodic domain reads the
-    // node's unstreamed slot, state t-1, from the edge store. Plane bases are wave-uniform;
-    // per-lane offsets are 32-bit (N < 2^31).
-    const T* src = static_cast<const T*>(p.src);
-    const int xm = gx == 0 ? (p.px ? p.nx - 1 : -1) : gx - 1, xp = gx == p.nx - 1 ? (p.px ? 0 : -1) : gx + 1;
-    const int ym = gy == 0 ? (p.py ? p.ny - 1 : -1) : gy - 1, yp = gy == p.ny - 1 ? (p.py ? 0 : -1) : gy + 1;
-    const uint32_t rm = uint32_t(ym) * uint32_t(p.nx), r0 = uint32_t(gy) * uint32_t(p.nx), rp = uint32_t(yp) * uint32_t(p.nx);
+    // pull sources; a source outside a non-periodic domain reads the node's unstreamed slot,
+    // state t-1, from the edge store
     float g[Q];
-    if (!ex && !ey) {
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            const uint32_t xs = uint32_t(CX[q] > 0 ? xm : (CX[q] < 0 ? xp : gx));
-            const uint32_t row = CY[q] > 0 ? rm : (CY[q] < 0 ? rp : r0);
-            g[q] = St::load1(src + q * p.plane + (row + xs), q, p.rho0);
-        }
-    } else {
-        const T* prev = static_cast<const T*>(p.edge_prev);
-        const uint32_t eslot = uint32_t(edge_slot(gx, gy, p.nx, p.ny, p.px, p.py));
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            const int xs = CX[q] > 0 ? xm : (CX[q] < 0 ? xp : gx);
-            const int ys = CY[q] > 0 ? ym : (CY[q] < 0 ? yp : gy);
-            const uint32_t row = CY[q] > 0 ? rm : (CY[q] < 0 ? rp : r0);
-            g[q] = (xs < 0 || ys < 0) ? St::load1(prev + q * p.n_edge + eslot, q, p.rho0)
-                                      : St::load1(src + q * p.plane + (row + uint32_t(xs)), q, p.rho0);
-        }
-    }
+    step2_gather<STORE>(p, gx, gy, g, fast_x && clear_of_y_edges(p, gy));
     if (fl != LBM_FLUID) apply_local_bc(g, fl, gx, gy, node_args(p));
     float fx, fy;
     node_force(p, gx, gy, node, fx, fy);
@@ -686,7 +677,7 @@ LBM_HD void step2_phase_a(const StepParams& p, typename Elem<STORE>::T* tile, in
 }
 
 template <int COLL, int STORE, int TY>
-LBM_HD void step2_phase_b(const StepParams& p, const typename Elem<STORE>::T* tile, int t, int x0, int y0) {
+LBM_HD void step2_phase_b(const StepParams& p, const typename Elem<STORE>::T* tile, int t, int x0, int y0, bool fast_x) {
     using St = Elem<STORE>;
     using T = typename St::T;
     constexpr int NH = Tile2<TY>::NH;
@@ -696,9 +687,8 @@ LBM_HD void step2_phase_b(const StepParams& p, const typename Elem<STORE>::T* ti
     if (x >= p.nx || y >= p.ny) return;
     const uint32_t node = uint32_t(y) * uint32_t(p.nx) + uint32_t(x);
     const T* src = static_cast<const T*>(p.src);
-    const bool ex = !p.px && (x == 0 || x == p.nx - 1), ey = !p.py && (y == 0 || y == p.ny - 1);
     float g[Q];
-    if (!ex && !ey) {
+    if (fast_x && clear_of_y_edges(p, y)) {
 #pragma unroll
         for (int q = 0; q < Q; q++) g[q] = St::load1(tile + q * NH + t - CY[q] * kT2L - CX[q], q, p.rho0);
     } else {
@@ -739,9 +729,12 @@ LBM_HD void step2_phase_b(const StepParams& p, const typename Elem<STORE>::T* ti
 // Unstreamed slots are handled as in the tile form (edge store for phase A, source buffer
 // for phase B).
 
-// Pull sources of phase-A node (gx, gy) (already wrapped and inside the domain).
+// Pull sources of phase-A node (gx, gy) (already wrapped and inside the domain). `fast`,
+// the same for a whole wavefront, promises that no node of the wavefront lies on a
+// non-periodic domain edge; otherwise every node takes the edge-aware path (which loads the
+// same values for interior nodes), so a boundary wavefront runs one gather, not both.
 template <int STORE>
-LBM_HD void step2_gather(const StepParams& p, int gx, int gy, float* g) {
+LBM_HD void step2_gather(const StepParams& p, int gx, int gy, float* g, bool fast) {
     using St = Elem<STORE>;
     using T = typename St::T;
     const bool ex = !p.px && (gx == 0 || gx == p.nx - 1), ey = !p.py && (gy == 0 || gy == p.ny - 1);
@@ -749,7 +742,7 @@ LBM_HD void step2_gather(const StepParams& p, int gx, int gy, float* g) {
     const int xm = gx == 0 ? (p.px ? p.nx - 1 : -1) : gx - 1, xp = gx == p.nx - 1 ? (p.px ? 0 : -1) : gx + 1;
     const int ym = gy == 0 ? (p.py ? p.ny - 1 : -1) : gy - 1, yp = gy == p.ny - 1 ? (p.py ? 0 : -1) : gy + 1;
     const uint32_t rm = uint32_t(ym) * uint32_t(p.nx), r0 = uint32_t(gy) * uint32_t(p.nx), rp = uint32_t(yp) * uint32_t(p.nx);
-    if (!ex && !ey) {
+    if (fast) {
 #pragma unroll
         for (int q = 0; q < Q; q++) {
             const uint32_t xs = uint32_t(CX[q] > 0 ? xm : (CX[q] < 0 ? xp : gx));
@@ -758,7 +751,7 @@ LBM_HD void step2_gather(const StepParams& p, int gx, int gy, float* g) {
         }
     } else {
         const T* prev = static_cast<const T*>(p.edge_prev);
-        const uint32_t eslot = uint32_t(edge_slot(gx, gy, p.nx, p.ny, p.px, p.py));
+        const uint32_t eslot = (ex || ey) ? uint32_t(edge_slot(gx, gy, p.nx, p.ny, p.px, p.py)) : 0u;
 #pragma unroll
         for (int q = 0; q < Q; q++) {
             const int xs = CX[q] > 0 ? xm : (CX[q] < 0 ? xp : gx);
@@ -828,7 +821,7 @@ LBM_HD void step2s_phase_a(const StepParams& p, typename Elem<STORE>::T* ring, i
 // Phase B of strip row j (grid row y0 + j), lane l: pulls from ring rows j, j+1, j+2.
 template <int COLL, int STORE>
 LBM_HD void step2s_phase_b(const StepParams& p, const typename Elem<STORE>::T* ring, int ring_rows, int l, int j,
-                           int x0, int y0) {
+                           int x0, int y0, bool fast_x) {
     using St = Elem<STORE>;
     using T = typename St::T;
     if (l < 1 || l > kT2W) return;
@@ -839,13 +832,13 @@ LBM_HD void step2s_phase_b(const StepParams& p, const typename Elem<STORE>::T* r
     const int stride = ring_rows * kT2L;
     // ring rows of strip rows j (below), j + 1 (own), j + 2 (above)
     const int s_lo = j % ring_rows, s_mid = (j + 1) % ring_rows, s_hi = (j + 2) % ring_rows;
-    const bool ex = !p.px && (x == 0 || x == p.nx - 1), ey = !p.py && (y == 0 || y == p.ny - 1);
+    const bool fast = fast_x && clear_of_y_edges(p, y);  // fast_x = clear_of_x_edges(p, x0)
     float g[Q];
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         const int srow = CY[q] > 0 ? s_lo : (CY[q] < 0 ? s_hi : s_mid);
         const T* a = ring + q * stride + srow * kT2L + (l - CX[q]);
-        if (!ex && !ey) {
+        if (fast) {
             g[q] = St::load1(a, q, p.rho0);
         } else {
             const int xs = x - CX[q], ys = y - CY[q];
