@@ -63,7 +63,8 @@ __global__ __launch_bounds__(kMaxStepBlock) void fused_step_kernel(StepParams p,
     }
 }
 
-// Two time steps per launch; see step2_phase_a / step2_phase_b.
+// Two time steps per launch; see step2_phase_a / step2_phase_b. Tiles serve cache-resident
+// grids only, so their arrays always fit one buffer resource.
 template <int COLL, int STORE, int TY>
 __global__ __launch_bounds__(Tile2<TY>::THREADS) void fused_step2_kernel(StepParams p, long long nb_padded, int remap,
                                                                          int tiles_x, long long n_tiles) {
@@ -73,14 +74,14 @@ __global__ __launch_bounds__(Tile2<TY>::THREADS) void fused_step2_kernel(StepPar
     if (lb >= n_tiles) return;  // whole workgroup: remap padding
     const int by = int(lb / tiles_x), bx = int(lb - (long long)by * tiles_x);
     const int x0 = bx * kT2W, y0 = by * TY;
-    const bool fast_x = clear_of_x_edges(p, x0);
-    step2_phase_a<COLL, STORE, TY>(p, tile, threadIdx.x, x0, y0, fast_x);
+    step2_phase_a<COLL, STORE, TY, true>(p, tile, threadIdx.x, x0, y0, interior_cols(p, x0));
     __syncthreads();
-    step2_phase_b<COLL, STORE, TY>(p, tile, threadIdx.x, x0, y0, fast_x);
+    step2_phase_b<COLL, STORE, TY, true>(p, tile, threadIdx.x, x0, y0, clear_of_x_edges(p, x0));
 }
 
-// Two time steps per launch, marching strips; see step2s_phase_a / step2s_phase_b.
-template <int COLL, int STORE, int W>
+// Two time steps per launch, marching strips; see step2s_phase_a / step2s_phase_b. BUF:
+// population arrays addressed through buffer resources (arrays below 4 GiB).
+template <int COLL, int STORE, int W, bool BUF>
 __global__ __launch_bounds__(kT2L * W) void fused_step2s_kernel(StepParams p, long long nb_padded, int remap,
                                                                  int strips_x, long long n_strips, int H) {
     using T = typename Elem<STORE>::T;
@@ -93,22 +94,30 @@ __global__ __launch_bounds__(kT2L * W) void fused_step2s_kernel(StepParams p, lo
     const int hs = p.ny - y0 < H ? p.ny - y0 : H;  // output rows of this strip
     const int l = threadIdx.x & (kT2L - 1), w = threadIdx.x / kT2L;
     const int K = (hs + 1) / W + 1;  // chunks: phase-A rows 0 .. hs + 1, phase-B rows 0 .. hs - 1
-    const bool fast_x = clear_of_x_edges(p, x0);
+    const bool interior_x = interior_cols(p, x0), clear_x = clear_of_x_edges(p, x0);
     float g[Q];
     int i = w, gx = 0, gy = 0;
     bool va = i <= hs + 1 && strip_node(p, x0, y0, l, i, gx, gy);
-    if (va) step2_gather<STORE>(p, gx, gy, g, fast_x && clear_of_y_edges(p, gy));
+    if (va) step2_gather<STORE, BUF>(p, gx, gy, g, interior_x && interior_row(p, y0 - 1 + i));
+    StepParams pl = p;
     for (int k = 0; k < K; k++) {
-        if (va) step2s_phase_a<COLL, STORE>(p, ring, R, i % R, l, i, x0, y0, hs, gx, gy, g);
+        // Opaque per iteration: values derived from these (plane offsets, pull shifts,
+        // buffer resources) are then rebuilt inside the march with scalar ALU instead of
+        // being held across it, which exceeds the SGPR file and spills to VGPR lanes.
+        asm volatile("" : "+s"(pl.plane), "+s"(pl.nx), "+s"(pl.n_edge), "+s"(pl.src), "+s"(pl.dst), "+s"(pl.mid),
+                     "+s"(pl.edge_prev), "+s"(pl.edge_next));
+        asm volatile("" : "+s"(pl.px), "+s"(pl.py), "+s"(pl.ny), "+s"(pl.flags), "+s"(pl.forced), "+s"(pl.force_field),
+                     "+s"(pl.ibm_force), "+s"(pl.out_rho), "+s"(pl.quirks));
+        if (va) step2s_phase_a<COLL, STORE, BUF>(pl, ring, R, i % R, l, i, x0, y0, hs, gx, gy, g);
         __syncthreads();
         // the next chunk's pull sources, loaded before this chunk's phase B
         const int in = (k + 1) * W + w;
         int nxg = 0, nyg = 0;
         const bool vn = k + 1 < K && in <= hs + 1 && strip_node(p, x0, y0, l, in, nxg, nyg);
         float gn[Q];
-        if (vn) step2_gather<STORE>(p, nxg, nyg, gn, fast_x && clear_of_y_edges(p, nyg));
+        if (vn) step2_gather<STORE, BUF>(pl, nxg, nyg, gn, interior_x && interior_row(pl, y0 - 1 + in));
         const int j = k * W - 2 + w;
-        if (j >= 0 && j < hs) step2s_phase_b<COLL, STORE>(p, ring, R, l, j, x0, y0, fast_x);
+        if (j >= 0 && j < hs) step2s_phase_b<COLL, STORE, BUF>(pl, ring, R, l, j, x0, y0, clear_x);
         __syncthreads();
         i = in;
         va = vn;
@@ -355,21 +364,21 @@ long long step2_grid_blocks(const StepParams& p, const Step2Policy& lp) {
     return lp.remap ? (tiles + kXcds - 1) / kXcds * kXcds : tiles;
 }
 
-template <int COLL, int STORE, int W>
+template <int COLL, int STORE, int W, bool BUF>
 static hipError_t launch_step2s_t(const StepParams& p, const Step2Policy& lp, hipStream_t s) {
     const int strips_x = (p.nx + kT2W - 1) / kT2W;
     const long long n_strips = (long long)strips_x * ((p.ny + lp.h - 1) / lp.h);
     const long long nb = step2_grid_blocks(p, lp);
-    hipLaunchKernelGGL((fused_step2s_kernel<COLL, STORE, W>), dim3(unsigned(nb)), dim3(kT2L * W), 0, s, p, nb, lp.remap,
-                       strips_x, n_strips, lp.h);
+    hipLaunchKernelGGL((fused_step2s_kernel<COLL, STORE, W, BUF>), dim3(unsigned(nb)), dim3(kT2L * W), 0, s, p, nb,
+                       lp.remap, strips_x, n_strips, lp.h);
     return hipGetLastError();
 }
 
-template <int STORE, int W>
+template <int STORE, int W, bool BUF>
 static hipError_t launch_step2s_c(int coll, const StepParams& p, const Step2Policy& lp, hipStream_t s) {
-    if (coll == LBM_BGK) return launch_step2s_t<LBM_BGK, STORE, W>(p, lp, s);
-    if (coll == LBM_MRT) return launch_step2s_t<LBM_MRT, STORE, W>(p, lp, s);
-    return launch_step2s_t<LBM_CM, STORE, W>(p, lp, s);
+    if (coll == LBM_BGK) return launch_step2s_t<LBM_BGK, STORE, W, BUF>(p, lp, s);
+    if (coll == LBM_MRT) return launch_step2s_t<LBM_MRT, STORE, W, BUF>(p, lp, s);
+    return launch_step2s_t<LBM_CM, STORE, W, BUF>(p, lp, s);
 }
 
 template <int COLL, int STORE, int TY>
@@ -391,12 +400,17 @@ static hipError_t launch_step2_c(int coll, const StepParams& p, const Step2Polic
 
 template <int STORE>
 static hipError_t launch_step2_s(int coll, const StepParams& p, const Step2Policy& lp, hipStream_t s) {
+    using T = typename Elem<STORE>::T;
+    // population buffers (and the intermediate-state buffer) span 9 planes; edge stores are
+    // far smaller
+    const bool fits = buf_fits<T>(Q, p.plane);
     if (lp.strip) {
         if (lp.h < 1) return hipErrorInvalidValue;
-        if (lp.w == 4) return launch_step2s_c<STORE, 4>(coll, p, lp, s);
-        if (lp.w == 8) return launch_step2s_c<STORE, 8>(coll, p, lp, s);
+        if (lp.w == 4) return fits ? launch_step2s_c<STORE, 4, true>(coll, p, lp, s) : launch_step2s_c<STORE, 4, false>(coll, p, lp, s);
+        if (lp.w == 8 && fits) return launch_step2s_c<STORE, 8, true>(coll, p, lp, s);
         return hipErrorInvalidValue;
     }
+    if (!fits) return hipErrorInvalidValue;  // tiles serve cache-resident grids only
     if (lp.ty == 6) return launch_step2_c<STORE, 6>(coll, p, lp, s);
     if (lp.ty == 14) return launch_step2_c<STORE, 14>(coll, p, lp, s);
     return hipErrorInvalidValue;

@@ -205,6 +205,8 @@ struct Storage<LBM_F32, V_> {
         __builtin_memcpy(d, r.v, 4 * V);
         store_dwords<V>(p, d, nt);
     }
+    LBM_HD static float dec1(T raw, int, float) { return raw; }
+    LBM_HD static T enc1(float v, int, float) { return v; }
     LBM_HD static float load1(const T* p, int, float) { return *p; }
     LBM_HD static void store1(T* p, int, float, float v) { *p = v; }
 };
@@ -258,8 +260,10 @@ struct Shifted16 {
         w = (j & 1) ? ((w & 0xffffu) | (h << 16)) : ((w & 0xffff0000u) | h);
     }
     LBM_HD static void store_vec(T* p, const Vec& r, bool nt) { store_dwords<W>(p, r.d, nt); }
-    LBM_HD static float load1(const T* p, int q, float rho0) { return dec(*p) + wgt(q) * rho0; }
-    LBM_HD static void store1(T* p, int q, float rho0, float v) { *p = enc(v - wgt(q) * rho0); }
+    LBM_HD static float dec1(T raw, int q, float rho0) { return dec(raw) + wgt(q) * rho0; }
+    LBM_HD static T enc1(float v, int q, float rho0) { return enc(v - wgt(q) * rho0); }
+    LBM_HD static float load1(const T* p, int q, float rho0) { return dec1(*p, q, rho0); }
+    LBM_HD static void store1(T* p, int q, float rho0, float v) { *p = enc1(v, q, rho0); }
 };
 template <int V>
 struct Storage<LBM_F16_SHIFTED, V> : Shifted16<LBM_F16_SHIFTED, V> {};
@@ -274,6 +278,67 @@ constexpr int kSegNarrow = STORE == LBM_F32 ? 1 : 2;
 // Single-element access (element type, load1/store1) does not depend on the width.
 template <int STORE>
 using Elem = Storage<STORE, kSegNarrow<STORE>>;
+
+// ---- population arrays addressed as plane offset + node offset --------------------------
+//
+// The two-step kernels address a population array ([9][plane], or an edge store
+// [9][n_edge]) as element `node` of plane q, with the plane offset (plus, for a pull, the
+// uniform shift to the upstream node) a wavefront-uniform value. With BUF = true the GPU
+// build makes the array a raw buffer resource: every access is one buffer_load/store with
+// the node's byte offset in a VGPR and the plane offset in an SGPR (soffset), so there is
+// no 64-bit address arithmetic per plane and no per-plane base pointer held in SGPRs.
+// Buffer offsets are 32-bit, so BUF needs every byte offset below 4 GiB (buf_fits);
+// BUF = false keeps 64-bit pointers for larger arrays. The host build (the CPU emulation)
+// uses a plain pointer with the same offsets.
+template <class T>
+LBM_HD bool buf_fits(long long planes, long long plane) {
+    return (unsigned long long)(planes * plane) * sizeof(T) < 0xffffffffull;
+}
+// Word 3 of a gfx9 raw buffer resource: DATA_FORMAT = 32 (untyped dword access).
+constexpr int kBufWord3 = 0x00020000;
+
+template <class T, bool BUF>
+struct PopBuf {
+    static_assert(sizeof(T) == 2 || sizeof(T) == 4, "16- or 32-bit populations");
+    char* b;
+#if defined(__HIP_DEVICE_COMPILE__)
+    __amdgpu_buffer_rsrc_t r;
+#endif
+    LBM_HD explicit PopBuf(const void* ptr) : b(static_cast<char*>(const_cast<void*>(ptr))) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        if constexpr (BUF) r = __builtin_amdgcn_make_buffer_rsrc(b, 0, -1, kBufWord3);
+#endif
+    }
+    // element `node` of the array at element offset `off` (a plane offset, possibly plus a
+    // uniform pull shift; off + node >= 0 always)
+    LBM_HD T ld(uint32_t node, long long off) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+        if constexpr (BUF) {
+            const int vo = int(node * uint32_t(sizeof(T))), so = int(uint32_t(off) * uint32_t(sizeof(T)));
+            if constexpr (sizeof(T) == 4)
+                return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+            else
+                return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0));
+        }
+#endif
+        T v;
+        __builtin_memcpy(&v, b + (off + (long long)node) * (long long)sizeof(T), sizeof(T));
+        return v;
+    }
+    LBM_HD void st(uint32_t node, long long off, T v) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+        if constexpr (BUF) {
+            const int vo = int(node * uint32_t(sizeof(T))), so = int(uint32_t(off) * uint32_t(sizeof(T)));
+            if constexpr (sizeof(T) == 4)
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, vo, so, 0);
+            else
+                __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, v), r, vo, so, 0);
+            return;
+        }
+#endif
+        __builtin_memcpy(b + (off + (long long)node) * (long long)sizeof(T), &v, sizeof(T));
+    }
+};
 
 // Moment partial sums for the OptimalAdapter averages (FP64).
 struct MomAcc {
@@ -612,27 +677,33 @@ struct Tile2 {
 };
 
 // True when the columns x0 - 1 .. x0 + kT2W of a tile or strip hold no node of a
-// non-periodic x edge (uniform over the workgroup).
+// non-periodic x edge (uniform over the workgroup): phase B then pulls only from LDS.
 LBM_HD bool clear_of_x_edges(const StepParams& p, int x0) { return p.px || (x0 - 1 > 0 && x0 + kT2W < p.nx - 1); }
 // True when grid row y (in the domain) is not a non-periodic y edge (uniform over a
 // wavefront: one row per wavefront in both two-step forms).
 LBM_HD bool clear_of_y_edges(const StepParams& p, int y) { return p.py || (y > 0 && y < p.ny - 1); }
+// True when every pull source of columns x0 - 1 .. x0 + kT2W lies in the grid without a
+// periodic wrap (columns x0 - 2 .. x0 + kT2W + 1), so none of these columns is an x edge.
+LBM_HD bool interior_cols(const StepParams& p, int x0) { return x0 >= 2 && x0 + kT2W + 1 < p.nx; }
+// The same for an unwrapped row y: rows y - 1 .. y + 1 in the grid.
+LBM_HD bool interior_row(const StepParams& p, int y) { return y >= 1 && y < p.ny - 1; }
 
 // Pull sources of phase-A node (gx, gy) (already wrapped and inside the domain); defined
 // with the strip form below.
-template <int STORE>
-LBM_HD void step2_gather(const StepParams& p, int gx, int gy, float* g, bool fast);
+template <int STORE, bool BUF>
+LBM_HD void step2_gather(const StepParams& p, int gx, int gy, float* g, bool affine);
 
 // Phase A of a tile, thread t: node (x0 - 1 + t % 64, y0 - 1 + t / 64) at state t+1 into
-// the tile. fast_x = clear_of_x_edges(p, x0).
-template <int COLL, int STORE, int TY>
-LBM_HD void step2_phase_a(const StepParams& p, typename Elem<STORE>::T* tile, int t, int x0, int y0, bool fast_x) {
+// the tile. interior_x = interior_cols(p, x0).
+template <int COLL, int STORE, int TY, bool BUF>
+LBM_HD void step2_phase_a(const StepParams& p, typename Elem<STORE>::T* tile, int t, int x0, int y0, bool interior_x) {
     using St = Elem<STORE>;
     using T = typename St::T;
     constexpr int NH = Tile2<TY>::NH;
     const int l = t % kT2L, w = t / kT2L;
     int gx = x0 - 1 + l, gy = y0 - 1 + w;
     const bool own = l >= 1 && l <= kT2W && w >= 1 && w <= TY && gx < p.nx && gy < p.ny;
+    const bool affine = interior_x && interior_row(p, gy);  // uniform per wavefront
     if (gx < 0 || gx >= p.nx) {
         if (!p.px) return;
         gx = ((gx % p.nx) + p.nx) % p.nx;
@@ -647,7 +718,7 @@ LBM_HD void step2_phase_a(const StepParams& p, typename Elem<STORE>::T* tile, in
     // pull sources; a source outside a non-periodic domain reads the node's unstreamed slot,
     // state t-1, from the edge store
     float g[Q];
-    step2_gather<STORE>(p, gx, gy, g, fast_x && clear_of_y_edges(p, gy));
+    step2_gather<STORE, BUF>(p, gx, gy, g, affine);
     if (fl != LBM_FLUID) apply_local_bc(g, fl, gx, gy, node_args(p));
     float fx, fy;
     node_force(p, gx, gy, node, fx, fy);
@@ -657,12 +728,12 @@ LBM_HD void step2_phase_a(const StepParams& p, typename Elem<STORE>::T* tile, in
 #pragma unroll
     for (int q = 0; q < Q; q++) St::store1(tile + q * NH + t, q, p.rho0, g[q]);
     if (own && p.mid) {
-        T* md = static_cast<T*>(p.mid);
+        const PopBuf<T, BUF> md(p.mid);
 #pragma unroll
-        for (int q = 0; q < Q; q++) St::store1(md + q * p.plane + node, q, p.rho0, g[q]);
+        for (int q = 0; q < Q; q++) md.st(node, q * p.plane, St::enc1(g[q], q, p.rho0));
     }
     if (own && (ex || ey)) {
-        T* en = static_cast<T*>(p.edge_next);
+        const PopBuf<T, BUF> en(p.edge_next);
         const int base = p.py ? 0 : 2 * p.nx;
         int slot[2] = {-1, -1};
         if (ey) slot[0] = gy == 0 ? gx : p.nx + gx;
@@ -671,13 +742,15 @@ LBM_HD void step2_phase_a(const StepParams& p, typename Elem<STORE>::T* tile, in
         for (int k = 0; k < 2; k++) {
             if (slot[k] < 0) continue;
 #pragma unroll
-            for (int q = 0; q < Q; q++) St::store1(en + (long long)q * p.n_edge + slot[k], q, p.rho0, g[q]);
+            for (int q = 0; q < Q; q++) en.st(uint32_t(slot[k]), q * (long long)p.n_edge, St::enc1(g[q], q, p.rho0));
         }
     }
 }
 
-template <int COLL, int STORE, int TY>
-LBM_HD void step2_phase_b(const StepParams& p, const typename Elem<STORE>::T* tile, int t, int x0, int y0, bool fast_x) {
+// Phase B of a tile, thread t: state t+2 of the inner node from the tile to HBM.
+// clear_x = clear_of_x_edges(p, x0).
+template <int COLL, int STORE, int TY, bool BUF>
+LBM_HD void step2_phase_b(const StepParams& p, const typename Elem<STORE>::T* tile, int t, int x0, int y0, bool clear_x) {
     using St = Elem<STORE>;
     using T = typename St::T;
     constexpr int NH = Tile2<TY>::NH;
@@ -686,18 +759,20 @@ LBM_HD void step2_phase_b(const StepParams& p, const typename Elem<STORE>::T* ti
     const int x = x0 - 1 + l, y = y0 - 1 + w;
     if (x >= p.nx || y >= p.ny) return;
     const uint32_t node = uint32_t(y) * uint32_t(p.nx) + uint32_t(x);
-    const T* src = static_cast<const T*>(p.src);
     float g[Q];
-    if (fast_x && clear_of_y_edges(p, y)) {
+    if (clear_x && clear_of_y_edges(p, y)) {
 #pragma unroll
         for (int q = 0; q < Q; q++) g[q] = St::load1(tile + q * NH + t - CY[q] * kT2L - CX[q], q, p.rho0);
     } else {
+        const PopBuf<T, BUF> src(p.src);
 #pragma unroll
         for (int q = 0; q < Q; q++) {
             const int xs = x - CX[q], ys = y - CY[q];
             const bool out = (!p.px && (xs < 0 || xs >= p.nx)) || (!p.py && (ys < 0 || ys >= p.ny));
-            g[q] = out ? St::load1(src + q * p.plane + node, q, p.rho0)
-                       : St::load1(tile + q * NH + t - CY[q] * kT2L - CX[q], q, p.rho0);
+            if (out)
+                g[q] = St::dec1(src.ld(node, q * p.plane), q, p.rho0);
+            else
+                g[q] = St::load1(tile + q * NH + t - CY[q] * kT2L - CX[q], q, p.rho0);
         }
     }
     const uint8_t fl = p.flags ? p.flags[node] : uint8_t(LBM_FLUID);
@@ -707,9 +782,9 @@ LBM_HD void step2_phase_b(const StepParams& p, const typename Elem<STORE>::T* ti
     Macro m;
     MomAcc acc{0.0, 0.0, 0.0};
     node_update<COLL, 0>(p, g, fx, fy, m, acc);
-    T* dst = static_cast<T*>(p.dst);
+    const PopBuf<T, BUF> dst(p.dst);
 #pragma unroll
-    for (int q = 0; q < Q; q++) St::store1(dst + q * p.plane + node, q, p.rho0, g[q]);
+    for (int q = 0; q < Q; q++) dst.st(node, q * p.plane, St::enc1(g[q], q, p.rho0));
     if (p.out_rho) {
         p.out_rho[node] = m.rho;
         p.out_ux[node] = m.ux;
@@ -729,37 +804,40 @@ LBM_HD void step2_phase_b(const StepParams& p, const typename Elem<STORE>::T* ti
 // Unstreamed slots are handled as in the tile form (edge store for phase A, source buffer
 // for phase B).
 
-// Pull sources of phase-A node (gx, gy) (already wrapped and inside the domain). `fast`,
-// the same for a whole wavefront, promises that no node of the wavefront lies on a
-// non-periodic domain edge; otherwise every node takes the edge-aware path (which loads the
-// same values for interior nodes), so a boundary wavefront runs one gather, not both.
-template <int STORE>
-LBM_HD void step2_gather(const StepParams& p, int gx, int gy, float* g, bool fast) {
+// Pull sources of phase-A node (gx, gy) (already wrapped and inside the domain). `affine`,
+// the same for a whole wavefront, promises that every pull source of the wavefront is in
+// the grid without a wrap (interior_cols and interior_row): the source of direction q is
+// then node - (c_x + c_y nx), one uniform shift per plane, and the load needs no per-lane
+// address arithmetic. Otherwise every node takes the general path (periodic wrap, and the
+// edge store for sources outside a non-periodic domain), which loads the same values for
+// interior nodes; a wavefront runs one of the two, never both.
+template <int STORE, bool BUF>
+LBM_HD void step2_gather(const StepParams& p, int gx, int gy, float* g, bool affine) {
     using St = Elem<STORE>;
     using T = typename St::T;
+    const PopBuf<T, BUF> src(p.src);
+    if (affine) {
+        const uint32_t node = uint32_t(gy) * uint32_t(p.nx) + uint32_t(gx);
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+            g[q] = St::dec1(src.ld(node, q * p.plane - CX[q] - CY[q] * (long long)p.nx), q, p.rho0);
+        return;
+    }
     const bool ex = !p.px && (gx == 0 || gx == p.nx - 1), ey = !p.py && (gy == 0 || gy == p.ny - 1);
-    const T* src = static_cast<const T*>(p.src);
     const int xm = gx == 0 ? (p.px ? p.nx - 1 : -1) : gx - 1, xp = gx == p.nx - 1 ? (p.px ? 0 : -1) : gx + 1;
     const int ym = gy == 0 ? (p.py ? p.ny - 1 : -1) : gy - 1, yp = gy == p.ny - 1 ? (p.py ? 0 : -1) : gy + 1;
     const uint32_t rm = uint32_t(ym) * uint32_t(p.nx), r0 = uint32_t(gy) * uint32_t(p.nx), rp = uint32_t(yp) * uint32_t(p.nx);
-    if (fast) {
+    const PopBuf<T, BUF> prev(p.edge_prev);
+    const uint32_t eslot = (ex || ey) ? uint32_t(edge_slot(gx, gy, p.nx, p.ny, p.px, p.py)) : 0u;
 #pragma unroll
-        for (int q = 0; q < Q; q++) {
-            const uint32_t xs = uint32_t(CX[q] > 0 ? xm : (CX[q] < 0 ? xp : gx));
-            const uint32_t row = CY[q] > 0 ? rm : (CY[q] < 0 ? rp : r0);
-            g[q] = St::load1(src + q * p.plane + (row + xs), q, p.rho0);
-        }
-    } else {
-        const T* prev = static_cast<const T*>(p.edge_prev);
-        const uint32_t eslot = (ex || ey) ? uint32_t(edge_slot(gx, gy, p.nx, p.ny, p.px, p.py)) : 0u;
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            const int xs = CX[q] > 0 ? xm : (CX[q] < 0 ? xp : gx);
-            const int ys = CY[q] > 0 ? ym : (CY[q] < 0 ? yp : gy);
-            const uint32_t row = CY[q] > 0 ? rm : (CY[q] < 0 ? rp : r0);
-            g[q] = (xs < 0 || ys < 0) ? St::load1(prev + q * p.n_edge + eslot, q, p.rho0)
-                                      : St::load1(src + q * p.plane + (row + uint32_t(xs)), q, p.rho0);
-        }
+    for (int q = 0; q < Q; q++) {
+        const int xs = CX[q] > 0 ? xm : (CX[q] < 0 ? xp : gx);
+        const int ys = CY[q] > 0 ? ym : (CY[q] < 0 ? yp : gy);
+        const uint32_t row = CY[q] > 0 ? rm : (CY[q] < 0 ? rp : r0);
+        if (xs < 0 || ys < 0)
+            g[q] = St::dec1(prev.ld(eslot, q * (long long)p.n_edge), q, p.rho0);
+        else
+            g[q] = St::dec1(src.ld(row + uint32_t(xs), q * p.plane), q, p.rho0);
     }
 }
 
@@ -780,7 +858,7 @@ LBM_HD bool strip_node(const StepParams& p, int x0, int y0, int l, int i, int& g
 
 // Phase A of strip node (l, i) from its gathered sources g: collide, store into the ring
 // row `slot` and, for the strip's own edge nodes, into the next edge store.
-template <int COLL, int STORE>
+template <int COLL, int STORE, bool BUF>
 LBM_HD void step2s_phase_a(const StepParams& p, typename Elem<STORE>::T* ring, int ring_rows, int slot, int l, int i,
                            int x0, int y0, int hs, int gx, int gy, float* g) {
     using St = Elem<STORE>;
@@ -799,12 +877,12 @@ LBM_HD void step2s_phase_a(const StepParams& p, typename Elem<STORE>::T* ring, i
     const bool ex = !p.px && (gx == 0 || gx == p.nx - 1), ey = !p.py && (gy == 0 || gy == p.ny - 1);
     const bool own = l >= 1 && l <= kT2W && i >= 1 && i <= hs && x0 - 1 + l < p.nx;
     if (own && p.mid) {
-        T* md = static_cast<T*>(p.mid);
+        const PopBuf<T, BUF> md(p.mid);
 #pragma unroll
-        for (int q = 0; q < Q; q++) St::store1(md + q * p.plane + node, q, p.rho0, g[q]);
+        for (int q = 0; q < Q; q++) md.st(node, q * p.plane, St::enc1(g[q], q, p.rho0));
     }
     if (own && (ex || ey)) {
-        T* en = static_cast<T*>(p.edge_next);
+        const PopBuf<T, BUF> en(p.edge_next);
         const int base = p.py ? 0 : 2 * p.nx;
         int es[2] = {-1, -1};
         if (ey) es[0] = gy == 0 ? gx : p.nx + gx;
@@ -813,37 +891,43 @@ LBM_HD void step2s_phase_a(const StepParams& p, typename Elem<STORE>::T* ring, i
         for (int k = 0; k < 2; k++) {
             if (es[k] < 0) continue;
 #pragma unroll
-            for (int q = 0; q < Q; q++) St::store1(en + (long long)q * p.n_edge + es[k], q, p.rho0, g[q]);
+            for (int q = 0; q < Q; q++) en.st(uint32_t(es[k]), q * (long long)p.n_edge, St::enc1(g[q], q, p.rho0));
         }
     }
 }
 
 // Phase B of strip row j (grid row y0 + j), lane l: pulls from ring rows j, j+1, j+2.
-template <int COLL, int STORE>
+// clear_x = clear_of_x_edges(p, x0).
+template <int COLL, int STORE, bool BUF>
 LBM_HD void step2s_phase_b(const StepParams& p, const typename Elem<STORE>::T* ring, int ring_rows, int l, int j,
-                           int x0, int y0, bool fast_x) {
+                           int x0, int y0, bool clear_x) {
     using St = Elem<STORE>;
     using T = typename St::T;
     if (l < 1 || l > kT2W) return;
     const int x = x0 - 1 + l, y = y0 + j;
     if (x >= p.nx) return;
     const uint32_t node = uint32_t(y) * uint32_t(p.nx) + uint32_t(x);
-    const T* src = static_cast<const T*>(p.src);
     const int stride = ring_rows * kT2L;
     // ring rows of strip rows j (below), j + 1 (own), j + 2 (above)
     const int s_lo = j % ring_rows, s_mid = (j + 1) % ring_rows, s_hi = (j + 2) % ring_rows;
-    const bool fast = fast_x && clear_of_y_edges(p, y);  // fast_x = clear_of_x_edges(p, x0)
     float g[Q];
+    if (clear_x && clear_of_y_edges(p, y)) {
 #pragma unroll
-    for (int q = 0; q < Q; q++) {
-        const int srow = CY[q] > 0 ? s_lo : (CY[q] < 0 ? s_hi : s_mid);
-        const T* a = ring + q * stride + srow * kT2L + (l - CX[q]);
-        if (fast) {
-            g[q] = St::load1(a, q, p.rho0);
-        } else {
+        for (int q = 0; q < Q; q++) {
+            const int srow = CY[q] > 0 ? s_lo : (CY[q] < 0 ? s_hi : s_mid);
+            g[q] = St::load1(ring + q * stride + srow * kT2L + (l - CX[q]), q, p.rho0);
+        }
+    } else {
+        const PopBuf<T, BUF> src(p.src);
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int srow = CY[q] > 0 ? s_lo : (CY[q] < 0 ? s_hi : s_mid);
             const int xs = x - CX[q], ys = y - CY[q];
             const bool out = (!p.px && (xs < 0 || xs >= p.nx)) || (!p.py && (ys < 0 || ys >= p.ny));
-            g[q] = out ? St::load1(src + q * p.plane + node, q, p.rho0) : St::load1(a, q, p.rho0);
+            if (out)
+                g[q] = St::dec1(src.ld(node, q * p.plane), q, p.rho0);
+            else
+                g[q] = St::load1(ring + q * stride + srow * kT2L + (l - CX[q]), q, p.rho0);
         }
     }
     const uint8_t fl = p.flags ? p.flags[node] : uint8_t(LBM_FLUID);
@@ -853,9 +937,9 @@ LBM_HD void step2s_phase_b(const StepParams& p, const typename Elem<STORE>::T* r
     Macro m;
     MomAcc acc{0.0, 0.0, 0.0};
     node_update<COLL, 0>(p, g, fx, fy, m, acc);
-    T* dst = static_cast<T*>(p.dst);
+    const PopBuf<T, BUF> dst(p.dst);
 #pragma unroll
-    for (int q = 0; q < Q; q++) St::store1(dst + q * p.plane + node, q, p.rho0, g[q]);
+    for (int q = 0; q < Q; q++) dst.st(node, q * p.plane, St::enc1(g[q], q, p.rho0));
     if (p.out_rho) {
         p.out_rho[node] = m.rho;
         p.out_ux[node] = m.ux;
