@@ -76,7 +76,7 @@ __global__ __launch_bounds__(Tile2<TY>::THREADS) void fused_step2_kernel(StepPar
     const int x0 = bx * kT2W, y0 = by * TY;
     step2_phase_a<COLL, STORE, TY, true>(p, tile, threadIdx.x, x0, y0, interior_cols(p, x0));
     __syncthreads();
-    step2_phase_b<COLL, STORE, TY, true>(p, tile, threadIdx.x, x0, y0, clear_of_x_edges(p, x0));
+    step2_phase_b<COLL, STORE, TY, true>(p, tile, threadIdx.x, x0, y0);
 }
 
 // Two time steps per launch, marching strips; see step2s_phase_a / step2s_phase_b. BUF:
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(kT2L * W) void fused_step2s_kernel(StepParams p, lo
     const int hs = p.ny - y0 < H ? p.ny - y0 : H;  // output rows of this strip
     const int l = threadIdx.x & (kT2L - 1), w = threadIdx.x / kT2L;
     const int K = (hs + 1) / W + 1;  // chunks: phase-A rows 0 .. hs + 1, phase-B rows 0 .. hs - 1
-    const bool interior_x = interior_cols(p, x0), clear_x = clear_of_x_edges(p, x0);
+    const bool interior_x = interior_cols(p, x0);
     float g[Q];
     int i = w, gx = 0, gy = 0;
     bool va = i <= hs + 1 && strip_node(p, x0, y0, l, i, gx, gy);
@@ -108,7 +108,10 @@ __global__ __launch_bounds__(kT2L * W) void fused_step2s_kernel(StepParams p, lo
                      "+s"(pl.edge_prev), "+s"(pl.edge_next));
         asm volatile("" : "+s"(pl.px), "+s"(pl.py), "+s"(pl.ny), "+s"(pl.flags), "+s"(pl.forced), "+s"(pl.force_field),
                      "+s"(pl.ibm_force), "+s"(pl.out_rho), "+s"(pl.quirks));
-        if (va) step2s_phase_a<COLL, STORE, BUF>(pl, ring, R, i % R, l, i, x0, y0, hs, gx, gy, g);
+        if (va)
+            step2s_phase_a<COLL, STORE, BUF>(pl, ring, R, i % R, l, i, x0, y0, hs, gx, gy, g);
+        else if (i <= hs + 1)  // outside a non-periodic domain
+            step2_fill_halo<STORE, BUF>(pl, ring + (i % R) * kT2L + l, R * kT2L, x0 - 1 + l, y0 - 1 + i);
         __syncthreads();
         // the next chunk's pull sources, loaded before this chunk's phase B
         const int in = (k + 1) * W + w;
@@ -117,7 +120,7 @@ __global__ __launch_bounds__(kT2L * W) void fused_step2s_kernel(StepParams p, lo
         float gn[Q];
         if (vn) step2_gather<STORE, BUF>(pl, nxg, nyg, gn, interior_x && interior_row(pl, y0 - 1 + in));
         const int j = k * W - 2 + w;
-        if (j >= 0 && j < hs) step2s_phase_b<COLL, STORE, BUF>(pl, ring, R, l, j, x0, y0, clear_x);
+        if (j >= 0 && j < hs) step2s_phase_b<COLL, STORE, BUF>(pl, ring, R, l, j, x0, y0);
         __syncthreads();
         i = in;
         va = vn;

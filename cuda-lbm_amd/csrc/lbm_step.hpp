@@ -662,10 +662,11 @@ LBM_HD void step_segment(const StepParams& p, long long seg, MomAcc& acc) {
 // costs 64 (TY + 2) / (kT2W TY) phase-A node updates per node.
 //
 // Unstreamed slots: a one-step launch reads them from the buffer it writes (state t-1 at
-// step t+1, state t at step t+2). Here phase B reads state t from the source buffer, and
-// phase A reads state t-1 from a small edge store, because the destination buffer is being
-// overwritten by other workgroups during the launch; phase A also saves state t+1 of the
-// tile's own edge nodes into the next edge store. Boundary functors that read a neighbour's
+// step t+1, state t at step t+2). Here phase A reads state t-1 from a small edge store,
+// because the destination buffer is being overwritten by other workgroups during the
+// launch, and saves state t+1 of the tile's own edge nodes into the next edge store. For
+// phase B, the halo lanes outside a non-periodic domain copy the edge nodes' state-t slots
+// from the source buffer into LDS (step2_fill_halo), so phase B pulls only from LDS. Boundary functors that read a neighbour's
 // populations (ZG outflow, regularized corner) and the stale-rest quirk need the one-step
 // launch.
 constexpr int kT2L = 64;          // lanes of a tile row (one wavefront)
@@ -676,12 +677,6 @@ struct Tile2 {
     static constexpr int ROWS = TY + 2, NH = kT2L * ROWS, THREADS = NH;
 };
 
-// True when the columns x0 - 1 .. x0 + kT2W of a tile or strip hold no node of a
-// non-periodic x edge (uniform over the workgroup): phase B then pulls only from LDS.
-LBM_HD bool clear_of_x_edges(const StepParams& p, int x0) { return p.px || (x0 - 1 > 0 && x0 + kT2W < p.nx - 1); }
-// True when grid row y (in the domain) is not a non-periodic y edge (uniform over a
-// wavefront: one row per wavefront in both two-step forms).
-LBM_HD bool clear_of_y_edges(const StepParams& p, int y) { return p.py || (y > 0 && y < p.ny - 1); }
 // True when every pull source of columns x0 - 1 .. x0 + kT2W lies in the grid without a
 // periodic wrap (columns x0 - 2 .. x0 + kT2W + 1), so none of these columns is an x edge.
 LBM_HD bool interior_cols(const StepParams& p, int x0) { return x0 >= 2 && x0 + kT2W + 1 < p.nx; }
@@ -692,6 +687,25 @@ LBM_HD bool interior_row(const StepParams& p, int y) { return y >= 1 && y < p.ny
 // with the strip form below.
 template <int STORE, bool BUF>
 LBM_HD void step2_gather(const StepParams& p, int gx, int gy, float* g, bool affine);
+
+// Phase-A slot of a halo position (xh, yh) outside a non-periodic domain (unwrapped
+// coordinates; lds points at its direction-0 entry, planes `stride` apart). The only node
+// that pulls direction q from here is (xh + c_x, yh + c_y); its upstream node is outside
+// the domain, so it keeps its unstreamed slot, its own state-t population src[q][node]
+// (the buffer a one-step launch would write at step t+2 holds state t). Storing that value
+// (raw, in the storage format) lets phase B pull every direction from LDS, edge nodes
+// included. Entries whose puller lies outside the domain are never read.
+template <int STORE, bool BUF>
+LBM_HD void step2_fill_halo(const StepParams& p, typename Elem<STORE>::T* lds, int stride, int xh, int yh) {
+    using T = typename Elem<STORE>::T;
+    const PopBuf<T, BUF> src(p.src);
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int tx = xh + CX[q], ty = yh + CY[q];
+        if (tx < 0 || tx >= p.nx || ty < 0 || ty >= p.ny) continue;
+        lds[q * stride] = src.ld(uint32_t(ty) * uint32_t(p.nx) + uint32_t(tx), q * p.plane);
+    }
+}
 
 // Phase A of a tile, thread t: node (x0 - 1 + t % 64, y0 - 1 + t / 64) at state t+1 into
 // the tile. interior_x = interior_cols(p, x0).
@@ -704,14 +718,12 @@ LBM_HD void step2_phase_a(const StepParams& p, typename Elem<STORE>::T* tile, in
     int gx = x0 - 1 + l, gy = y0 - 1 + w;
     const bool own = l >= 1 && l <= kT2W && w >= 1 && w <= TY && gx < p.nx && gy < p.ny;
     const bool affine = interior_x && interior_row(p, gy);  // uniform per wavefront
-    if (gx < 0 || gx >= p.nx) {
-        if (!p.px) return;
-        gx = ((gx % p.nx) + p.nx) % p.nx;
+    if ((!p.px && (gx < 0 || gx >= p.nx)) || (!p.py && (gy < 0 || gy >= p.ny))) {
+        step2_fill_halo<STORE, BUF>(p, tile + t, NH, gx, gy);
+        return;
     }
-    if (gy < 0 || gy >= p.ny) {
-        if (!p.py) return;
-        gy = ((gy % p.ny) + p.ny) % p.ny;
-    }
+    if (gx < 0 || gx >= p.nx) gx = ((gx % p.nx) + p.nx) % p.nx;
+    if (gy < 0 || gy >= p.ny) gy = ((gy % p.ny) + p.ny) % p.ny;
     const uint32_t node = uint32_t(gy) * uint32_t(p.nx) + uint32_t(gx);
     const uint8_t fl = p.flags ? p.flags[node] : uint8_t(LBM_FLUID);
     const bool ex = !p.px && (gx == 0 || gx == p.nx - 1), ey = !p.py && (gy == 0 || gy == p.ny - 1);
@@ -747,10 +759,10 @@ LBM_HD void step2_phase_a(const StepParams& p, typename Elem<STORE>::T* tile, in
     }
 }
 
-// Phase B of a tile, thread t: state t+2 of the inner node from the tile to HBM.
-// clear_x = clear_of_x_edges(p, x0).
+// Phase B of a tile, thread t: state t+2 of the inner node from the tile to HBM. Every
+// pull comes from the tile: unstreamed slots were placed there by step2_fill_halo.
 template <int COLL, int STORE, int TY, bool BUF>
-LBM_HD void step2_phase_b(const StepParams& p, const typename Elem<STORE>::T* tile, int t, int x0, int y0, bool clear_x) {
+LBM_HD void step2_phase_b(const StepParams& p, const typename Elem<STORE>::T* tile, int t, int x0, int y0) {
     using St = Elem<STORE>;
     using T = typename St::T;
     constexpr int NH = Tile2<TY>::NH;
@@ -760,21 +772,8 @@ LBM_HD void step2_phase_b(const StepParams& p, const typename Elem<STORE>::T* ti
     if (x >= p.nx || y >= p.ny) return;
     const uint32_t node = uint32_t(y) * uint32_t(p.nx) + uint32_t(x);
     float g[Q];
-    if (clear_x && clear_of_y_edges(p, y)) {
 #pragma unroll
-        for (int q = 0; q < Q; q++) g[q] = St::load1(tile + q * NH + t - CY[q] * kT2L - CX[q], q, p.rho0);
-    } else {
-        const PopBuf<T, BUF> src(p.src);
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            const int xs = x - CX[q], ys = y - CY[q];
-            const bool out = (!p.px && (xs < 0 || xs >= p.nx)) || (!p.py && (ys < 0 || ys >= p.ny));
-            if (out)
-                g[q] = St::dec1(src.ld(node, q * p.plane), q, p.rho0);
-            else
-                g[q] = St::load1(tile + q * NH + t - CY[q] * kT2L - CX[q], q, p.rho0);
-        }
-    }
+    for (int q = 0; q < Q; q++) g[q] = St::load1(tile + q * NH + t - CY[q] * kT2L - CX[q], q, p.rho0);
     const uint8_t fl = p.flags ? p.flags[node] : uint8_t(LBM_FLUID);
     if (fl != LBM_FLUID) apply_local_bc(g, fl, x, y, node_args(p));
     float fx, fy;
@@ -801,8 +800,8 @@ LBM_HD void step2_phase_b(const StepParams& p, const typename Elem<STORE>::T* ti
 // ring rows each needs. A phase-A row is computed once per strip (the tile form recomputes
 // the two halo rows of every tile), and the pull sources of the next chunk are loaded into
 // registers before the current chunk's phase B, so their latency overlaps that compute.
-// Unstreamed slots are handled as in the tile form (edge store for phase A, source buffer
-// for phase B).
+// Unstreamed slots are handled as in the tile form (edge store for phase A, halo slots
+// filled from the source buffer for phase B).
 
 // Pull sources of phase-A node (gx, gy) (already wrapped and inside the domain). `affine`,
 // the same for a whole wavefront, promises that every pull source of the wavefront is in
@@ -896,11 +895,11 @@ LBM_HD void step2s_phase_a(const StepParams& p, typename Elem<STORE>::T* ring, i
     }
 }
 
-// Phase B of strip row j (grid row y0 + j), lane l: pulls from ring rows j, j+1, j+2.
-// clear_x = clear_of_x_edges(p, x0).
+// Phase B of strip row j (grid row y0 + j), lane l: pulls from ring rows j, j+1, j+2 (the
+// unstreamed slots of edge nodes are there too, from step2_fill_halo).
 template <int COLL, int STORE, bool BUF>
 LBM_HD void step2s_phase_b(const StepParams& p, const typename Elem<STORE>::T* ring, int ring_rows, int l, int j,
-                           int x0, int y0, bool clear_x) {
+                           int x0, int y0) {
     using St = Elem<STORE>;
     using T = typename St::T;
     if (l < 1 || l > kT2W) return;
@@ -911,24 +910,10 @@ LBM_HD void step2s_phase_b(const StepParams& p, const typename Elem<STORE>::T* r
     // ring rows of strip rows j (below), j + 1 (own), j + 2 (above)
     const int s_lo = j % ring_rows, s_mid = (j + 1) % ring_rows, s_hi = (j + 2) % ring_rows;
     float g[Q];
-    if (clear_x && clear_of_y_edges(p, y)) {
 #pragma unroll
-        for (int q = 0; q < Q; q++) {
-            const int srow = CY[q] > 0 ? s_lo : (CY[q] < 0 ? s_hi : s_mid);
-            g[q] = St::load1(ring + q * stride + srow * kT2L + (l - CX[q]), q, p.rho0);
-        }
-    } else {
-        const PopBuf<T, BUF> src(p.src);
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            const int srow = CY[q] > 0 ? s_lo : (CY[q] < 0 ? s_hi : s_mid);
-            const int xs = x - CX[q], ys = y - CY[q];
-            const bool out = (!p.px && (xs < 0 || xs >= p.nx)) || (!p.py && (ys < 0 || ys >= p.ny));
-            if (out)
-                g[q] = St::dec1(src.ld(node, q * p.plane), q, p.rho0);
-            else
-                g[q] = St::load1(ring + q * stride + srow * kT2L + (l - CX[q]), q, p.rho0);
-        }
+    for (int q = 0; q < Q; q++) {
+        const int srow = CY[q] > 0 ? s_lo : (CY[q] < 0 ? s_hi : s_mid);
+        g[q] = St::load1(ring + q * stride + srow * kT2L + (l - CX[q]), q, p.rho0);
     }
     const uint8_t fl = p.flags ? p.flags[node] : uint8_t(LBM_FLUID);
     if (fl != LBM_FLUID) apply_local_bc(g, fl, x, y, node_args(p));

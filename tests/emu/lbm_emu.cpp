@@ -45,11 +45,11 @@ void launch2_emu(const StepParams& p, int reverse) {
         const long long lb = reverse ? nt - 1 - k : k;
         const int by = int(lb / tiles_x), bx = int(lb - (long long)by * tiles_x);
         std::fill(tile.begin(), tile.end(), poison<T>());
-        const bool fast_x = clear_of_x_edges(p, bx * kT2W);
+        const int x0 = bx * kT2W;
+        const bool interior_x = interior_cols(p, x0);
         for (int t = 0; t < Tile2<TY>::THREADS; t++)
-            step2_phase_a<COLL, STORE, TY>(p, tile.data(), t, bx * kT2W, by * TY, fast_x);
-        for (int t = 0; t < Tile2<TY>::THREADS; t++)
-            step2_phase_b<COLL, STORE, TY>(p, tile.data(), t, bx * kT2W, by * TY, fast_x);
+            step2_phase_a<COLL, STORE, TY, true>(p, tile.data(), t, x0, by * TY, interior_x);
+        for (int t = 0; t < Tile2<TY>::THREADS; t++) step2_phase_b<COLL, STORE, TY, true>(p, tile.data(), t, x0, by * TY);
     }
 }
 
@@ -69,21 +69,25 @@ void launch2s_emu(const StepParams& p, int H, int reverse) {
         const int hs = p.ny - y0 < H ? p.ny - y0 : H;
         const int K = (hs + 1) / W + 1;
         std::fill(ring.begin(), ring.end(), poison<T>());
-        const bool fast_x = clear_of_x_edges(p, x0);
+        const bool interior_x = interior_cols(p, x0);
         for (int k = 0; k < K; k++) {
             for (int w = 0; w < W; w++)
                 for (int l = 0; l < kT2L; l++) {
                     const int i = k * W + w;
                     int gx, gy;
-                    if (i > hs + 1 || !strip_node(p, x0, y0, l, i, gx, gy)) continue;
+                    if (i > hs + 1) continue;
+                    if (!strip_node(p, x0, y0, l, i, gx, gy)) {  // outside a non-periodic domain
+                        step2_fill_halo<STORE, true>(p, ring.data() + (i % R) * kT2L + l, R * kT2L, x0 - 1 + l, y0 - 1 + i);
+                        continue;
+                    }
                     float g[Q];
-                    step2_gather<STORE>(p, gx, gy, g, fast_x && clear_of_y_edges(p, gy));
-                    step2s_phase_a<COLL, STORE>(p, ring.data(), R, i % R, l, i, x0, y0, hs, gx, gy, g);
+                    step2_gather<STORE, true>(p, gx, gy, g, interior_x && interior_row(p, y0 - 1 + i));
+                    step2s_phase_a<COLL, STORE, true>(p, ring.data(), R, i % R, l, i, x0, y0, hs, gx, gy, g);
                 }
             for (int w = 0; w < W; w++)
                 for (int l = 0; l < kT2L; l++) {
                     const int j = k * W - 2 + w;
-                    if (j >= 0 && j < hs) step2s_phase_b<COLL, STORE>(p, ring.data(), R, l, j, x0, y0, fast_x);
+                    if (j >= 0 && j < hs) step2s_phase_b<COLL, STORE, true>(p, ring.data(), R, l, j, x0, y0);
                 }
         }
     }
