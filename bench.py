@@ -192,7 +192,7 @@ def copy_calibration(torch, mb):
     return {"read": 4 * n, "write": 4 * n}
 
 
-def cpu_baseline(sample_steps=120):
+def cpu_baseline(sample_steps=800):
     """The FP32 oracle (OpenMP over rows) on the host cores: a bounded sample of the
     same cavity (1024^2 MRT, `sample_steps` steps)."""
     sys.path.insert(0, str(ROOT / "oracle"))

@@ -427,12 +427,17 @@ hipError_t launch_step2(int coll, int storage, const StepParams& p, const Step2P
     return launch_step2_s<LBM_F32>(coll, p, lp, s);
 }
 
-// Measured on MI355X (us per step, two-step launches):
-//   1024^2 FP32 BGK: tiles of 6 rows 8.6; strips of 4 wavefronts x 16 rows 11.0
-//   8192^2 FP32 BGK: tiles 553; strips 4 x 32 539, 4 x 64 592-610 (MRT: tiles 646, 4 x 32 598)
-//   8192^2 FP16 BGK: tiles 529; strips 4 x 64 473
+// Measured on MI355X (us per step, two-step launches, periodic grids):
+//   1024^2 FP32 BGK: tiles of 6 rows 9.6, of 14 rows 10.3; strips of 4 wavefronts x 8 rows
+//     10.6, 4 x 16 10.7, 8 x 16 13.4 (MRT: tiles 9.5, strips 4 x 16 10.9)
+//   8192^2 FP32 BGK: tiles 620; strips 4 x 12 523, 4 x 16 520, 4 x 32 555, 4 x 64 660,
+//     8 x 64 569 (MRT: tiles 625, 4 x 16 530, 4 x 32 557)
+//   8192^2 FP16 BGK: tiles 551; strips 4 x 16 440, 4 x 32 399, 4 x 64 385, 8 x 128 485
+//     (MRT: tiles 562, 4 x 64 460)
 // Cache-resident grids have too few rows per strip for the march to pay; HBM grids gain
-// from computing each phase-A row once and from the prefetch.
+// from computing each phase-A row once and from the prefetch. FP32 strips are bound by
+// load latency (shorter strips: more workgroups in flight), FP16 strips by VALU (longer
+// strips: fewer recomputed halo rows).
 Step2Policy step2_policy(long long plane, int storage) {
     const double bytes = 2.0 * 9.0 * double(plane) * (storage == LBM_F32 ? 4.0 : 2.0);
     Step2Policy lp;
@@ -443,7 +448,7 @@ Step2Policy step2_policy(long long plane, int storage) {
     } else {
         lp.strip = 1;
         lp.w = 4;
-        lp.h = storage == LBM_F32 ? 32 : 64;
+        lp.h = storage == LBM_F32 ? 16 : 64;
     }
     return lp;
 }
