@@ -290,7 +290,7 @@ int lbm_adjoint_backward(lbm_ctx* ctx, const float* dL_du, float* dL_dforce, flo
             if (pairs && i + 1 < len) {
                 sp.mid = seg + size_t(i) * pf;
                 sp.dst = seg + size_t(i + 1) * pf;
-                e = launch_step2(ctx->cfg.collision, LBM_F32, sp, step2_policy(ctx->plane, LBM_F32), ctx->stream);
+                e = launch_step2(ctx->cfg.collision, LBM_F32, sp, step2_policy(ctx->plane, LBM_F32, ctx->cfg.collision), ctx->stream);
                 i += 2;
             } else {
                 sp.dst = seg + size_t(i) * pf;

@@ -180,7 +180,7 @@ int lbm_ctx::two_step(bool write_macros) {
     p.edge_prev = d_edge[edge_cur];
     p.edge_next = d_edge[1 - edge_cur];
     p.n_edge = n_edge;
-    e = launch_step2(cfg.collision, cfg.storage, p, step2_policy(plane, cfg.storage), stream);
+    e = launch_step2(cfg.collision, cfg.storage, p, step2_policy(plane, cfg.storage, cfg.collision), stream);
     if (e != hipSuccess) return hip_fail(e, "two-step launch");
     if (n_edge > 0) {
         edge_cur = 1 - edge_cur;

@@ -81,9 +81,17 @@ __global__ __launch_bounds__(Tile2<TY>::THREADS) void fused_step2_kernel(StepPar
 
 // Two time steps per launch, marching strips; see step2s_phase_a / step2s_phase_b. BUF:
 // population arrays addressed through buffer resources (arrays below 4 GiB).
+// Minimum wavefronts per SIMD requested for the strip kernels. 16-bit strips are VALU-bound
+// and gain from more wavefronts in flight: at 7 (72 VGPRs, a few spilled to AGPRs) the
+// 8192^2 FP16 step takes 335 us (BGK) / 363 us (MRT) against 385 / 460 us at the
+// unconstrained 88-104 VGPRs. FP32 strips lose at any cap (6: 536 us, 7: 561 us, 8: 585 us
+// against 518 us), so they keep the compiler's choice (1 = no constraint).
+template <int STORE>
+constexpr int kStripWavesPerEu = STORE == LBM_F32 ? 1 : 7;
+
 template <int COLL, int STORE, int W, bool BUF>
-__global__ __launch_bounds__(kT2L * W) void fused_step2s_kernel(StepParams p, long long nb_padded, int remap,
-                                                                 int strips_x, long long n_strips, int H) {
+__global__ __launch_bounds__(kT2L * W) __attribute__((amdgpu_waves_per_eu(kStripWavesPerEu<STORE>)))
+void fused_step2s_kernel(StepParams p, long long nb_padded, int remap, int strips_x, long long n_strips, int H) {
     using T = typename Elem<STORE>::T;
     constexpr int R = W + 2;  // ring rows
     __shared__ T ring[Q * R * kT2L];
@@ -432,13 +440,14 @@ hipError_t launch_step2(int coll, int storage, const StepParams& p, const Step2P
 //     10.6, 4 x 16 10.7, 8 x 16 13.4 (MRT: tiles 9.5, strips 4 x 16 10.9)
 //   8192^2 FP32 BGK: tiles 620; strips 4 x 12 523, 4 x 16 520, 4 x 32 555, 4 x 64 660,
 //     8 x 64 569 (MRT: tiles 625, 4 x 16 530, 4 x 32 557)
-//   8192^2 FP16 BGK: tiles 551; strips 4 x 16 440, 4 x 32 399, 4 x 64 385, 8 x 128 485
-//     (MRT: tiles 562, 4 x 64 460)
+//     CM: strips 4 x 16 601, 4 x 32 572, 4 x 64 599
+//   8192^2 FP16 (7 wavefronts per SIMD, see kStripWavesPerEu): strips 4 x 16 384, 4 x 32 343,
+//     4 x 64 336 (MRT 4 x 64 364, CM 411; BF16 BGK 346, MRT 378, CM 427)
 // Cache-resident grids have too few rows per strip for the march to pay; HBM grids gain
 // from computing each phase-A row once and from the prefetch. FP32 strips are bound by
 // load latency (shorter strips: more workgroups in flight), FP16 strips by VALU (longer
 // strips: fewer recomputed halo rows).
-Step2Policy step2_policy(long long plane, int storage) {
+Step2Policy step2_policy(long long plane, int storage, int coll) {
     const double bytes = 2.0 * 9.0 * double(plane) * (storage == LBM_F32 ? 4.0 : 2.0);
     Step2Policy lp;
     lp.remap = 1;
@@ -448,7 +457,7 @@ Step2Policy step2_policy(long long plane, int storage) {
     } else {
         lp.strip = 1;
         lp.w = 4;
-        lp.h = storage == LBM_F32 ? 16 : 64;
+        lp.h = storage != LBM_F32 ? 64 : (coll == LBM_CM ? 32 : 16);
     }
     return lp;
 }

@@ -41,7 +41,7 @@ struct Step2Policy {
     int w = 8;
     int h = 64;
 };
-Step2Policy step2_policy(long long plane, int storage);
+Step2Policy step2_policy(long long plane, int storage, int coll);
 long long step2_grid_blocks(const StepParams& p, const Step2Policy& lp);
 hipError_t launch_step2(int coll, int storage, const StepParams& p, const Step2Policy& lp, hipStream_t s);
 // to_store = 1: edge nodes of `buf` -> store; 0: store -> edge nodes of `buf`.
