@@ -161,6 +161,49 @@ def run_accuracy(torch):
     return out
 
 
+def run_poiseuille(torch, steps=400, warmup=20, adapter_steps=100):
+    """The force-driven Poiseuille channel (4096x1024 CM, periodic x, bounce-back walls,
+    nu = 0.05, Guo force for u_max = 0.05), from rest.
+
+    - CM<NoAdapter>: MLUPS per storage (two steps per launch) and the 16-bit storages'
+      velocity against FP32 after the same steps.
+    - CM<OptimalAdapter>, the configuration as specified: MLUPS per storage over its first
+      warmup + adapter_steps steps. Each step is a moments pass, an FP64 finalize of the
+      domain means and the fused step (one step per launch). The reference's adapter
+      diverges on this channel (non-finite after ~160-280 steps, in the FP64 oracle too), so
+      it is timed only over that finite window and its velocities are not compared."""
+    import numpy as np
+
+    from lbm_amd import _abi as A
+    from lbm_amd import scenarios as sc
+
+    stores = (("f32", A.LBM_F32), ("bf16", A.LBM_BF16_SHIFTED), ("f16", A.LBM_F16_SHIFTED))
+    plain, u = {}, {}
+    for name, st in stores:
+        case = sc.poiseuille(4096, 1024, storage=st, adapter=0)
+        s, r = run_case(case, steps, warmup, torch)
+        _, u[name] = s.get_macros()
+        s.close()
+        plain[name] = r
+    umax = float(np.abs(u["f32"][..., 0]).max())
+    for name in ("bf16", "f16"):
+        plain[name]["u_rel_l2_vs_fp32"] = rel_l2(u[name], u["f32"])
+        plain[name]["speedup_vs_fp32"] = round(plain[name]["mlups"] / plain["f32"]["mlups"], 3)
+    plain["workload"] = (f"CM<NoAdapter>, {warmup + steps} steps from rest (max u_x {umax:.2e}: the flow is "
+                         f"still accelerating; the steady parabola needs ~ny^2/nu = 2e7 steps)")
+    adapt = {}
+    for name, st in stores:
+        case = sc.poiseuille(4096, 1024, storage=st)
+        s, r = run_case(case, adapter_steps, warmup, torch)
+        _, ua = s.get_macros()
+        s.close()
+        adapt[name] = dict(r, finite=bool(np.isfinite(ua).all()))
+    adapt["workload"] = (f"CM<OptimalAdapter> (same-step means), first {warmup + adapter_steps} steps; per step: "
+                         f"moments pass + FP64 means + fused step")
+    return {"cm_no_adapter": plain, "cm_optimal_adapter": adapt,
+            "workload": "Poiseuille channel 4096x1024, nu=0.05, Guo force for u_max=0.05, periodic x, bounce-back walls"}
+
+
 def rel_l2(a, b):
     import numpy as np
 
@@ -219,7 +262,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-extras", action="store_true", help="only the headline workload")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["hbm_grid", "adjoint"], default=None,
+    ap.add_argument("--only", choices=["hbm_grid", "adjoint", "poiseuille"], default=None,
                     help="run one extra section alone (for profiling its kernels); prints that section")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="also run one plain device copy of MB megabytes (for scaling profiler byte counters)")
@@ -249,6 +292,9 @@ def main():
         return
     if args.only == "adjoint":
         print(json.dumps({"adjoint": run_adjoint(torch)}))
+        return
+    if args.only == "poiseuille":
+        print(json.dumps({"poiseuille": run_poiseuille(torch)}))
         return
 
     # headline: LDC 1024^2 MRT FP32
@@ -308,6 +354,7 @@ def main():
         s.close()
         out["ibm_cylinder"] = dict(rc, workload="IBM cylinder 2048x512 CM, 201 markers, 3 IBM iterations per step "
                                                 "(IBM kernel + fused step per step)")
+        out["poiseuille"] = run_poiseuille(torch)
         out["adjoint"] = run_adjoint(torch)
         out["accuracy"] = run_accuracy(torch)
     if args.calibrate_copy:

@@ -427,6 +427,27 @@ def test_poiseuille_profile_is_a_parabola():
     assert abs(p[0] + F / (2 * nu)) < 1e-6 * F / nu  # curvature -F/(2 nu) of the steady solution
 
 
+def test_optimal_adapter_diverges_on_the_force_driven_channel():
+    # The reference's OptimalAdapter (adapters.cuh:46-111; its authors' own note there:
+    # "maybe this isn't working") relaxes the third- and fourth-order central moments at
+    # 1 / (3 tau* + 1/2), with tau* from a linear model clamped to 0.005 where it goes
+    # negative, i.e. at ~1.94. On the force-driven channel from rest that is unstable in
+    # exact arithmetic: the FP64 oracle turns non-finite within a few hundred steps, in
+    # both adapter modes, while the same channel without the adapter stays finite. A NaN in
+    # a CM<OptimalAdapter> channel run is therefore the method's, not the kernel's.
+    def finite_after(steps, **kw):
+        case = sc.poiseuille(256, 64, **kw)
+        o = orc.Oracle(case.config.to_c(), precision=64)
+        case.apply_oracle(o)
+        o.step(steps)
+        _, u = o.get_macros()
+        return bool(np.isfinite(u).all())
+
+    assert finite_after(600, adapter=0)
+    assert not finite_after(600)
+    assert not finite_after(600, adapter_mode=A.LBM_ADAPTER_LAGGED)
+
+
 def test_ghia_error_formula_is_zero_on_the_table():
     n = 129
     u = np.zeros((n, n, 2))
