@@ -435,29 +435,40 @@ hipError_t launch_step2(int coll, int storage, const StepParams& p, const Step2P
     return launch_step2_s<LBM_F32>(coll, p, lp, s);
 }
 
-// Measured on MI355X (us per step, two-step launches, periodic grids):
-//   1024^2 FP32 BGK: tiles of 6 rows 9.6, of 14 rows 10.3; strips of 4 wavefronts x 8 rows
-//     10.6, 4 x 16 10.7, 8 x 16 13.4 (MRT: tiles 9.5, strips 4 x 16 10.9)
-//   8192^2 FP32 BGK: tiles 620; strips 4 x 12 523, 4 x 16 520, 4 x 32 555, 4 x 64 660,
-//     8 x 64 569 (MRT: tiles 625, 4 x 16 530, 4 x 32 557)
-//     CM: strips 4 x 16 601, 4 x 32 572, 4 x 64 599
-//   8192^2 FP16 (7 wavefronts per SIMD, see kStripWavesPerEu): strips 4 x 16 384, 4 x 32 343,
-//     4 x 64 336 (MRT 4 x 64 364, CM 411; BF16 BGK 346, MRT 378, CM 427)
-// Cache-resident grids have too few rows per strip for the march to pay; HBM grids gain
-// from computing each phase-A row once and from the prefetch. FP32 strips are bound by
-// load latency (shorter strips: more workgroups in flight), FP16 strips by VALU (longer
-// strips: fewer recomputed halo rows).
+// Measured on MI355X (us per step, two-step launches, periodic grids; population bytes =
+// both buffers; strips of 4 wavefronts x H rows):
+//   FP32            tiles   H=16   H=32   H=64      FP16            tiles   H=16   H=32   H=64
+//   1448^2  151 MB   16.2   19.0   19.0   27.8      1448^2   75 MB   16.4   17.1   18.5   27.7
+//   2048^2  302 MB   38.5   38.1   39.0   44.9      2048^2  151 MB   30.6   27.2   27.7   32.0
+//   2896^2  604 MB   76.5   74.4   78.6   85.0      2896^2  302 MB   62.0   53.5   51.7   59.2
+//   4096^2  1.2 GB  151.4  139.6  144.3  162.7      4096^2  604 MB  132.9  107.9   96.0   97.0
+//   5792^2  2.4 GB  294.3  270.0  283.3  316.2      5792^2  1.2 GB  255.0  197.9  175.9  179.6
+//   8192^2  4.8 GB  567.1  521.5  557.5  659.7      8192^2  2.4 GB  506.4  387.3  343.6  337.5
+// (BGK; MRT follows BGK; CM prefers H=32 over 16 in FP32 from ~0.7 GB: 8192^2 572.9 vs
+// 598.8). Tiles win while the grid stays in the 256 MB Infinity Cache: strips then have too
+// few rows per workgroup for the march to pay. FP32 strips are bound by load latency (short
+// strips: more workgroups in flight), 16-bit strips by VALU (long strips: fewer recomputed
+// halo rows).
 Step2Policy step2_policy(long long plane, int storage, int coll) {
     const double bytes = 2.0 * 9.0 * double(plane) * (storage == LBM_F32 ? 4.0 : 2.0);
     Step2Policy lp;
     lp.remap = 1;
-    if (bytes <= 192.0e6) {
-        lp.strip = 0;
-        lp.ty = 6;
+    lp.strip = 1;
+    lp.w = 4;
+    if (storage == LBM_F32) {
+        if (bytes <= 256.0e6) {
+            lp.strip = 0;
+            lp.ty = 6;
+        } else {
+            lp.h = (coll == LBM_CM && bytes > 700.0e6) ? 32 : 16;
+        }
     } else {
-        lp.strip = 1;
-        lp.w = 4;
-        lp.h = storage != LBM_F32 ? 64 : (coll == LBM_CM ? 32 : 16);
+        if (bytes <= 100.0e6) {
+            lp.strip = 0;
+            lp.ty = 6;
+        } else {
+            lp.h = bytes <= 200.0e6 ? 16 : (bytes <= 1.6e9 ? 32 : 64);
+        }
     }
     return lp;
 }
