@@ -81,13 +81,12 @@ __global__ __launch_bounds__(Tile2<TY>::THREADS) void fused_step2_kernel(StepPar
 
 // Two time steps per launch, marching strips; see step2s_phase_a / step2s_phase_b. BUF:
 // population arrays addressed through buffer resources (arrays below 4 GiB).
-// Minimum wavefronts per SIMD requested for the strip kernels. 16-bit strips are VALU-bound
-// and gain from more wavefronts in flight: at 7 (72 VGPRs, a few spilled to AGPRs) the
-// 8192^2 FP16 step takes 335 us (BGK) / 363 us (MRT) against 385 / 460 us at the
-// unconstrained 88-104 VGPRs. FP32 strips lose at any cap (6: 536 us, 7: 561 us, 8: 585 us
-// against 518 us), so they keep the compiler's choice (1 = no constraint).
+// Minimum wavefronts per SIMD requested for the strip kernels (a register cap: 80 VGPRs at
+// 6, 72 at 7). 8192^2, us per step at the best strip height, against the unconstrained
+// kernel: FP32 BGK 508 vs 516, MRT 509 vs 510, CM 522 vs 546 (at 6); FP16 BGK 317 vs 335,
+// MRT 343 vs 365, CM 392 vs 410 (at 7, 16-bit MRT/CM spilling 8-10 VGPRs to scratch).
 template <int STORE>
-constexpr int kStripWavesPerEu = STORE == LBM_F32 ? 1 : 7;
+constexpr int kStripWavesPerEu = STORE == LBM_F32 ? 6 : 7;
 
 template <int COLL, int STORE, int W, bool BUF>
 __global__ __launch_bounds__(kT2L * W) __attribute__((amdgpu_waves_per_eu(kStripWavesPerEu<STORE>)))
@@ -103,30 +102,29 @@ void fused_step2s_kernel(StepParams p, long long nb_padded, int remap, int strip
     const int l = threadIdx.x & (kT2L - 1), w = threadIdx.x / kT2L;
     const int K = (hs + 1) / W + 1;  // chunks: phase-A rows 0 .. hs + 1, phase-B rows 0 .. hs - 1
     const bool interior_x = interior_cols(p, x0);
-    float g[Q];
     int i = w, gx = 0, gy = 0;
     bool va = i <= hs + 1 && strip_node(p, x0, y0, l, i, gx, gy);
-    if (va) step2_gather<STORE, BUF>(p, gx, gy, g, interior_x && interior_row(p, y0 - 1 + i));
     StepParams pl = p;
     for (int k = 0; k < K; k++) {
         // Opaque per iteration: values derived from these (plane offsets, pull shifts,
-        // buffer resources) are then rebuilt inside the march with scalar ALU instead of
-        // being held across it, which exceeds the SGPR file and spills to VGPR lanes.
+        // buffer resources, uniform flags) are then rebuilt inside the march with scalar
+        // ALU instead of being held across it, which exceeds the SGPR file and spills to
+        // VGPR lanes.
         asm volatile("" : "+s"(pl.plane), "+s"(pl.nx), "+s"(pl.n_edge), "+s"(pl.src), "+s"(pl.dst), "+s"(pl.mid),
                      "+s"(pl.edge_prev), "+s"(pl.edge_next));
         asm volatile("" : "+s"(pl.px), "+s"(pl.py), "+s"(pl.ny), "+s"(pl.flags), "+s"(pl.forced), "+s"(pl.force_field),
                      "+s"(pl.ibm_force), "+s"(pl.out_rho), "+s"(pl.quirks));
-        if (va)
+        if (va) {
+            float g[Q];
+            step2_gather<STORE, BUF>(pl, gx, gy, g, interior_x && interior_row(pl, y0 - 1 + i));
             step2s_phase_a<COLL, STORE, BUF>(pl, ring, R, i % R, l, i, x0, y0, hs, gx, gy, g);
-        else if (i <= hs + 1)  // outside a non-periodic domain
+        } else if (i <= hs + 1) {  // outside a non-periodic domain
             step2_fill_halo<STORE, BUF>(pl, ring + (i % R) * kT2L + l, R * kT2L, x0 - 1 + l, y0 - 1 + i);
+        }
         __syncthreads();
-        // the next chunk's pull sources, loaded before this chunk's phase B
         const int in = (k + 1) * W + w;
         int nxg = 0, nyg = 0;
         const bool vn = k + 1 < K && in <= hs + 1 && strip_node(p, x0, y0, l, in, nxg, nyg);
-        float gn[Q];
-        if (vn) step2_gather<STORE, BUF>(pl, nxg, nyg, gn, interior_x && interior_row(pl, y0 - 1 + in));
         const int j = k * W - 2 + w;
         if (j >= 0 && j < hs) step2s_phase_b<COLL, STORE, BUF>(pl, ring, R, l, j, x0, y0);
         __syncthreads();
@@ -134,8 +132,6 @@ void fused_step2s_kernel(StepParams p, long long nb_padded, int remap, int strip
         va = vn;
         gx = nxg;
         gy = nyg;
-#pragma unroll
-        for (int q = 0; q < Q; q++) g[q] = gn[q];
     }
 }
 
@@ -436,19 +432,26 @@ hipError_t launch_step2(int coll, int storage, const StepParams& p, const Step2P
 }
 
 // Measured on MI355X (us per step, two-step launches, periodic grids; population bytes =
-// both buffers; strips of 4 wavefronts x H rows):
-//   FP32            tiles   H=16   H=32   H=64      FP16            tiles   H=16   H=32   H=64
-//   1448^2  151 MB   16.2   19.0   19.0   27.8      1448^2   75 MB   16.4   17.1   18.5   27.7
-//   2048^2  302 MB   38.5   38.1   39.0   44.9      2048^2  151 MB   30.6   27.2   27.7   32.0
-//   2896^2  604 MB   76.5   74.4   78.6   85.0      2896^2  302 MB   62.0   53.5   51.7   59.2
-//   4096^2  1.2 GB  151.4  139.6  144.3  162.7      4096^2  604 MB  132.9  107.9   96.0   97.0
-//   5792^2  2.4 GB  294.3  270.0  283.3  316.2      5792^2  1.2 GB  255.0  197.9  175.9  179.6
-//   8192^2  4.8 GB  567.1  521.5  557.5  659.7      8192^2  2.4 GB  506.4  387.3  343.6  337.5
-// (BGK; MRT follows BGK; CM prefers H=32 over 16 in FP32 from ~0.7 GB: 8192^2 572.9 vs
-// 598.8). Tiles win while the grid stays in the 256 MB Infinity Cache: strips then have too
-// few rows per workgroup for the march to pay. FP32 strips are bound by load latency (short
-// strips: more workgroups in flight), 16-bit strips by VALU (long strips: fewer recomputed
-// halo rows).
+// both buffers; strips of 4 wavefronts x H rows, 6 (FP32) / 7 (16-bit) wavefronts per SIMD):
+//   FP32 BGK        tiles   H=12   H=16   H=24   H=32      CM: tiles  H=12   H=16
+//   1448^2  151 MB   16.4   18.0   17.0   16.1   16.6          19.1   17.5   17.4
+//   2048^2  302 MB   39.3   35.3   35.0   37.3   36.5          44.0   37.3   36.5
+//   2896^2  604 MB   77.1   69.8   72.3   78.9   80.4          87.8   71.6   72.4
+//   4096^2  1.2 GB  151.5  134.6  139.3  149.5  153.8         170.1  137.2  138.4
+//   5792^2  2.4 GB  297.8  258.5  271.3  302.8  308.7         337.5  270.4  270.4
+//   8192^2  4.8 GB  580.6  508.0  522.5  591.3  615.7         659.3  531.8  521.6
+//   (MRT follows BGK within 2 %.)
+//   16-bit BGK      tiles   H=16   H=24   H=32   H=48   H=64   (MRT, CM best H in brackets)
+//   1448^2   75 MB   16.4   16.6   16.4   18.1   21.5   27.3   (16, 16)
+//   2048^2  151 MB   30.8   26.1   25.9   26.3   27.6   31.5   (16, 16)
+//   2896^2  302 MB   65.6   47.8   47.7   48.4   50.7   56.9   (32, 32)
+//   4096^2  604 MB  132.8   94.1   90.2   88.3   92.8   92.9   (32, 32)
+//   5792^2  1.2 GB  258.4  177.2  168.3  164.6  167.5  171.1   (48, 48)
+//   8192^2  2.4 GB  508.7  354.1  328.3  324.2  320.8  321.9   (64, 64)
+// Tiles win only for grids well inside the 256 MB Infinity Cache (1024^2: 9.5 us against
+// 10.6 for strips), where a strip has too few rows for the march to pay. FP32 strips are
+// bound by load latency (short strips: more workgroups in flight), 16-bit strips by VALU
+// (longer strips: fewer recomputed halo rows).
 Step2Policy step2_policy(long long plane, int storage, int coll) {
     const double bytes = 2.0 * 9.0 * double(plane) * (storage == LBM_F32 ? 4.0 : 2.0);
     Step2Policy lp;
@@ -456,18 +459,18 @@ Step2Policy step2_policy(long long plane, int storage, int coll) {
     lp.strip = 1;
     lp.w = 4;
     if (storage == LBM_F32) {
-        if (bytes <= 256.0e6) {
+        if (bytes <= 100.0e6 || (bytes <= 256.0e6 && coll != LBM_CM)) {
             lp.strip = 0;
             lp.ty = 6;
         } else {
-            lp.h = (coll == LBM_CM && bytes > 700.0e6) ? 32 : 16;
+            lp.h = coll == LBM_CM ? 16 : 12;
         }
     } else {
-        if (bytes <= 100.0e6) {
+        if (bytes <= 64.0e6) {
             lp.strip = 0;
             lp.ty = 6;
         } else {
-            lp.h = bytes <= 200.0e6 ? 16 : (bytes <= 1.6e9 ? 32 : 64);
+            lp.h = bytes <= 200.0e6 ? 16 : (bytes <= 700.0e6 ? 32 : (bytes <= 1.6e9 ? 48 : 64));
         }
     }
     return lp;
