@@ -204,6 +204,17 @@ def run_poiseuille(torch, steps=400, warmup=20, adapter_steps=100):
             "workload": "Poiseuille channel 4096x1024, nu=0.05, Guo force for u_max=0.05, periodic x, bounce-back walls"}
 
 
+def run_ibm_cylinder(torch, steps=200, warmup=20):
+    """The IBM cylinder channel (2048x512 CM, 201 markers, 3 multi-direct forcing iterations
+    per step): the IBM kernel and a one-step fused launch per step."""
+    from lbm_amd import scenarios as sc
+
+    s, r = run_case(sc.ibm_cylinder(), steps, warmup, torch)
+    s.close()
+    return dict(r, workload="IBM cylinder 2048x512 CM, 201 markers, 3 IBM iterations per step "
+                            "(IBM kernel + fused step per step)")
+
+
 def rel_l2(a, b):
     import numpy as np
 
@@ -262,7 +273,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-extras", action="store_true", help="only the headline workload")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["hbm_grid", "adjoint", "poiseuille"], default=None,
+    ap.add_argument("--only", choices=["hbm_grid", "adjoint", "poiseuille", "ibm_cylinder"], default=None,
                     help="run one extra section alone (for profiling its kernels); prints that section")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="also run one plain device copy of MB megabytes (for scaling profiler byte counters)")
@@ -295,6 +306,9 @@ def main():
         return
     if args.only == "poiseuille":
         print(json.dumps({"poiseuille": run_poiseuille(torch)}))
+        return
+    if args.only == "ibm_cylinder":
+        print(json.dumps({"ibm_cylinder": run_ibm_cylinder(torch)}))
         return
 
     # headline: LDC 1024^2 MRT FP32
@@ -349,11 +363,7 @@ def main():
         grid["speedup_f16_vs_f32"] = round(grid["f16"]["mlups"] / grid["f32"]["mlups"], 3)
         grid["workload"] = "periodic BGK 8192x8192 (2 x 2.4 GB FP32 population buffers)"
         out["hbm_grid"] = grid
-        cyl = sc.ibm_cylinder()
-        s, rc = run_case(cyl, 200, 20, torch)
-        s.close()
-        out["ibm_cylinder"] = dict(rc, workload="IBM cylinder 2048x512 CM, 201 markers, 3 IBM iterations per step "
-                                                "(IBM kernel + fused step per step)")
+        out["ibm_cylinder"] = run_ibm_cylinder(torch)
         out["poiseuille"] = run_poiseuille(torch)
         out["adjoint"] = run_adjoint(torch)
         out["accuracy"] = run_accuracy(torch)

@@ -192,15 +192,58 @@ int lbm_ctx::two_step(bool write_macros) {
     return LBM_OK;
 }
 
+// One step of an IBM run whose box avoids non-periodic edges, in two concurrent parts: the
+// step outside the IBM box on aux_stream (it reads neither the IBM force nor anything the
+// IBM pass writes) and, on `stream`, the IBM pass followed by the box's step. Both parts
+// read buf[cur] and write disjoint nodes of buf[1-cur]; `stream` waits for aux_stream
+// before anything later runs.
+int lbm_ctx::split_ibm_step(bool write_macros) {
+    hipError_t e;
+    if (!aux_stream) {
+        e = hipStreamCreateWithFlags(&aux_stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_join, hipEventDisableTiming);
+        if (e != hipSuccess) return hip_fail(e, "IBM stream set-up");
+    }
+    StepParams p = step_params(cur, write_macros);
+    LaunchPolicy lp = step_policy(plane, cfg.storage);
+    // The IBM pass (one workgroup) is submitted before the outside step, so that it has its
+    // compute unit before the step's workgroups fill the device.
+    e = hipEventRecord(ev_fork, stream);
+    if (e != hipSuccess) return hip_fail(e, "IBM fork");
+    int rc = ibm_pass(cur);
+    if (rc) return rc;
+    e = hipStreamWaitEvent(aux_stream, ev_fork, 0);
+    if (e != hipSuccess) return hip_fail(e, "IBM fork");
+    p.region = 1;
+    e = launch_step(cfg.collision, cfg.storage, 0, p, lp, partials, aux_stream);
+    if (e != hipSuccess) return hip_fail(e, "step launch (outside the IBM box)");
+    e = hipEventRecord(ev_join, aux_stream);
+    if (e != hipSuccess) return hip_fail(e, "IBM join");
+    p.region = 2;
+    e = launch_step(cfg.collision, cfg.storage, 0, p, lp, partials, stream);
+    if (e != hipSuccess) {
+        (void)hipStreamWaitEvent(stream, ev_join, 0);
+        return hip_fail(e, "step launch (IBM box)");
+    }
+    e = hipStreamWaitEvent(stream, ev_join, 0);
+    if (e != hipSuccess) return hip_fail(e, "IBM join");
+    cur = 1 - cur;
+    timestep++;
+    if (write_macros) macros_valid = true;
+    return LBM_OK;
+}
+
 // One step from buf[cur] into buf[1-cur].
 int lbm_ctx::one_step(bool write_macros) {
     int rc = restore_edges();
     if (rc) return rc;
+    bool adapt = cfg.collision == LBM_CM && cfg.adapter == LBM_OPTIMAL_ADAPTER;
+    if (ibm.active && ibm.split_ok && !adapt) return split_ibm_step(write_macros);
     if (ibm.active) {
         rc = ibm_pass(cur);
         if (rc) return rc;
     }
-    bool adapt = cfg.collision == LBM_CM && cfg.adapter == LBM_OPTIMAL_ADAPTER;
     bool lagged = adapt && cfg.adapter_mode == LBM_ADAPTER_LAGGED;
     if (adapt && (!lagged || !avg_valid)) {
         rc = adapter_pass(cur);
@@ -230,12 +273,16 @@ lbm_ctx::~lbm_ctx() {
     (void)hipSetDevice(device);
     if (own_stream) (void)hipStreamSynchronize(own_stream);
     if (stream != own_stream) (void)hipStreamSynchronize(stream);
+    if (aux_stream) (void)hipStreamSynchronize(aux_stream);
     void* ptrs[] = {buf[0],  buf[1], d_flags, d_row_class, d_force_field, d_rho,     d_ux,
                     d_uy,    d_avg,  d_avg64, partials,    tmp,           d_edge[0], d_edge[1]};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     ibm_free();
     adjoint_free();
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (aux_stream) (void)hipStreamDestroy(aux_stream);
     if (own_stream) (void)hipStreamDestroy(own_stream);
 }
 
@@ -467,6 +514,7 @@ int lbm_set_force_field(lbm_ctx* ctx, const float* force, int on_device) {
     if (!ctx) return LBM_E_INVALID;
     int rc = ctx->activate();
     if (rc) return rc;
+    ctx->ibm.box_valid = false;  // the IBM box's external force changes with the field
     if (!force) {
         if (ctx->d_force_field) {
             (void)hipStreamSynchronize(ctx->stream);

@@ -35,7 +35,11 @@ struct IbmState {
     int* d_box_map = nullptr;              // band index or -1 per cell of the box
     float* d_lag_force = nullptr;          // [n][2] summed over the iterations of the last step
     double* d_force_sum = nullptr;         // [2]
-    size_t shmem = 0;
+    bool box_valid = false;                // box cells outside the band hold the external force
+    bool split_ok = false;                 // the box avoids non-periodic domain edges
+    int n_ent = 0;                         // stencil (= spread) table entries
+    size_t shmem = 0;                      // LDS of a pass reading its tables from memory
+    size_t shmem_tables = 0;               // LDS of a pass with the tables staged, or 0 if too large
     std::vector<float> h_ds;               // arc lengths (host copy, for the force sum)
 };
 
@@ -58,6 +62,10 @@ struct lbm_ctx {
     int device = 0;
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
+    // IBM steps run the step outside the IBM box on aux_stream, concurrently with the IBM
+    // pass and the box's step on `stream` (fork and join through the two events)
+    hipStream_t aux_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     size_t N = 0;
     long long plane = 0;
     void* buf[2] = {nullptr, nullptr};
@@ -97,6 +105,7 @@ struct lbm_ctx {
     lbm::StepParams step_params(int src_idx, bool write_macros) const;
     int adapter_pass(int src_idx);
     int one_step(bool write_macros);
+    int split_ibm_step(bool write_macros);
     bool two_step_ok() const;
     int two_step(bool write_macros);
     int restore_edges();

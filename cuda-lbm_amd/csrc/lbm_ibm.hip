@@ -9,7 +9,9 @@
 // workgroup for all iterations. Spreading is a gather by band node over a precomputed
 // (marker, weight) list in marker order, so the sum is deterministic and equals the
 // reference's scatter order. The result is the total node force on the band's bounding
-// box, which the fused step reads there.
+// box, which the fused step reads there. Box cells outside the band hold the external force,
+// which changes only with the force field: they are written on the first pass after the
+// markers or the field change, and later passes write the band alone.
 #include "lbm_ctx.hpp"
 
 #include <algorithm>
@@ -27,7 +29,9 @@ struct IbmParams {
     NodeArgs na;
     int storage;
     int n_markers, n_band, iters, clamp;
+    int n_ent;                    // stencil entries (= spread entries)
     int x0, y0, bw, bh;
+    int write_box;                // 1: every box cell, 0: the band's cells only
     float fx, fy;                 // uniform force
     const float* force_field;     // AoS or null
     const int* band_node;
@@ -42,7 +46,9 @@ struct IbmParams {
     float* lag_force;             // AoS [n][2], summed over iterations
 };
 
-template <int STORE>
+// TABLES_LDS: the interpolation and spreading tables are staged in LDS once per pass, so
+// the iterations run from LDS only (used whenever they fit beside the band data).
+template <int STORE, bool TABLES_LDS>
 __global__ __launch_bounds__(kIbmBlock) void ibm_kernel(IbmParams p) {
     extern __shared__ float lds[];
     const int nb = p.n_band, nm = p.n_markers;
@@ -55,12 +61,42 @@ __global__ __launch_bounds__(kIbmBlock) void ibm_kernel(IbmParams p) {
     float* s_ly = s_lx + nm;
     float* s_sx = s_ly + nm;   // summed over iterations
     float* s_sy = s_sx + nm;
+    int* s_node = reinterpret_cast<int*>(s_sy + nm);  // global node of each band entry
     const int nx = p.na.nx;
+    const int* mk_ptr = p.mk_ptr;
+    const int* mk_band = p.mk_band;
+    const float* mk_w = p.mk_w;
+    const int* bn_ptr = p.bn_ptr;
+    const int* bn_mk = p.bn_mk;
+    const float* bn_w = p.bn_w;
+    if constexpr (TABLES_LDS) {
+        int* t_mk_ptr = s_node + nb;
+        int* t_bn_ptr = t_mk_ptr + nm + 1;
+        int* t_mk_band = t_bn_ptr + nb + 1;
+        int* t_bn_mk = t_mk_band + p.n_ent;
+        float* t_mk_w = reinterpret_cast<float*>(t_bn_mk + p.n_ent);
+        float* t_bn_w = t_mk_w + p.n_ent;
+        for (int i = threadIdx.x; i <= nm; i += kIbmBlock) t_mk_ptr[i] = p.mk_ptr[i];
+        for (int i = threadIdx.x; i <= nb; i += kIbmBlock) t_bn_ptr[i] = p.bn_ptr[i];
+        for (int i = threadIdx.x; i < p.n_ent; i += kIbmBlock) {
+            t_mk_band[i] = p.mk_band[i];
+            t_bn_mk[i] = p.bn_mk[i];
+            t_mk_w[i] = p.mk_w[i];
+            t_bn_w[i] = p.bn_w[i];
+        }
+        mk_ptr = t_mk_ptr;
+        mk_band = t_mk_band;
+        mk_w = t_mk_w;
+        bn_ptr = t_bn_ptr;
+        bn_mk = t_bn_mk;
+        bn_w = t_bn_w;
+    }
 
     // post-stream macroscopics of the band (uncorrected_macroscopics_kernel<2>,
     // macroscopics.cu:5-38) and the external force (reset_forces, macroscopics.cuh:13-48)
     for (int b = threadIdx.x; b < nb; b += kIbmBlock) {
         int node = p.band_node[b];
+        s_node[b] = node;
         int y = node / nx, x = node - y * nx;
         Pop9 g = gather_node<STORE>(p.na, x, y);
         float r = 0.f, jx = 0.f, jy = 0.f;
@@ -90,9 +126,9 @@ __global__ __launch_bounds__(kIbmBlock) void ibm_kernel(IbmParams p) {
         // IBM_impl.cuh:7-26): u_L, rho_L over the stencil, F_L = 2 rho_L (0 - u_L)
         for (int l = threadIdx.x; l < nm; l += kIbmBlock) {
             float ux = 0.f, uy = 0.f, r = 0.f;
-            for (int e = p.mk_ptr[l]; e < p.mk_ptr[l + 1]; e++) {
-                float k = p.mk_w[e];
-                int b = p.mk_band[e];
+            for (int e = mk_ptr[l]; e < mk_ptr[l + 1]; e++) {
+                float k = mk_w[e];
+                int b = mk_band[e];
                 r += k * s_rho[b];
                 ux += k * s_ux[b];
                 uy += k * s_uy[b];
@@ -112,9 +148,9 @@ __global__ __launch_bounds__(kIbmBlock) void ibm_kernel(IbmParams p) {
         // correct_velocities_kernel and accumulate_forces_kernel (IBM_impl.cuh:28-68)
         for (int b = threadIdx.x; b < nb; b += kIbmBlock) {
             float fx = 0.f, fy = 0.f;
-            for (int e = p.bn_ptr[b]; e < p.bn_ptr[b + 1]; e++) {
-                float w = p.bn_w[e];
-                int l = p.bn_mk[e];
+            for (int e = bn_ptr[b]; e < bn_ptr[b + 1]; e++) {
+                float w = bn_w[e];
+                int l = bn_mk[e];
                 fx += w * s_lx[l];
                 fy += w * s_ly[l];
             }
@@ -133,7 +169,14 @@ __global__ __launch_bounds__(kIbmBlock) void ibm_kernel(IbmParams p) {
     }
 
     // total force on the band's box: band nodes from LDS, the rest the external force
-    const int ncell = p.bw * p.bh;
+    if (!p.write_box) {
+        for (int b = threadIdx.x; b < nb; b += kIbmBlock) {
+            const long long node = s_node[b];
+            p.force_total[2 * node] = s_fx[b];
+            p.force_total[2 * node + 1] = s_fy[b];
+        }
+    }
+    const int ncell = p.write_box ? p.bw * p.bh : 0;
     for (int c = threadIdx.x; c < ncell; c += kIbmBlock) {
         int yy = p.y0 + c / p.bw, xx = p.x0 + c % p.bw;
         long long node = (long long)yy * nx + xx;
@@ -212,15 +255,23 @@ int lbm_ctx::ibm_pass(int src_idx) {
     p.box_map = ibm.d_box_map;
     p.force_total = ibm.d_force_total;
     p.lag_force = ibm.d_lag_force;
-    size_t sh = ibm.shmem;
-    if (cfg.storage == LBM_F16_SHIFTED)
-        hipLaunchKernelGGL(ibm_kernel<LBM_F16_SHIFTED>, dim3(1), dim3(kIbmBlock), sh, stream, p);
-    else if (cfg.storage == LBM_BF16_SHIFTED)
-        hipLaunchKernelGGL(ibm_kernel<LBM_BF16_SHIFTED>, dim3(1), dim3(kIbmBlock), sh, stream, p);
-    else
-        hipLaunchKernelGGL(ibm_kernel<LBM_F32>, dim3(1), dim3(kIbmBlock), sh, stream, p);
+    p.write_box = ibm.box_valid ? 0 : 1;
+    p.n_ent = ibm.n_ent;
+    const bool tl = ibm.shmem_tables > 0;
+    size_t sh = tl ? ibm.shmem_tables : ibm.shmem;
+    if (cfg.storage == LBM_F16_SHIFTED) {
+        if (tl) hipLaunchKernelGGL((ibm_kernel<LBM_F16_SHIFTED, true>), dim3(1), dim3(kIbmBlock), sh, stream, p);
+        else hipLaunchKernelGGL((ibm_kernel<LBM_F16_SHIFTED, false>), dim3(1), dim3(kIbmBlock), sh, stream, p);
+    } else if (cfg.storage == LBM_BF16_SHIFTED) {
+        if (tl) hipLaunchKernelGGL((ibm_kernel<LBM_BF16_SHIFTED, true>), dim3(1), dim3(kIbmBlock), sh, stream, p);
+        else hipLaunchKernelGGL((ibm_kernel<LBM_BF16_SHIFTED, false>), dim3(1), dim3(kIbmBlock), sh, stream, p);
+    } else {
+        if (tl) hipLaunchKernelGGL((ibm_kernel<LBM_F32, true>), dim3(1), dim3(kIbmBlock), sh, stream, p);
+        else hipLaunchKernelGGL((ibm_kernel<LBM_F32, false>), dim3(1), dim3(kIbmBlock), sh, stream, p);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "IBM launch");
+    ibm.box_valid = true;
     return LBM_OK;
 }
 
@@ -292,6 +343,9 @@ int lbm_ibm_set_markers(lbm_ctx* ctx, const float* xy, const float* ds, int32_t 
         }
         bn_ptr[b + 1] = int(bn_mk.size());
     }
+    // the box's x range in whole 4-node segments, so that one-step launches can split at it
+    xmin = xmin / 4 * 4;
+    xmax = std::min(nx, (xmax + 1 + 3) / 4 * 4) - 1;
     const int bw = xmax - xmin + 1, bh = ymax - ymin + 1;
     std::vector<int> box_map(size_t(bw) * bh, -1);
     for (int b = 0; b < nb; b++) {
@@ -310,7 +364,7 @@ int lbm_ibm_set_markers(lbm_ctx* ctx, const float* xy, const float* ds, int32_t 
     for (int v : box_map)
         if (v < -1 || v >= nb) return ctx->fail(LBM_E_INVALID, "IBM box map out of range");
     if (xmin < 0 || ymin < 0 || xmax >= nx || ymax >= ny) return ctx->fail(LBM_E_INVALID, "IBM box outside the grid");
-    size_t shmem = (size_t(5) * nb + size_t(4) * n) * sizeof(float);
+    size_t shmem = (size_t(5) * nb + size_t(4) * n) * sizeof(float) + size_t(nb) * sizeof(int);
     int maxsh = 0;
     (void)hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
     if (shmem > size_t(maxsh))
@@ -346,6 +400,26 @@ int lbm_ibm_set_markers(lbm_ctx* ctx, const float* xy, const float* ds, int32_t 
     I.y0 = ymin;
     I.y1 = ymax + 1;
     I.shmem = shmem;
+    I.split_ok = (ctx->cfg.periodic_x || (I.x0 > 0 && I.x1 < nx)) && (ctx->cfg.periodic_y || (I.y0 > 0 && I.y1 < ny));
+    I.n_ent = int(mk_band.size());
+    // the tables beside the band data: mk_ptr, bn_ptr, mk_band, bn_mk (ints), mk_w, bn_w
+    const size_t tables = (size_t(n) + 1 + size_t(nb) + 1 + 4 * mk_band.size()) * 4;
+    I.shmem_tables = shmem + tables <= size_t(maxsh) ? shmem + tables : 0;
+    if (I.shmem_tables > 0) {
+        const int want = int(I.shmem_tables);
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&ibm_kernel<LBM_F32, true>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, want);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&ibm_kernel<LBM_F16_SHIFTED, true>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, want);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&ibm_kernel<LBM_BF16_SHIFTED, true>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, want);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            I.shmem_tables = 0;  // read the tables from memory instead
+        }
+    }
     I.h_ds = dsv;
     I.active = true;
     return LBM_OK;

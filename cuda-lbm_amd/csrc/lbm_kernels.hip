@@ -332,6 +332,11 @@ long long step_grid_blocks(const StepParams& p, const LaunchPolicy& lp) {
 template <int COLL, int STORE, int MODE, int SEG>
 static hipError_t launch_step_t(StepParams p, const LaunchPolicy& lp, double* partials, hipStream_t s) {
     long long nb = step_grid_blocks(p, lp);
+    if (p.region == 2) {  // the IBM box only
+        const long long nseg = (long long)((p.ibm_x1 - p.ibm_x0 + SEG - 1) / SEG) * (p.ibm_y1 - p.ibm_y0);
+        nb = (nseg + lp.block - 1) / lp.block;
+        if (lp.remap) nb = (nb + kXcds - 1) / kXcds * kXcds;
+    }
     p.ntx = (p.nx + SEG - 1) / SEG;
     p.nt = lp.nt;
     hipLaunchKernelGGL((fused_step_kernel<COLL, STORE, MODE, SEG>), dim3(unsigned(nb)), dim3(lp.block), 0, s, p, partials,
@@ -360,6 +365,10 @@ static hipError_t launch_step_s(int coll, int mode, const StepParams& p, const L
 hipError_t launch_step(int coll, int storage, int mode, const StepParams& p, const LaunchPolicy& lp, double* partials,
                        hipStream_t s) {
     if (lp.block < 64 || lp.block > kMaxStepBlock || lp.block % 64 != 0) return hipErrorInvalidValue;
+    // a split step needs an IBM box whose x range is whole segments, and no partial sums
+    if (p.region != 0 && (mode != 0 || !p.ibm_force || p.ibm_x0 % 4 != 0 || (p.ibm_x1 % 4 != 0 && p.ibm_x1 != p.nx) ||
+                          p.ibm_x0 >= p.ibm_x1 || p.ibm_y0 >= p.ibm_y1))
+        return hipErrorInvalidValue;
     if (storage == LBM_F16_SHIFTED) return launch_step_s<LBM_F16_SHIFTED>(coll, mode, p, lp, partials, s);
     if (storage == LBM_BF16_SHIFTED) return launch_step_s<LBM_BF16_SHIFTED>(coll, mode, p, lp, partials, s);
     return launch_step_s<LBM_F32>(coll, mode, p, lp, partials, s);
@@ -442,6 +451,8 @@ hipError_t launch_step2(int coll, int storage, const StepParams& p, const Step2P
 //   8192^2  4.8 GB  580.6  508.0  522.5  591.3  615.7         659.3  531.8  521.6
 //   (MRT follows BGK within 2 %.)
 //   16-bit BGK      tiles   H=16   H=24   H=32   H=48   H=64   (MRT, CM best H in brackets)
+//   1024^2   38 MB    9.0   10.3     -      -      -      -     (tiles, tiles)
+//   1280^2   59 MB   12.7   12.2     -      -      -      -     (16, 16)
 //   1448^2   75 MB   16.4   16.6   16.4   18.1   21.5   27.3   (16, 16)
 //   2048^2  151 MB   30.8   26.1   25.9   26.3   27.6   31.5   (16, 16)
 //   2896^2  302 MB   65.6   47.8   47.7   48.4   50.7   56.9   (32, 32)
@@ -466,7 +477,7 @@ Step2Policy step2_policy(long long plane, int storage, int coll) {
             lp.h = coll == LBM_CM ? 16 : 12;
         }
     } else {
-        if (bytes <= 64.0e6) {
+        if (bytes <= 48.0e6) {
             lp.strip = 0;
             lp.ty = 6;
         } else {

@@ -42,6 +42,8 @@ struct StepParams {
     const float* force_field;    // AoS [N][2] or null
     const float* ibm_force;      // AoS [N][2] total force inside the IBM box, or null
     int ibm_x0, ibm_x1, ibm_y0, ibm_y1;
+    int region;                  // one-step launches: 0 every node, 1 the nodes outside the IBM
+                                 // box, 2 the box only (its x range is a multiple of 4 nodes)
     float wall_ux, wall_uy, u_lid;
     unsigned quirks;
     const float* avg;            // adapter averages (3) or null
@@ -569,10 +571,20 @@ LBM_HD void step_segment(const StepParams& p, long long seg, MomAcc& acc) {
     using T = typename St::T;
     using Vec = typename St::Vec;
     constexpr int V = St::V;
-    const long long nseg = (long long)p.ntx * p.ny;
-    if (seg >= nseg) return;
-    const int y = int(seg / p.ntx);
-    const int x0 = int(seg - (long long)y * p.ntx) * V;
+    int y, x0;
+    if (p.region == 2) {  // segments of the IBM box, row by row
+        const int bs = (p.ibm_x1 - p.ibm_x0 + V - 1) / V;
+        if (seg >= (long long)bs * (p.ibm_y1 - p.ibm_y0)) return;
+        const int r = int(seg / bs);
+        y = p.ibm_y0 + r;
+        x0 = p.ibm_x0 + int(seg - (long long)r * bs) * V;
+    } else {
+        const long long nseg = (long long)p.ntx * p.ny;
+        if (seg >= nseg) return;
+        y = int(seg / p.ntx);
+        x0 = int(seg - (long long)y * p.ntx) * V;
+        if (p.region == 1 && y >= p.ibm_y0 && y < p.ibm_y1 && x0 >= p.ibm_x0 && x0 < p.ibm_x1) return;
+    }
     const long long row0 = (long long)y * p.nx;
     const T* src = static_cast<const T*>(p.src);
     T* dst = static_cast<T*>(p.dst);

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 using namespace lbm;
@@ -293,6 +294,67 @@ extern "C" int emu_run(const lbm_config* c, int seg, int two_ty, int reverse, co
     return run_s<LBM_F32>(*c, seg, flags, rho, u, force, nsteps, f_out, rho_out, u_out, avg_out, two_ty, reverse);
 }
 
+// One one-step launch over every node (f_full) against the same step split at an IBM box
+// (x range a multiple of 4 nodes, or ending at nx): every node outside it, then the box
+// alone (f_split, on a poisoned destination). FP32 storage, periodic, all FLUID, with a
+// force field inside the box standing in for the IBM force. seg: nodes per thread (1 or 4).
+template <int COLL, int SEG>
+void region_run(StepParams p, float* dst_full, float* dst_split, long long N, int bx0, int bx1, int by0, int by1) {
+    MomAcc dummy{0.0, 0.0, 0.0};
+    p.ntx = (p.nx + SEG - 1) / SEG;
+    const long long nseg = (long long)p.ntx * p.ny;
+    p.dst = dst_full;
+    p.region = 0;
+    for (long long sg = 0; sg < nseg; sg++) step_segment<COLL, LBM_F32, 0, SEG>(p, sg, dummy);
+    p.dst = dst_split;
+    p.region = 1;
+    for (long long sg = 0; sg < nseg; sg++) step_segment<COLL, LBM_F32, 0, SEG>(p, sg, dummy);
+    p.region = 2;
+    const long long nbox = (long long)((bx1 - bx0 + SEG - 1) / SEG) * (by1 - by0);
+    for (long long sg = 0; sg < nbox; sg++) step_segment<COLL, LBM_F32, 0, SEG>(p, sg, dummy);
+    (void)N;
+}
+
+extern "C" int emu_region_split(const lbm_config* c, int seg, int bx0, int bx1, int by0, int by1, const float* f_in,
+                                const float* box_force, float* f_full, float* f_split) {
+    if (c->storage != LBM_F32 || !c->periodic_x || !c->periodic_y) return -1;
+    if (bx0 % 4 != 0 || (bx1 % 4 != 0 && bx1 != c->nx) || bx0 >= bx1 || by0 >= by1 || bx1 > c->nx || by1 > c->ny)
+        return -2;
+    const long long N = (long long)c->nx * c->ny;
+    std::vector<float> a(f_in, f_in + 9 * N), full(size_t(9) * N, poison<float>()), split(size_t(9) * N, poison<float>());
+    std::vector<float> ibm(box_force, box_force + 2 * N);
+    StepParams p{};
+    p.nx = c->nx;
+    p.ny = c->ny;
+    p.px = 1;
+    p.py = 1;
+    p.plane = N;
+    p.src = a.data();
+    p.omega = c->omega;
+    for (int k = 0; k < 9; k++) p.S[k] = c->S[k];
+    p.rho0 = c->rho0;
+    p.fx = c->force[0];
+    p.fy = c->force[1];
+    p.forced = 1;
+    p.quirks = c->quirks;
+    p.ibm_force = ibm.data();
+    p.ibm_x0 = bx0;
+    p.ibm_x1 = bx1;
+    p.ibm_y0 = by0;
+    p.ibm_y1 = by1;
+    auto go = [&](auto coll_tag) {
+        constexpr int COLL = decltype(coll_tag)::value;
+        if (seg == 4) region_run<COLL, 4>(p, full.data(), split.data(), N, bx0, bx1, by0, by1);
+        else region_run<COLL, 1>(p, full.data(), split.data(), N, bx0, bx1, by0, by1);
+    };
+    if (c->collision == LBM_BGK) go(std::integral_constant<int, LBM_BGK>{});
+    else if (c->collision == LBM_MRT) go(std::integral_constant<int, LBM_MRT>{});
+    else go(std::integral_constant<int, LBM_CM>{});
+    std::memcpy(f_full, full.data(), 9 * N * sizeof(float));
+    std::memcpy(f_split, split.data(), 9 * N * sizeof(float));
+    return 0;
+}
+
 // One two-step launch from FP32 populations f_in ([9][N], fully periodic, all FLUID, FP32
 // storage) with the intermediate-state output on: f_mid receives state t+1, f_out state t+2.
 // form 0: tiles of 6 rows; 1: strips of 4 wavefronts x 8 rows.
@@ -437,6 +499,34 @@ int main() {
             }
         }
     }
+    // split IBM steps: every node outside the box, then the box, on exact-size buffers
+    for (int coll = 0; coll < 3; coll++)
+        for (int sg : {1, 4}) {
+            const int boxes[][6] = {{45, 19, 8, 24, 5, 14}, {45, 19, 0, 8, 0, 3}, {45, 19, 36, 45, 10, 19},
+                                    {9, 4, 4, 9, 0, 4}, {130, 9, 64, 128, 2, 7}};
+            for (auto& b : boxes) {
+                lbm_config c;
+                std::memset(&c, 0, sizeof(c));
+                c.nx = b[0];
+                c.ny = b[1];
+                c.collision = coll;
+                c.omega = 1.2f;
+                for (int k = 0; k < 9; k++) c.S[k] = 1.1f;
+                c.rho0 = 1.0f;
+                c.periodic_x = c.periodic_y = 1;
+                c.force[0] = 1e-5f;
+                const long long N = (long long)c.nx * c.ny;
+                std::vector<float> f(size_t(9) * N), F(size_t(2) * N, 1e-5f), o1(size_t(9) * N), o2(size_t(9) * N);
+                for (long long n = 0; n < 9 * N; n++) f[size_t(n)] = wgt(int(n / N)) * (1.0f + 1e-3f * float(n % 7));
+                if (emu_region_split(&c, sg, b[2], b[3], b[4], b[5], f.data(), F.data(), o1.data(), o2.data()) != 0 ||
+                    std::memcmp(o1.data(), o2.data(), o1.size() * sizeof(float)) != 0) {
+                    printf("region split mismatch: %dx%d box %d..%d x %d..%d seg %d coll %d\n", b[0], b[1], b[2], b[3],
+                           b[4], b[5], sg, coll);
+                    return 1;
+                }
+                cases++;
+            }
+        }
     std::printf("emu bounds matrix: %d cases ok\n", cases);
     return 0;
 }

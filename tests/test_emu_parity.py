@@ -141,3 +141,35 @@ def test_emulated_two_step_intermediate_state(form, coll):
     c = case.config.to_c()
     assert L.emu_two_step_mid(ctypes.byref(c), form, fi.ctypes.data, fm.ctypes.data, fo.ctypes.data) == 0
     assert np.array_equal(fm, f1) and np.array_equal(fo, f2)
+
+
+@pytest.mark.parametrize("seg", [1, 4])
+@pytest.mark.parametrize("box", [(8, 24, 5, 14), (0, 8, 0, 3), (36, 45, 10, 19)], ids=["inner", "corner", "to_nx"])
+@pytest.mark.parametrize("coll", [A.LBM_BGK, A.LBM_MRT, A.LBM_CM])
+def test_emulated_ibm_split_step_equals_whole_step(coll, box, seg):
+    # IBM steps run the nodes outside the IBM box and the box itself as two launches (the
+    # former concurrently with the IBM pass): together they must write every node exactly as
+    # one launch does. 45 columns: the "to_nx" box ends at a ragged nx.
+    import ctypes
+
+    from helpers import emu_lib
+
+    case = layout_case("periodic", 45, 19, coll)
+    f0, *_ = run_emu(case, 2, seg=1)
+    rng = np.random.default_rng(3)
+    force = np.zeros((19, 45, 2), np.float32)
+    bx0, bx1, by0, by1 = box
+    force[by0:by1, bx0:bx1] = rng.normal(0.0, 1e-4, (by1 - by0, bx1 - bx0, 2)).astype(np.float32)
+    L = emu_lib()
+    L.emu_region_split.restype = ctypes.c_int
+    L.emu_region_split.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5 + [ctypes.c_void_p] * 4
+    fi = np.ascontiguousarray(f0, np.float32)
+    full = np.zeros_like(fi)
+    split = np.zeros_like(fi)
+    c = case.config.to_c()
+    c.force = (ctypes.c_float * 2)(1e-5, 0.0)
+    fz = np.ascontiguousarray(force)
+    rc = L.emu_region_split(ctypes.byref(c), seg, bx0, bx1, by0, by1, fi.ctypes.data, fz.ctypes.data,
+                            full.ctypes.data, split.ctypes.data)
+    assert rc == 0
+    assert np.all(np.isfinite(full)) and np.array_equal(full, split)
