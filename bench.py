@@ -215,6 +215,52 @@ def run_ibm_cylinder(torch, steps=200, warmup=20):
                             "(IBM kernel + fused step per step)")
 
 
+def run_d3q27(torch, steps=200, warmup=20):
+    """The D3Q27 BGK step (one fused pull-scheme launch per step, 27 planes in and out):
+    Taylor-Green 3-D at the reference scenario's 128^3, a 256^3 periodic grid whose 2 x 1.8 GB
+    FP32 buffers stream from HBM (FP32 and FP16 storage), and a 256x128x96 channel between
+    bounce-back walls (the reference wind tunnel's grid)."""
+    import numpy as np
+
+    from lbm_amd import _abi as A
+    from lbm_amd.solver3d import Config3D, Solver3D
+
+    def tg_init(n):
+        k = 2 * np.pi / n
+        z, y, x = np.meshgrid(*(np.arange(n, dtype=np.float32),) * 3, indexing="ij")
+        u = np.stack([0.05 * np.sin(k * x) * np.cos(k * y) * np.cos(k * z),
+                      -0.05 * np.cos(k * x) * np.sin(k * y) * np.cos(k * z), np.zeros_like(x)], -1)
+        return np.ones((n, n, n), np.float32), u.astype(np.float32)
+
+    def one(cfg, init, flags=None, n_steps=steps):
+        with Solver3D(cfg) as s:
+            s.init_from_macros(*init)
+            if flags is not None:
+                s.set_flags(flags)
+            secs = time_steps(s, n_steps, warmup, torch)
+            B = 2 * 27 * (4 if cfg.storage == A.LBM_F32 else 2)
+            gbs = B * cfg.N / (secs / n_steps) / 1e9
+            return {"mlups": round(cfg.N * n_steps / secs / 1e6, 1), "us_per_step": round(1e6 * secs / n_steps, 2),
+                    "GB_s": round(gbs, 1), "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_node_update": B}
+
+    per = dict(periodic_x=True, periodic_y=True, periodic_z=True)
+    res = {}
+    res["tg3d_128_f32"] = one(Config3D(128, 128, 128, omega=1.8, **per), tg_init(128))
+    for name, st in (("hbm_256_f32", A.LBM_F32), ("hbm_256_f16", A.LBM_F16_SHIFTED)):
+        res[name] = one(Config3D(256, 256, 256, omega=1.8, storage=st, **per), tg_init(256), n_steps=max(20, steps // 2))
+    res["speedup_f16_vs_f32"] = round(res["hbm_256_f16"]["mlups"] / res["hbm_256_f32"]["mlups"], 3)
+    nx, ny, nz = 256, 128, 96
+    fl = np.zeros((nz, ny, nx), np.uint8)
+    fl[:, 0, :] = fl[:, -1, :] = A.BOUNCE_BACK
+    fl[0], fl[-1] = A.BOUNCE_BACK, A.BOUNCE_BACK
+    ch = Config3D(nx, ny, nz, omega=1.6, periodic_x=True, force=(1e-6, 0.0, 0.0))
+    res["channel_256x128x96_f32"] = one(ch, (np.ones((nz, ny, nx), np.float32), np.zeros((nz, ny, nx, 3), np.float32)), fl)
+    res["workload"] = ("D3Q27 BGK with Guo forcing, FP32 collision; Taylor-Green 3-D (u_max 0.05) periodic, and a "
+                       "force-driven channel with bounce-back walls on y and z; 216 B (FP32) / 108 B (FP16) per "
+                       "node update")
+    return res
+
+
 def rel_l2(a, b):
     import numpy as np
 
@@ -273,7 +319,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-extras", action="store_true", help="only the headline workload")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["hbm_grid", "adjoint", "poiseuille", "ibm_cylinder"], default=None,
+    ap.add_argument("--only", choices=["hbm_grid", "adjoint", "poiseuille", "ibm_cylinder", "d3q27"], default=None,
                     help="run one extra section alone (for profiling its kernels); prints that section")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="also run one plain device copy of MB megabytes (for scaling profiler byte counters)")
@@ -309,6 +355,9 @@ def main():
         return
     if args.only == "ibm_cylinder":
         print(json.dumps({"ibm_cylinder": run_ibm_cylinder(torch)}))
+        return
+    if args.only == "d3q27":
+        print(json.dumps({"d3q27": run_d3q27(torch)}))
         return
 
     # headline: LDC 1024^2 MRT FP32
@@ -367,6 +416,7 @@ def main():
         out["poiseuille"] = run_poiseuille(torch)
         out["adjoint"] = run_adjoint(torch)
         out["accuracy"] = run_accuracy(torch)
+        out["d3q27"] = run_d3q27(torch)
     if args.calibrate_copy:
         out["copy_calibration_bytes"] = copy_calibration(torch, args.calibrate_copy)
     if not args.no_cpu:

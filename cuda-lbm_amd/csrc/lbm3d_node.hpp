@@ -1,0 +1,151 @@
+// lbm3d_node.hpp — node-local D3Q27 physics of the fused 3-D step, FP32 in registers.
+//
+// The 27 populations of one node live in registers. Every loop over directions is unrolled
+// against the constexpr velocity table, so c_i u becomes a signed sum of u components and
+// the c = 0 terms disappear at compile time. The functions are __host__ __device__: the CPU
+// emulation runs the same code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <utility>
+
+#include "../../include/lbm3d_hip.h"
+
+#define LBM3_HD __host__ __device__ __forceinline__
+
+namespace lbm3 {
+
+constexpr int Q = 27;
+// Velocity set in the reference's order: rest, 6 faces, 12 edges, 8 corners
+// (lbm_constants.cuh:76-104), and the opposite directions (lbm_constants.cuh:106-114).
+constexpr int CX[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1};
+constexpr int CY[Q] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, -1};
+constexpr int CZ[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, 1, 1, -1, -1, -1, -1};
+constexpr int OPP[Q] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15, 26, 25, 24, 23, 22, 21, 20, 19};
+
+// Weights by shell (lbm_constants.cuh:64-71).
+LBM3_HD constexpr float w3(int q) {
+    return q == 0 ? 8.0f / 27.0f : (q < 7 ? 2.0f / 27.0f : (q < 19 ? 1.0f / 54.0f : 1.0f / 216.0f));
+}
+
+// sum_k c_k v_k for a compile-time direction, as signed adds (no multiply by 0 or +-1).
+template <int q>
+LBM3_HD float cdot(float vx, float vy, float vz) {
+    float s = 0.0f;
+    bool first = true;
+    auto add = [&](int c, float v) {
+        if (c == 0) return;
+        if (first) {
+            s = c > 0 ? v : -v;
+            first = false;
+        } else {
+            s = c > 0 ? s + v : s - v;
+        }
+    };
+    add(CX[q], vx);
+    add(CY[q], vy);
+    add(CZ[q], vz);
+    return s;
+}
+
+struct Macro3 {
+    float rho, ux, uy, uz;
+};
+
+// uncorrected_macroscopics_kernel<3> (macroscopics.cu:39-100) followed by the Guo half-force
+// correction (macroscopics.cu:112-128): sums over i = 0..26 in the reference's order (the
+// c = 0 terms add an exact zero and are left out), one reciprocal for the three divisions.
+template <int q>
+LBM3_HD void mom_acc(const float* g, float& r, float& jx, float& jy, float& jz) {
+    r += g[q];
+    if (CX[q] > 0) jx += g[q];
+    if (CX[q] < 0) jx -= g[q];
+    if (CY[q] > 0) jy += g[q];
+    if (CY[q] < 0) jy -= g[q];
+    if (CZ[q] > 0) jz += g[q];
+    if (CZ[q] < 0) jz -= g[q];
+}
+template <int... qs>
+LBM3_HD void mom_all(const float* g, float& r, float& jx, float& jy, float& jz, std::integer_sequence<int, qs...>) {
+    (mom_acc<qs>(g, r, jx, jy, jz), ...);
+}
+LBM3_HD Macro3 macroscopics3(const float* g, float fx, float fy, float fz) {
+    float r = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
+    mom_all(g, r, jx, jy, jz, std::make_integer_sequence<int, Q>{});
+    const float inv = 1.0f / r;
+    Macro3 m;
+    m.rho = r;
+    m.ux = (jx + 0.5f * fx) * inv;
+    m.uy = (jy + 0.5f * fy) * inv;
+    m.uz = (jz + 0.5f * fz) * inv;
+    return m;
+}
+
+// Equilibrium (equilibrium.cu:41-96) and BGK<3>::apply with Guo forcing (BGK.cuh:55-96):
+//   f_i <- (1 - omega) f_i + omega w_i rho (1 + lin_i c.u + 4.5 (c.u)^2 - 1.5 u^2)
+//          + w_i (1 - omega/2) (3 (c - u).F + 9 (c.u)(c.F)),
+// lin_i = 3, or `lin_ec` on the edge and corner directions (1 reproduces the reference's
+// De Rosis expression, LBM_QUIRK_EQ3D_LINEAR).
+struct BgkK {
+    float rho, ux, uy, uz, fx, fy, fz;
+    float om, om1, k3, k9, uF3, base, lin_ec;
+};
+template <int q>
+LBM3_HD void bgk_dir(float* g, const BgkK& K) {
+    const float cu = cdot<q>(K.ux, K.uy, K.uz);
+    const float cF = cdot<q>(K.fx, K.fy, K.fz);
+    const float lin = q < 7 ? 3.0f : K.lin_ec;
+    const float w = w3(q);
+    const float feq = w * (K.base + K.rho * cu * (lin + 4.5f * cu));
+    const float src = w * (K.k3 * cF - K.uF3 + K.k9 * cu * cF);
+    g[q] = K.om1 * g[q] + K.om * feq + src;
+}
+template <int... qs>
+LBM3_HD void bgk_all(float* g, const BgkK& K, std::integer_sequence<int, qs...>) {
+    (bgk_dir<qs>(g, K), ...);
+}
+LBM3_HD void collide_bgk3(float* g, const Macro3& m, float fx, float fy, float fz, float omega, float lin_ec) {
+    BgkK K;
+    K.rho = m.rho;
+    K.ux = m.ux;
+    K.uy = m.uy;
+    K.uz = m.uz;
+    K.fx = fx;
+    K.fy = fy;
+    K.fz = fz;
+    K.om = omega;
+    K.om1 = 1.0f - omega;
+    const float k = 1.0f - 0.5f * omega;
+    K.k3 = 3.0f * k;
+    K.k9 = 9.0f * k;
+    K.uF3 = K.k3 * (m.ux * fx + m.uy * fy + m.uz * fz);
+    K.base = m.rho * (1.0f - 1.5f * (m.ux * m.ux + m.uy * m.uy + m.uz * m.uz));
+    K.lin_ec = lin_ec;
+    bgk_all(g, K, std::make_integer_sequence<int, Q>{});
+}
+
+// BounceBack<3>::apply (bbDomainBoundary.cuh:52-90): for i = 1..26 in order, a direction whose
+// neighbour lies outside the domain on a non-periodic axis copies g_i into g_opp(i).
+// Sequential as written, so edge and corner nodes match the reference's loop order.
+struct Box3 {
+    int nx, ny, nz, px, py, pz;
+};
+template <int i>
+LBM3_HD void bb_dir(float* g, int x, int y, int z, const Box3& B) {
+    bool out = false;
+    if (CX[i] != 0 && !B.px) out = out || (x + CX[i] < 0 || x + CX[i] >= B.nx);
+    if (CY[i] != 0 && !B.py) out = out || (y + CY[i] < 0 || y + CY[i] >= B.ny);
+    if (CZ[i] != 0 && !B.pz) out = out || (z + CZ[i] < 0 || z + CZ[i] >= B.nz);
+    if (out) g[OPP[i]] = g[i];
+}
+template <int... is>
+LBM3_HD void bb_all(float* g, int x, int y, int z, const Box3& B, std::integer_sequence<int, is...>) {
+    (bb_dir<is + 1>(g, x, y, z, B), ...);
+}
+LBM3_HD void bc_bounce_back3(float* g, int x, int y, int z, const Box3& B) {
+    bb_all(g, x, y, z, B, std::make_integer_sequence<int, Q - 1>{});
+}
+
+}  // namespace lbm3

@@ -1,0 +1,138 @@
+// lbm3d_step.hpp — per-thread body of the fused D3Q27 pull-scheme step.
+//
+// One thread owns one node; a wavefront owns 64 consecutive nodes of one x-row, so each of
+// the 27 pulls is one wavefront-wide load of 64 consecutive elements of a population plane,
+// shifted by the direction's upstream offset, and each of the 27 stores is 64 aligned
+// consecutive elements. One read and one write of every population per step replace the
+// reference's stream, boundaries, macroscopics, correction, equilibrium and collide kernels
+// (main.cu:98-114) and its f_eq buffer.
+//
+// Streaming rule (streaming.cu:35-65, pulled): population q at node x comes from x - c_q,
+// wrapped on periodic axes; when x - c_q lies outside a non-periodic domain the push kernel
+// never writes the slot, so it keeps what the destination buffer already holds (the state of
+// two steps before), and the pull reads exactly that slot. LBM_QUIRK_STALE_REST does the same
+// for the rest population (streaming.cu:43 starts at i = 1).
+//
+// The body is __host__ __device__: lbm3d.hip launches it on the GPU, and the CPU tests run it
+// over the same grid in a host emulation (bounds and parity checks before any GPU launch).
+#pragma once
+
+#include "lbm3d_node.hpp"
+#include "lbm_step.hpp"
+
+namespace lbm3 {
+
+// Kernel arguments (passed by value).
+struct Step3Params {
+    const void* src;       // post-collision populations of step t-1, [27][plane]
+    void* dst;             // receives step t; holds step t-2 (stale slots)
+    const uint8_t* flags;  // per-node flag (FLUID / BOUNCE_BACK), or null (all FLUID)
+    const float* force;    // SoA [3][plane] force field, or null (uniform fx, fy, fz)
+    float* out_rho;        // macroscopic outputs, rho [plane] and u SoA [3][plane], or null
+    float* out_u;
+    int nx, ny, nz;
+    int px, py, pz;
+    long long plane;       // elements between population planes (N rounded up to 64)
+    float omega, rho0, fx, fy, fz;
+    float lin_ec;          // linear equilibrium coefficient of edge/corner directions (3 or 1)
+    int stale_rest;        // LBM_QUIRK_STALE_REST
+    int nbx, nby;          // workgroups along x (64 nodes) and y (kRows rows)
+};
+
+constexpr int kRows = 4;  // x-rows (wavefronts) per 256-thread workgroup
+
+// Population storage: FP32, or FP16/BF16 of the shifted deviation f - w rho0.
+template <int STORE>
+struct Store3 {
+    using T = uint16_t;
+    LBM3_HD static float dec(T raw, int q, float rho0) {
+        const float d = STORE == LBM_F16_SHIFTED ? lbm::half_bits_to_float(raw) : lbm::bf16_bits_to_float(raw);
+        return d + w3(q) * rho0;
+    }
+    LBM3_HD static T enc(float v, int q, float rho0) {
+        const float d = v - w3(q) * rho0;
+        return STORE == LBM_F16_SHIFTED ? lbm::float_to_half_bits(d) : lbm::float_to_bf16_bits(d);
+    }
+};
+template <>
+struct Store3<LBM_F32> {
+    using T = float;
+    LBM3_HD static float dec(T raw, int, float) { return raw; }
+    LBM3_HD static T enc(float v, int, float) { return v; }
+};
+
+// Offsets (in elements) of the upstream node along one axis, for c = -1, 0, +1 (index c + 1);
+// ok[c + 1] is false when the upstream node lies outside a non-periodic domain.
+LBM3_HD void axis_pull(int x, int n, int periodic, int stride, int* off, bool* ok) {
+    off[1] = 0;
+    ok[1] = true;
+    // c = +1: upstream x - 1
+    ok[2] = x > 0 || periodic;
+    off[2] = x > 0 ? -stride : (periodic ? (n - 1) * stride : 0);
+    // c = -1: upstream x + 1
+    ok[0] = x < n - 1 || periodic;
+    off[0] = x < n - 1 ? stride : (periodic ? -(n - 1) * stride : 0);
+}
+
+template <int STORE, bool BUF, int q>
+LBM3_HD void pull_dir(float* g, const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src,
+                      const lbm::PopBuf<typename Store3<STORE>::T, BUF>& dst, uint32_t node, const int* ox,
+                      const bool* vx, const int* oy, const bool* vy, const int* oz, const bool* vz) {
+    using T = typename Store3<STORE>::T;
+    constexpr int a = CX[q] + 1, b = CY[q] + 1, c = CZ[q] + 1;
+    const long long off = (long long)q * p.plane;
+    bool ok = vx[a] && vy[b] && vz[c];
+    if (q == 0) ok = !p.stale_rest;
+    T raw;
+    if (ok) raw = src.ld(uint32_t(int(node) + ox[a] + oy[b] + oz[c]), off);
+    else raw = dst.ld(node, off);
+    g[q] = Store3<STORE>::dec(raw, q, p.rho0);
+}
+template <int STORE, bool BUF, int... qs>
+LBM3_HD void pull_all(float* g, const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src,
+                      const lbm::PopBuf<typename Store3<STORE>::T, BUF>& dst, uint32_t node, const int* ox,
+                      const bool* vx, const int* oy, const bool* vy, const int* oz, const bool* vz,
+                      std::integer_sequence<int, qs...>) {
+    (pull_dir<STORE, BUF, qs>(g, p, src, dst, node, ox, vx, oy, vy, oz, vz), ...);
+}
+
+// One node of one time step: pull, boundary functor, macroscopics, Guo-forced BGK, store.
+// Requires 0 <= x < nx, 0 <= y < ny, 0 <= z < nz and nx*ny*nz < 2^31.
+template <int STORE, bool BUF>
+LBM3_HD void step3_node(const Step3Params& p, int x, int y, int z) {
+    using St = Store3<STORE>;
+    using T = typename St::T;
+    const int nxy = p.nx * p.ny;
+    const uint32_t node = uint32_t(z * nxy + y * p.nx + x);
+    int ox[3], oy[3], oz[3];
+    bool vx[3], vy[3], vz[3];
+    axis_pull(x, p.nx, p.px, 1, ox, vx);
+    axis_pull(y, p.ny, p.py, p.nx, oy, vy);
+    axis_pull(z, p.nz, p.pz, nxy, oz, vz);
+    const lbm::PopBuf<T, BUF> src(p.src), dst(p.dst);
+    float g[Q];
+    pull_all<STORE, BUF>(g, p, src, dst, node, ox, vx, oy, vy, oz, vz, std::make_integer_sequence<int, Q>{});
+
+    if (p.flags && p.flags[node] == LBM_BOUNCE_BACK) {
+        const Box3 B{p.nx, p.ny, p.nz, p.px, p.py, p.pz};
+        bc_bounce_back3(g, x, y, z, B);
+    }
+    float fx = p.fx, fy = p.fy, fz = p.fz;
+    if (p.force) {
+        fx = p.force[node];
+        fy = p.force[p.plane + node];
+        fz = p.force[2 * p.plane + node];
+    }
+    const Macro3 m = macroscopics3(g, fx, fy, fz);
+    if (p.out_rho) {
+        p.out_rho[node] = m.rho;
+        p.out_u[node] = m.ux;
+        p.out_u[p.plane + node] = m.uy;
+        p.out_u[2 * p.plane + node] = m.uz;
+    }
+    collide_bgk3(g, m, fx, fy, fz, p.omega, p.lin_ec);
+#pragma unroll
+    for (int q = 0; q < Q; q++) dst.st(node, (long long)q * p.plane, St::enc(g[q], q, p.rho0));
+}
+
+}  // namespace lbm3

@@ -54,6 +54,7 @@ LBM_QUIRK_MRT_FORCE_ROWS = 1 << 2
 LBM_QUIRK_IBM_2X2 = 1 << 3
 LBM_QUIRK_IBM_CLAMP = 1 << 4
 LBM_QUIRK_IBM_NO_DS = 1 << 5
+LBM_QUIRK_EQ3D_LINEAR = 1 << 6   # include/lbm3d_hip.h
 
 # reductions
 LBM_REDUCE_MASS = 0
@@ -87,7 +88,34 @@ class LbmConfig(ctypes.Structure):
     ]
 
 
-# (name, restype, argtypes) of every entry point include/lbm_hip.h declares.
+class Lbm3dConfig(ctypes.Structure):
+    """struct lbm3d_config (include/lbm3d_hip.h)."""
+
+    _fields_ = [
+        ("nx", ctypes.c_int32),
+        ("ny", ctypes.c_int32),
+        ("nz", ctypes.c_int32),
+        ("collision", ctypes.c_int32),
+        ("omega", ctypes.c_float),
+        ("rho0", ctypes.c_float),
+        ("storage", ctypes.c_int32),
+        ("periodic_x", ctypes.c_int32),
+        ("periodic_y", ctypes.c_int32),
+        ("periodic_z", ctypes.c_int32),
+        ("force", ctypes.c_float * 3),
+        ("quirks", ctypes.c_uint32),
+        ("reserved", ctypes.c_int32 * 8),
+    ]
+
+
+# 3-D reductions (include/lbm3d_hip.h)
+LBM3D_REDUCE_MASS = 0
+LBM3D_REDUCE_MOMENTUM = 1
+LBM3D_REDUCE_KINETIC_ENERGY = 2
+
+
+# (name, restype, argtypes) of every entry point include/lbm_hip.h and include/lbm3d_hip.h
+# declare.
 _P = ctypes.c_void_p
 _F = ctypes.POINTER(ctypes.c_float)
 _D = ctypes.POINTER(ctypes.c_double)
@@ -114,6 +142,22 @@ PROTOTYPES = [
     ("lbm_adjoint_backward", _I, [_P, _P, _P, _P, _I]),
     ("lbm_last_error", ctypes.c_char_p, [_P]),
     ("lbm_version", ctypes.c_char_p, []),
+    # include/lbm3d_hip.h
+    ("lbm3d_config_default", _I, [ctypes.POINTER(Lbm3dConfig), _I, _I, _I]),
+    ("lbm3d_create", _I, [ctypes.POINTER(Lbm3dConfig), ctypes.POINTER(_P)]),
+    ("lbm3d_destroy", _I, [_P]),
+    ("lbm3d_set_stream", _I, [_P, _P]),
+    ("lbm3d_set_flags", _I, [_P, _P, _I]),
+    ("lbm3d_init_from_macros", _I, [_P, _P, _P, _I]),
+    ("lbm3d_set_force_field", _I, [_P, _P, _I]),
+    ("lbm3d_step", _I, [_P, _I]),
+    ("lbm3d_sync", _I, [_P]),
+    ("lbm3d_get_timestep", _I, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    ("lbm3d_get_macros", _I, [_P, _P, _P, _I]),
+    ("lbm3d_get_pops", _I, [_P, _P, _I]),
+    ("lbm3d_set_pops", _I, [_P, _P, _I]),
+    ("lbm3d_reduce", _I, [_P, _I, _D]),
+    ("lbm3d_last_error", ctypes.c_char_p, [_P]),
 ]
 
 _lib = None

@@ -1,0 +1,103 @@
+/*
+ * lbm3d_hip.h — C ABI of the D3Q27 path of liblbm_hip.so (MI355X, gfx950).
+ *
+ * The reference's default build is D3Q27 (src/defines.hpp:4): `LBM<3>` (src/core/lbm.cuh:33-382)
+ * with 27 SoA population planes, node = z*NX*NY + y*NX + x (src/core/lbm_constants.cuh:351-358),
+ * stepped by main() (src/main.cu:96-114). This header is the 3-D counterpart of lbm_hip.h and
+ * keeps its conventions: every function returns 0 or a negative LBM_E* code (lbm_hip.h) with
+ * the message in lbm3d_last_error(ctx); a context owns its device memory and one stream;
+ * host-facing vector fields are AoS [node][3]; populations crossing the boundary are FP32 SoA
+ * [27][node]; `on_device` = 1 marks a device pointer (the call is then stream-ordered and does
+ * not synchronise).
+ *
+ * Built: BGK<3> with Guo forcing (src/core/collision/BGK/BGK.cuh:55-96), the FLUID and
+ * BOUNCE_BACK flags (src/core/boundaries/boundaries.cuh:89-138, BounceBack<3>
+ * src/functors/boundaryConditions/bbDomainBoundary.cuh:52-90), runtime periodic axes, FP32 or
+ * 16-bit shifted population storage. MRT<3>, CM<3> and the other 3-D boundary functors are
+ * rejected with LBM_E_UNSUPPORTED.
+ */
+#ifndef LBM3D_HIP_H
+#define LBM3D_HIP_H
+
+#include <stdint.h>
+
+#include "lbm_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Reference-literal 3-D equilibrium (bit in lbm3d_config.quirks). The reference's De Rosis
+ * form (src/core/equilibrium/equilibrium.cu:70-96) gives the 12 edge and 8 corner directions
+ * a linear coefficient 1 where the second-order equilibrium has 3 (1/c_s^2), so its first
+ * moment is (17/27) rho u instead of rho u. Default (0): the second-order equilibrium. */
+#define LBM_QUIRK_EQ3D_LINEAR (1u << 6)
+
+typedef struct lbm3d_config {
+    int32_t  nx, ny, nz;
+    int32_t  collision;      /* LBM_BGK (the only 3-D operator built)                        */
+    float    omega;          /* 1/tau (ScenarioTrait::omega, src/scenarios/scenario.cuh:36-38) */
+    float    rho0;           /* reference density of the shifted storage                      */
+    int32_t  storage;        /* LBM_F32 | LBM_F16_SHIFTED | LBM_BF16_SHIFTED                  */
+    int32_t  periodic_x;     /* streaming wraps on the axis (PERIODIC_X/Y/Z of               */
+    int32_t  periodic_y;     /*   src/core/streaming/streaming.cu:45-53, never defined there) */
+    int32_t  periodic_z;
+    float    force[3];       /* uniform Guo body force, used where no force field is set      */
+    uint32_t quirks;         /* LBM_QUIRK_STALE_REST | LBM_QUIRK_EQ3D_LINEAR                   */
+    int32_t  reserved[8];
+} lbm3d_config;
+
+typedef struct lbm3d_ctx lbm3d_ctx;
+
+/* Defaults: BGK, tau = 1 (omega = 1), rho0 = 1, FP32, not periodic, no force. */
+int lbm3d_config_default(lbm3d_config* cfg, int32_t nx, int32_t ny, int32_t nz);
+
+/* Replaces LBM<3>::allocate (lbm.cuh:92-125) + send_consts (lbm.cuh:63-82). */
+int lbm3d_create(const lbm3d_config* cfg, lbm3d_ctx** out);
+/* Replaces LBM<3>::free (lbm.cuh:127-140). NULL is accepted. */
+int lbm3d_destroy(lbm3d_ctx* ctx);
+/* Run on `hip_stream` (a hipStream_t; NULL = the HIP null stream). */
+int lbm3d_set_stream(lbm3d_ctx* ctx, void* hip_stream);
+
+/* Replaces setup_boundary_flags (boundaries.cuh:170-197) for 3-D: one flag per node,
+ * LBM_FLUID or LBM_BOUNCE_BACK; other flags give LBM_E_UNSUPPORTED. NULL = all FLUID. */
+int lbm3d_set_flags(lbm3d_ctx* ctx, const uint8_t* flags, int on_device);
+
+/* Replaces init_kernel<3> + init_node (src/core/init/init.cuh:10-43): f = f_back = f_eq.
+ * rho [N]; u AoS [N][3]. Resets the step counter. */
+int lbm3d_init_from_macros(lbm3d_ctx* ctx, const float* rho, const float* u, int on_device);
+
+/* Replaces reset_forces + Init::apply_forces (macroscopics.cuh:13-48) for a non-uniform
+ * force, AoS [N][3]. NULL removes the field (the uniform cfg.force applies). */
+int lbm3d_set_force_field(lbm3d_ctx* ctx, const float* force, int on_device);
+
+/* `nsteps` iterations of main.cu:96-114 (stream, boundaries, macroscopics, forces, Guo
+ * correction, equilibrium, BGK collision), one fused pull-scheme kernel per step.
+ * Asynchronous; the last step also stores rho and the corrected u. */
+int lbm3d_step(lbm3d_ctx* ctx, int32_t nsteps);
+int lbm3d_sync(lbm3d_ctx* ctx);
+int lbm3d_get_timestep(const lbm3d_ctx* ctx, int64_t* t);
+
+/* Replaces update_macroscopics (lbm.cuh:148-154): rho [N], corrected u AoS [N][3] of the
+ * last step's post-stream state. Either pointer may be NULL. */
+int lbm3d_get_macros(lbm3d_ctx* ctx, float* rho, float* u, int on_device);
+
+/* Raw state, FP32 SoA [27][N]. lbm3d_set_pops writes both buffers (as init does). */
+int lbm3d_get_pops(lbm3d_ctx* ctx, float* f, int on_device);
+int lbm3d_set_pops(lbm3d_ctx* ctx, const float* f, int on_device);
+
+/* Reductions over the last stored macroscopic fields, FP64, deterministic. Synchronises. */
+#define LBM3D_REDUCE_MASS           0 /* out[0] = sum rho                                      */
+#define LBM3D_REDUCE_MOMENTUM       1 /* out[0..2] = sum rho*u                                 */
+#define LBM3D_REDUCE_KINETIC_ENERGY 2 /* out[0] = sum 0.5*rho*|u|^2; compute_kinetic_energy_kernel
+                                         (src/scenarios/taylorGreen3D/postprocessing/
+                                         kineticEnergy.cuh:7-42) before its /(N*0.05^2)  */
+int lbm3d_reduce(lbm3d_ctx* ctx, int32_t kind, double* out);
+
+const char* lbm3d_last_error(const lbm3d_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM3D_HIP_H */

@@ -1,0 +1,269 @@
+// lbm_oracle3d.hpp — CPU restatement of the reference's D3Q27 BGK path.
+//
+// TEST INFRASTRUCTURE ONLY. This is the correctness oracle for the 3-D step. Only tests/,
+// the smoke check in __graft_entry__.py and the cpu_baseline leg of bench.py may load or call
+// it, and only as the checker. The product (liblbm_hip.so) never links or calls this code.
+//
+// It follows the reference's unfused stage order (src/main.cu:96-114) on separate buffers, as
+// `LBM<3>` does (src/core/lbm.cuh:36-43): push stream -> swap -> boundaries -> uncorrected
+// macroscopics -> reset forces -> Guo velocity correction -> equilibrium -> BGK<3> collision,
+// templated on the arithmetic type (FP32 oracle, FP64 host run).
+//
+// Parity pinning (tests/test_oracle3d_pinning.py), each against something this file does not
+// restate: the equilibrium's moments up to second order (exact identities), the literal
+// equilibrium's first moment (17/27) rho u (closed form), the Guo source's moments,
+// periodic streaming against numpy.roll, BGK's conserved moments, the rest state as a fixed
+// point of a closed bounce-back box, mirror symmetry of the box, viscous decay of a small
+// Taylor-Green vortex against exp(-6 nu k^2 t), and the curvature -F/nu of a force-driven
+// channel. Parity unpinned: the FP32 rounding sequence of the reference's De Rosis expression
+// (equilibrium.cu:70-96), which acts at FP32 rounding level.
+#pragma once
+
+#include "../include/lbm3d_hip.h"
+#include "lbm_oracle.hpp"
+
+#include <cmath>
+#include <stdexcept>
+#include <vector>
+
+namespace oracle3d {
+
+constexpr int Q = 27;
+// Velocity set in the reference's order: rest, 6 faces, 12 edges, 8 corners
+// (lbm_constants.cuh:76-104).
+constexpr int CX[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1};
+constexpr int CY[Q] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, -1};
+constexpr int CZ[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, 1, 1, -1, -1, -1, -1};
+// Opposite directions (lbm_constants.cuh:106-114).
+constexpr int OPP[Q] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15, 26, 25, 24, 23, 22, 21, 20, 19};
+
+// Weights 8/27, 2/27, 1/54, 1/216 by |c|^2 (lbm_constants.cuh:64-71).
+template <class R> inline R weight(int q) {
+    if (q == 0) return R(8) / R(27);
+    if (q < 7) return R(2) / R(27);
+    if (q < 19) return R(1) / R(54);
+    return R(1) / R(216);
+}
+
+// LBM<3>::equilibrium_node (equilibrium.cu:41-96). Default: w rho (1 + 3cu + 4.5cu^2 - 1.5u^2)
+// in R. literal == true restates the De Rosis (2017) form as written: the rest and face
+// directions are the second-order equilibrium, the edge and corner directions compute
+// cs2*s*(s + 2cs4) + 2cs2cs4 - cs4*u^2 scaled by rho/(2 cs2 cs4) w, i.e. a linear
+// coefficient 1 instead of 3 (quirk LBM_QUIRK_EQ3D_LINEAR).
+template <class R>
+inline void equilibrium(R rho, R ux, R uy, R uz, R* feq, bool literal) {
+    const R uu = ux * ux + uy * uy + uz * uz;
+    if (!literal) {
+        for (int q = 0; q < Q; q++) {
+            R cu = R(CX[q]) * ux + R(CY[q]) * uy + R(CZ[q]) * uz;
+            feq[q] = weight<R>(q) * rho * (R(1) + R(3) * cu + R(4.5) * cu * cu - R(1.5) * uu);
+        }
+        return;
+    }
+    const R cs2 = R(1) / R(3);
+    const R cs4 = cs2 * cs2;
+    const R two_cs4 = R(2) * cs4;
+    const R cs2_cs4 = cs2 * cs4;
+    const R inv_cs2_cs4 = R(1) / cs2_cs4;
+    const R two_cs2_cs4 = R(2) * cs2 * cs4;
+    const R half_rho = R(0.5) * rho;
+    const R ux2 = ux * ux, uy2 = uy * uy, uz2 = uz * uz;
+    feq[0] = half_rho * weight<R>(0) * (R(2) * cs2 - ux2 - uy2 - uz2) / cs2;
+    for (int q = 1; q < Q; q++) {
+        R s = R(CX[q]) * ux + R(CY[q]) * uy + R(CZ[q]) * uz;
+        R bracket;
+        if (q < 7) bracket = cs2 * s * s + two_cs2_cs4 + two_cs4 * s - cs4 * (ux2 + uy2 + uz2);
+        else bracket = cs2 * s * (s + two_cs4) + two_cs2_cs4 - cs4 * (ux2 + uy2 + uz2);
+        feq[q] = inv_cs2_cs4 * half_rho * weight<R>(q) * bracket;
+    }
+}
+
+// Guo source term of BGK<3>::apply (BGK.cuh:55-96):
+// w (1 - omega/2) [ ((c - u)/cs2 + (c.u) c / cs4) . F ].
+template <class R>
+inline R guo_source(int q, R ux, R uy, R uz, R fx, R fy, R fz, R omega) {
+    const R cs2 = R(1) / R(3);
+    const R cx = R(CX[q]), cy = R(CY[q]), cz = R(CZ[q]);
+    const R cu = cx * ux + cy * uy + cz * uz;
+    const R k = R(1) - R(0.5) * omega;
+    return weight<R>(q) * (k * ((cx - ux) / cs2 + cu * cx / (cs2 * cs2)) * fx +
+                           k * ((cy - uy) / cs2 + cu * cy / (cs2 * cs2)) * fy +
+                           k * ((cz - uz) / cs2 + cu * cz / (cs2 * cs2)) * fz);
+}
+
+template <class R>
+struct Solver3 {
+    lbm3d_config cfg;
+    int nx, ny, nz;
+    size_t N;
+    bool eq_literal = false;
+    std::vector<R> f, f_back;      // SoA [q][node] (lbm_constants.cuh:320-328)
+    std::vector<R> rho, u, force;  // u, force SoA [c][node]
+    std::vector<uint8_t> flags;
+    std::vector<R> force_field;    // AoS [node][3]; empty => uniform cfg.force
+    int64_t timestep = 0;
+
+    explicit Solver3(const lbm3d_config& c)
+        : cfg(c), nx(c.nx), ny(c.ny), nz(c.nz), N(size_t(c.nx) * size_t(c.ny) * size_t(c.nz)) {
+        if (nx < 3 || ny < 3 || nz < 3) throw std::runtime_error("grid must be at least 3x3x3");
+        eq_literal = (c.quirks & LBM_QUIRK_EQ3D_LINEAR) != 0;
+        f.assign(Q * N, R(0));
+        f_back.assign(Q * N, R(0));
+        rho.assign(N, R(1));
+        u.assign(3 * N, R(0));
+        force.assign(3 * N, R(0));
+        flags.assign(N, 0);
+    }
+
+    size_t node(int x, int y, int z) const { return (size_t(z) * ny + y) * nx + x; }  // lbm_constants.cuh:351-358
+
+    // Storage rounding of the shifted deviation f - w rho0 (FP32 arithmetic, as the GPU).
+    R store_round(int q, R v) const {
+        if (cfg.storage == LBM_F32) return v;
+        float w0 = weight<float>(q) * cfg.rho0;
+        float d = float(v) - w0;
+        float back = cfg.storage == LBM_F16_SHIFTED ? oracle::f16_to_f32(oracle::f32_to_f16(d))
+                                                    : oracle::bf16_to_f32(oracle::f32_to_bf16(d));
+        return R(back + w0);
+    }
+    void quantize(std::vector<R>& buf) {
+        if (cfg.storage == LBM_F32) return;
+        for (int q = 0; q < Q; q++)
+            for (size_t n = 0; n < N; n++) buf[q * N + n] = store_round(q, buf[q * N + n]);
+    }
+
+    // init_kernel<3> + init_node (init.cuh:10-43): f = f_back = f_eq(rho, u).
+    void init_from_macros(const R* rho_in, const R* u_aos) {
+        for (size_t n = 0; n < N; n++) {
+            rho[n] = rho_in[n];
+            for (int c = 0; c < 3; c++) u[c * N + n] = u_aos[3 * n + c];
+            R feq[Q];
+            equilibrium<R>(rho[n], u[n], u[N + n], u[2 * N + n], feq, eq_literal);
+            for (int q = 0; q < Q; q++) f[q * N + n] = f_back[q * N + n] = feq[q];
+        }
+        quantize(f);
+        quantize(f_back);
+        timestep = 0;
+    }
+
+    // LBM<3>::stream_node (streaming.cu:35-65): push f[node][i] to the neighbour's f_back slot,
+    // wrapping on periodic axes; out-of-domain targets are skipped (the slot stays stale).
+    // The rest population is copied unless LBM_QUIRK_STALE_REST (loop from i = 1).
+    void stream() {
+        const int i0 = (cfg.quirks & LBM_QUIRK_STALE_REST) ? 1 : 0;
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int z = 0; z < nz; z++) {
+            for (int y = 0; y < ny; y++) {
+                for (int x = 0; x < nx; x++) {
+                    const size_t n = node(x, y, z);
+                    for (int i = i0; i < Q; i++) {
+                        int xn = x + CX[i], yn = y + CY[i], zn = z + CZ[i];
+                        if (cfg.periodic_x) xn = (xn + nx) % nx;
+                        if (cfg.periodic_y) yn = (yn + ny) % ny;
+                        if (cfg.periodic_z) zn = (zn + nz) % nz;
+                        if (xn < 0 || xn >= nx || yn < 0 || yn >= ny || zn < 0 || zn >= nz) continue;
+                        f_back[i * N + node(xn, yn, zn)] = f[i * N + n];
+                    }
+                }
+            }
+        }
+    }
+
+    // BounceBack<3>::apply (bbDomainBoundary.cuh:52-90): for i = 1..26 in order, a direction
+    // whose neighbour lies outside the domain on a reflecting axis copies f_i into f_opp(i).
+    // The reference enables all three axes (boundaries.cuh:98); here the reflecting axes are
+    // the non-periodic ones, which is the same whenever no axis is periodic.
+    void bc_bounce_back(size_t n, int x, int y, int z) {
+        for (int i = 1; i < Q; i++) {
+            const int xn = x + CX[i], yn = y + CY[i], zn = z + CZ[i];
+            const bool bx = !cfg.periodic_x && (xn < 0 || xn >= nx);
+            const bool by = !cfg.periodic_y && (yn < 0 || yn >= ny);
+            const bool bz = !cfg.periodic_z && (zn < 0 || zn >= nz);
+            if (bx || by || bz) f[OPP[i] * N + n] = f[i * N + n];
+        }
+    }
+
+    // Flags restated here: FLUID and BOUNCE_BACK.
+    void set_flags(const uint8_t* fl) {
+        for (size_t n = 0; n < N; n++)
+            if (fl[n] != LBM_FLUID && fl[n] != LBM_BOUNCE_BACK) throw std::runtime_error("3-D flag not restated");
+        flags.assign(fl, fl + N);
+    }
+
+    // boundaries_kernel_3D (boundaries.cuh:89-138) for the FLUID and BOUNCE_BACK flags.
+    void apply_boundaries() {
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int z = 0; z < nz; z++)
+            for (int y = 0; y < ny; y++)
+                for (int x = 0; x < nx; x++) {
+                    const size_t n = node(x, y, z);
+                    if (flags[n] == LBM_BOUNCE_BACK) bc_bounce_back(n, x, y, z);
+                }
+    }
+
+    // uncorrected_macroscopics_kernel<3> (macroscopics.cu:39-100): sums over i = 0..26 in
+    // order, u scaled by 1/rho.
+    void uncorrected_macroscopics() {
+#pragma omp parallel for schedule(static)
+        for (long long ln = 0; ln < (long long)N; ln++) {
+            const size_t n = size_t(ln);
+            R r = 0, jx = 0, jy = 0, jz = 0;
+            for (int i = 0; i < Q; i++) {
+                const R fi = f[i * N + n];
+                r += fi;
+                jx += fi * R(CX[i]);
+                jy += fi * R(CY[i]);
+                jz += fi * R(CZ[i]);
+            }
+            rho[n] = r;
+            u[n] = jx * (R(1) / r);
+            u[N + n] = jy * (R(1) / r);
+            u[2 * N + n] = jz * (R(1) / r);
+        }
+    }
+
+    // reset_forces + Init::apply_forces (macroscopics.cuh:13-48): the uniform force or the
+    // per-node field.
+    void reset_forces() {
+        for (size_t n = 0; n < N; n++)
+            for (int c = 0; c < 3; c++)
+                force[c * N + n] = force_field.empty() ? R(cfg.force[c]) : force_field[3 * n + c];
+    }
+
+    // correct_macroscopics_kernel<3> (macroscopics.cu:112-128): u += F / (2 rho).
+    void correct_macroscopics() {
+        for (size_t n = 0; n < N; n++)
+            for (int c = 0; c < 3; c++) u[c * N + n] += R(0.5) * force[c * N + n] / rho[n];
+    }
+
+    // equilibrium + BGK<3>::apply (BGK.cuh:55-96): f - omega (f - f_eq) + Guo source.
+    void collide() {
+        const R om = R(cfg.omega);
+#pragma omp parallel for schedule(static)
+        for (long long ln = 0; ln < (long long)N; ln++) {
+            const size_t n = size_t(ln);
+            const R ux = u[n], uy = u[N + n], uz = u[2 * N + n];
+            R feq[Q];
+            equilibrium<R>(rho[n], ux, uy, uz, feq, eq_literal);
+            for (int q = 0; q < Q; q++) {
+                const R ft = guo_source<R>(q, ux, uy, uz, force[n], force[N + n], force[2 * N + n], om);
+                R& fi = f[q * N + n];
+                fi = fi - om * (fi - feq[q]) + ft;
+            }
+        }
+    }
+
+    void step() {
+        timestep++;
+        stream();
+        std::swap(f, f_back);
+        apply_boundaries();
+        uncorrected_macroscopics();
+        reset_forces();
+        correct_macroscopics();
+        collide();
+        quantize(f);
+    }
+};
+
+}  // namespace oracle3d

@@ -63,6 +63,22 @@ def lib():
             "oracle_f16_to_f32": (ctypes.c_float, [ctypes.c_uint16]),
             "oracle_f32_to_bf16": (ctypes.c_uint16, [ctypes.c_float]),
             "oracle_bf16_to_f32": (ctypes.c_float, [ctypes.c_uint16]),
+            # D3Q27 (lbm_oracle3d.cpp)
+            "oracle3d_create": (_P, [_P, ctypes.c_int]),
+            "oracle3d_destroy": (None, [_P]),
+            "oracle3d_set_flags": (ctypes.c_int, [_P, _P]),
+            "oracle3d_init_from_macros": (ctypes.c_int, [_P, _D, _D]),
+            "oracle3d_set_force_field": (ctypes.c_int, [_P, _D]),
+            "oracle3d_step": (ctypes.c_int, [_P, ctypes.c_int]),
+            "oracle3d_get_macros": (ctypes.c_int, [_P, _D, _D]),
+            "oracle3d_get_pops": (ctypes.c_int, [_P, _D]),
+            "oracle3d_set_pops": (ctypes.c_int, [_P, _D]),
+            "oracle3d_stream_only": (ctypes.c_int, [_P]),
+            "oracle3d_last_error": (ctypes.c_char_p, [_P]),
+            "oracle3d_tables": (None, [_D, _P, _P, _P, _P]),
+            "oracle3d_equilibrium": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_double, _D]),
+            "oracle3d_guo_source": (None, [ctypes.c_double] * 7 + [_D]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -248,3 +264,87 @@ def num_threads():
 
 def set_num_threads(n):
     lib().oracle_set_num_threads(int(n))
+
+
+# ---- D3Q27 -------------------------------------------------------------------------------
+
+def tables3d():
+    """(w, cx, cy, cz, opp) of the 27-velocity set, reference order."""
+    w = np.zeros(27)
+    c = [np.zeros(27, np.int32) for _ in range(4)]
+    lib().oracle3d_tables(_dp(w), *[a.ctypes.data_as(_P) for a in c])
+    return (w, *c)
+
+
+def equilibrium3d(rho, ux, uy, uz, prec=64, literal=False):
+    out = np.zeros(27)
+    lib().oracle3d_equilibrium(prec, int(literal), rho, ux, uy, uz, _dp(out))
+    return out
+
+
+def guo_source3d(u, F, omega):
+    out = np.zeros(27)
+    lib().oracle3d_guo_source(*[float(v) for v in u], *[float(v) for v in F], float(omega), _dp(out))
+    return out
+
+
+class Oracle3D:
+    """One D3Q27 oracle solver (precision 32 or 64) in the reference's stage order.
+
+    Arrays are indexed [z, y, x] (node = (z*NY + y)*NX + x); populations [27, z, y, x]."""
+
+    def __init__(self, cfg_c, precision=64):
+        self.L = lib()
+        self.cfg = cfg_c
+        self.nx, self.ny, self.nz = cfg_c.nx, cfg_c.ny, cfg_c.nz
+        self.N = self.nx * self.ny * self.nz
+        self.shape = (self.nz, self.ny, self.nx)
+        self.h = self.L.oracle3d_create(ctypes.byref(cfg_c), precision)
+        if not self.h:
+            raise RuntimeError("oracle3d_create failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.oracle3d_destroy(self.h)
+            self.h = None
+
+    def _ck(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.L.oracle3d_last_error(self.h).decode())
+
+    def set_flags(self, flags):
+        f = np.ascontiguousarray(flags, np.uint8).reshape(-1)
+        self._ck(self.L.oracle3d_set_flags(self.h, f.ctypes.data_as(_P)))
+
+    def init_from_macros(self, rho, u):
+        r = np.ascontiguousarray(rho, np.float64).reshape(-1)
+        uu = np.ascontiguousarray(u, np.float64).reshape(-1)
+        self._ck(self.L.oracle3d_init_from_macros(self.h, _dp(r), _dp(uu)))
+
+    def set_force_field(self, F):
+        if F is None:
+            self._ck(self.L.oracle3d_set_force_field(self.h, None))
+        else:
+            a = np.ascontiguousarray(F, np.float64).reshape(-1)
+            self._ck(self.L.oracle3d_set_force_field(self.h, _dp(a)))
+
+    def step(self, n=1):
+        self._ck(self.L.oracle3d_step(self.h, int(n)))
+
+    def stream_only(self):
+        self._ck(self.L.oracle3d_stream_only(self.h))
+
+    def get_macros(self):
+        r = np.zeros(self.N)
+        u = np.zeros(3 * self.N)
+        self._ck(self.L.oracle3d_get_macros(self.h, _dp(r), _dp(u)))
+        return r.reshape(self.shape), u.reshape(*self.shape, 3)
+
+    def get_pops(self):
+        f = np.zeros(27 * self.N)
+        self._ck(self.L.oracle3d_get_pops(self.h, _dp(f)))
+        return f.reshape(27, *self.shape)
+
+    def set_pops(self, f):
+        a = np.ascontiguousarray(f, np.float64).reshape(-1)
+        self._ck(self.L.oracle3d_set_pops(self.h, _dp(a)))

@@ -1,0 +1,137 @@
+"""The D3Q27 kernel body (cuda-lbm_amd/csrc/lbm3d_step.hpp) run by the host emulation over the
+kernel's own grid, against the FP32 oracle (oracle/lbm_oracle3d.hpp). CPU only."""
+import ctypes
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+from lbm_amd import _abi
+
+ROOT = Path(__file__).resolve().parent.parent
+EMU = ROOT / "tests" / "emu" / "liblbm3d_emu.so"
+W = O.tables3d()[0]
+
+
+def emu_lib():
+    if not EMU.exists():
+        pytest.fail("tests/emu/liblbm3d_emu.so is not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(str(EMU))
+    L.emu3d_run.restype = ctypes.c_int
+    P = ctypes.c_void_p
+    L.emu3d_run.argtypes = [P, P, P, P, ctypes.c_int, P, P, P]
+    return L
+
+
+def make_cfg(shape, periodic, omega, storage=_abi.LBM_F32, force=(0.0, 0.0, 0.0), quirks=0):
+    c = _abi.Lbm3dConfig()
+    c.nx, c.ny, c.nz = shape
+    c.collision = _abi.LBM_BGK
+    c.omega = omega
+    c.rho0 = 1.0
+    c.storage = storage
+    c.periodic_x, c.periodic_y, c.periodic_z = periodic
+    for i in range(3):
+        c.force[i] = force[i]
+    c.quirks = quirks
+    return c
+
+
+def emu_run(c, f0, flags, force, steps):
+    L = emu_lib()
+    N = c.nx * c.ny * c.nz
+    f_in = np.ascontiguousarray(f0, np.float32).reshape(-1)
+    out = np.zeros(27 * N, np.float32)
+    rho = np.zeros(N, np.float32)
+    u = np.zeros(3 * N, np.float32)
+    fl = None if flags is None else np.ascontiguousarray(flags, np.uint8).reshape(-1)
+    F = None if force is None else np.ascontiguousarray(force, np.float32).reshape(-1)
+    p = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+    assert L.emu3d_run(ctypes.byref(c), p(f_in), p(fl), p(F), steps, p(out), p(rho), p(u)) == 0
+    shp = (c.nz, c.ny, c.nx)
+    return out.reshape(27, *shp), rho.reshape(shp), u.reshape(*shp, 3)
+
+
+def initial_state(shape, seed):
+    nx, ny, nz = shape
+    rng = np.random.default_rng(seed)
+    rho = 1 + 0.02 * rng.standard_normal((nz, ny, nx))
+    u = 0.03 * rng.standard_normal((nz, ny, nx, 3))
+    f = np.empty((27, nz, ny, nx))
+    for z in range(nz):
+        for y in range(ny):
+            for x in range(nx):
+                f[:, z, y, x] = O.equilibrium3d(rho[z, y, x], *u[z, y, x])
+    f *= 1 + 0.01 * rng.standard_normal(f.shape)  # a non-equilibrium part
+    return f
+
+
+def box_flags(shape, faces):
+    nx, ny, nz = shape
+    fl = np.zeros((nz, ny, nx), np.uint8)
+    if "x" in faces:
+        fl[:, :, 0] = fl[:, :, -1] = _abi.BOUNCE_BACK
+    if "y" in faces:
+        fl[:, 0, :] = fl[:, -1, :] = _abi.BOUNCE_BACK
+    if "z" in faces:
+        fl[0] = fl[-1] = _abi.BOUNCE_BACK
+    return fl
+
+
+CASES = [
+    # shape, periodic, omega, faces with bounce-back, force field, quirks
+    ((6, 5, 4), (1, 1, 1), 1.0, "", False, 0),
+    ((7, 5, 6), (1, 1, 1), 1.7, "", True, 0),
+    ((5, 6, 4), (0, 0, 0), 1.3, "xyz", False, 0),
+    ((66, 5, 3), (1, 0, 1), 0.8, "y", True, 0),
+    ((5, 7, 6), (0, 1, 0), 1.5, "xz", False, 0),
+    ((5, 4, 5), (0, 0, 0), 1.2, "", False, 0),          # open edges: stale slots only
+    ((6, 5, 4), (1, 0, 0), 1.1, "yz", False, _abi.LBM_QUIRK_STALE_REST),
+    ((6, 5, 4), (1, 1, 1), 1.4, "", True, _abi.LBM_QUIRK_EQ3D_LINEAR),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_emulated_step_matches_fp32_oracle(case):
+    shape, periodic, omega, faces, field, quirks = CASES[case]
+    nx, ny, nz = shape
+    c = make_cfg(shape, periodic, omega, force=(1e-5, -2e-5, 5e-6), quirks=quirks)
+    f0 = initial_state(shape, case)
+    flags = box_flags(shape, faces) if faces else None
+    force = None
+    if field:
+        force = 1e-5 * np.random.default_rng(100 + case).standard_normal((nz, ny, nx, 3))
+    o = O.Oracle3D(c, 32)
+    o.set_pops(f0.astype(np.float32))
+    if flags is not None:
+        o.set_flags(flags)
+    if force is not None:
+        o.set_force_field(force.astype(np.float32))
+    steps = 6
+    o.step(steps)
+    g, rho, u = emu_run(c, f0, flags, force, steps)
+    ref = o.get_pops()
+    assert np.abs(g - ref).max() < 1e-5
+    r_rho, r_u = o.get_macros()
+    assert np.abs(rho - r_rho).max() < 1e-5
+    assert np.abs(u - r_u).max() < 1e-6
+
+
+@pytest.mark.parametrize("storage,eps", [(_abi.LBM_F16_SHIFTED, 2.0 ** -11), (_abi.LBM_BF16_SHIFTED, 2.0 ** -8)])
+def test_emulated_reduced_storage_matches_oracle(storage, eps):
+    """16-bit storage: FP32 rounding differences may move a stored deviation f - w rho0 across a
+    rounding boundary, so the bound is 4 units of the format's relative precision eps times the
+    largest deviation."""
+    shape = (6, 5, 4)
+    c = make_cfg(shape, (1, 0, 1), 1.3, storage=storage, force=(1e-5, 0, 0))
+    f0 = initial_state(shape, 7)
+    flags = box_flags(shape, "y")
+    o = O.Oracle3D(c, 32)
+    o.set_pops(f0.astype(np.float32))
+    o.set_flags(flags)
+    o.step(5)
+    g, _, _ = emu_run(c, f0, flags, None, 5)
+    ref = o.get_pops()
+    dev = np.abs(ref - W[:, None, None, None]).max()
+    assert np.abs(g - ref).max() <= 4 * eps * dev

@@ -1,0 +1,235 @@
+"""Pins the D3Q27 oracle (oracle/lbm_oracle3d.hpp) against what its code does not restate:
+lattice isotropy, moment identities, closed forms, numpy.roll, conservation laws and two
+analytic flows. CPU only."""
+import ctypes
+import itertools
+
+import numpy as np
+import pytest
+
+from lbm_amd import _abi
+import oracle as O
+
+W, CX, CY, CZ, OPP = O.tables3d()
+C = np.stack([CX, CY, CZ], 1).astype(np.float64)
+
+
+def cfg3(nx, ny, nz, omega=1.0, periodic=(1, 1, 1), force=(0.0, 0.0, 0.0), storage=_abi.LBM_F32, quirks=0):
+    c = _abi.Lbm3dConfig()
+    c.nx, c.ny, c.nz = nx, ny, nz
+    c.collision = _abi.LBM_BGK
+    c.omega = omega
+    c.rho0 = 1.0
+    c.storage = storage
+    c.periodic_x, c.periodic_y, c.periodic_z = periodic
+    for i in range(3):
+        c.force[i] = force[i]
+    c.quirks = quirks
+    return c
+
+
+def test_velocity_set_is_the_full_27_stencil_in_shell_order():
+    assert len({tuple(v) for v in C.astype(int)}) == 27
+    assert set(map(tuple, C.astype(int))) == set(itertools.product((-1, 0, 1), repeat=3))
+    shells = (C ** 2).sum(1)
+    assert shells[0] == 0 and all(shells[1:7] == 1) and all(shells[7:19] == 2) and all(shells[19:] == 3)
+    assert np.array_equal(C[OPP], -C)
+    assert all(OPP[OPP] == np.arange(27))
+
+
+def test_weights_give_an_isotropic_fourth_order_lattice():
+    assert abs(W.sum() - 1) < 1e-15
+    d = np.eye(3)
+    assert np.allclose(np.einsum("q,qa->a", W, C), 0, atol=1e-15)
+    assert np.allclose(np.einsum("q,qa,qb->ab", W, C, C), d / 3, atol=1e-15)
+    assert np.allclose(np.einsum("q,qa,qb,qc->abc", W, C, C, C), 0, atol=1e-15)
+    iso4 = (np.einsum("ab,cd->abcd", d, d) + np.einsum("ac,bd->abcd", d, d) + np.einsum("ad,bc->abcd", d, d)) / 9
+    assert np.allclose(np.einsum("q,qa,qb,qc,qd->abcd", W, C, C, C, C), iso4, atol=1e-15)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_equilibrium_moments_up_to_second_order(seed):
+    rng = np.random.default_rng(seed)
+    rho = 1 + 0.2 * rng.standard_normal()
+    u = 0.1 * rng.standard_normal(3)
+    feq = O.equilibrium3d(rho, *u)
+    assert abs(feq.sum() - rho) < 1e-14
+    assert np.allclose(feq @ C, rho * u, atol=1e-14)
+    assert np.allclose(np.einsum("q,qa,qb->ab", feq, C, C), rho * (np.outer(u, u) + np.eye(3) / 3), atol=1e-14)
+    # FP32 evaluation agrees with FP64 to float rounding
+    assert np.allclose(O.equilibrium3d(rho, *u, prec=32), feq, rtol=0, atol=1e-7)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_literal_equilibrium_carries_17_27_of_the_momentum(seed):
+    """The reference's De Rosis form (equilibrium.cu:70-96) has linear coefficient 1 on the edge
+    and corner directions: its first moment is (2/27*3*2 + 1/54*8 + 1/216*8) rho u = 17/27 rho u,
+    and it differs from the second-order equilibrium by -2 w rho (c.u) on those directions only."""
+    rng = np.random.default_rng(10 + seed)
+    rho = 1 + 0.1 * rng.standard_normal()
+    u = 0.05 * rng.standard_normal(3)
+    lit = O.equilibrium3d(rho, *u, literal=True)
+    ref = O.equilibrium3d(rho, *u)
+    assert abs(lit.sum() - rho) < 1e-14
+    assert np.allclose(lit @ C, 17 / 27 * rho * u, atol=1e-14)
+    assert np.allclose(np.einsum("q,qa,qb->ab", lit, C, C), np.einsum("q,qa,qb->ab", ref, C, C), atol=1e-14)
+    expect = np.where(np.arange(27) >= 7, -2 * W * rho * (C @ u), 0.0)
+    assert np.allclose(lit - ref, expect, atol=1e-15)
+
+
+@pytest.mark.parametrize("omega", [0.7, 1.0, 1.9])
+def test_guo_source_moments(omega):
+    rng = np.random.default_rng(3)
+    u = 0.05 * rng.standard_normal(3)
+    F = 1e-3 * rng.standard_normal(3)
+    S = O.guo_source3d(u, F, omega)
+    k = 1 - omega / 2
+    assert abs(S.sum()) < 1e-17
+    assert np.allclose(S @ C, k * F, atol=1e-17)
+    assert np.allclose(np.einsum("q,qa,qb->ab", S, C, C), k * (np.outer(u, F) + np.outer(F, u)), atol=1e-17)
+
+
+def test_periodic_streaming_is_a_roll_of_each_plane():
+    nx, ny, nz = 7, 5, 6
+    o = O.Oracle3D(cfg3(nx, ny, nz), 64)
+    f = np.random.default_rng(0).random((27, nz, ny, nx))
+    o.set_pops(f)
+    o.stream_only()
+    g = o.get_pops()
+    for q in range(27):
+        assert np.array_equal(g[q], np.roll(f[q], (CZ[q], CY[q], CX[q]), axis=(0, 1, 2)))
+
+
+def test_non_periodic_streaming_leaves_unreached_slots_stale():
+    nx, ny, nz = 5, 4, 6
+    o = O.Oracle3D(cfg3(nx, ny, nz, periodic=(0, 1, 0)), 64)
+    f = np.random.default_rng(1).random((27, nz, ny, nx))
+    o.set_pops(f)  # both buffers hold f
+    o.stream_only()
+    g = o.get_pops()
+    for q in range(27):
+        rolled = np.roll(f[q], (CZ[q], CY[q], CX[q]), axis=(0, 1, 2))
+        z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+        xs, zs = x - CX[q], z - CZ[q]
+        reached = (xs >= 0) & (xs < nx) & (zs >= 0) & (zs < nz)
+        assert np.array_equal(g[q][reached], rolled[reached])
+        assert np.array_equal(g[q][~reached], f[q][~reached])
+
+
+def test_unforced_periodic_bgk_conserves_mass_and_momentum():
+    nx, ny, nz = 8, 6, 5
+    o = O.Oracle3D(cfg3(nx, ny, nz, omega=1.6), 64)
+    rng = np.random.default_rng(2)
+    rho = 1 + 0.05 * rng.standard_normal((nz, ny, nx))
+    u = 0.05 * rng.standard_normal((nz, ny, nx, 3))
+    o.init_from_macros(rho, u)
+    f0 = o.get_pops()
+    o.step(20)
+    f1 = o.get_pops()
+    assert abs(f1.sum() - f0.sum()) < 1e-11
+    assert np.allclose(np.einsum("qzyx,qa->a", f1, C), np.einsum("qzyx,qa->a", f0, C), atol=1e-12)
+
+
+def test_uniform_force_adds_n_times_f_of_momentum_per_step():
+    nx, ny, nz = 6, 5, 4
+    F = np.array([2e-5, -1e-5, 3e-5])
+    o = O.Oracle3D(cfg3(nx, ny, nz, omega=1.2, force=F), 64)
+    o.init_from_macros(np.ones((nz, ny, nx)), np.zeros((nz, ny, nx, 3)))
+    o.step(10)
+    p = np.einsum("qzyx,qa->a", o.get_pops(), C)
+    F32 = F.astype(np.float32).astype(np.float64)  # lbm3d_config.force is float
+    assert np.allclose(p, 10 * F32 * nx * ny * nz, rtol=1e-10, atol=0)
+
+
+def test_rest_state_is_a_fixed_point_of_a_closed_bounce_back_box():
+    nx, ny, nz = 5, 4, 6
+    o = O.Oracle3D(cfg3(nx, ny, nz, omega=1.3, periodic=(0, 0, 0)), 64)
+    fl = np.zeros((nz, ny, nx), np.uint8)
+    fl[0], fl[-1], fl[:, 0], fl[:, -1], fl[:, :, 0], fl[:, :, -1] = (_abi.BOUNCE_BACK,) * 6
+    o.set_flags(fl)
+    o.init_from_macros(np.ones((nz, ny, nx)), np.zeros((nz, ny, nx, 3)))
+    o.step(7)
+    f = o.get_pops()
+    assert np.abs(f - W[:, None, None, None]).max() < 1e-15
+
+
+def test_force_driven_channel_has_curvature_minus_f_over_nu():
+    """Walls at y = 0 and y = NY-1 (BOUNCE_BACK), periodic x and z, body force along x: the
+    steady profile satisfies nu u'' = -F in the bulk and is symmetric about the centre."""
+    nx, ny, nz = 3, 18, 3
+    omega = 1.0
+    nu = (1 / omega - 0.5) / 3
+    Fx = 1e-6
+    o = O.Oracle3D(cfg3(nx, ny, nz, omega=omega, periodic=(1, 0, 1), force=(Fx, 0, 0)), 64)
+    fl = np.zeros((nz, ny, nx), np.uint8)
+    fl[:, 0], fl[:, -1] = _abi.BOUNCE_BACK, _abi.BOUNCE_BACK
+    o.set_flags(fl)
+    o.init_from_macros(np.ones((nz, ny, nx)), np.zeros((nz, ny, nx, 3)))
+    o.step(4000)
+    rho, u = o.get_macros()
+    ux = u[..., 0].mean(axis=(0, 2))
+    assert np.abs(u[..., 1:]).max() < 1e-12 * 1e6 * Fx
+    assert np.allclose(ux, ux[::-1], rtol=1e-9, atol=0)
+    d2 = ux[3:-1] - 2 * ux[2:-2] + ux[1:-3]
+    assert np.allclose(d2, -Fx / nu, rtol=2e-3)
+    assert ux.max() > 30 * abs(ux[0])
+
+
+def test_small_taylor_green_vortex_decays_at_the_viscous_rate():
+    """u = U (sin kx cos ky cos kz, -cos kx sin ky cos kz, 0) is an eigenfield of the Laplacian
+    with eigenvalue -3k^2: at small U its kinetic energy decays as exp(-6 nu k^2 t)."""
+    n = 24
+    omega = 1.7
+    nu = (1 / omega - 0.5) / 3
+    k = 2 * np.pi / n
+    o = O.Oracle3D(cfg3(n, n, n, omega=omega), 64)
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    U = 1e-4
+    u = np.stack([U * np.sin(k * x) * np.cos(k * y) * np.cos(k * z),
+                  -U * np.cos(k * x) * np.sin(k * y) * np.cos(k * z),
+                  np.zeros_like(x, dtype=float)], -1)
+    o.init_from_macros(np.ones((n, n, n)), u)
+
+    def energy():
+        rho, uu = o.get_macros()
+        return 0.5 * (rho * (uu ** 2).sum(-1)).sum()
+
+    o.step(20)
+    e0 = energy()
+    o.step(60)
+    e1 = energy()
+    rate = -np.log(e1 / e0) / 60
+    assert abs(rate / (6 * nu * k * k) - 1) < 0.02
+
+
+def test_literal_equilibrium_breaks_momentum_conservation():
+    nx, ny, nz = 6, 5, 4
+    rng = np.random.default_rng(4)
+    rho = np.ones((nz, ny, nx))
+    u = 0.02 * rng.standard_normal((nz, ny, nx, 3))
+    o = O.Oracle3D(cfg3(nx, ny, nz, omega=1.0, quirks=_abi.LBM_QUIRK_EQ3D_LINEAR), 64)
+    o.init_from_macros(rho, u)
+    p0 = np.einsum("qzyx,qa->a", o.get_pops(), C)
+    assert np.allclose(p0, 17 / 27 * (rho[..., None] * u).sum((0, 1, 2)), atol=1e-13)
+    o.step(1)
+    p1 = np.einsum("qzyx,qa->a", o.get_pops(), C)
+    # omega = 1: f* = f_eq(rho, u(f)) whose momentum is 17/27 of that of f
+    assert np.allclose(p1, 17 / 27 * p0, atol=1e-13)
+
+
+def test_reduced_storage_rounds_the_shifted_deviation():
+    """16-bit storage keeps f - w rho0 to half (2^-11) or bfloat16 (2^-8) relative rounding: the
+    error bound scales with the deviation from w, not with f (plus half the FP16 subnormal
+    quantum 2^-25 and the FP32 rounding of f and w)."""
+    nx, ny, nz = 4, 3, 3
+    for st, eps in [(_abi.LBM_F16_SHIFTED, 2.0 ** -11), (_abi.LBM_BF16_SHIFTED, 2.0 ** -8)]:
+        o = O.Oracle3D(cfg3(nx, ny, nz, storage=st), 32)
+        rng = np.random.default_rng(5)
+        f = W[:, None, None, None] * (1 + 0.05 * rng.standard_normal((27, nz, ny, nx)))
+        o.set_pops(f)
+        g = o.get_pops()
+        dev = np.abs(f - W[:, None, None, None])
+        assert np.all(np.abs(g - f) <= eps * dev + 2.0 ** -25 + 2.0 ** -22 * W[:, None, None, None])
+        assert np.abs(g - f).max() > 0
+        o.set_pops(g)  # idempotent
+        assert np.array_equal(o.get_pops(), g)
