@@ -249,15 +249,19 @@ def run_d3q27(torch, steps=200, warmup=20):
     for name, st in (("hbm_256_f32", A.LBM_F32), ("hbm_256_f16", A.LBM_F16_SHIFTED)):
         res[name] = one(Config3D(256, 256, 256, omega=1.8, storage=st, **per), tg_init(256), n_steps=max(20, steps // 2))
     res["speedup_f16_vs_f32"] = round(res["hbm_256_f16"]["mlups"] / res["hbm_256_f32"]["mlups"], 3)
+    S = [1.0] * 27
+    S[4:9] = [1.8] * 5
+    res["hbm_256_mrt_f32"] = one(Config3D(256, 256, 256, collision=A.LBM_MRT, S=tuple(S), **per), tg_init(256),
+                                 n_steps=max(20, steps // 2))
     nx, ny, nz = 256, 128, 96
     fl = np.zeros((nz, ny, nx), np.uint8)
     fl[:, 0, :] = fl[:, -1, :] = A.BOUNCE_BACK
     fl[0], fl[-1] = A.BOUNCE_BACK, A.BOUNCE_BACK
     ch = Config3D(nx, ny, nz, omega=1.6, periodic_x=True, force=(1e-6, 0.0, 0.0))
     res["channel_256x128x96_f32"] = one(ch, (np.ones((nz, ny, nx), np.float32), np.zeros((nz, ny, nx, 3), np.float32)), fl)
-    res["workload"] = ("D3Q27 BGK with Guo forcing, FP32 collision; Taylor-Green 3-D (u_max 0.05) periodic, and a "
-                       "force-driven channel with bounce-back walls on y and z; 216 B (FP32) / 108 B (FP16) per "
-                       "node update")
+    res["workload"] = ("D3Q27 BGK with Guo forcing (and MRT<3> at 256^3), FP32 collision; Taylor-Green 3-D "
+                       "(u_max 0.05) periodic, and a force-driven channel with bounce-back walls on y and z; 216 B "
+                       "(FP32) / 108 B (FP16) per node update")
     return res
 
 
@@ -357,7 +361,10 @@ def main():
         print(json.dumps({"ibm_cylinder": run_ibm_cylinder(torch)}))
         return
     if args.only == "d3q27":
-        print(json.dumps({"d3q27": run_d3q27(torch)}))
+        res = run_d3q27(torch, steps=max(20, args.steps // 10))
+        if args.calibrate_copy:
+            res["copy_calibration_bytes"] = copy_calibration(torch, args.calibrate_copy)
+        print(json.dumps({"d3q27": res}))
         return
 
     # headline: LDC 1024^2 MRT FP32
@@ -416,7 +423,7 @@ def main():
         out["poiseuille"] = run_poiseuille(torch)
         out["adjoint"] = run_adjoint(torch)
         out["accuracy"] = run_accuracy(torch)
-        out["d3q27"] = run_d3q27(torch)
+        out["d3q27"] = run_d3q27(torch, steps=max(20, args.steps // 10))
     if args.calibrate_copy:
         out["copy_calibration_bytes"] = copy_calibration(torch, args.calibrate_copy)
     if not args.no_cpu:

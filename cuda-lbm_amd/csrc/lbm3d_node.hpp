@@ -126,6 +126,137 @@ LBM3_HD void collide_bgk3(float* g, const Macro3& m, float fx, float fy, float f
     bgk_all(g, K, std::make_integer_sequence<int, Q>{});
 }
 
+// MRT<3>::apply (MRT.cu:397-456) with compute_m_post_component (MRT.cu:155-201), without the
+// 27x27 matrices. Every post-collision moment the reference sets from (rho, u) is a raw moment
+// of the product-form equilibrium f^p = rho * phi_x(c_x) phi_y(c_y) phi_z(c_z), phi(0) = 2/3 -
+// u_a^2, phi(+-1) = (1/3 + u_a^2 +- u_a)/2 (1-D moments 1, u_a, 1/3 + u_a^2). So
+//   f* = f^p + M^-1 d,
+// where d, the post-collision moments minus those of f^p, is non-zero only in the relaxed
+// deviatoric second moments 4-8 ((1 - S)(m - m^eq) + (1 - S/2) F_k) and the forcing terms
+// (F_k / 2, F_k of MRT.cu:119-153). M^-1 d is taken in two stages: the basis combinations are
+// undone into tensor moments T_abc = sum f c_x^a c_y^b c_z^c, which are then inverted one axis
+// at a time (h(0) = T_0 - T_2, h(+-1) = (T_2 +- T_1)/2). `literal` keeps the reference's
+// momentum without F/2 and its (2 ux^2 + 1) in moment 19 (LBM_QUIRK_MRT3_LITERAL).
+struct Mrt3Rates {
+    float s[5];  // S[4..8]
+    int literal;
+};
+constexpr int q_of(int cx, int cy, int cz) {
+    for (int q = 0; q < Q; q++)
+        if (CX[q] == cx && CY[q] == cy && CZ[q] == cz) return q;
+    return -1;
+}
+template <int q>
+LBM3_HD void m2_acc(const float* g, float& m4, float& m5, float& m6, float& m7, float& m8) {
+    constexpr int xy = CX[q] * CY[q], xz = CX[q] * CZ[q], yz = CY[q] * CZ[q];
+    constexpr int d7 = CX[q] * CX[q] - CY[q] * CY[q], d8 = CX[q] * CX[q] - CZ[q] * CZ[q];
+    if (xy > 0) m4 += g[q];
+    if (xy < 0) m4 -= g[q];
+    if (xz > 0) m5 += g[q];
+    if (xz < 0) m5 -= g[q];
+    if (yz > 0) m6 += g[q];
+    if (yz < 0) m6 -= g[q];
+    if (d7 > 0) m7 += g[q];
+    if (d7 < 0) m7 -= g[q];
+    if (d8 > 0) m8 += g[q];
+    if (d8 < 0) m8 -= g[q];
+}
+template <int... qs>
+LBM3_HD void m2_all(const float* g, float& m4, float& m5, float& m6, float& m7, float& m8, std::integer_sequence<int, qs...>) {
+    (m2_acc<qs>(g, m4, m5, m6, m7, m8), ...);
+}
+LBM3_HD void collide_mrt3(float* g, const Macro3& m, float fx, float fy, float fz, const Mrt3Rates& R) {
+    const float rho = m.rho, ux = m.ux, uy = m.uy, uz = m.uz;
+    float m4 = 0.0f, m5 = 0.0f, m6 = 0.0f, m7 = 0.0f, m8 = 0.0f;
+    m2_all(g, m4, m5, m6, m7, m8, std::make_integer_sequence<int, Q>{});
+    const float ux2 = ux * ux, uy2 = uy * uy, uz2 = uz * uz;
+    const float Fu = fx * ux + fy * uy + fz * uz;
+    auto dev = [&](int k, float mk, float meq, float Fk) {
+        const float sk = R.s[k - 4];
+        return (1.0f - sk) * (mk - meq) + (1.0f - 0.5f * sk) * Fk;
+    };
+    const float half = R.literal ? 0.0f : 0.5f;
+    const float d4 = dev(4, m4, rho * ux * uy, fx * uy + fy * ux);
+    const float d5 = dev(5, m5, rho * ux * uz, fx * uz + fz * ux);
+    const float d6 = dev(6, m6, rho * uy * uz, fy * uz + fz * uy);
+    const float d7 = dev(7, m7, rho * (ux2 - uy2), 2.0f * (fx * ux - fy * uy));
+    const float d8 = dev(8, m8, rho * (ux2 - uz2), 2.0f * (fx * ux - fz * uz));
+    const float d9 = Fu;
+    const float d17 = (2.0f / 3.0f) * Fu, d18 = (2.0f / 3.0f) * fx * ux;
+    float d19 = (1.0f / 3.0f) * (fy * uy - fz * uz);
+    if (R.literal) d19 -= (1.0f / 3.0f) * rho * ux2 * (uy2 - uz2);
+    // tensor moments T[a][b][c] of M^-1 d
+    float T[3][3][3];
+    T[0][0][0] = 0.0f;
+    T[1][0][0] = half * fx;
+    T[0][1][0] = half * fy;
+    T[0][0][1] = half * fz;
+    T[1][1][0] = d4;
+    T[1][0][1] = d5;
+    T[0][1][1] = d6;
+    T[2][0][0] = (d7 + d8 + d9) * (1.0f / 3.0f);
+    T[0][2][0] = T[2][0][0] - d7;
+    T[0][0][2] = T[2][0][0] - d8;
+    T[1][2][0] = T[1][0][2] = fx * (1.0f / 6.0f);  // (d10 +- d13)/2, d10 = fx/3, d13 = 0
+    T[2][1][0] = T[0][1][2] = fy * (1.0f / 6.0f);
+    T[2][0][1] = T[0][2][1] = fz * (1.0f / 6.0f);
+    T[1][1][1] = 0.0f;
+    const float sxy = 0.5f * (d17 + d18);
+    T[0][2][2] = 0.5f * (d17 - d18);
+    T[2][2][0] = 0.5f * (sxy + d19);
+    T[2][0][2] = 0.5f * (sxy - d19);
+    T[2][1][1] = (1.0f / 6.0f) * (fy * uz + fz * uy);
+    T[1][2][1] = (1.0f / 6.0f) * (fx * uz + fz * ux);
+    T[1][1][2] = (1.0f / 6.0f) * (fx * uy + fy * ux);
+    T[1][2][2] = fx * (1.0f / 18.0f);
+    T[2][1][2] = fy * (1.0f / 18.0f);
+    T[2][2][1] = fz * (1.0f / 18.0f);
+    T[2][2][2] = (1.0f / 9.0f) * Fu;
+    // invert along z, y, x: index 0 -> c = -1, 1 -> c = 0, 2 -> c = +1
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            const float t0 = T[a][b][0], t1 = T[a][b][1], t2 = T[a][b][2];
+            T[a][b][0] = 0.5f * (t2 - t1);
+            T[a][b][1] = t0 - t2;
+            T[a][b][2] = 0.5f * (t2 + t1);
+        }
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float t0 = T[a][0][c], t1 = T[a][1][c], t2 = T[a][2][c];
+            T[a][0][c] = 0.5f * (t2 - t1);
+            T[a][1][c] = t0 - t2;
+            T[a][2][c] = 0.5f * (t2 + t1);
+        }
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float t0 = T[0][b][c], t1 = T[1][b][c], t2 = T[2][b][c];
+            T[0][b][c] = 0.5f * (t2 - t1);
+            T[1][b][c] = t0 - t2;
+            T[2][b][c] = 0.5f * (t2 + t1);
+        }
+    // product-form equilibrium
+    const float third = 1.0f / 3.0f;
+    const float px[3] = {0.5f * (third + ux2 - ux), 2.0f * third - ux2, 0.5f * (third + ux2 + ux)};
+    const float py[3] = {0.5f * (third + uy2 - uy), 2.0f * third - uy2, 0.5f * (third + uy2 + uy)};
+    const float pz[3] = {0.5f * (third + uz2 - uz), 2.0f * third - uz2, 0.5f * (third + uz2 + uz)};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float ra = rho * px[a];
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            const float rab = ra * py[b];
+#pragma unroll
+            for (int c = 0; c < 3; c++) g[q_of(a - 1, b - 1, c - 1)] = rab * pz[c] + T[a][b][c];
+        }
+    }
+}
+
 // BounceBack<3>::apply (bbDomainBoundary.cuh:52-90): for i = 1..26 in order, a direction whose
 // neighbour lies outside the domain on a non-periodic axis copies g_i into g_opp(i).
 // Sequential as written, so edge and corner nodes match the reference's loop order.

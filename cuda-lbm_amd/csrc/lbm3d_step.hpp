@@ -35,11 +35,29 @@ struct Step3Params {
     long long plane;       // elements between population planes (N rounded up to 64)
     float omega, rho0, fx, fy, fz;
     float lin_ec;          // linear equilibrium coefficient of edge/corner directions (3 or 1)
+    Mrt3Rates mrt;         // MRT<3> rates S[4..8] and the literal flag
     int stale_rest;        // LBM_QUIRK_STALE_REST
-    int nbx, nby;          // workgroups along x (64 nodes) and y (kRows rows)
+    // Workgroup shape: 256 threads = wx wavefronts along x (64*wx consecutive nodes of a row)
+    // times 4/wx rows; nbx, nby workgroups along x and y, nz along z.
+    int wx;
+    int nbx, nby;
 };
 
-constexpr int kRows = 4;  // x-rows (wavefronts) per 256-thread workgroup
+// Workgroup of 256 threads: wx = min(4, ceil(nx/64)) wavefronts along x, so a workgroup holds
+// whole rows up to nx = 256 and the partial cache lines of the x-shifted pulls are shared
+// between wavefronts of one workgroup.
+LBM3_HD int wave_cols(int nx) { return nx > 192 ? 4 : (nx > 64 ? 2 : 1); }
+
+// (x, y, z) of thread t of logical workgroup L.
+LBM3_HD void thread_node(const Step3Params& p, long long L, int t, int& x, int& y, int& z) {
+    const int bx = int(L % p.nbx);
+    const long long r = L / p.nbx;
+    const int by = int(r % p.nby);
+    z = int(r / p.nby);
+    const int cols = 64 * p.wx;
+    x = bx * cols + t % cols;
+    y = by * (4 / p.wx) + t / cols;
+}
 
 // Population storage: FP32, or FP16/BF16 of the shifted deviation f - w rho0.
 template <int STORE>
@@ -74,31 +92,52 @@ LBM3_HD void axis_pull(int x, int n, int periodic, int stride, int* off, bool* o
     off[0] = x < n - 1 ? stride : (periodic ? -(n - 1) * stride : 0);
 }
 
-template <int STORE, bool BUF, int q>
+// True when `v` holds on every active lane of the wavefront (the host emulation runs one lane).
+LBM3_HD bool wave_all(bool v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_ballot_w64(!v) == 0;
+#else
+    return v;
+#endif
+}
+
+// Pull of population q. FAST: every upstream node of the wavefront is inside the domain or
+// wrapped, so every lane loads from the source buffer. Otherwise each lane loads from the
+// source at an in-domain index, and lanes whose upstream node lies outside a non-periodic
+// domain then read their own stale slot of the destination buffer through a plain pointer
+// (a per-lane choice between two buffer resources would make the compiler loop over the
+// distinct resources of the wavefront).
+template <int STORE, bool BUF, bool FAST, int q>
 LBM3_HD void pull_dir(float* g, const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src,
-                      const lbm::PopBuf<typename Store3<STORE>::T, BUF>& dst, uint32_t node, const int* ox,
-                      const bool* vx, const int* oy, const bool* vy, const int* oz, const bool* vz) {
+                      uint32_t node, const int* ox, const bool* vx, const int* oy, const bool* vy, const int* oz,
+                      const bool* vz) {
     using T = typename Store3<STORE>::T;
     constexpr int a = CX[q] + 1, b = CY[q] + 1, c = CZ[q] + 1;
     const long long off = (long long)q * p.plane;
-    bool ok = vx[a] && vy[b] && vz[c];
-    if (q == 0) ok = !p.stale_rest;
+    const uint32_t up = uint32_t(int(node) + ox[a] + oy[b] + oz[c]);
+    const T* stale = static_cast<const T*>(p.dst) + off + node;
     T raw;
-    if (ok) raw = src.ld(uint32_t(int(node) + ox[a] + oy[b] + oz[c]), off);
-    else raw = dst.ld(node, off);
+    if (q == 0 && p.stale_rest) {
+        raw = *stale;
+    } else if (FAST) {
+        raw = src.ld(up, off);
+    } else {
+        const bool ok = vx[a] && vy[b] && vz[c];
+        raw = src.ld(ok ? up : node, off);
+        if (!ok) raw = *stale;
+    }
     g[q] = Store3<STORE>::dec(raw, q, p.rho0);
 }
-template <int STORE, bool BUF, int... qs>
+template <int STORE, bool BUF, bool FAST, int... qs>
 LBM3_HD void pull_all(float* g, const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src,
-                      const lbm::PopBuf<typename Store3<STORE>::T, BUF>& dst, uint32_t node, const int* ox,
-                      const bool* vx, const int* oy, const bool* vy, const int* oz, const bool* vz,
-                      std::integer_sequence<int, qs...>) {
-    (pull_dir<STORE, BUF, qs>(g, p, src, dst, node, ox, vx, oy, vy, oz, vz), ...);
+                      uint32_t node, const int* ox, const bool* vx, const int* oy, const bool* vy, const int* oz,
+                      const bool* vz, std::integer_sequence<int, qs...>) {
+    (pull_dir<STORE, BUF, FAST, qs>(g, p, src, node, ox, vx, oy, vy, oz, vz), ...);
 }
 
-// One node of one time step: pull, boundary functor, macroscopics, Guo-forced BGK, store.
-// Requires 0 <= x < nx, 0 <= y < ny, 0 <= z < nz and nx*ny*nz < 2^31.
-template <int STORE, bool BUF>
+// One node of one time step: pull, boundary functor, macroscopics, Guo-forced BGK or MRT,
+// store. Requires 0 <= x < nx, 0 <= y < ny, 0 <= z < nz and nx*ny*nz < 2^31.
+template <int COLL, int STORE, bool BUF>
 LBM3_HD void step3_node(const Step3Params& p, int x, int y, int z) {
     using St = Store3<STORE>;
     using T = typename St::T;
@@ -111,7 +150,11 @@ LBM3_HD void step3_node(const Step3Params& p, int x, int y, int z) {
     axis_pull(z, p.nz, p.pz, nxy, oz, vz);
     const lbm::PopBuf<T, BUF> src(p.src), dst(p.dst);
     float g[Q];
-    pull_all<STORE, BUF>(g, p, src, dst, node, ox, vx, oy, vy, oz, vz, std::make_integer_sequence<int, Q>{});
+    const bool lane_in = vx[0] && vx[2] && vy[0] && vy[2] && vz[0] && vz[2];
+    if (wave_all(lane_in))
+        pull_all<STORE, BUF, true>(g, p, src, node, ox, vx, oy, vy, oz, vz, std::make_integer_sequence<int, Q>{});
+    else
+        pull_all<STORE, BUF, false>(g, p, src, node, ox, vx, oy, vy, oz, vz, std::make_integer_sequence<int, Q>{});
 
     if (p.flags && p.flags[node] == LBM_BOUNCE_BACK) {
         const Box3 B{p.nx, p.ny, p.nz, p.px, p.py, p.pz};
@@ -130,7 +173,8 @@ LBM3_HD void step3_node(const Step3Params& p, int x, int y, int z) {
         p.out_u[p.plane + node] = m.uy;
         p.out_u[2 * p.plane + node] = m.uz;
     }
-    collide_bgk3(g, m, fx, fy, fz, p.omega, p.lin_ec);
+    if (COLL == LBM_MRT) collide_mrt3(g, m, fx, fy, fz, p.mrt);
+    else collide_bgk3(g, m, fx, fy, fz, p.omega, p.lin_ec);
 #pragma unroll
     for (int q = 0; q < Q; q++) dst.st(node, (long long)q * p.plane, St::enc(g[q], q, p.rho0));
 }

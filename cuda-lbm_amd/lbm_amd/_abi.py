@@ -55,6 +55,7 @@ LBM_QUIRK_IBM_2X2 = 1 << 3
 LBM_QUIRK_IBM_CLAMP = 1 << 4
 LBM_QUIRK_IBM_NO_DS = 1 << 5
 LBM_QUIRK_EQ3D_LINEAR = 1 << 6   # include/lbm3d_hip.h
+LBM_QUIRK_MRT3_LITERAL = 1 << 7
 
 # reductions
 LBM_REDUCE_MASS = 0
@@ -97,6 +98,7 @@ class Lbm3dConfig(ctypes.Structure):
         ("nz", ctypes.c_int32),
         ("collision", ctypes.c_int32),
         ("omega", ctypes.c_float),
+        ("S", ctypes.c_float * 27),
         ("rho0", ctypes.c_float),
         ("storage", ctypes.c_int32),
         ("periodic_x", ctypes.c_int32),

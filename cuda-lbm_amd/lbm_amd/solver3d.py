@@ -24,8 +24,9 @@ class Config3D:
     nx: int
     ny: int
     nz: int
-    collision: int = A.LBM_BGK
+    collision: int = A.LBM_BGK  # or A.LBM_MRT (rates S[4..8])
     omega: float = 1.0
+    S: tuple = (1.0,) * 27
     rho0: float = 1.0
     storage: int = A.LBM_F32
     periodic_x: bool = False
@@ -39,6 +40,8 @@ class Config3D:
         c.nx, c.ny, c.nz = self.nx, self.ny, self.nz
         c.collision = self.collision
         c.omega = self.omega
+        for k in range(27):
+            c.S[k] = self.S[k]
         c.rho0 = self.rho0
         c.storage = self.storage
         c.periodic_x, c.periodic_y, c.periodic_z = int(self.periodic_x), int(self.periodic_y), int(self.periodic_z)

@@ -10,11 +10,12 @@
  * [27][node]; `on_device` = 1 marks a device pointer (the call is then stream-ordered and does
  * not synchronise).
  *
- * Built: BGK<3> with Guo forcing (src/core/collision/BGK/BGK.cuh:55-96), the FLUID and
- * BOUNCE_BACK flags (src/core/boundaries/boundaries.cuh:89-138, BounceBack<3>
+ * Built: BGK<3> with Guo forcing (src/core/collision/BGK/BGK.cuh:55-96), MRT<3> in the De Rosis
+ * raw-moment basis (src/core/collision/MRT/MRT.cu:79-456, basis src/core/lbm_constants.cuh:218-246),
+ * the FLUID and BOUNCE_BACK flags (src/core/boundaries/boundaries.cuh:89-138, BounceBack<3>
  * src/functors/boundaryConditions/bbDomainBoundary.cuh:52-90), runtime periodic axes, FP32 or
- * 16-bit shifted population storage. MRT<3>, CM<3> and the other 3-D boundary functors are
- * rejected with LBM_E_UNSUPPORTED.
+ * 16-bit shifted population storage. CM<3> and the other 3-D boundary functors are rejected
+ * with LBM_E_UNSUPPORTED.
  */
 #ifndef LBM3D_HIP_H
 #define LBM3D_HIP_H
@@ -33,23 +34,34 @@ extern "C" {
  * moment is (17/27) rho u instead of rho u. Default (0): the second-order equilibrium. */
 #define LBM_QUIRK_EQ3D_LINEAR (1u << 6)
 
+/* Reference-literal MRT<3> post-collision moments (bit in lbm3d_config.quirks), as
+ * compute_m_post_component writes them (src/core/collision/MRT/MRT.cu:155-201): the momentum
+ * moments are rho*u with no forcing term (:168-171), so a step adds F/2 of momentum instead of F,
+ * and the equilibrium of moment 19 (cx^2 cy^2 - cx^2 cz^2) carries (2 ux^2 + 1) where the moment
+ * of the equilibrium has (3 ux^2 + 1) (:188). Default (0): rho*u + F/2 (the (1 - S/2) F rule
+ * with S = 1 applied to every other moment) and (3 ux^2 + 1). The reference's active path,
+ * compute_m_post_combined (MRT.cu:203-395), leaves moments 8-26 unset and is not restated. */
+#define LBM_QUIRK_MRT3_LITERAL (1u << 7)
+
 typedef struct lbm3d_config {
     int32_t  nx, ny, nz;
-    int32_t  collision;      /* LBM_BGK (the only 3-D operator built)                        */
-    float    omega;          /* 1/tau (ScenarioTrait::omega, src/scenarios/scenario.cuh:36-38) */
+    int32_t  collision;      /* LBM_BGK | LBM_MRT                                             */
+    float    omega;          /* BGK rate 1/tau (ScenarioTrait::omega, src/scenarios/scenario.cuh:36-38) */
+    float    S[27];          /* MRT<3> rates in the basis order (ScenarioTrait::S); S[4..8], the
+                                deviatoric second moments, are the ones MRT<3> reads            */
     float    rho0;           /* reference density of the shifted storage                      */
     int32_t  storage;        /* LBM_F32 | LBM_F16_SHIFTED | LBM_BF16_SHIFTED                  */
     int32_t  periodic_x;     /* streaming wraps on the axis (PERIODIC_X/Y/Z of               */
     int32_t  periodic_y;     /*   src/core/streaming/streaming.cu:45-53, never defined there) */
     int32_t  periodic_z;
     float    force[3];       /* uniform Guo body force, used where no force field is set      */
-    uint32_t quirks;         /* LBM_QUIRK_STALE_REST | LBM_QUIRK_EQ3D_LINEAR                   */
+    uint32_t quirks;         /* LBM_QUIRK_STALE_REST | LBM_QUIRK_EQ3D_LINEAR | LBM_QUIRK_MRT3_LITERAL */
     int32_t  reserved[8];
 } lbm3d_config;
 
 typedef struct lbm3d_ctx lbm3d_ctx;
 
-/* Defaults: BGK, tau = 1 (omega = 1), rho0 = 1, FP32, not periodic, no force. */
+/* Defaults: BGK, tau = 1 (omega = 1), S = 1 everywhere, rho0 = 1, FP32, not periodic, no force. */
 int lbm3d_config_default(lbm3d_config* cfg, int32_t nx, int32_t ny, int32_t nz);
 
 /* Replaces LBM<3>::allocate (lbm.cuh:92-125) + send_consts (lbm.cuh:63-82). */
@@ -72,7 +84,7 @@ int lbm3d_init_from_macros(lbm3d_ctx* ctx, const float* rho, const float* u, int
 int lbm3d_set_force_field(lbm3d_ctx* ctx, const float* force, int on_device);
 
 /* `nsteps` iterations of main.cu:96-114 (stream, boundaries, macroscopics, forces, Guo
- * correction, equilibrium, BGK collision), one fused pull-scheme kernel per step.
+ * correction, equilibrium, BGK or MRT collision), one fused pull-scheme kernel per step.
  * Asynchronous; the last step also stores rho and the corrected u. */
 int lbm3d_step(lbm3d_ctx* ctx, int32_t nsteps);
 int lbm3d_sync(lbm3d_ctx* ctx);

@@ -131,6 +131,24 @@ void oracle3d_equilibrium(int prec, int literal, double rho, double ux, double u
     }
 }
 
+// MRT<3> basis M and its inverse (row-major 27x27, FP64).
+void oracle3d_mrt_basis(double* M, double* Minv) {
+    const auto& b = mrt_basis();
+    for (int i = 0; i < Q * Q; i++) {
+        M[i] = b.M[i];
+        Minv[i] = b.Minv[i];
+    }
+}
+
+// One node through MRT<3> (FP64): g post-stream populations, u corrected velocity, F force,
+// S[27] rates.
+void oracle3d_collide_mrt(const double* g, const double* u, const double* F, const float* S, int literal, double* out) {
+    double f[Q];
+    for (int q = 0; q < Q; q++) f[q] = g[q];
+    collide_mrt3<double>(f, u[0], u[1], u[2], F[0], F[1], F[2], S, literal != 0);
+    for (int q = 0; q < Q; q++) out[q] = f[q];
+}
+
 // Guo source of BGK<3> for all 27 directions (FP64).
 void oracle3d_guo_source(double ux, double uy, double uz, double fx, double fy, double fz, double omega, double* out) {
     for (int q = 0; q < Q; q++) out[q] = guo_source<double>(q, ux, uy, uz, fx, fy, fz, omega);

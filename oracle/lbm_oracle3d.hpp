@@ -1,4 +1,4 @@
-// lbm_oracle3d.hpp — CPU restatement of the reference's D3Q27 BGK path.
+// lbm_oracle3d.hpp — CPU restatement of the reference's D3Q27 BGK and MRT path.
 //
 // TEST INFRASTRUCTURE ONLY. This is the correctness oracle for the 3-D step. Only tests/,
 // the smoke check in __graft_entry__.py and the cpu_baseline leg of bench.py may load or call
@@ -6,17 +6,20 @@
 //
 // It follows the reference's unfused stage order (src/main.cu:96-114) on separate buffers, as
 // `LBM<3>` does (src/core/lbm.cuh:36-43): push stream -> swap -> boundaries -> uncorrected
-// macroscopics -> reset forces -> Guo velocity correction -> equilibrium -> BGK<3> collision,
-// templated on the arithmetic type (FP32 oracle, FP64 host run).
+// macroscopics -> reset forces -> Guo velocity correction -> equilibrium -> BGK<3> or MRT<3>
+// collision, templated on the arithmetic type (FP32 oracle, FP64 host run). MRT<3> is the dense
+// form of the reference: m = M f, m* per moment, f = M^-1 m*.
 //
 // Parity pinning (tests/test_oracle3d_pinning.py), each against something this file does not
 // restate: the equilibrium's moments up to second order (exact identities), the literal
 // equilibrium's first moment (17/27) rho u (closed form), the Guo source's moments,
 // periodic streaming against numpy.roll, BGK's conserved moments, the rest state as a fixed
-// point of a closed bounce-back box, mirror symmetry of the box, viscous decay of a small
+// point of a closed bounce-back box, viscous decay of a small
 // Taylor-Green vortex against exp(-6 nu k^2 t), and the curvature -F/nu of a force-driven
-// channel. Parity unpinned: the FP32 rounding sequence of the reference's De Rosis expression
-// (equilibrium.cu:70-96), which acts at FP32 rounding level.
+// channel; for MRT<3>: the basis against its inverse, the collision at unit rates against the
+// product-form equilibrium rho*prod_a phi(c_a, u_a), the relaxed second moments, the momentum
+// gain per step and the viscous decay rate. Parity unpinned: the FP32 rounding sequence of the
+// reference's De Rosis expression (equilibrium.cu:70-96), which acts at FP32 rounding level.
 #pragma once
 
 #include "../include/lbm3d_hip.h"
@@ -91,6 +94,166 @@ inline R guo_source(int q, R ux, R uy, R uz, R fx, R fy, R fz, R omega) {
                            k * ((cz - uz) / cs2 + cu * cz / (cs2 * cs2)) * fz);
 }
 
+// The De Rosis (2019) raw-moment basis of MRT<3> (lbm_constants.cuh:218-246): row k of M is the
+// polynomial P_k(c) evaluated on the velocity set, in the reference's row order.
+template <class R>
+inline R basis_poly(int k, int cx, int cy, int cz) {
+    const R x = R(cx), y = R(cy), z = R(cz), x2 = x * x, y2 = y * y, z2 = z * z;
+    switch (k) {
+        case 0: return R(1);
+        case 1: return x;
+        case 2: return y;
+        case 3: return z;
+        case 4: return x * y;
+        case 5: return x * z;
+        case 6: return y * z;
+        case 7: return x2 - y2;
+        case 8: return x2 - z2;
+        case 9: return x2 + y2 + z2;
+        case 10: return x * y2 + x * z2;
+        case 11: return x2 * y + y * z2;
+        case 12: return x2 * z + y2 * z;
+        case 13: return x * y2 - x * z2;
+        case 14: return x2 * y - y * z2;
+        case 15: return x2 * z - y2 * z;
+        case 16: return x * y * z;
+        case 17: return x2 * y2 + x2 * z2 + y2 * z2;
+        case 18: return x2 * y2 + x2 * z2 - y2 * z2;
+        case 19: return x2 * y2 - x2 * z2;
+        case 20: return x2 * y * z;
+        case 21: return x * y2 * z;
+        case 22: return x * y * z2;
+        case 23: return x * y2 * z2;
+        case 24: return x2 * y * z2;
+        case 25: return x2 * y2 * z;
+        default: return x2 * y2 * z2;
+    }
+}
+
+// M and M^-1 (Gauss-Jordan with partial pivoting in double), row-major 27x27.
+struct MrtBasis {
+    double M[Q * Q], Minv[Q * Q];
+    MrtBasis() {
+        double A[Q][2 * Q];
+        for (int k = 0; k < Q; k++)
+            for (int i = 0; i < Q; i++) {
+                M[k * Q + i] = basis_poly<double>(k, CX[i], CY[i], CZ[i]);
+                A[k][i] = M[k * Q + i];
+                A[k][Q + i] = k == i ? 1.0 : 0.0;
+            }
+        for (int c = 0; c < Q; c++) {
+            int piv = c;
+            for (int r = c + 1; r < Q; r++)
+                if (std::fabs(A[r][c]) > std::fabs(A[piv][c])) piv = r;
+            if (std::fabs(A[piv][c]) < 1e-12) throw std::runtime_error("MRT<3> basis is singular");
+            for (int j = 0; j < 2 * Q; j++) std::swap(A[c][j], A[piv][j]);
+            const double d = A[c][c];
+            for (int j = 0; j < 2 * Q; j++) A[c][j] /= d;
+            for (int r = 0; r < Q; r++) {
+                if (r == c) continue;
+                const double m = A[r][c];
+                if (m != 0.0)
+                    for (int j = 0; j < 2 * Q; j++) A[r][j] -= m * A[c][j];
+            }
+        }
+        for (int k = 0; k < Q; k++)
+            for (int i = 0; i < Q; i++) Minv[k * Q + i] = A[k][Q + i];
+    }
+};
+inline const MrtBasis& mrt_basis() {
+    static const MrtBasis b;
+    return b;
+}
+
+// MRT<3>::compute_forcing_term / compute_forcing_term_component (MRT.cu:79-153): the forcing
+// moments F_k in the basis order.
+template <class R>
+inline void mrt3_forcing(R* F, R ux, R uy, R uz, R fx, R fy, R fz) {
+    const R t = R(1) / R(3), n = R(1) / R(9);
+    F[0] = 0;
+    F[1] = fx;
+    F[2] = fy;
+    F[3] = fz;
+    F[4] = fx * uy + fy * ux;
+    F[5] = fx * uz + fz * ux;
+    F[6] = fy * uz + fz * uy;
+    F[7] = R(2) * fx * ux - R(2) * fy * uy;
+    F[8] = R(2) * fx * ux - R(2) * fz * uz;
+    F[9] = R(2) * fx * ux + R(2) * fy * uy + R(2) * fz * uz;
+    F[10] = R(2) * t * fx;
+    F[11] = R(2) * t * fy;
+    F[12] = R(2) * t * fz;
+    F[13] = F[14] = F[15] = F[16] = 0;
+    F[17] = R(4) * t * (fx * ux + fy * uy + fz * uz);
+    F[18] = R(4) * t * fx * ux;
+    F[19] = R(2) * t * (fy * uy - fz * uz);
+    F[20] = t * (fy * uz + fz * uy);
+    F[21] = t * (fx * uz + fz * ux);
+    F[22] = t * (fx * uy + fy * ux);
+    F[23] = n * fx;
+    F[24] = n * fy;
+    F[25] = n * fz;
+    F[26] = R(2) * n * (fx * ux + fy * uy + fz * uz);
+}
+
+// MRT<3>::apply (MRT.cu:397-456) with compute_m_post_component (MRT.cu:155-201): moments
+// 0-3 conserved (rho, rho u, plus F/2 unless literal), 4-8 relaxed with S[4..8] plus
+// (1 - S/2) F, 9-26 set to their equilibria plus F/2; f = M^-1 m*. literal keeps the
+// reference's (2 ux^2 + 1) in the equilibrium of moment 19 (MRT.cu:188).
+template <class R>
+inline void collide_mrt3(R* f, R ux, R uy, R uz, R fx, R fy, R fz, const float* S, bool literal) {
+    const auto& B = mrt_basis();
+    R rho = 0, m4 = 0, m5 = 0, m6 = 0, m7 = 0, m8 = 0;
+    for (int i = 0; i < Q; i++) {
+        const R fi = f[i], cx = R(CX[i]), cy = R(CY[i]), cz = R(CZ[i]);
+        rho += fi;
+        m4 += fi * cx * cy;
+        m5 += fi * cx * cz;
+        m6 += fi * cy * cz;
+        m7 += fi * (cx * cx - cy * cy);
+        m8 += fi * (cx * cx - cz * cz);
+    }
+    R F[Q];
+    mrt3_forcing<R>(F, ux, uy, uz, fx, fy, fz);
+    const R ux2 = ux * ux, uy2 = uy * uy, uz2 = uz * uz, cs2 = R(1) / R(3), h = R(0.5);
+    const R mom = literal ? R(0) : h;
+    auto relax = [&](int k, R m, R meq) {
+        const R s = R(S[k]);
+        return (R(1) - s) * m + s * meq + (R(1) - h * s) * F[k];
+    };
+    R mp[Q];
+    mp[0] = rho;
+    mp[1] = rho * ux + mom * F[1];
+    mp[2] = rho * uy + mom * F[2];
+    mp[3] = rho * uz + mom * F[3];
+    mp[4] = relax(4, m4, rho * ux * uy);
+    mp[5] = relax(5, m5, rho * ux * uz);
+    mp[6] = relax(6, m6, rho * uy * uz);
+    mp[7] = relax(7, m7, rho * (ux2 - uy2));
+    mp[8] = relax(8, m8, rho * (ux2 - uz2));
+    mp[9] = rho * (ux2 + uy2 + uz2 + R(1)) + h * F[9];
+    mp[10] = rho * cs2 * ux * (R(3) * uy2 + R(3) * uz2 + R(2)) + h * F[10];
+    mp[11] = rho * cs2 * uy * (R(3) * ux2 + R(3) * uz2 + R(2)) + h * F[11];
+    mp[12] = rho * cs2 * uz * (R(3) * ux2 + R(3) * uy2 + R(2)) + h * F[12];
+    mp[13] = rho * ux * (uy2 - uz2) + h * F[13];
+    mp[14] = rho * uy * (ux2 - uz2) + h * F[14];
+    mp[15] = rho * uz * (ux2 - uy2) + h * F[15];
+    mp[16] = rho * ux * uy * uz + h * F[16];
+    mp[17] = rho * cs2 * (R(3) * (ux2 * uy2 + ux2 * uz2 + uy2 * uz2) + R(2) * (ux2 + uy2 + uz2) + R(1)) + h * F[17];
+    mp[18] = rho * cs2 * cs2 * (R(9) * (ux2 * uy2 + ux2 * uz2 - uy2 * uz2) + R(6) * ux2 + R(1)) + h * F[18];
+    mp[19] = rho * cs2 * (uy2 - uz2) * ((literal ? R(2) : R(3)) * ux2 + R(1)) + h * F[19];
+    mp[20] = rho * cs2 * uy * uz * (R(3) * ux2 + R(1)) + h * F[20];
+    mp[21] = rho * cs2 * ux * uz * (R(3) * uy2 + R(1)) + h * F[21];
+    mp[22] = rho * cs2 * ux * uy * (R(3) * uz2 + R(1)) + h * F[22];
+    mp[23] = rho * cs2 * cs2 * ux * (R(3) * uy2 + R(1)) * (R(3) * uz2 + R(1)) + h * F[23];
+    mp[24] = rho * cs2 * cs2 * uy * (R(3) * ux2 + R(1)) * (R(3) * uz2 + R(1)) + h * F[24];
+    mp[25] = rho * cs2 * cs2 * uz * (R(3) * ux2 + R(1)) * (R(3) * uy2 + R(1)) + h * F[25];
+    mp[26] = rho * cs2 * cs2 * cs2 * (R(3) * ux2 + R(1)) * (R(3) * uy2 + R(1)) * (R(3) * uz2 + R(1)) + h * F[26];
+    for (int k = 0; k < Q; k++) f[k] = 0;
+    for (int i = 0; i < Q; i++)
+        for (int k = 0; k < Q; k++) f[k] += R(B.Minv[k * Q + i]) * mp[i];
+}
+
 template <class R>
 struct Solver3 {
     lbm3d_config cfg;
@@ -106,6 +269,8 @@ struct Solver3 {
     explicit Solver3(const lbm3d_config& c)
         : cfg(c), nx(c.nx), ny(c.ny), nz(c.nz), N(size_t(c.nx) * size_t(c.ny) * size_t(c.nz)) {
         if (nx < 3 || ny < 3 || nz < 3) throw std::runtime_error("grid must be at least 3x3x3");
+        if (c.collision != LBM_BGK && c.collision != LBM_MRT) throw std::runtime_error("3-D operator not restated");
+        mrt_basis();
         eq_literal = (c.quirks & LBM_QUIRK_EQ3D_LINEAR) != 0;
         f.assign(Q * N, R(0));
         f_back.assign(Q * N, R(0));
@@ -236,8 +401,21 @@ struct Solver3 {
             for (int c = 0; c < 3; c++) u[c * N + n] += R(0.5) * force[c * N + n] / rho[n];
     }
 
-    // equilibrium + BGK<3>::apply (BGK.cuh:55-96): f - omega (f - f_eq) + Guo source.
+    // equilibrium + BGK<3>::apply (BGK.cuh:55-96): f - omega (f - f_eq) + Guo source;
+    // or MRT<3>::apply.
     void collide() {
+        if (cfg.collision == LBM_MRT) {
+            const bool half = (cfg.quirks & LBM_QUIRK_MRT3_LITERAL) != 0;
+#pragma omp parallel for schedule(static)
+            for (long long ln = 0; ln < (long long)N; ln++) {
+                const size_t n = size_t(ln);
+                R g[Q];
+                for (int q = 0; q < Q; q++) g[q] = f[q * N + n];
+                collide_mrt3<R>(g, u[n], u[N + n], u[2 * N + n], force[n], force[N + n], force[2 * N + n], cfg.S, half);
+                for (int q = 0; q < Q; q++) f[q * N + n] = g[q];
+            }
+            return;
+        }
         const R om = R(cfg.omega);
 #pragma omp parallel for schedule(static)
         for (long long ln = 0; ln < (long long)N; ln++) {

@@ -79,6 +79,8 @@ def lib():
             "oracle3d_equilibrium": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                             ctypes.c_double, ctypes.c_double, _D]),
             "oracle3d_guo_source": (None, [ctypes.c_double] * 7 + [_D]),
+            "oracle3d_mrt_basis": (None, [_D, _D]),
+            "oracle3d_collide_mrt": (None, [_D, _D, _D, ctypes.POINTER(ctypes.c_float), ctypes.c_int, _D]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -285,6 +287,25 @@ def equilibrium3d(rho, ux, uy, uz, prec=64, literal=False):
 def guo_source3d(u, F, omega):
     out = np.zeros(27)
     lib().oracle3d_guo_source(*[float(v) for v in u], *[float(v) for v in F], float(omega), _dp(out))
+    return out
+
+
+def mrt3_basis():
+    M = np.zeros(27 * 27)
+    Mi = np.zeros(27 * 27)
+    lib().oracle3d_mrt_basis(_dp(M), _dp(Mi))
+    return M.reshape(27, 27), Mi.reshape(27, 27)
+
+
+def collide_mrt3(g, u, F, S, literal=False):
+    """One node through MRT<3> (FP64): g post-stream populations, u corrected velocity;
+    literal = the reference's momentum and moment-19 forms (LBM_QUIRK_MRT3_LITERAL)."""
+    gg = np.ascontiguousarray(g, np.float64)
+    uu = np.ascontiguousarray(u, np.float64)
+    FF = np.ascontiguousarray(F, np.float64)
+    SS = (ctypes.c_float * 27)(*[float(v) for v in S])
+    out = np.zeros(27)
+    lib().oracle3d_collide_mrt(_dp(gg), _dp(uu), _dp(FF), SS, int(literal), _dp(out))
     return out
 
 

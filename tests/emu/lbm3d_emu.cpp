@@ -17,7 +17,7 @@ using namespace lbm3;
 
 namespace {
 
-template <int STORE>
+template <int COLL, int STORE>
 int run3(const lbm3d_config& c, const float* f_in, const uint8_t* flags, const float* force_aos, int nsteps,
          float* f_out, float* rho_out, float* u_aos_out) {
     using St = Store3<STORE>;
@@ -49,9 +49,12 @@ int run3(const lbm3d_config& c, const float* f_in, const uint8_t* flags, const f
     p.fy = c.force[1];
     p.fz = c.force[2];
     p.lin_ec = (c.quirks & LBM_QUIRK_EQ3D_LINEAR) ? 1.0f : 3.0f;
+    for (int k = 0; k < 5; k++) p.mrt.s[k] = c.S[4 + k];
+    p.mrt.literal = (c.quirks & LBM_QUIRK_MRT3_LITERAL) ? 1 : 0;
     p.stale_rest = (c.quirks & LBM_QUIRK_STALE_REST) ? 1 : 0;
-    p.nbx = (c.nx + 63) / 64;
-    p.nby = (c.ny + kRows - 1) / kRows;
+    p.wx = wave_cols(c.nx);
+    p.nbx = (c.nx + 64 * p.wx - 1) / (64 * p.wx);
+    p.nby = (c.ny + 4 / p.wx - 1) / (4 / p.wx);
     T* cur = a.data();
     T* nxt = b.data();
     for (int s = 0; s < nsteps; s++) {
@@ -60,15 +63,23 @@ int run3(const lbm3d_config& c, const float* f_in, const uint8_t* flags, const f
         const bool last = s == nsteps - 1;
         p.out_rho = last ? rho.data() : nullptr;
         p.out_u = last ? u.data() : nullptr;
-        // the kernel's grid: workgroup (bx, by, z) of 64 x kRows threads, idle lanes return
-        for (int z = 0; z < c.nz; z++)
-            for (int by = 0; by < p.nby; by++)
-                for (int bx = 0; bx < p.nbx; bx++)
-                    for (int t = 0; t < 64 * kRows; t++) {
-                        const int x = bx * 64 + (t & 63), y = by * kRows + (t >> 6);
-                        if (x >= p.nx || y >= p.ny) continue;
-                        step3_node<STORE, false>(p, x, y, z);
-                    }
+        // the kernel's grid: every workgroup (in the kernel's XCD order) of 256 threads, idle
+        // lanes return; each node must be visited exactly once
+        const long long nb = (long long)p.nbx * p.nby * p.nz, per = nb / 8;
+        std::vector<uint8_t> seen(size_t(N), 0);
+        for (long long b = 0; b < nb; b++) {
+            const long long L = b < per * 8 ? (b % 8) * per + b / 8 : b;
+            for (int t = 0; t < 256; t++) {
+                int x, y, z;
+                thread_node(p, L, t, x, y, z);
+                if (x >= p.nx || y >= p.ny) continue;
+                if (z < 0 || z >= p.nz || x < 0 || y < 0) std::abort();
+                if (seen[size_t((z * p.ny + y) * p.nx + x)]++) std::abort();
+                step3_node<COLL, STORE, false>(p, x, y, z);
+            }
+        }
+        for (long long n = 0; n < N; n++)
+            if (seen[size_t(n)] != 1) std::abort();
         T* t = cur;
         cur = nxt;
         nxt = t;
@@ -91,10 +102,17 @@ extern "C" {
 // f_in / f_out: FP32 SoA [27][N]; both buffers start from f_in (as lbm3d_set_pops does).
 int emu3d_run(const lbm3d_config* c, const float* f_in, const uint8_t* flags, const float* force_aos, int nsteps,
               float* f_out, float* rho_out, float* u_aos_out) {
+    if (c->collision == LBM_MRT) {
+        switch (c->storage) {
+            case LBM_F32: return run3<LBM_MRT, LBM_F32>(*c, f_in, flags, force_aos, nsteps, f_out, rho_out, u_aos_out);
+            case LBM_F16_SHIFTED: return run3<LBM_MRT, LBM_F16_SHIFTED>(*c, f_in, flags, force_aos, nsteps, f_out, rho_out, u_aos_out);
+            default: return run3<LBM_MRT, LBM_BF16_SHIFTED>(*c, f_in, flags, force_aos, nsteps, f_out, rho_out, u_aos_out);
+        }
+    }
     switch (c->storage) {
-        case LBM_F32: return run3<LBM_F32>(*c, f_in, flags, force_aos, nsteps, f_out, rho_out, u_aos_out);
-        case LBM_F16_SHIFTED: return run3<LBM_F16_SHIFTED>(*c, f_in, flags, force_aos, nsteps, f_out, rho_out, u_aos_out);
-        default: return run3<LBM_BF16_SHIFTED>(*c, f_in, flags, force_aos, nsteps, f_out, rho_out, u_aos_out);
+        case LBM_F32: return run3<LBM_BGK, LBM_F32>(*c, f_in, flags, force_aos, nsteps, f_out, rho_out, u_aos_out);
+        case LBM_F16_SHIFTED: return run3<LBM_BGK, LBM_F16_SHIFTED>(*c, f_in, flags, force_aos, nsteps, f_out, rho_out, u_aos_out);
+        default: return run3<LBM_BGK, LBM_BF16_SHIFTED>(*c, f_in, flags, force_aos, nsteps, f_out, rho_out, u_aos_out);
     }
 }
 
@@ -105,26 +123,27 @@ int emu3d_run(const lbm3d_config* c, const float* f_in, const uint8_t* flags, co
 // shapes below, at and above one wavefront row, every periodicity pattern, bounce-back on
 // all faces, a force field, every storage and both quirks.
 int main() {
-    const int shapes[][3] = {{3, 3, 3}, {5, 4, 3}, {64, 3, 4}, {65, 5, 3}, {130, 6, 4}, {7, 9, 11}};
+    const int shapes[][3] = {{3, 3, 3}, {5, 4, 3}, {64, 3, 4}, {65, 5, 3}, {130, 6, 4}, {7, 9, 11}, {193, 3, 5}, {260, 5, 3}};
     int cases = 0;
     for (const auto& s : shapes)
         for (int per = 0; per < 8; per++)
             for (int st = 0; st <= 2; st++)
-                for (int var = 0; var < 3; var++) {
+                for (int var = 0; var < 4; var++) {
                     lbm3d_config c;
                     std::memset(&c, 0, sizeof c);
                     c.nx = s[0];
                     c.ny = s[1];
                     c.nz = s[2];
-                    c.collision = LBM_BGK;
+                    c.collision = var == 3 ? LBM_MRT : LBM_BGK;
                     c.omega = 1.3f;
+                    for (int k = 0; k < 27; k++) c.S[k] = k >= 4 && k <= 8 ? 1.3f : 1.0f;
                     c.rho0 = 1.0f;
                     c.storage = st;
                     c.periodic_x = per & 1;
                     c.periodic_y = (per >> 1) & 1;
                     c.periodic_z = (per >> 2) & 1;
                     c.force[0] = 1e-5f;
-                    c.quirks = var == 2 ? (LBM_QUIRK_STALE_REST | LBM_QUIRK_EQ3D_LINEAR) : 0;
+                    c.quirks = var == 2 ? (LBM_QUIRK_STALE_REST | LBM_QUIRK_EQ3D_LINEAR) : (var == 3 ? LBM_QUIRK_MRT3_LITERAL : 0);
                     const long long N = (long long)c.nx * c.ny * c.nz;
                     std::vector<float> f(static_cast<size_t>(Q * N)), out(static_cast<size_t>(Q * N));
                     std::vector<float> rho(static_cast<size_t>(N)), u(static_cast<size_t>(3 * N));
@@ -137,7 +156,7 @@ int main() {
                                 if (x == 0 || y == 0 || z == 0 || x == c.nx - 1 || y == c.ny - 1 || z == c.nz - 1)
                                     flags[size_t((z * c.ny + y) * c.nx + x)] = LBM_BOUNCE_BACK;
                     std::vector<float> force(size_t(3 * N), 2e-6f);
-                    emu3d_run(&c, f.data(), var >= 1 ? flags.data() : nullptr, var == 1 ? force.data() : nullptr, 3,
+                    emu3d_run(&c, f.data(), var >= 1 ? flags.data() : nullptr, var == 1 || var == 3 ? force.data() : nullptr, 3,
                               out.data(), rho.data(), u.data());
                     cases++;
                 }

@@ -48,8 +48,8 @@ def test_config_struct_layout_matches_header():
     # 5 int32 + omega + S[9] + rho0 + storage + 2 periodic + force[2] + wall_u[2] + u_lid + quirks
     # + steps_per_launch + reserved[7]
     assert ctypes.sizeof(_abi.LbmConfig) == 4 * (5 + 1 + 9 + 1 + 1 + 2 + 2 + 2 + 1 + 1 + 8)
-    # nx, ny, nz, collision, omega, rho0, storage, 3 periodic, force[3], quirks, reserved[8]
-    assert ctypes.sizeof(_abi.Lbm3dConfig) == 4 * (4 + 1 + 1 + 1 + 3 + 3 + 1 + 8)
+    # nx, ny, nz, collision, omega, S[27], rho0, storage, 3 periodic, force[3], quirks, reserved[8]
+    assert ctypes.sizeof(_abi.Lbm3dConfig) == 4 * (4 + 1 + 27 + 1 + 1 + 3 + 3 + 1 + 8)
 
 
 def test_version_and_defaults_without_gpu():
@@ -71,3 +71,4 @@ def test_3d_defaults_without_gpu():
     assert lib.lbm3d_config_default(ctypes.byref(c), 16, 8, 4) == 0
     assert (c.nx, c.ny, c.nz) == (16, 8, 4) and c.omega == 1.0 and c.rho0 == 1.0
     assert c.collision == _abi.LBM_BGK and c.storage == _abi.LBM_F32 and c.quirks == 0
+    assert list(c.S) == [1.0] * 27

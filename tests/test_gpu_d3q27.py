@@ -7,15 +7,34 @@ import oracle as O
 from lbm_amd import _abi as A
 from lbm_amd.solver import LbmError
 from lbm_amd.solver3d import Config3D, Solver3D
-from test_emu3d_parity import CASES, box_flags, initial_state, make_cfg
+from test_emu3d_parity import CASES, MRT_CASES, box_flags, initial_state, make_cfg, mrt_case, oracle_run
 
 pytestmark = pytest.mark.gpu
 W = O.tables3d()[0]
 
 
 def gpu_config(c):
-    return Config3D(c.nx, c.ny, c.nz, omega=c.omega, storage=c.storage, periodic_x=bool(c.periodic_x),
-                    periodic_y=bool(c.periodic_y), periodic_z=bool(c.periodic_z), force=tuple(c.force), quirks=c.quirks)
+    return Config3D(c.nx, c.ny, c.nz, collision=c.collision, omega=c.omega, S=tuple(c.S), storage=c.storage,
+                    periodic_x=bool(c.periodic_x), periodic_y=bool(c.periodic_y), periodic_z=bool(c.periodic_z),
+                    force=tuple(c.force), quirks=c.quirks)
+
+
+@pytest.mark.parametrize("case", range(len(MRT_CASES)))
+def test_gpu_mrt3_matches_dense_oracle(gpu_available, case):
+    c, f0, flags, force = mrt_case(case)
+    f0 = f0.astype(np.float32)
+    o = oracle_run(c, f0, flags, force, 8)
+    with Solver3D(gpu_config(c)) as s:
+        s.set_pops(f0)
+        s.set_flags(flags)
+        s.set_force_field(None if force is None else force.astype(np.float32))
+        s.step(8)
+        g = s.get_pops()
+        rho, u = s.get_macros()
+    assert np.abs(g - o.get_pops()).max() < 1e-5
+    r_rho, r_u = o.get_macros()
+    assert np.abs(rho - r_rho).max() < 1e-5
+    assert np.abs(u - r_u).max() < 1e-6
 
 
 @pytest.mark.parametrize("case", range(len(CASES)))
@@ -139,6 +158,30 @@ def test_gpu_taylor_green_3d_kinetic_energy(gpu_available):
     assert abs(rate / (6 * nu * k * k) - 1) < 0.01
 
 
+def test_gpu_mrt3_taylor_green_decay_and_f16_storage(gpu_available):
+    """MRT<3> with S[4..8] = 1.5 on 64^3: the kinetic energy decays at exp(-6 nu k^2 t),
+    nu = (1/1.5 - 1/2)/3, in FP32 and in FP16 storage."""
+    n, s48 = 64, 1.5
+    nu = (1 / s48 - 0.5) / 3
+    k = 2 * np.pi / n
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    U = 0.01
+    u = np.stack([U * np.sin(k * x) * np.cos(k * y) * np.cos(k * z), -U * np.cos(k * x) * np.sin(k * y) * np.cos(k * z),
+                  np.zeros_like(x, dtype=float)], -1).astype(np.float32)
+    S = [1.0] * 27
+    S[4:9] = [s48] * 5
+    for storage in (A.LBM_F32, A.LBM_F16_SHIFTED):
+        cfg = Config3D(n, n, n, collision=A.LBM_MRT, S=tuple(S), storage=storage, periodic_x=True, periodic_y=True,
+                       periodic_z=True)
+        with Solver3D(cfg) as s:
+            s.init_from_macros(np.ones((n, n, n), np.float32), u)
+            s.step(40)
+            e0 = s.kinetic_energy(None)
+            s.step(200)
+            e1 = s.kinetic_energy(None)
+        assert abs(-np.log(e1 / e0) / 200 / (6 * nu * k * k) - 1) < 0.01
+
+
 def test_gpu_d3q27_reductions_and_conservation(gpu_available):
     shape = (67, 10, 9)
     f0 = initial_state(shape, 11).astype(np.float32)
@@ -156,7 +199,7 @@ def test_gpu_d3q27_reductions_and_conservation(gpu_available):
 
 def test_gpu_d3q27_errors_are_reported(gpu_available):
     with pytest.raises(LbmError) as e:
-        Solver3D(Config3D(8, 8, 8, collision=A.LBM_MRT))
+        Solver3D(Config3D(8, 8, 8, collision=A.LBM_CM))
     assert e.value.code == A.LBM_E_UNSUPPORTED
     with pytest.raises(LbmError) as e:
         Solver3D(Config3D(2, 8, 8))

@@ -233,3 +233,122 @@ def test_reduced_storage_rounds_the_shifted_deviation():
         assert np.abs(g - f).max() > 0
         o.set_pops(g)  # idempotent
         assert np.array_equal(o.get_pops(), g)
+
+
+# ---- MRT<3> ---------------------------------------------------------------------------------
+
+def product_equilibrium(rho, u):
+    """rho * prod_a phi(c_a, u_a), phi(0) = 2/3 - u^2, phi(+-1) = (1/3 + u^2 +- u)/2: the
+    distribution whose 1-D moments are 1, u, 1/3 + u^2 along every axis."""
+    def phi(c, ua):
+        return np.where(c == 0, 2 / 3 - ua * ua, (1 / 3 + ua * ua + c * ua) / 2)
+    return rho * phi(CX, u[0]) * phi(CY, u[1]) * phi(CZ, u[2])
+
+
+def mrt_cfg_S(s4to8):
+    S = [1.0] * 27
+    S[4:9] = s4to8
+    return S
+
+
+def test_mrt3_basis_is_invertible_and_integer():
+    M, Mi = O.mrt3_basis()
+    assert np.allclose(M @ Mi, np.eye(27), atol=1e-13) and np.allclose(Mi @ M, np.eye(27), atol=1e-13)
+    assert np.array_equal(M, np.round(M)) and np.all(M[0] == 1)
+    assert np.array_equal(M[1:4], C.T)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_mrt3_at_unit_rates_is_the_product_equilibrium(seed):
+    rng = np.random.default_rng(20 + seed)
+    g = W * (1 + 0.1 * rng.standard_normal(27))
+    u = 0.08 * rng.standard_normal(3)
+    out = O.collide_mrt3(g, u, (0, 0, 0), mrt_cfg_S([1.0] * 5))
+    assert np.allclose(out, product_equilibrium(g.sum(), u), atol=1e-15)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_mrt3_moments_after_collision(seed):
+    rng = np.random.default_rng(30 + seed)
+    g = W * (1 + 0.1 * rng.standard_normal(27))
+    rho = g.sum()
+    F = 1e-3 * rng.standard_normal(3)
+    u = (g @ C + F / 2) / rho  # the corrected velocity the step passes in
+    s = rng.uniform(0.3, 1.9, 5).astype(np.float32).astype(np.float64)  # rates cross as float
+    out = O.collide_mrt3(g, u, F, mrt_cfg_S(s))
+    x, y, z = C.T
+    assert abs(out.sum() - rho) < 1e-15
+    assert np.allclose(out @ C, g @ C + F, atol=1e-15)  # Guo: a step adds F
+    pairs = [(x * y, u[0] * u[1], F[0] * u[1] + F[1] * u[0]), (x * z, u[0] * u[2], F[0] * u[2] + F[2] * u[0]),
+             (y * z, u[1] * u[2], F[1] * u[2] + F[2] * u[1]),
+             (x * x - y * y, u[0] ** 2 - u[1] ** 2, 2 * (F[0] * u[0] - F[1] * u[1])),
+             (x * x - z * z, u[0] ** 2 - u[2] ** 2, 2 * (F[0] * u[0] - F[2] * u[2]))]
+    for k, (poly, ueq, Fk) in enumerate(pairs):
+        expect = (1 - s[k]) * (g @ poly) + s[k] * rho * ueq + (1 - s[k] / 2) * Fk
+        assert abs(out @ poly - expect) < 1e-15
+    # unforced: every moment above second order at its product-equilibrium value
+    out0 = O.collide_mrt3(g, u, (0, 0, 0), mrt_cfg_S(s))
+    prod = product_equilibrium(rho, u)
+    for poly in (x * y * z, x * y * y, x * x * y * y, x * x * y * y * z * z, x * y * z * z, x * x + y * y + z * z):
+        assert abs(out0 @ poly - prod @ poly) < 1e-15
+
+
+def test_mrt3_literal_moment_19_differs_by_rho_cs2_ux2_duyz():
+    """The reference's moment-19 equilibrium (2 ux^2 + 1) vs the moment of the equilibrium
+    (3 ux^2 + 1): the literal collision at unit rates differs from the product equilibrium
+    only in moment 19, by -rho/3 ux^2 (uy^2 - uz^2)."""
+    rng = np.random.default_rng(40)
+    g = W * (1 + 0.1 * rng.standard_normal(27))
+    u = 0.08 * rng.standard_normal(3)
+    M, _ = O.mrt3_basis()
+    lit = O.collide_mrt3(g, u, (0, 0, 0), mrt_cfg_S([1.0] * 5), literal=True)
+    d = M @ (lit - product_equilibrium(g.sum(), u))
+    expect = np.zeros(27)
+    expect[19] = -g.sum() / 3 * u[0] ** 2 * (u[1] ** 2 - u[2] ** 2)
+    assert np.allclose(d, expect, atol=1e-15)
+
+
+def test_mrt3_literal_quirk_adds_half_the_momentum():
+    nx, ny, nz = 5, 4, 6
+    F = np.array([2.0 ** -16, -(2.0 ** -17), 2.0 ** -15])
+    for quirk, share in ((0, 1.0), (_abi.LBM_QUIRK_MRT3_LITERAL, 0.5)):
+        c = cfg3(nx, ny, nz, force=F, quirks=quirk)
+        c.collision = _abi.LBM_MRT
+        for k in range(27):
+            c.S[k] = 1.0
+        for k in range(4, 9):
+            c.S[k] = 1.3
+        o = O.Oracle3D(c, 64)
+        o.init_from_macros(np.ones((nz, ny, nx)), np.zeros((nz, ny, nx, 3)))
+        o.step(6)
+        p = np.einsum("qzyx,qa->a", o.get_pops(), C)
+        assert np.allclose(p, share * 6 * F * nx * ny * nz, rtol=1e-11, atol=0)
+
+
+def test_mrt3_taylor_green_decays_at_the_viscous_rate():
+    n = 24
+    s = 1.6
+    nu = (1 / s - 0.5) / 3
+    k = 2 * np.pi / n
+    c = cfg3(n, n, n)
+    c.collision = _abi.LBM_MRT
+    for i in range(27):
+        c.S[i] = 1.0
+    for i in range(4, 9):
+        c.S[i] = s
+    o = O.Oracle3D(c, 64)
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    U = 1e-4
+    u = np.stack([U * np.sin(k * x) * np.cos(k * y) * np.cos(k * z),
+                  -U * np.cos(k * x) * np.sin(k * y) * np.cos(k * z), np.zeros_like(x, dtype=float)], -1)
+    o.init_from_macros(np.ones((n, n, n)), u)
+
+    def energy():
+        rho, uu = o.get_macros()
+        return 0.5 * (rho * (uu ** 2).sum(-1)).sum()
+
+    o.step(20)
+    e0 = energy()
+    o.step(60)
+    e1 = energy()
+    assert abs(-np.log(e1 / e0) / 60 / (6 * nu * k * k) - 1) < 0.02
