@@ -253,13 +253,15 @@ def run_d3q27(torch, steps=200, warmup=20):
     S[4:9] = [1.8] * 5
     res["hbm_256_mrt_f32"] = one(Config3D(256, 256, 256, collision=A.LBM_MRT, S=tuple(S), **per), tg_init(256),
                                  n_steps=max(20, steps // 2))
+    res["hbm_256_cm_f32"] = one(Config3D(256, 256, 256, collision=A.LBM_CM, omega=1.8, **per), tg_init(256),
+                                n_steps=max(20, steps // 2))
     nx, ny, nz = 256, 128, 96
     fl = np.zeros((nz, ny, nx), np.uint8)
     fl[:, 0, :] = fl[:, -1, :] = A.BOUNCE_BACK
     fl[0], fl[-1] = A.BOUNCE_BACK, A.BOUNCE_BACK
     ch = Config3D(nx, ny, nz, omega=1.6, periodic_x=True, force=(1e-6, 0.0, 0.0))
     res["channel_256x128x96_f32"] = one(ch, (np.ones((nz, ny, nx), np.float32), np.zeros((nz, ny, nx, 3), np.float32)), fl)
-    res["workload"] = ("D3Q27 BGK with Guo forcing (and MRT<3> at 256^3), FP32 collision; Taylor-Green 3-D "
+    res["workload"] = ("D3Q27 BGK with Guo forcing (and MRT<3>, CM<3> at 256^3), FP32 collision; Taylor-Green 3-D "
                        "(u_max 0.05) periodic, and a force-driven channel with bounce-back walls on y and z; 216 B "
                        "(FP32) / 108 B (FP16) per node update")
     return res

@@ -1,10 +1,10 @@
 // lbm3d.hip — D3Q27 kernels and the C ABI of include/lbm3d_hip.h.
 //
-// Replaces LBM<3> (src/core/lbm.cuh:33-382) for BGK<3>: its per-stage kernels and f_eq buffer
-// and MRT<3> become one fused pull-scheme kernel per step (lbm3d_step.hpp), launched as
+// Replaces LBM<3> (src/core/lbm.cuh:33-382) for BGK<3>, MRT<3> and CM<3>: its per-stage kernels
+// and f_eq buffer become one fused pull-scheme kernel per step (lbm3d_step.hpp), launched as
 // 256-thread workgroups holding whole x-rows up to 256 nodes. The step is HBM-bound: 27
 // population reads and 27 writes per node update (216 B in FP32, 108 B in 16-bit storage)
-// against ~330 VALU (BGK) or ~350 (MRT, no 27x27 matrices).
+// against ~330-370 VALU (BGK, MRT and CM without 27x27 matrices).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -261,6 +261,8 @@ int lbm3d_ctx::one_step(bool write_macros) {
     }
     if (cfg.collision == LBM_MRT) {
         LBM3_LAUNCH_S(LBM_MRT)
+    } else if (cfg.collision == LBM_CM) {
+        LBM3_LAUNCH_S(LBM_CM)
     } else {
         LBM3_LAUNCH_S(LBM_BGK)
     }
@@ -302,9 +304,11 @@ int lbm3d_create(const lbm3d_config* cfg, lbm3d_ctx** out) {
     };
     if (k.nx < 3 || k.ny < 3 || k.nz < 3) return bad(LBM_E_INVALID, "grid must be at least 3x3x3");
     if ((long long)k.nx * k.ny * k.nz > (1LL << 31) - 64) return bad(LBM_E_INVALID, "grid exceeds 2^31 nodes");
-    if (k.collision == LBM_CM) return bad(LBM_E_UNSUPPORTED, "CM<3> is not built for D3Q27 (BGK<3> and MRT<3> are)");
-    if (k.collision != LBM_BGK && k.collision != LBM_MRT) return bad(LBM_E_INVALID, "unknown collision operator");
-    if (k.collision == LBM_BGK && !(k.omega > 0.0f && k.omega < 2.0f)) return bad(LBM_E_INVALID, "BGK needs 0 < omega < 2 (tau > 0.5)");
+    if (k.collision != LBM_BGK && k.collision != LBM_MRT && k.collision != LBM_CM) return bad(LBM_E_INVALID, "unknown collision operator");
+    if (k.adapter == LBM_OPTIMAL_ADAPTER || k.adapter == LBM3D_EMPIRICAL_ADAPTER)
+        return bad(LBM_E_UNSUPPORTED, "the D3Q27 adapters (ACM, E-ACM) are not built; CM<3> runs with NoAdapter");
+    if (k.adapter != LBM_NO_ADAPTER) return bad(LBM_E_INVALID, "unknown adapter");
+    if (k.collision != LBM_MRT && !(k.omega > 0.0f && k.omega < 2.0f)) return bad(LBM_E_INVALID, "BGK and CM need 0 < omega < 2 (tau > 0.5)");
     for (int i = 0; i < 27; i++)
         if (!std::isfinite(k.S[i])) return bad(LBM_E_INVALID, "relaxation rates must be finite");
     if (k.storage < LBM_F32 || k.storage > LBM_BF16_SHIFTED) return bad(LBM_E_INVALID, "unknown storage");

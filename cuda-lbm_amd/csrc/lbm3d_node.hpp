@@ -257,6 +257,76 @@ LBM3_HD void collide_mrt3(float* g, const Macro3& m, float fx, float fy, float f
     }
 }
 
+// CM<3, NoAdapter>::apply (CM_D3Q27.cuh:1-105; De Rosis 2019 universal central moments), in
+// factorised form. Central moments K_abc = sum f (c-u)_x^a (c-u)_y^b (c-u)_z^c about the
+// corrected u. After collision, with e = (1, 0, 1/3) per axis:
+//   K*_abc = rho e_a e_b e_c + (F_x/2)[a=1] e_b e_c + (F_y/2) e_a [b=1] e_c + (F_z/2) e_a e_b [c=1],
+// except the deviatoric second moments K110, K101, K011, K200-K020, K200-K002, which become
+// (1 - omega) times their pre-collision values (trace K200 + K020 + K002 = rho). Those five come
+// from the raw second moments shifted by u (j = rho u - F/2). f* is then rebuilt one axis at a
+// time from central moments (M0, M1, M2) about u_a:
+//   h(0) = M0 (1 - u^2) - 2u M1 - M2,   h(+-1) = (M0 (u^2 +- u) + M1 (2u +- 1) + M2) / 2.
+LBM3_HD void cm_shift_inv(float& m0, float& m1, float& m2, float u) {
+    const float u2 = u * u;
+    const float hm = 0.5f * (m0 * (u2 - u) + m1 * (2.0f * u - 1.0f) + m2);
+    const float h0 = m0 * (1.0f - u2) - 2.0f * u * m1 - m2;
+    const float hp = 0.5f * (m0 * (u2 + u) + m1 * (2.0f * u + 1.0f) + m2);
+    m0 = hm;
+    m1 = h0;
+    m2 = hp;
+}
+LBM3_HD void collide_cm3(float* g, const Macro3& m, float fx, float fy, float fz, float omega) {
+    const float rho = m.rho, ux = m.ux, uy = m.uy, uz = m.uz;
+    float m4 = 0.0f, m5 = 0.0f, m6 = 0.0f, m7 = 0.0f, m8 = 0.0f;
+    m2_all(g, m4, m5, m6, m7, m8, std::make_integer_sequence<int, Q>{});
+    const float jx = rho * ux - 0.5f * fx, jy = rho * uy - 0.5f * fy, jz = rho * uz - 0.5f * fz;
+    const float w = 1.0f - omega;
+    const float k4 = m4 - ux * jy - uy * jx + rho * ux * uy;
+    const float k5 = m5 - ux * jz - uz * jx + rho * ux * uz;
+    const float k6 = m6 - uy * jz - uz * jy + rho * uy * uz;
+    const float k7 = m7 - 2.0f * (ux * jx - uy * jy) + rho * (ux * ux - uy * uy);
+    const float k8 = m8 - 2.0f * (ux * jx - uz * jz) + rho * (ux * ux - uz * uz);
+    const float e[3] = {1.0f, 0.0f, 1.0f / 3.0f};
+    const float hf[3] = {0.5f * fx, 0.5f * fy, 0.5f * fz};
+    float K[3][3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                float v = rho * e[a] * e[b] * e[c];
+                if (a == 1) v += hf[0] * e[b] * e[c];
+                if (b == 1) v += hf[1] * e[a] * e[c];
+                if (c == 1) v += hf[2] * e[a] * e[b];
+                K[a][b][c] = v;
+            }
+    K[1][1][0] = w * k4;
+    K[1][0][1] = w * k5;
+    K[0][1][1] = w * k6;
+    K[2][0][0] = (1.0f / 3.0f) * (rho + w * (k7 + k8));
+    K[0][2][0] = K[2][0][0] - w * k7;
+    K[0][0][2] = K[2][0][0] - w * k8;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) cm_shift_inv(K[a][b][0], K[a][b][1], K[a][b][2], uz);
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) cm_shift_inv(K[a][0][c], K[a][1][c], K[a][2][c], uy);
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) cm_shift_inv(K[0][b][c], K[1][b][c], K[2][b][c], ux);
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) g[q_of(a - 1, b - 1, c - 1)] = K[a][b][c];
+}
+
 // BounceBack<3>::apply (bbDomainBoundary.cuh:52-90): for i = 1..26 in order, a direction whose
 // neighbour lies outside the domain on a non-periodic axis copies g_i into g_opp(i).
 // Sequential as written, so edge and corner nodes match the reference's loop order.

@@ -135,7 +135,7 @@ LBM3_HD void pull_all(float* g, const Step3Params& p, const lbm::PopBuf<typename
     (pull_dir<STORE, BUF, FAST, qs>(g, p, src, node, ox, vx, oy, vy, oz, vz), ...);
 }
 
-// One node of one time step: pull, boundary functor, macroscopics, Guo-forced BGK or MRT,
+// One node of one time step: pull, boundary functor, macroscopics, Guo-forced BGK, MRT or CM,
 // store. Requires 0 <= x < nx, 0 <= y < ny, 0 <= z < nz and nx*ny*nz < 2^31.
 template <int COLL, int STORE, bool BUF>
 LBM3_HD void step3_node(const Step3Params& p, int x, int y, int z) {
@@ -174,6 +174,7 @@ LBM3_HD void step3_node(const Step3Params& p, int x, int y, int z) {
         p.out_u[2 * p.plane + node] = m.uz;
     }
     if (COLL == LBM_MRT) collide_mrt3(g, m, fx, fy, fz, p.mrt);
+    else if (COLL == LBM_CM) collide_cm3(g, m, fx, fy, fz, p.omega);
     else collide_bgk3(g, m, fx, fy, fz, p.omega, p.lin_ec);
 #pragma unroll
     for (int q = 0; q < Q; q++) dst.st(node, (long long)q * p.plane, St::enc(g[q], q, p.rho0));

@@ -97,6 +97,7 @@ class Lbm3dConfig(ctypes.Structure):
         ("ny", ctypes.c_int32),
         ("nz", ctypes.c_int32),
         ("collision", ctypes.c_int32),
+        ("adapter", ctypes.c_int32),
         ("omega", ctypes.c_float),
         ("S", ctypes.c_float * 27),
         ("rho0", ctypes.c_float),
@@ -109,6 +110,8 @@ class Lbm3dConfig(ctypes.Structure):
         ("reserved", ctypes.c_int32 * 8),
     ]
 
+
+LBM3D_EMPIRICAL_ADAPTER = 2  # CM<3> adapter ids (include/lbm3d_hip.h); only LBM_NO_ADAPTER is built
 
 # 3-D reductions (include/lbm3d_hip.h)
 LBM3D_REDUCE_MASS = 0

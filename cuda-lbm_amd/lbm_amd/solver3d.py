@@ -1,7 +1,7 @@
 """Python handle on one D3Q27 context of liblbm_hip.so (include/lbm3d_hip.h).
 
 Mirrors the host-side surface of the reference's `LBM<3>` (src/core/lbm.cuh:33-382) for
-BGK<3>: allocate/init (`Solver3D(...)`, `init_from_macros`), the time loop (`step`),
+BGK<3>, MRT<3> and CM<3>: allocate/init (`Solver3D(...)`, `init_from_macros`), the time loop (`step`),
 `update_macroscopics` (`get_macros`) and the kinetic-energy validator of the 3-D Taylor-Green
 scenario (`kinetic_energy`). Arrays are indexed [z, y, x]; host arrays are numpy, device arrays
 torch CUDA tensors (passed by pointer, stream-ordered on the context's stream).
@@ -24,7 +24,8 @@ class Config3D:
     nx: int
     ny: int
     nz: int
-    collision: int = A.LBM_BGK  # or A.LBM_MRT (rates S[4..8])
+    collision: int = A.LBM_BGK  # or A.LBM_MRT (rates S[4..8]) or A.LBM_CM (omega)
+    adapter: int = A.LBM_NO_ADAPTER
     omega: float = 1.0
     S: tuple = (1.0,) * 27
     rho0: float = 1.0
@@ -39,6 +40,7 @@ class Config3D:
         c = A.Lbm3dConfig()
         c.nx, c.ny, c.nz = self.nx, self.ny, self.nz
         c.collision = self.collision
+        c.adapter = self.adapter
         c.omega = self.omega
         for k in range(27):
             c.S[k] = self.S[k]

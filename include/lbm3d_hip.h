@@ -12,10 +12,11 @@
  *
  * Built: BGK<3> with Guo forcing (src/core/collision/BGK/BGK.cuh:55-96), MRT<3> in the De Rosis
  * raw-moment basis (src/core/collision/MRT/MRT.cu:79-456, basis src/core/lbm_constants.cuh:218-246),
- * the FLUID and BOUNCE_BACK flags (src/core/boundaries/boundaries.cuh:89-138, BounceBack<3>
+ * CM<3, NoAdapter> (src/core/collision/CM/CM_D3Q27.cuh:1-105), the FLUID and BOUNCE_BACK flags
+ * (src/core/boundaries/boundaries.cuh:89-138, BounceBack<3>
  * src/functors/boundaryConditions/bbDomainBoundary.cuh:52-90), runtime periodic axes, FP32 or
- * 16-bit shifted population storage. CM<3> and the other 3-D boundary functors are rejected
- * with LBM_E_UNSUPPORTED.
+ * 16-bit shifted population storage. The CM<3> adapters (ACM_D3Q27.cuh, E_ACM_D3Q27.cuh) and the
+ * other 3-D boundary functors are rejected with LBM_E_UNSUPPORTED.
  */
 #ifndef LBM3D_HIP_H
 #define LBM3D_HIP_H
@@ -43,10 +44,15 @@ extern "C" {
  * compute_m_post_combined (MRT.cu:203-395), leaves moments 8-26 unset and is not restated. */
 #define LBM_QUIRK_MRT3_LITERAL (1u << 7)
 
+/* CM<3> adapters (src/core/collision/adapters.cuh): LBM_NO_ADAPTER is built; LBM_OPTIMAL_ADAPTER
+ * (CM<3, OptimalAdapter>, ACM_D3Q27.cuh) and LBM3D_EMPIRICAL_ADAPTER (E_ACM_D3Q27.cuh) are not. */
+#define LBM3D_EMPIRICAL_ADAPTER 2
+
 typedef struct lbm3d_config {
     int32_t  nx, ny, nz;
-    int32_t  collision;      /* LBM_BGK | LBM_MRT                                             */
-    float    omega;          /* BGK rate 1/tau (ScenarioTrait::omega, src/scenarios/scenario.cuh:36-38) */
+    int32_t  collision;      /* LBM_BGK | LBM_MRT | LBM_CM                                    */
+    int32_t  adapter;        /* LBM_NO_ADAPTER (CM<3> adapters are not built)                  */
+    float    omega;          /* BGK and CM rate 1/tau (ScenarioTrait::omega, src/scenarios/scenario.cuh:36-38) */
     float    S[27];          /* MRT<3> rates in the basis order (ScenarioTrait::S); S[4..8], the
                                 deviatoric second moments, are the ones MRT<3> reads            */
     float    rho0;           /* reference density of the shifted storage                      */
@@ -61,7 +67,8 @@ typedef struct lbm3d_config {
 
 typedef struct lbm3d_ctx lbm3d_ctx;
 
-/* Defaults: BGK, tau = 1 (omega = 1), S = 1 everywhere, rho0 = 1, FP32, not periodic, no force. */
+/* Defaults: BGK, no adapter, tau = 1 (omega = 1), S = 1 everywhere, rho0 = 1, FP32, not periodic,
+ * no force. */
 int lbm3d_config_default(lbm3d_config* cfg, int32_t nx, int32_t ny, int32_t nz);
 
 /* Replaces LBM<3>::allocate (lbm.cuh:92-125) + send_consts (lbm.cuh:63-82). */
@@ -84,7 +91,7 @@ int lbm3d_init_from_macros(lbm3d_ctx* ctx, const float* rho, const float* u, int
 int lbm3d_set_force_field(lbm3d_ctx* ctx, const float* force, int on_device);
 
 /* `nsteps` iterations of main.cu:96-114 (stream, boundaries, macroscopics, forces, Guo
- * correction, equilibrium, BGK or MRT collision), one fused pull-scheme kernel per step.
+ * correction, equilibrium, BGK, MRT or CM collision), one fused pull-scheme kernel per step.
  * Asynchronous; the last step also stores rho and the corrected u. */
 int lbm3d_step(lbm3d_ctx* ctx, int32_t nsteps);
 int lbm3d_sync(lbm3d_ctx* ctx);

@@ -149,6 +149,14 @@ void oracle3d_collide_mrt(const double* g, const double* u, const double* F, con
     for (int q = 0; q < Q; q++) out[q] = f[q];
 }
 
+// One node through CM<3, NoAdapter> (FP64): g post-stream populations, u corrected velocity.
+void oracle3d_collide_cm(const double* g, const double* u, const double* F, double omega, double* out) {
+    double f[Q];
+    for (int q = 0; q < Q; q++) f[q] = g[q];
+    collide_cm3<double>(f, u[0], u[1], u[2], F[0], F[1], F[2], omega);
+    for (int q = 0; q < Q; q++) out[q] = f[q];
+}
+
 // Guo source of BGK<3> for all 27 directions (FP64).
 void oracle3d_guo_source(double ux, double uy, double uz, double fx, double fy, double fz, double omega, double* out) {
     for (int q = 0; q < Q; q++) out[q] = guo_source<double>(q, ux, uy, uz, fx, fy, fz, omega);

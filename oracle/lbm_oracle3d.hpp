@@ -1,4 +1,4 @@
-// lbm_oracle3d.hpp — CPU restatement of the reference's D3Q27 BGK and MRT path.
+// lbm_oracle3d.hpp — CPU restatement of the reference's D3Q27 BGK, MRT and CM path.
 //
 // TEST INFRASTRUCTURE ONLY. This is the correctness oracle for the 3-D step. Only tests/,
 // the smoke check in __graft_entry__.py and the cpu_baseline leg of bench.py may load or call
@@ -6,9 +6,9 @@
 //
 // It follows the reference's unfused stage order (src/main.cu:96-114) on separate buffers, as
 // `LBM<3>` does (src/core/lbm.cuh:36-43): push stream -> swap -> boundaries -> uncorrected
-// macroscopics -> reset forces -> Guo velocity correction -> equilibrium -> BGK<3> or MRT<3>
-// collision, templated on the arithmetic type (FP32 oracle, FP64 host run). MRT<3> is the dense
-// form of the reference: m = M f, m* per moment, f = M^-1 m*.
+// macroscopics -> reset forces -> Guo velocity correction -> equilibrium -> BGK<3>, MRT<3> or
+// CM<3> collision, templated on the arithmetic type (FP32 oracle, FP64 host run). MRT<3> and CM<3>
+// are dense: m = M f, m* per moment, f = M^-1 m* (CM: the central-moment matrix T(u) per node).
 //
 // Parity pinning (tests/test_oracle3d_pinning.py), each against something this file does not
 // restate: the equilibrium's moments up to second order (exact identities), the literal
@@ -18,7 +18,9 @@
 // Taylor-Green vortex against exp(-6 nu k^2 t), and the curvature -F/nu of a force-driven
 // channel; for MRT<3>: the basis against its inverse, the collision at unit rates against the
 // product-form equilibrium rho*prod_a phi(c_a, u_a), the relaxed second moments, the momentum
-// gain per step and the viscous decay rate. Parity unpinned: the FP32 rounding sequence of the
+// gain per step and the viscous decay rate; for CM<3>: unit omega against the product-form
+// equilibrium, the relaxed central second moments and trace, momentum (+F), the forcing central
+// moments and the viscous decay rate. Parity unpinned: the FP32 rounding sequence of the
 // reference's De Rosis expression (equilibrium.cu:70-96), which acts at FP32 rounding level.
 #pragma once
 
@@ -254,6 +256,75 @@ inline void collide_mrt3(R* f, R ux, R uy, R uz, R fx, R fy, R fz, const float* 
         for (int k = 0; k < Q; k++) f[k] += R(B.Minv[k * Q + i]) * mp[i];
 }
 
+// CM<3, NoAdapter>::apply (CM_D3Q27.cuh:1-105, the De Rosis (2019) universal central-moment
+// operator; its sympy-expanded expressions are restated here by their central moments). With
+// K_abc = sum f (c_x - u_x)^a (c_y - u_y)^b (c_z - u_z)^c about the corrected u and e = (1, 0, cs2)
+// per axis, the post-collision central moments are
+//   K*_abc = rho e_a e_b e_c + (F_x/2) [a=1] e_b e_c + (F_y/2) e_a [b=1] e_c + (F_z/2) e_a e_b [c=1]
+// except the five deviatoric second moments k4 = K110, k5 = K101, k6 = K011, k7 = K200 - K020,
+// k8 = K200 - K002, which become (1 - omega) k (the trace K200 + K020 + K002 is set to rho).
+// f* = T(u)^-1 K*, with T(u) the 27x27 central-moment matrix inverted per node (Gauss-Jordan).
+template <class R>
+inline void collide_cm3(R* f, R ux, R uy, R uz, R fx, R fy, R fz, R omega) {
+    const R u[3] = {ux, uy, uz}, F[3] = {fx, fy, fz};
+    R rho = 0, k4 = 0, k5 = 0, k6 = 0, k7 = 0, k8 = 0;
+    for (int i = 0; i < Q; i++) {
+        const R cx = R(CX[i]) - ux, cy = R(CY[i]) - uy, cz = R(CZ[i]) - uz;
+        rho += f[i];
+        k4 += f[i] * cx * cy;
+        k5 += f[i] * cx * cz;
+        k6 += f[i] * cy * cz;
+        k7 += f[i] * (cx * cx - cy * cy);
+        k8 += f[i] * (cx * cx - cz * cz);
+    }
+    const R e[3] = {R(1), R(0), R(1) / R(3)};
+    R K[3][3][3];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+            for (int c = 0; c < 3; c++) {
+                R v = rho * e[a] * e[b] * e[c];
+                if (a == 1) v += R(0.5) * F[0] * e[b] * e[c];
+                if (b == 1) v += R(0.5) * F[1] * e[a] * e[c];
+                if (c == 1) v += R(0.5) * F[2] * e[a] * e[b];
+                K[a][b][c] = v;
+            }
+    const R w = R(1) - omega;
+    K[1][1][0] = w * k4;
+    K[1][0][1] = w * k5;
+    K[0][1][1] = w * k6;
+    K[2][0][0] = (rho + w * (k7 + k8)) / R(3);
+    K[0][2][0] = K[2][0][0] - w * k7;
+    K[0][0][2] = K[2][0][0] - w * k8;
+    // T(u): row (a,b,c), column i: (c_x-u_x)^a (c_y-u_y)^b (c_z-u_z)^c; solve T f = K
+    double A[Q][Q + 1];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+            for (int c = 0; c < 3; c++) {
+                const int r = (a * 3 + b) * 3 + c;
+                for (int i = 0; i < Q; i++) {
+                    const double dx = double(CX[i]) - double(u[0]), dy = double(CY[i]) - double(u[1]),
+                                 dz = double(CZ[i]) - double(u[2]);
+                    A[r][i] = std::pow(dx, a) * std::pow(dy, b) * std::pow(dz, c);
+                }
+                A[r][Q] = double(K[a][b][c]);
+            }
+    for (int col = 0; col < Q; col++) {
+        int piv = col;
+        for (int r = col + 1; r < Q; r++)
+            if (std::fabs(A[r][col]) > std::fabs(A[piv][col])) piv = r;
+        for (int j = 0; j <= Q; j++) std::swap(A[col][j], A[piv][j]);
+        const double d = A[col][col];
+        for (int j = 0; j <= Q; j++) A[col][j] /= d;
+        for (int r = 0; r < Q; r++) {
+            if (r == col) continue;
+            const double m = A[r][col];
+            if (m != 0.0)
+                for (int j = 0; j <= Q; j++) A[r][j] -= m * A[col][j];
+        }
+    }
+    for (int i = 0; i < Q; i++) f[i] = R(A[i][Q]);
+}
+
 template <class R>
 struct Solver3 {
     lbm3d_config cfg;
@@ -269,7 +340,8 @@ struct Solver3 {
     explicit Solver3(const lbm3d_config& c)
         : cfg(c), nx(c.nx), ny(c.ny), nz(c.nz), N(size_t(c.nx) * size_t(c.ny) * size_t(c.nz)) {
         if (nx < 3 || ny < 3 || nz < 3) throw std::runtime_error("grid must be at least 3x3x3");
-        if (c.collision != LBM_BGK && c.collision != LBM_MRT) throw std::runtime_error("3-D operator not restated");
+        if (c.collision != LBM_BGK && c.collision != LBM_MRT && c.collision != LBM_CM)
+            throw std::runtime_error("3-D operator not restated");
         mrt_basis();
         eq_literal = (c.quirks & LBM_QUIRK_EQ3D_LINEAR) != 0;
         f.assign(Q * N, R(0));
@@ -404,6 +476,17 @@ struct Solver3 {
     // equilibrium + BGK<3>::apply (BGK.cuh:55-96): f - omega (f - f_eq) + Guo source;
     // or MRT<3>::apply.
     void collide() {
+        if (cfg.collision == LBM_CM) {
+#pragma omp parallel for schedule(static)
+            for (long long ln = 0; ln < (long long)N; ln++) {
+                const size_t n = size_t(ln);
+                R g[Q];
+                for (int q = 0; q < Q; q++) g[q] = f[q * N + n];
+                collide_cm3<R>(g, u[n], u[N + n], u[2 * N + n], force[n], force[N + n], force[2 * N + n], R(cfg.omega));
+                for (int q = 0; q < Q; q++) f[q * N + n] = g[q];
+            }
+            return;
+        }
         if (cfg.collision == LBM_MRT) {
             const bool half = (cfg.quirks & LBM_QUIRK_MRT3_LITERAL) != 0;
 #pragma omp parallel for schedule(static)

@@ -80,6 +80,7 @@ def lib():
                                             ctypes.c_double, ctypes.c_double, _D]),
             "oracle3d_guo_source": (None, [ctypes.c_double] * 7 + [_D]),
             "oracle3d_mrt_basis": (None, [_D, _D]),
+            "oracle3d_collide_cm": (None, [_D, _D, _D, ctypes.c_double, _D]),
             "oracle3d_collide_mrt": (None, [_D, _D, _D, ctypes.POINTER(ctypes.c_float), ctypes.c_int, _D]),
         }
         for name, (res, args) in sig.items():
@@ -306,6 +307,16 @@ def collide_mrt3(g, u, F, S, literal=False):
     SS = (ctypes.c_float * 27)(*[float(v) for v in S])
     out = np.zeros(27)
     lib().oracle3d_collide_mrt(_dp(gg), _dp(uu), _dp(FF), SS, int(literal), _dp(out))
+    return out
+
+
+def collide_cm3(g, u, F, omega):
+    """One node through CM<3, NoAdapter> (FP64): g post-stream populations, u corrected velocity."""
+    gg = np.ascontiguousarray(g, np.float64)
+    uu = np.ascontiguousarray(u, np.float64)
+    FF = np.ascontiguousarray(F, np.float64)
+    out = np.zeros(27)
+    lib().oracle3d_collide_cm(_dp(gg), _dp(uu), _dp(FF), float(omega), _dp(out))
     return out
 
 

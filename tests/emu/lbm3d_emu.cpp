@@ -102,6 +102,13 @@ extern "C" {
 // f_in / f_out: FP32 SoA [27][N]; both buffers start from f_in (as lbm3d_set_pops does).
 int emu3d_run(const lbm3d_config* c, const float* f_in, const uint8_t* flags, const float* force_aos, int nsteps,
               float* f_out, float* rho_out, float* u_aos_out) {
+    if (c->collision == LBM_CM) {
+        switch (c->storage) {
+            case LBM_F32: return run3<LBM_CM, LBM_F32>(*c, f_in, flags, force_aos, nsteps, f_out, rho_out, u_aos_out);
+            case LBM_F16_SHIFTED: return run3<LBM_CM, LBM_F16_SHIFTED>(*c, f_in, flags, force_aos, nsteps, f_out, rho_out, u_aos_out);
+            default: return run3<LBM_CM, LBM_BF16_SHIFTED>(*c, f_in, flags, force_aos, nsteps, f_out, rho_out, u_aos_out);
+        }
+    }
     if (c->collision == LBM_MRT) {
         switch (c->storage) {
             case LBM_F32: return run3<LBM_MRT, LBM_F32>(*c, f_in, flags, force_aos, nsteps, f_out, rho_out, u_aos_out);
@@ -128,13 +135,13 @@ int main() {
     for (const auto& s : shapes)
         for (int per = 0; per < 8; per++)
             for (int st = 0; st <= 2; st++)
-                for (int var = 0; var < 4; var++) {
+                for (int var = 0; var < 5; var++) {
                     lbm3d_config c;
                     std::memset(&c, 0, sizeof c);
                     c.nx = s[0];
                     c.ny = s[1];
                     c.nz = s[2];
-                    c.collision = var == 3 ? LBM_MRT : LBM_BGK;
+                    c.collision = var == 3 ? LBM_MRT : (var == 4 ? LBM_CM : LBM_BGK);
                     c.omega = 1.3f;
                     for (int k = 0; k < 27; k++) c.S[k] = k >= 4 && k <= 8 ? 1.3f : 1.0f;
                     c.rho0 = 1.0f;
@@ -156,7 +163,7 @@ int main() {
                                 if (x == 0 || y == 0 || z == 0 || x == c.nx - 1 || y == c.ny - 1 || z == c.nz - 1)
                                     flags[size_t((z * c.ny + y) * c.nx + x)] = LBM_BOUNCE_BACK;
                     std::vector<float> force(size_t(3 * N), 2e-6f);
-                    emu3d_run(&c, f.data(), var >= 1 ? flags.data() : nullptr, var == 1 || var == 3 ? force.data() : nullptr, 3,
+                    emu3d_run(&c, f.data(), var >= 1 ? flags.data() : nullptr, var == 1 || var >= 3 ? force.data() : nullptr, 3,
                               out.data(), rho.data(), u.data());
                     cases++;
                 }

@@ -167,6 +167,37 @@ def test_emulated_mrt3_matches_dense_oracle(case):
     assert np.abs(u - r_u).max() < 1e-6
 
 
+CM_CASES = [
+    # shape, periodic, omega, faces with bounce-back, force field
+    ((6, 5, 4), (1, 1, 1), 1.0, "", False),
+    ((7, 5, 6), (1, 1, 1), 1.7, "", True),
+    ((66, 5, 3), (1, 0, 1), 0.8, "y", True),
+    ((5, 6, 4), (0, 0, 0), 1.4, "xyz", False),
+]
+
+
+def cm_case(case):
+    shape, periodic, omega, faces, field = CM_CASES[case]
+    nx, ny, nz = shape
+    c = make_cfg(shape, periodic, omega, force=(2e-5, -1e-5, 1e-5), collision=_abi.LBM_CM)
+    f0 = initial_state(shape, 80 + case)
+    flags = box_flags(shape, faces) if faces else None
+    force = 1e-5 * np.random.default_rng(300 + case).standard_normal((nz, ny, nx, 3)) if field else None
+    return c, f0, flags, force
+
+
+@pytest.mark.parametrize("case", range(len(CM_CASES)))
+def test_emulated_cm3_matches_dense_oracle(case):
+    """The kernel's CM<3> (factorised central shifts) against the oracle's per-node dense T(u)^-1."""
+    c, f0, flags, force = cm_case(case)
+    o = oracle_run(c, f0, flags, force, 6)
+    g, rho, u = emu_run(c, f0, flags, force, 6)
+    assert np.abs(g - o.get_pops()).max() < 1e-5
+    r_rho, r_u = o.get_macros()
+    assert np.abs(rho - r_rho).max() < 1e-5
+    assert np.abs(u - r_u).max() < 1e-6
+
+
 @pytest.mark.parametrize("storage,eps", [(_abi.LBM_F16_SHIFTED, 2.0 ** -11), (_abi.LBM_BF16_SHIFTED, 2.0 ** -8)])
 def test_emulated_reduced_storage_matches_oracle(storage, eps):
     """16-bit storage: FP32 rounding differences may move a stored deviation f - w rho0 across a

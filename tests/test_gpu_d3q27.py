@@ -7,16 +7,36 @@ import oracle as O
 from lbm_amd import _abi as A
 from lbm_amd.solver import LbmError
 from lbm_amd.solver3d import Config3D, Solver3D
-from test_emu3d_parity import CASES, MRT_CASES, box_flags, initial_state, make_cfg, mrt_case, oracle_run
+from test_emu3d_parity import (CASES, CM_CASES, MRT_CASES, box_flags, cm_case, initial_state, make_cfg, mrt_case,
+                               oracle_run)
 
 pytestmark = pytest.mark.gpu
 W = O.tables3d()[0]
 
 
 def gpu_config(c):
-    return Config3D(c.nx, c.ny, c.nz, collision=c.collision, omega=c.omega, S=tuple(c.S), storage=c.storage,
+    return Config3D(c.nx, c.ny, c.nz, collision=c.collision, adapter=c.adapter, omega=c.omega, S=tuple(c.S),
+                    storage=c.storage,
                     periodic_x=bool(c.periodic_x), periodic_y=bool(c.periodic_y), periodic_z=bool(c.periodic_z),
                     force=tuple(c.force), quirks=c.quirks)
+
+
+@pytest.mark.parametrize("case", range(len(CM_CASES)))
+def test_gpu_cm3_matches_dense_oracle(gpu_available, case):
+    c, f0, flags, force = cm_case(case)
+    f0 = f0.astype(np.float32)
+    o = oracle_run(c, f0, flags, force, 8)
+    with Solver3D(gpu_config(c)) as s:
+        s.set_pops(f0)
+        s.set_flags(flags)
+        s.set_force_field(None if force is None else force.astype(np.float32))
+        s.step(8)
+        g = s.get_pops()
+        rho, u = s.get_macros()
+    assert np.abs(g - o.get_pops()).max() < 1e-5
+    r_rho, r_u = o.get_macros()
+    assert np.abs(rho - r_rho).max() < 1e-5
+    assert np.abs(u - r_u).max() < 1e-6
 
 
 @pytest.mark.parametrize("case", range(len(MRT_CASES)))
@@ -158,9 +178,9 @@ def test_gpu_taylor_green_3d_kinetic_energy(gpu_available):
     assert abs(rate / (6 * nu * k * k) - 1) < 0.01
 
 
-def test_gpu_mrt3_taylor_green_decay_and_f16_storage(gpu_available):
-    """MRT<3> with S[4..8] = 1.5 on 64^3: the kinetic energy decays at exp(-6 nu k^2 t),
-    nu = (1/1.5 - 1/2)/3, in FP32 and in FP16 storage."""
+def test_gpu_mrt3_and_cm3_taylor_green_decay_and_f16_storage(gpu_available):
+    """MRT<3> with S[4..8] = 1.5 and CM<3> with omega = 1.5 on 64^3: the kinetic energy decays at
+    exp(-6 nu k^2 t), nu = (1/1.5 - 1/2)/3, in FP32 and in FP16 storage."""
     n, s48 = 64, 1.5
     nu = (1 / s48 - 0.5) / 3
     k = 2 * np.pi / n
@@ -170,9 +190,10 @@ def test_gpu_mrt3_taylor_green_decay_and_f16_storage(gpu_available):
                   np.zeros_like(x, dtype=float)], -1).astype(np.float32)
     S = [1.0] * 27
     S[4:9] = [s48] * 5
-    for storage in (A.LBM_F32, A.LBM_F16_SHIFTED):
-        cfg = Config3D(n, n, n, collision=A.LBM_MRT, S=tuple(S), storage=storage, periodic_x=True, periodic_y=True,
-                       periodic_z=True)
+    for coll, storage in ((A.LBM_MRT, A.LBM_F32), (A.LBM_MRT, A.LBM_F16_SHIFTED), (A.LBM_CM, A.LBM_F32),
+                          (A.LBM_CM, A.LBM_F16_SHIFTED)):
+        cfg = Config3D(n, n, n, collision=coll, omega=s48, S=tuple(S), storage=storage, periodic_x=True,
+                       periodic_y=True, periodic_z=True)
         with Solver3D(cfg) as s:
             s.init_from_macros(np.ones((n, n, n), np.float32), u)
             s.step(40)
@@ -199,7 +220,7 @@ def test_gpu_d3q27_reductions_and_conservation(gpu_available):
 
 def test_gpu_d3q27_errors_are_reported(gpu_available):
     with pytest.raises(LbmError) as e:
-        Solver3D(Config3D(8, 8, 8, collision=A.LBM_CM))
+        Solver3D(Config3D(8, 8, 8, collision=A.LBM_CM, adapter=A.LBM_OPTIMAL_ADAPTER))
     assert e.value.code == A.LBM_E_UNSUPPORTED
     with pytest.raises(LbmError) as e:
         Solver3D(Config3D(2, 8, 8))

@@ -352,3 +352,67 @@ def test_mrt3_taylor_green_decays_at_the_viscous_rate():
     o.step(60)
     e1 = energy()
     assert abs(-np.log(e1 / e0) / 60 / (6 * nu * k * k) - 1) < 0.02
+
+
+# ---- CM<3, NoAdapter> ------------------------------------------------------------------------
+
+def central_moment(f, u, a, b, c):
+    return (f * (CX - u[0]) ** a * (CY - u[1]) ** b * (CZ - u[2]) ** c).sum()
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_cm3_at_unit_omega_is_the_product_equilibrium(seed):
+    rng = np.random.default_rng(60 + seed)
+    g = W * (1 + 0.1 * rng.standard_normal(27))
+    u = 0.08 * rng.standard_normal(3)
+    out = O.collide_cm3(g, u, (0, 0, 0), 1.0)
+    assert np.allclose(out, product_equilibrium(g.sum(), u), atol=1e-14)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_cm3_central_moments_after_collision(seed):
+    """Conserved rho, momentum + F, the deviatoric central second moments relaxed by (1 - omega),
+    the trace at rho (= 3 cs2 rho), third-order forcing moments cs2 F/2 and the sixth at rho cs2^3."""
+    rng = np.random.default_rng(70 + seed)
+    g = W * (1 + 0.1 * rng.standard_normal(27))
+    rho = g.sum()
+    F = 1e-3 * rng.standard_normal(3)
+    u = (g @ C + F / 2) / rho
+    om = float(np.float64(rng.uniform(0.3, 1.9)))
+    out = O.collide_cm3(g, u, F, om)
+    assert abs(out.sum() - rho) < 1e-14
+    assert np.allclose(out @ C, g @ C + F, atol=1e-14)
+    k = lambda f, a, b, c: central_moment(f, u, a, b, c)
+    for idx in [(1, 1, 0), (1, 0, 1), (0, 1, 1)]:
+        assert abs(k(out, *idx) - (1 - om) * k(g, *idx)) < 1e-14
+    dev_in = k(g, 2, 0, 0) - k(g, 0, 2, 0)
+    assert abs(k(out, 2, 0, 0) - k(out, 0, 2, 0) - (1 - om) * dev_in) < 1e-14
+    assert abs(k(out, 2, 0, 0) + k(out, 0, 2, 0) + k(out, 0, 0, 2) - rho) < 1e-14
+    assert np.allclose([k(out, 1, 2, 0), k(out, 2, 1, 0), k(out, 0, 2, 1)], [F[0] / 6, F[1] / 6, F[2] / 6], atol=1e-15)
+    assert abs(k(out, 1, 1, 1)) < 1e-15 and abs(k(out, 2, 2, 2) - rho / 27) < 1e-14
+    assert abs(k(out, 1, 2, 2) - F[0] / 18) < 1e-15
+
+
+def test_cm3_taylor_green_decays_at_the_viscous_rate():
+    n = 24
+    om = 1.6
+    nu = (1 / om - 0.5) / 3
+    k = 2 * np.pi / n
+    c = cfg3(n, n, n, omega=om)
+    c.collision = _abi.LBM_CM
+    o = O.Oracle3D(c, 64)
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    U = 1e-4
+    u = np.stack([U * np.sin(k * x) * np.cos(k * y) * np.cos(k * z),
+                  -U * np.cos(k * x) * np.sin(k * y) * np.cos(k * z), np.zeros_like(x, dtype=float)], -1)
+    o.init_from_macros(np.ones((n, n, n)), u)
+
+    def energy():
+        rho, uu = o.get_macros()
+        return 0.5 * (rho * (uu ** 2).sum(-1)).sum()
+
+    o.step(20)
+    e0 = energy()
+    o.step(60)
+    e1 = energy()
+    assert abs(-np.log(e1 / e0) / 60 / (6 * nu * k * k) - 1) < 0.02
