@@ -306,7 +306,7 @@ int lbm3d_create(const lbm3d_config* cfg, lbm3d_ctx** out) {
     if ((long long)k.nx * k.ny * k.nz > (1LL << 31) - 64) return bad(LBM_E_INVALID, "grid exceeds 2^31 nodes");
     if (k.collision != LBM_BGK && k.collision != LBM_MRT && k.collision != LBM_CM) return bad(LBM_E_INVALID, "unknown collision operator");
     if (k.adapter == LBM_OPTIMAL_ADAPTER || k.adapter == LBM3D_EMPIRICAL_ADAPTER)
-        return bad(LBM_E_UNSUPPORTED, "the D3Q27 adapters (ACM, E-ACM) are not built; CM<3> runs with NoAdapter");
+        return bad(LBM_E_UNSUPPORTED, "the reference's D3Q27 adapter operators (ACM, E-ACM) do not collide the node as shipped; CM<3> runs with NoAdapter");
     if (k.adapter != LBM_NO_ADAPTER) return bad(LBM_E_INVALID, "unknown adapter");
     if (k.collision != LBM_MRT && !(k.omega > 0.0f && k.omega < 2.0f)) return bad(LBM_E_INVALID, "BGK and CM need 0 < omega < 2 (tau > 0.5)");
     for (int i = 0; i < 27; i++)

@@ -44,8 +44,13 @@ extern "C" {
  * compute_m_post_combined (MRT.cu:203-395), leaves moments 8-26 unset and is not restated. */
 #define LBM_QUIRK_MRT3_LITERAL (1u << 7)
 
-/* CM<3> adapters (src/core/collision/adapters.cuh): LBM_NO_ADAPTER is built; LBM_OPTIMAL_ADAPTER
- * (CM<3, OptimalAdapter>, ACM_D3Q27.cuh) and LBM3D_EMPIRICAL_ADAPTER (E_ACM_D3Q27.cuh) are not. */
+/* CM<3> adapters (src/core/collision/adapters.cuh): LBM_NO_ADAPTER is built. LBM_OPTIMAL_ADAPTER
+ * (CM<3, OptimalAdapter>, src/core/collision/CM/ACM_D3Q27.cuh) and LBM3D_EMPIRICAL_ADAPTER
+ * (E_ACM_D3Q27.cuh) are rejected with LBM_E_UNSUPPORTED: as shipped, both assign instead of
+ * accumulate their central moments over the directions (ACM_D3Q27.cuh:30-52, E_ACM_D3Q27.cuh:40-65)
+ * and write their 27 results to f[0..26] of the whole population array instead of the node's
+ * own populations (ACM_D3Q27.cuh:100-126, E_ACM_D3Q27.cuh:103-129), so there is no operator to
+ * restate. */
 #define LBM3D_EMPIRICAL_ADAPTER 2
 
 typedef struct lbm3d_config {
