@@ -261,8 +261,18 @@ def run_d3q27(torch, steps=200, warmup=20):
     fl[0], fl[-1] = A.BOUNCE_BACK, A.BOUNCE_BACK
     ch = Config3D(nx, ny, nz, omega=1.6, periodic_x=True, force=(1e-6, 0.0, 0.0))
     res["channel_256x128x96_f32"] = one(ch, (np.ones((nz, ny, nx), np.float32), np.zeros((nz, ny, nx, 3), np.float32)), fl)
+    # the reference's default 3-D case (windTestScenario.cuh) without its IBM body: CM<3>, Guo
+    # velocity inlet / pressure outlet on x, bounce-back on y and z, plug flow u = 0.05
+    wt = fl.copy()
+    wt[:, :, 0] = A.GUO_VELOCITY_INLET
+    wt[:, :, -1] = A.GUO_PRESSURE_OUTLET
+    u0 = np.zeros((nz, ny, nx, 3), np.float32)
+    u0[..., 0] = 0.05
+    wcfg = Config3D(nx, ny, nz, collision=A.LBM_CM, omega=1.0 / (3.0 * 0.05 * ny / 1000.0 + 0.5), u_inlet=0.05)
+    res["wind_tunnel_256x128x96_cm_f32"] = one(wcfg, (np.ones((nz, ny, nx), np.float32), u0), wt)
     res["workload"] = ("D3Q27 BGK with Guo forcing (and MRT<3>, CM<3> at 256^3), FP32 collision; Taylor-Green 3-D "
-                       "(u_max 0.05) periodic, and a force-driven channel with bounce-back walls on y and z; 216 B "
+                       "(u_max 0.05) periodic, a force-driven channel with bounce-back walls on y and z, and the wind "
+                       "tunnel (CM<3>, Guo inlet/outlet, walls; planes launch + main launch per step); 216 B "
                        "(FP32) / 108 B (FP16) per node update")
     return res
 

@@ -41,7 +41,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void s
     int x, y, z;
     thread_node(p, L, int(threadIdx.x), x, y, z);
     if (x >= p.nx || y >= p.ny) return;
+    if (p.skip_x_edges && (x == 0 || x == p.nx - 1)) return;
     step3_node<COLL, STORE, BUF>(p, x, y, z);
+}
+
+// The x = 0 and x = NX-1 planes when they carry Guo inlet/outlet flags: thread i covers node
+// (side ? NX-1 : 0, y, z) of the 2 * NY * NZ plane nodes. Runs before step3_kernel, because
+// the Guo functors read their x +- 1 neighbours' stale slots, which that launch rewrites.
+template <int COLL, int STORE, bool BUF>
+__global__ __launch_bounds__(256) void step3_planes_kernel(Step3Params p) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long per = (long long)p.ny * p.nz;
+    if (i >= 2 * per) return;
+    const int side = int(i / per);
+    const long long r = i - side * per;
+    step3_node<COLL, STORE, BUF, true>(p, side ? p.nx - 1 : 0, int(r % p.ny), int(r / p.ny));
 }
 
 // init_kernel<3> + init_node (init.cuh:10-43): both buffers = f_eq(rho, u), u AoS [N][3].
@@ -144,6 +158,7 @@ struct lbm3d_ctx {
     void* buf[2] = {nullptr, nullptr};
     int cur = 0;                       // buffer holding the latest post-collision state
     uint8_t* d_flags = nullptr;        // null while every node is FLUID
+    bool has_guo = false;              // Guo inlet/outlet flags present (two launches per step)
     float* d_force = nullptr;          // SoA [3][plane] or null
     float* d_rho = nullptr;            // [plane]
     float* d_u = nullptr;              // SoA [3][plane]
@@ -242,6 +257,9 @@ int lbm3d_ctx::one_step(bool write_macros) {
     p.lin_ec = lin_ec();
     for (int k = 0; k < 5; k++) p.mrt.s[k] = cfg.S[4 + k];
     p.mrt.literal = (cfg.quirks & LBM_QUIRK_MRT3_LITERAL) ? 1 : 0;
+    p.u_inlet = cfg.u_inlet;
+    p.guo_own = (cfg.quirks & LBM_QUIRK_GUO3D_OWN_MOMENTS) ? 1 : 0;
+    p.skip_x_edges = has_guo ? 1 : 0;
     p.stale_rest = (cfg.quirks & LBM_QUIRK_STALE_REST) ? 1 : 0;
     p.wx = wave_cols(cfg.nx);
     p.nbx = (cfg.nx + 64 * p.wx - 1) / (64 * p.wx);
@@ -250,7 +268,13 @@ int lbm3d_ctx::one_step(bool write_macros) {
     if (nblk > 0x7fffffffLL) return fail(LBM_E_INVALID, "grid needs too many workgroups");
     const bool fits = cfg.storage == LBM_F32 ? lbm::buf_fits<float>(Q, plane) : lbm::buf_fits<uint16_t>(Q, plane);
     const dim3 grid{unsigned(nblk)}, block{256};
+    const long long nplane = 2LL * cfg.ny * cfg.nz;
+    const dim3 pgrid{unsigned((nplane + 255) / 256)};
 #define LBM3_LAUNCH(C, S)                                                                       \
+    if (has_guo) {                                                                              \
+        if (fits) hipLaunchKernelGGL((step3_planes_kernel<C, S, true>), pgrid, block, 0, stream, p);  \
+        else hipLaunchKernelGGL((step3_planes_kernel<C, S, false>), pgrid, block, 0, stream, p);      \
+    }                                                                                           \
     if (fits) hipLaunchKernelGGL((step3_kernel<C, S, true>), grid, block, 0, stream, p);        \
     else hipLaunchKernelGGL((step3_kernel<C, S, false>), grid, block, 0, stream, p);
 #define LBM3_LAUNCH_S(C)                                                                        \
@@ -315,8 +339,9 @@ int lbm3d_create(const lbm3d_config* cfg, lbm3d_ctx** out) {
     if (!(k.rho0 > 0.0f)) return bad(LBM_E_INVALID, "rho0 must be positive");
     for (int c = 0; c < 3; c++)
         if (!std::isfinite(k.force[c])) return bad(LBM_E_INVALID, "force must be finite");
-    if (k.quirks & ~(LBM_QUIRK_STALE_REST | LBM_QUIRK_EQ3D_LINEAR | LBM_QUIRK_MRT3_LITERAL))
-        return bad(LBM_E_UNSUPPORTED, "only the STALE_REST, EQ3D_LINEAR and MRT3_LITERAL quirks apply to D3Q27");
+    if (k.quirks & ~(LBM_QUIRK_STALE_REST | LBM_QUIRK_EQ3D_LINEAR | LBM_QUIRK_MRT3_LITERAL | LBM_QUIRK_GUO3D_OWN_MOMENTS))
+        return bad(LBM_E_UNSUPPORTED, "only the STALE_REST, EQ3D_LINEAR, MRT3_LITERAL and GUO3D_OWN_MOMENTS quirks apply to D3Q27");
+    if (!std::isfinite(k.u_inlet) || !(std::fabs(k.u_inlet) < 0.5f)) return bad(LBM_E_INVALID, "u_inlet must stay below 0.5");
 
     auto* c = new lbm3d_ctx();
     c->cfg = k;
@@ -376,6 +401,7 @@ int lbm3d_set_flags(lbm3d_ctx* ctx, const uint8_t* flags, int on_device) {
             (void)hipFree(ctx->d_flags);
             ctx->d_flags = nullptr;
         }
+        ctx->has_guo = false;
         return LBM_OK;
     }
     std::vector<uint8_t> h(size_t(ctx->N));
@@ -386,13 +412,24 @@ int lbm3d_set_flags(lbm3d_ctx* ctx, const uint8_t* flags, int on_device) {
     } else {
         std::memcpy(h.data(), flags, h.size());
     }
-    bool any = false;
+    bool any = false, guo = false;
+    const int nx = ctx->cfg.nx;
     for (size_t n = 0; n < h.size(); n++) {
-        if (h[n] == LBM_BOUNCE_BACK) any = true;
-        else if (h[n] != LBM_FLUID)
-            return ctx->fail(LBM_E_UNSUPPORTED, "flag %d at node %zu: only FLUID and BOUNCE_BACK are built for D3Q27", int(h[n]), n);
+        const int x = int(n % size_t(nx));
+        if (h[n] == LBM_FLUID) continue;
+        any = true;
+        if (h[n] == LBM_BOUNCE_BACK) continue;
+        if ((h[n] == LBM_GUO_VELOCITY_INLET && x == 0) || (h[n] == LBM_GUO_PRESSURE_OUTLET && x == nx - 1)) {
+            if (ctx->cfg.periodic_x) return ctx->fail(LBM_E_INVALID, "Guo inlet/outlet flags need a non-periodic x axis");
+            guo = true;
+            continue;
+        }
+        if (h[n] == LBM_GUO_VELOCITY_INLET || h[n] == LBM_GUO_PRESSURE_OUTLET)
+            return ctx->fail(LBM_E_UNSUPPORTED, "flag %d at node %zu: Guo inlets are built at x = 0 and outlets at x = NX-1", int(h[n]), n);
+        return ctx->fail(LBM_E_UNSUPPORTED, "flag %d at node %zu: not built for D3Q27 (FLUID, BOUNCE_BACK, GUO_VELOCITY_INLET, GUO_PRESSURE_OUTLET are)", int(h[n]), n);
     }
     if (!any) return lbm3d_set_flags(ctx, nullptr, 0);
+    ctx->has_guo = guo;
     if (!ctx->d_flags) {
         rc = ctx->dev_alloc(reinterpret_cast<void**>(&ctx->d_flags), size_t(ctx->plane), "flags");
         if (rc) return rc;

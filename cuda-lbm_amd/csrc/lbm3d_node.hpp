@@ -327,6 +327,73 @@ LBM3_HD void collide_cm3(float* g, const Macro3& m, float fx, float fy, float fz
             for (int c = 0; c < 3; c++) g[q_of(a - 1, b - 1, c - 1)] = K[a][b][c];
 }
 
+// Equilibrium of direction q (equilibrium.cu:41-96), lin_ec as in collide_bgk3; base is
+// rho (1 - 1.5 u^2).
+template <int q>
+LBM3_HD float feq3(float rho, float ux, float uy, float uz, float base, float lin_ec) {
+    const float cu = cdot<q>(ux, uy, uz);
+    const float lin = q < 7 ? 3.0f : lin_ec;
+    return w3(q) * (base + rho * cu * (lin + 4.5f * cu));
+}
+
+// The unknown populations of GuoVelocityInlet3D (c_x > 0 at x = 0, guoInlet3D.cuh:8-60) and
+// GuoPressureOutlet3D (c_x < 0 at x = NX-1, guoOutlet3D.cuh:6-66): f_eq(rho_b, u_b) plus the
+// neighbour's non-equilibrium part gn_q - f_eq_q(rho_f, u_f). The boundary state comes from the
+// node's own known populations: inlet rho_b = (sum_{c_x=0} g + 2 sum_{c_x<0} g) / (1 - u_in),
+// u_b = (u_in, 0, 0); outlet rho_b = 1, u_x,b = sum_{c_x=0} g + 2 sum_{c_x>0} g - 1 and
+// u_y,b, u_z,b the momentum of the c_x >= 0 populations.
+struct GuoBc {
+    bool inlet;
+    float u_in, lin_ec;
+    float rf, uxf, uyf, uzf;  // neighbour (or own) density and velocity
+};
+template <bool INLET, int q>
+LBM3_HD void guo_known(const float* g, float& sum, float& jy, float& jz) {
+    if (INLET ? CX[q] < 0 : CX[q] > 0) sum += 2.0f * g[q];
+    else if (CX[q] == 0) sum += g[q];
+    if (!INLET && CX[q] >= 0) {
+        if (CY[q] > 0) jy += g[q];
+        if (CY[q] < 0) jy -= g[q];
+        if (CZ[q] > 0) jz += g[q];
+        if (CZ[q] < 0) jz -= g[q];
+    }
+}
+template <bool INLET, int... qs>
+LBM3_HD void guo_known_all(const float* g, float& sum, float& jy, float& jz, std::integer_sequence<int, qs...>) {
+    (guo_known<INLET, qs>(g, sum, jy, jz), ...);
+}
+// gn_unknown[k]: the neighbour's post-stream value of the k-th unknown direction (increasing q).
+template <bool INLET>
+LBM3_HD void bc_guo3(float* g, const float* gn_unknown, const GuoBc& B) {
+    float sum = 0.0f, jy = 0.0f, jz = 0.0f;
+    guo_known_all<INLET>(g, sum, jy, jz, std::make_integer_sequence<int, Q>{});
+    float rb, uxb, uyb = 0.0f, uzb = 0.0f;
+    if (INLET) {
+        uxb = B.u_in;
+        rb = sum / (1.0f - uxb);
+    } else {
+        rb = 1.0f;
+        uxb = sum - 1.0f;
+        uyb = jy;
+        uzb = jz;
+    }
+    const float base_b = rb * (1.0f - 1.5f * (uxb * uxb + uyb * uyb + uzb * uzb));
+    const float base_f = B.rf * (1.0f - 1.5f * (B.uxf * B.uxf + B.uyf * B.uyf + B.uzf * B.uzf));
+    int k = 0;
+#define LBM3_GUO_DIR(q)                                                                                \
+    if (INLET ? CX[q] > 0 : CX[q] < 0) {                                                               \
+        g[q] = feq3<q>(rb, uxb, uyb, uzb, base_b, B.lin_ec) +                                          \
+               (gn_unknown[k] - feq3<q>(B.rf, B.uxf, B.uyf, B.uzf, base_f, B.lin_ec));                 \
+        k++;                                                                                           \
+    }
+    LBM3_GUO_DIR(0) LBM3_GUO_DIR(1) LBM3_GUO_DIR(2) LBM3_GUO_DIR(3) LBM3_GUO_DIR(4) LBM3_GUO_DIR(5)
+    LBM3_GUO_DIR(6) LBM3_GUO_DIR(7) LBM3_GUO_DIR(8) LBM3_GUO_DIR(9) LBM3_GUO_DIR(10) LBM3_GUO_DIR(11)
+    LBM3_GUO_DIR(12) LBM3_GUO_DIR(13) LBM3_GUO_DIR(14) LBM3_GUO_DIR(15) LBM3_GUO_DIR(16) LBM3_GUO_DIR(17)
+    LBM3_GUO_DIR(18) LBM3_GUO_DIR(19) LBM3_GUO_DIR(20) LBM3_GUO_DIR(21) LBM3_GUO_DIR(22) LBM3_GUO_DIR(23)
+    LBM3_GUO_DIR(24) LBM3_GUO_DIR(25) LBM3_GUO_DIR(26)
+#undef LBM3_GUO_DIR
+}
+
 // BounceBack<3>::apply (bbDomainBoundary.cuh:52-90): for i = 1..26 in order, a direction whose
 // neighbour lies outside the domain on a non-periodic axis copies g_i into g_opp(i).
 // Sequential as written, so edge and corner nodes match the reference's loop order.

@@ -36,6 +36,9 @@ struct Step3Params {
     float omega, rho0, fx, fy, fz;
     float lin_ec;          // linear equilibrium coefficient of edge/corner directions (3 or 1)
     Mrt3Rates mrt;         // MRT<3> rates S[4..8] and the literal flag
+    float u_inlet;         // GUO_VELOCITY_INLET velocity
+    int guo_own;           // LBM_QUIRK_GUO3D_OWN_MOMENTS
+    int skip_x_edges;      // the x = 0 and x = NX-1 planes run in their own launch (Guo flags)
     int stale_rest;        // LBM_QUIRK_STALE_REST
     // Workgroup shape: 256 threads = wx wavefronts along x (64*wx consecutive nodes of a row)
     // times 4/wx rows; nbx, nby workgroups along x and y, nz along z.
@@ -135,9 +138,73 @@ LBM3_HD void pull_all(float* g, const Step3Params& p, const lbm::PopBuf<typename
     (pull_dir<STORE, BUF, FAST, qs>(g, p, src, node, ox, vx, oy, vy, oz, vz), ...);
 }
 
+// Post-stream value of population q at node (x, y, z) by the streaming rule, for a functor that
+// reads a neighbour. Its stale slot is read from the destination buffer, which is why such
+// functors run in a launch before the one that rewrites those slots.
+template <int STORE, bool BUF, int q>
+LBM3_HD float pull_at(const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src, int x, int y, int z) {
+    using T = typename Store3<STORE>::T;
+    const int nxy = p.nx * p.ny;
+    const uint32_t node = uint32_t(z * nxy + y * p.nx + x);
+    int ox[3], oy[3], oz[3];
+    bool vx[3], vy[3], vz[3];
+    axis_pull(x, p.nx, p.px, 1, ox, vx);
+    axis_pull(y, p.ny, p.py, p.nx, oy, vy);
+    axis_pull(z, p.nz, p.pz, nxy, oz, vz);
+    constexpr int a = CX[q] + 1, b = CY[q] + 1, c = CZ[q] + 1;
+    const long long off = (long long)q * p.plane;
+    const bool ok = vx[a] && vy[b] && vz[c] && !(q == 0 && p.stale_rest);
+    T raw = src.ld(ok ? uint32_t(int(node) + ox[a] + oy[b] + oz[c]) : node, off);
+    if (!ok) raw = static_cast<const T*>(p.dst)[off + node];
+    return Store3<STORE>::dec(raw, q, p.rho0);
+}
+
+// The neighbour's unknown-direction values and the density and velocity used for f_eq_f.
+template <int STORE, bool BUF, bool INLET, int q>
+LBM3_HD void guo_neigh_dir(const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src, int xn, int y,
+                           int z, float* gu, int& k, float& r, float& jx, float& jy, float& jz) {
+    const float v = pull_at<STORE, BUF, q>(p, src, xn, y, z);
+    if (INLET ? CX[q] > 0 : CX[q] < 0) gu[k++] = v;
+    r += v;
+    if (CX[q] > 0) jx += v;
+    if (CX[q] < 0) jx -= v;
+    if (CY[q] > 0) jy += v;
+    if (CY[q] < 0) jy -= v;
+    if (CZ[q] > 0) jz += v;
+    if (CZ[q] < 0) jz -= v;
+}
+template <int STORE, bool BUF, bool INLET, int... qs>
+LBM3_HD void guo_neigh_all(const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src, int xn, int y,
+                           int z, float* gu, float& r, float& jx, float& jy, float& jz, std::integer_sequence<int, qs...>) {
+    int k = 0;
+    (guo_neigh_dir<STORE, BUF, INLET, qs>(p, src, xn, y, z, gu, k, r, jx, jy, jz), ...);
+}
+template <int STORE, bool BUF, bool INLET>
+LBM3_HD void apply_guo(const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src, float* g, int x, int y,
+                       int z) {
+    float gu[9];
+    float r = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
+    guo_neigh_all<STORE, BUF, INLET>(p, src, INLET ? x + 1 : x - 1, y, z, gu, r, jx, jy, jz,
+                                     std::make_integer_sequence<int, Q>{});
+    if (p.guo_own) {
+        r = jx = jy = jz = 0.0f;
+        mom_all(g, r, jx, jy, jz, std::make_integer_sequence<int, Q>{});
+    }
+    GuoBc B;
+    B.inlet = INLET;
+    B.u_in = p.u_inlet;
+    B.lin_ec = p.lin_ec;
+    B.rf = r;
+    B.uxf = jx / r;
+    B.uyf = jy / r;
+    B.uzf = jz / r;
+    bc_guo3<INLET>(g, gu, B);
+}
+
 // One node of one time step: pull, boundary functor, macroscopics, Guo-forced BGK, MRT or CM,
-// store. Requires 0 <= x < nx, 0 <= y < ny, 0 <= z < nz and nx*ny*nz < 2^31.
-template <int COLL, int STORE, bool BUF>
+// store. Requires 0 <= x < nx, 0 <= y < ny, 0 <= z < nz and nx*ny*nz < 2^31. GUO: the node may
+// carry a Guo inlet/outlet flag (the launch over the x = 0 and x = NX-1 planes).
+template <int COLL, int STORE, bool BUF, bool GUO = false>
 LBM3_HD void step3_node(const Step3Params& p, int x, int y, int z) {
     using St = Store3<STORE>;
     using T = typename St::T;
@@ -156,9 +223,14 @@ LBM3_HD void step3_node(const Step3Params& p, int x, int y, int z) {
     else
         pull_all<STORE, BUF, false>(g, p, src, node, ox, vx, oy, vy, oz, vz, std::make_integer_sequence<int, Q>{});
 
-    if (p.flags && p.flags[node] == LBM_BOUNCE_BACK) {
+    const uint8_t fl = p.flags ? p.flags[node] : uint8_t(LBM_FLUID);
+    if (fl == LBM_BOUNCE_BACK) {
         const Box3 B{p.nx, p.ny, p.nz, p.px, p.py, p.pz};
         bc_bounce_back3(g, x, y, z, B);
+    }
+    if (GUO) {
+        if (fl == LBM_GUO_VELOCITY_INLET) apply_guo<STORE, BUF, true>(p, src, g, x, y, z);
+        else if (fl == LBM_GUO_PRESSURE_OUTLET) apply_guo<STORE, BUF, false>(p, src, g, x, y, z);
     }
     float fx = p.fx, fy = p.fy, fz = p.fz;
     if (p.force) {

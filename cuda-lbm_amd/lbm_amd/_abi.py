@@ -56,6 +56,7 @@ LBM_QUIRK_IBM_CLAMP = 1 << 4
 LBM_QUIRK_IBM_NO_DS = 1 << 5
 LBM_QUIRK_EQ3D_LINEAR = 1 << 6   # include/lbm3d_hip.h
 LBM_QUIRK_MRT3_LITERAL = 1 << 7
+LBM_QUIRK_GUO3D_OWN_MOMENTS = 1 << 8
 
 # reductions
 LBM_REDUCE_MASS = 0
@@ -106,6 +107,7 @@ class Lbm3dConfig(ctypes.Structure):
         ("periodic_y", ctypes.c_int32),
         ("periodic_z", ctypes.c_int32),
         ("force", ctypes.c_float * 3),
+        ("u_inlet", ctypes.c_float),
         ("quirks", ctypes.c_uint32),
         ("reserved", ctypes.c_int32 * 8),
     ]

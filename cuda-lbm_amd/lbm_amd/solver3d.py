@@ -34,6 +34,7 @@ class Config3D:
     periodic_y: bool = False
     periodic_z: bool = False
     force: tuple = (0.0, 0.0, 0.0)
+    u_inlet: float = 0.0
     quirks: int = 0
 
     def to_c(self) -> A.Lbm3dConfig:
@@ -49,6 +50,7 @@ class Config3D:
         c.periodic_x, c.periodic_y, c.periodic_z = int(self.periodic_x), int(self.periodic_y), int(self.periodic_z)
         for k in range(3):
             c.force[k] = self.force[k]
+        c.u_inlet = self.u_inlet
         c.quirks = self.quirks
         return c
 

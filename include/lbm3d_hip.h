@@ -14,7 +14,8 @@
  * raw-moment basis (src/core/collision/MRT/MRT.cu:79-456, basis src/core/lbm_constants.cuh:218-246),
  * CM<3, NoAdapter> (src/core/collision/CM/CM_D3Q27.cuh:1-105), the FLUID and BOUNCE_BACK flags
  * (src/core/boundaries/boundaries.cuh:89-138, BounceBack<3>
- * src/functors/boundaryConditions/bbDomainBoundary.cuh:52-90), runtime periodic axes, FP32 or
+ * src/functors/boundaryConditions/bbDomainBoundary.cuh:52-90), GUO_VELOCITY_INLET at x = 0 and
+ * GUO_PRESSURE_OUTLET at x = NX-1 (guoInlet3D.cuh, guoOutlet3D.cuh), runtime periodic axes, FP32 or
  * 16-bit shifted population storage. The CM<3> adapters (ACM_D3Q27.cuh, E_ACM_D3Q27.cuh) and the
  * other 3-D boundary functors are rejected with LBM_E_UNSUPPORTED.
  */
@@ -44,6 +45,13 @@ extern "C" {
  * compute_m_post_combined (MRT.cu:203-395), leaves moments 8-26 unset and is not restated. */
 #define LBM_QUIRK_MRT3_LITERAL (1u << 7)
 
+/* Reference-literal Guo boundary moments (bit in lbm3d_config.quirks). GuoVelocityInlet3D and
+ * GuoPressureOutlet3D (src/functors/boundaryConditions/guoInlet3D.cuh:8-60, guoOutlet3D.cuh:6-66)
+ * name the interior neighbour fluid_node but compute its rho_f, u_f from the boundary node's
+ * own post-stream populations (unknown ones included); only f_neq = f(neighbour) - f_eq(rho_f,
+ * u_f) reads the neighbour. Default (0): rho_f, u_f of the neighbour. */
+#define LBM_QUIRK_GUO3D_OWN_MOMENTS (1u << 8)
+
 /* CM<3> adapters (src/core/collision/adapters.cuh): LBM_NO_ADAPTER is built. LBM_OPTIMAL_ADAPTER
  * (CM<3, OptimalAdapter>, src/core/collision/CM/ACM_D3Q27.cuh) and LBM3D_EMPIRICAL_ADAPTER
  * (E_ACM_D3Q27.cuh) are rejected with LBM_E_UNSUPPORTED: as shipped, both assign instead of
@@ -66,7 +74,9 @@ typedef struct lbm3d_config {
     int32_t  periodic_y;     /*   src/core/streaming/streaming.cu:45-53, never defined there) */
     int32_t  periodic_z;
     float    force[3];       /* uniform Guo body force, used where no force field is set      */
-    uint32_t quirks;         /* LBM_QUIRK_STALE_REST | LBM_QUIRK_EQ3D_LINEAR | LBM_QUIRK_MRT3_LITERAL */
+    float    u_inlet;        /* Scenario::u_max of GUO_VELOCITY_INLET (boundaries.cuh:123)     */
+    uint32_t quirks;         /* LBM_QUIRK_STALE_REST | LBM_QUIRK_EQ3D_LINEAR | LBM_QUIRK_MRT3_LITERAL
+                                | LBM_QUIRK_GUO3D_OWN_MOMENTS                                    */
     int32_t  reserved[8];
 } lbm3d_config;
 
@@ -83,8 +93,12 @@ int lbm3d_destroy(lbm3d_ctx* ctx);
 /* Run on `hip_stream` (a hipStream_t; NULL = the HIP null stream). */
 int lbm3d_set_stream(lbm3d_ctx* ctx, void* hip_stream);
 
-/* Replaces setup_boundary_flags (boundaries.cuh:170-197) for 3-D: one flag per node,
- * LBM_FLUID or LBM_BOUNCE_BACK; other flags give LBM_E_UNSUPPORTED. NULL = all FLUID. */
+/* Replaces setup_boundary_flags (boundaries.cuh:170-197) for 3-D: one flag per node, LBM_FLUID,
+ * LBM_BOUNCE_BACK, LBM_GUO_VELOCITY_INLET (x = 0 only) or LBM_GUO_PRESSURE_OUTLET (x = NX-1 only,
+ * both need a non-periodic x axis); other flags give LBM_E_UNSUPPORTED. NULL = all FLUID. With
+ * Guo flags a step is two launches: the inlet/outlet planes first (they read their neighbours'
+ * post-stream populations, including slots the neighbours' own update overwrites), then the
+ * rest. */
 int lbm3d_set_flags(lbm3d_ctx* ctx, const uint8_t* flags, int on_device);
 
 /* Replaces init_kernel<3> + init_node (src/core/init/init.cuh:10-43): f = f_back = f_eq.

@@ -420,21 +420,89 @@ struct Solver3 {
         }
     }
 
-    // Flags restated here: FLUID and BOUNCE_BACK.
+    // Flags restated here: FLUID, BOUNCE_BACK, GUO_VELOCITY_INLET (x = 0) and
+    // GUO_PRESSURE_OUTLET (x = NX-1), the placement of the reference's wind tunnel
+    // (windTestScenario.cuh:13-28).
+    bool has_guo = false;
     void set_flags(const uint8_t* fl) {
-        for (size_t n = 0; n < N; n++)
-            if (fl[n] != LBM_FLUID && fl[n] != LBM_BOUNCE_BACK) throw std::runtime_error("3-D flag not restated");
+        has_guo = false;
+        for (int z = 0; z < nz; z++)
+            for (int y = 0; y < ny; y++)
+                for (int x = 0; x < nx; x++) {
+                    const uint8_t v = fl[node(x, y, z)];
+                    if (v == LBM_FLUID || v == LBM_BOUNCE_BACK) continue;
+                    if ((v == LBM_GUO_VELOCITY_INLET && x == 0) || (v == LBM_GUO_PRESSURE_OUTLET && x == nx - 1)) {
+                        if (cfg.periodic_x) throw std::runtime_error("Guo inlet/outlet need a non-periodic x axis");
+                        has_guo = true;
+                        continue;
+                    }
+                    throw std::runtime_error("3-D flag not restated");
+                }
         flags.assign(fl, fl + N);
     }
 
-    // boundaries_kernel_3D (boundaries.cuh:89-138) for the FLUID and BOUNCE_BACK flags.
+    // GuoVelocityInlet3D::apply (guoInlet3D.cuh:8-60) and GuoPressureOutlet3D::apply
+    // (guoOutlet3D.cuh:6-66): the boundary state (rho_b, u_b) from the known populations, then
+    // the unknown ones (c_x > 0 at the inlet, < 0 at the outlet) = f_eq(rho_b, u_b) + the
+    // neighbour's non-equilibrium part f(x -+ 1) - f_eq(rho_f, u_f). The neighbour is read from
+    // `snap`, the post-stream field before any boundary functor (the reference's functors run
+    // concurrently, and one may read a neighbour another is rewriting). rho_f, u_f are the
+    // neighbour's, or the node's own under LBM_QUIRK_GUO3D_OWN_MOMENTS.
+    void bc_guo(size_t n, int x, int y, int z, bool inlet, const std::vector<R>& snap) {
+        R g[Q], gn[Q];
+        const size_t nb = node(inlet ? x + 1 : x - 1, y, z);
+        for (int q = 0; q < Q; q++) {
+            g[q] = f[q * N + n];
+            gn[q] = snap[q * N + nb];
+        }
+        R sum = 0;
+        for (int q = 0; q < Q; q++) {
+            if (inlet ? CX[q] < 0 : CX[q] > 0) sum += R(2) * g[q];
+            else if (CX[q] == 0) sum += g[q];
+        }
+        R rb, uxb, uyb = 0, uzb = 0;
+        if (inlet) {
+            uxb = R(cfg.u_inlet);
+            rb = sum / (R(1) - uxb);
+        } else {
+            rb = R(1);
+            uxb = sum / rb - R(1);
+            R jy = 0, jz = 0;
+            for (int q = 0; q < Q; q++)
+                if (CX[q] >= 0) {
+                    jy += g[q] * R(CY[q]);
+                    jz += g[q] * R(CZ[q]);
+                }
+            uyb = jy / rb;
+            uzb = jz / rb;
+        }
+        R feqb[Q], feqf[Q];
+        equilibrium<R>(rb, uxb, uyb, uzb, feqb, eq_literal);
+        const R* mom = (cfg.quirks & LBM_QUIRK_GUO3D_OWN_MOMENTS) ? g : gn;
+        R rf = 0, jxf = 0, jyf = 0, jzf = 0;
+        for (int q = 0; q < Q; q++) {
+            rf += mom[q];
+            jxf += mom[q] * R(CX[q]);
+            jyf += mom[q] * R(CY[q]);
+            jzf += mom[q] * R(CZ[q]);
+        }
+        equilibrium<R>(rf, jxf / rf, jyf / rf, jzf / rf, feqf, eq_literal);
+        for (int q = 0; q < Q; q++)
+            if (inlet ? CX[q] > 0 : CX[q] < 0) f[q * N + n] = feqb[q] + (gn[q] - feqf[q]);
+    }
+
+    // boundaries_kernel_3D (boundaries.cuh:89-138) for the restated flags.
     void apply_boundaries() {
+        std::vector<R> snap;
+        if (has_guo) snap = f;
 #pragma omp parallel for collapse(2) schedule(static)
         for (int z = 0; z < nz; z++)
             for (int y = 0; y < ny; y++)
                 for (int x = 0; x < nx; x++) {
                     const size_t n = node(x, y, z);
                     if (flags[n] == LBM_BOUNCE_BACK) bc_bounce_back(n, x, y, z);
+                    else if (flags[n] == LBM_GUO_VELOCITY_INLET) bc_guo(n, x, y, z, true, snap);
+                    else if (flags[n] == LBM_GUO_PRESSURE_OUTLET) bc_guo(n, x, y, z, false, snap);
                 }
     }
 

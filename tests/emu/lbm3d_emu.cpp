@@ -51,6 +51,12 @@ int run3(const lbm3d_config& c, const float* f_in, const uint8_t* flags, const f
     p.lin_ec = (c.quirks & LBM_QUIRK_EQ3D_LINEAR) ? 1.0f : 3.0f;
     for (int k = 0; k < 5; k++) p.mrt.s[k] = c.S[4 + k];
     p.mrt.literal = (c.quirks & LBM_QUIRK_MRT3_LITERAL) ? 1 : 0;
+    p.u_inlet = c.u_inlet;
+    p.guo_own = (c.quirks & LBM_QUIRK_GUO3D_OWN_MOMENTS) ? 1 : 0;
+    bool guo = false;
+    if (flags)
+        for (long long n = 0; n < N; n++) guo = guo || flags[n] == LBM_GUO_VELOCITY_INLET || flags[n] == LBM_GUO_PRESSURE_OUTLET;
+    p.skip_x_edges = guo ? 1 : 0;
     p.stale_rest = (c.quirks & LBM_QUIRK_STALE_REST) ? 1 : 0;
     p.wx = wave_cols(c.nx);
     p.nbx = (c.nx + 64 * p.wx - 1) / (64 * p.wx);
@@ -67,12 +73,23 @@ int run3(const lbm3d_config& c, const float* f_in, const uint8_t* flags, const f
         // lanes return; each node must be visited exactly once
         const long long nb = (long long)p.nbx * p.nby * p.nz, per = nb / 8;
         std::vector<uint8_t> seen(size_t(N), 0);
+        if (guo) {  // the planes launch first, as on the GPU
+            const long long pn = (long long)p.ny * p.nz;
+            for (long long i = 0; i < 2 * pn; i++) {
+                const int side = int(i / pn);
+                const long long r = i - side * pn;
+                const int x = side ? p.nx - 1 : 0, y = int(r % p.ny), z = int(r / p.ny);
+                if (seen[size_t((z * p.ny + y) * p.nx + x)]++) std::abort();
+                step3_node<COLL, STORE, false, true>(p, x, y, z);
+            }
+        }
         for (long long b = 0; b < nb; b++) {
             const long long L = b < per * 8 ? (b % 8) * per + b / 8 : b;
             for (int t = 0; t < 256; t++) {
                 int x, y, z;
                 thread_node(p, L, t, x, y, z);
                 if (x >= p.nx || y >= p.ny) continue;
+                if (p.skip_x_edges && (x == 0 || x == p.nx - 1)) continue;
                 if (z < 0 || z >= p.nz || x < 0 || y < 0) std::abort();
                 if (seen[size_t((z * p.ny + y) * p.nx + x)]++) std::abort();
                 step3_node<COLL, STORE, false>(p, x, y, z);
@@ -135,13 +152,15 @@ int main() {
     for (const auto& s : shapes)
         for (int per = 0; per < 8; per++)
             for (int st = 0; st <= 2; st++)
-                for (int var = 0; var < 5; var++) {
+                for (int var = 0; var < 6; var++) {
+                    if (var == 5 && (per & 1)) continue;  // Guo flags need a non-periodic x axis
                     lbm3d_config c;
                     std::memset(&c, 0, sizeof c);
                     c.nx = s[0];
                     c.ny = s[1];
                     c.nz = s[2];
-                    c.collision = var == 3 ? LBM_MRT : (var == 4 ? LBM_CM : LBM_BGK);
+                    c.collision = var == 3 ? LBM_MRT : (var == 4 || var == 5 ? LBM_CM : LBM_BGK);
+                    c.u_inlet = 0.03f;
                     c.omega = 1.3f;
                     for (int k = 0; k < 27; k++) c.S[k] = k >= 4 && k <= 8 ? 1.3f : 1.0f;
                     c.rho0 = 1.0f;
@@ -150,7 +169,8 @@ int main() {
                     c.periodic_y = (per >> 1) & 1;
                     c.periodic_z = (per >> 2) & 1;
                     c.force[0] = 1e-5f;
-                    c.quirks = var == 2 ? (LBM_QUIRK_STALE_REST | LBM_QUIRK_EQ3D_LINEAR) : (var == 3 ? LBM_QUIRK_MRT3_LITERAL : 0);
+                    c.quirks = var == 2 ? (LBM_QUIRK_STALE_REST | LBM_QUIRK_EQ3D_LINEAR)
+                                        : (var == 3 ? LBM_QUIRK_MRT3_LITERAL : (var == 5 && st == 1 ? LBM_QUIRK_GUO3D_OWN_MOMENTS : 0));
                     const long long N = (long long)c.nx * c.ny * c.nz;
                     std::vector<float> f(static_cast<size_t>(Q * N)), out(static_cast<size_t>(Q * N));
                     std::vector<float> rho(static_cast<size_t>(N)), u(static_cast<size_t>(3 * N));
@@ -161,7 +181,9 @@ int main() {
                         for (int y = 0; y < c.ny; y++)
                             for (int x = 0; x < c.nx; x++)
                                 if (x == 0 || y == 0 || z == 0 || x == c.nx - 1 || y == c.ny - 1 || z == c.nz - 1)
-                                    flags[size_t((z * c.ny + y) * c.nx + x)] = LBM_BOUNCE_BACK;
+                                    flags[size_t((z * c.ny + y) * c.nx + x)] =
+                                        var == 5 && x == 0 ? LBM_GUO_VELOCITY_INLET
+                                                           : (var == 5 && x == c.nx - 1 ? LBM_GUO_PRESSURE_OUTLET : LBM_BOUNCE_BACK);
                     std::vector<float> force(size_t(3 * N), 2e-6f);
                     emu3d_run(&c, f.data(), var >= 1 ? flags.data() : nullptr, var == 1 || var >= 3 ? force.data() : nullptr, 3,
                               out.data(), rho.data(), u.data());

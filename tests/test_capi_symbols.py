@@ -48,8 +48,9 @@ def test_config_struct_layout_matches_header():
     # 5 int32 + omega + S[9] + rho0 + storage + 2 periodic + force[2] + wall_u[2] + u_lid + quirks
     # + steps_per_launch + reserved[7]
     assert ctypes.sizeof(_abi.LbmConfig) == 4 * (5 + 1 + 9 + 1 + 1 + 2 + 2 + 2 + 1 + 1 + 8)
-    # nx, ny, nz, collision, adapter, omega, S[27], rho0, storage, 3 periodic, force[3], quirks, reserved[8]
-    assert ctypes.sizeof(_abi.Lbm3dConfig) == 4 * (5 + 1 + 27 + 1 + 1 + 3 + 3 + 1 + 8)
+    # nx, ny, nz, collision, adapter, omega, S[27], rho0, storage, 3 periodic, force[3], u_inlet, quirks,
+    # reserved[8]
+    assert ctypes.sizeof(_abi.Lbm3dConfig) == 4 * (5 + 1 + 27 + 1 + 1 + 3 + 3 + 1 + 1 + 8)
 
 
 def test_version_and_defaults_without_gpu():

@@ -7,8 +7,8 @@ import oracle as O
 from lbm_amd import _abi as A
 from lbm_amd.solver import LbmError
 from lbm_amd.solver3d import Config3D, Solver3D
-from test_emu3d_parity import (CASES, CM_CASES, MRT_CASES, box_flags, cm_case, initial_state, make_cfg, mrt_case,
-                               oracle_run)
+from test_emu3d_parity import (CASES, CM_CASES, GUO_CASES, MRT_CASES, box_flags, cm_case, guo_case, initial_state,
+                               make_cfg, mrt_case, oracle_run)
 
 pytestmark = pytest.mark.gpu
 W = O.tables3d()[0]
@@ -18,7 +18,50 @@ def gpu_config(c):
     return Config3D(c.nx, c.ny, c.nz, collision=c.collision, adapter=c.adapter, omega=c.omega, S=tuple(c.S),
                     storage=c.storage,
                     periodic_x=bool(c.periodic_x), periodic_y=bool(c.periodic_y), periodic_z=bool(c.periodic_z),
-                    force=tuple(c.force), quirks=c.quirks)
+                    force=tuple(c.force), u_inlet=c.u_inlet, quirks=c.quirks)
+
+
+@pytest.mark.parametrize("case", range(len(GUO_CASES)))
+def test_gpu_guo_inlet_outlet_matches_oracle(gpu_available, case):
+    c, f0, fl = guo_case(case)
+    f0 = f0.astype(np.float32)
+    o = oracle_run(c, f0, fl, None, 8)
+    with Solver3D(gpu_config(c)) as s:
+        s.set_pops(f0)
+        s.set_flags(fl)
+        s.step(8)
+        g = s.get_pops()
+        rho, u = s.get_macros()
+    assert np.abs(g - o.get_pops()).max() < 1e-5
+    r_rho, r_u = o.get_macros()
+    assert np.abs(rho - r_rho).max() < 1e-5
+    assert np.abs(u - r_u).max() < 1e-6
+
+
+def test_gpu_wind_tunnel_runs_and_carries_the_inlet_flux(gpu_available):
+    """The reference's default 3-D case (windTestScenario.cuh: 256x128x96, CM<3, NoAdapter>, Guo
+    velocity inlet u = 0.05 at x = 0, pressure outlet at x = NX-1, bounce-back on y and z; no IBM
+    body): 400 steps from the plug-flow initial state stay finite, the inlet plane keeps
+    u_x = 0.05 on average and the flow stays a plug flow within 10 % away from the walls."""
+    nx, ny, nz = 256, 128, 96
+    nu = 0.05 * ny / 1000.0
+    omega = 1.0 / (3.0 * nu + 0.5)
+    fl = np.zeros((nz, ny, nx), np.uint8)
+    fl[:, 0, :] = fl[:, -1, :] = A.BOUNCE_BACK
+    fl[0], fl[-1] = A.BOUNCE_BACK, A.BOUNCE_BACK
+    fl[:, :, 0] = A.GUO_VELOCITY_INLET
+    fl[:, :, -1] = A.GUO_PRESSURE_OUTLET
+    u0 = np.zeros((nz, ny, nx, 3), np.float32)
+    u0[..., 0] = 0.05
+    with Solver3D(Config3D(nx, ny, nz, collision=A.LBM_CM, omega=omega, u_inlet=0.05)) as s:
+        s.set_flags(fl)
+        s.init_from_macros(np.ones((nz, ny, nx), np.float32), u0)
+        s.step(400)
+        rho, u = s.get_macros()
+    assert np.isfinite(u).all() and np.isfinite(rho).all()
+    core = u[8:-8, 8:-8, 1:-1, 0]
+    assert abs(core.mean() / 0.05 - 1) < 0.1
+    assert abs(rho.mean() - 1) < 0.05
 
 
 @pytest.mark.parametrize("case", range(len(CM_CASES)))
@@ -222,6 +265,12 @@ def test_gpu_d3q27_errors_are_reported(gpu_available):
     with pytest.raises(LbmError) as e:
         Solver3D(Config3D(8, 8, 8, collision=A.LBM_CM, adapter=A.LBM_OPTIMAL_ADAPTER))
     assert e.value.code == A.LBM_E_UNSUPPORTED
+    with Solver3D(Config3D(8, 8, 8)) as s:
+        fl = np.zeros((8, 8, 8), np.uint8)
+        fl[:, :, 3] = A.GUO_VELOCITY_INLET  # inlets are built at x = 0 only
+        with pytest.raises(LbmError) as e:
+            s.set_flags(fl)
+        assert e.value.code == A.LBM_E_UNSUPPORTED
     with pytest.raises(LbmError) as e:
         Solver3D(Config3D(2, 8, 8))
     assert e.value.code == A.LBM_E_INVALID

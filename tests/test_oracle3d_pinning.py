@@ -416,3 +416,63 @@ def test_cm3_taylor_green_decays_at_the_viscous_rate():
     o.step(60)
     e1 = energy()
     assert abs(-np.log(e1 / e0) / 60 / (6 * nu * k * k) - 1) < 0.02
+
+
+# ---- Guo velocity inlet / pressure outlet ---------------------------------------------------
+
+def guo_flags(nx, ny, nz, walls):
+    fl = np.zeros((nz, ny, nx), np.uint8)
+    if walls:
+        fl[:, 0, :] = fl[:, -1, :] = _abi.BOUNCE_BACK
+        fl[0], fl[-1] = _abi.BOUNCE_BACK, _abi.BOUNCE_BACK
+    fl[:, :, 0] = _abi.GUO_VELOCITY_INLET
+    fl[:, :, -1] = _abi.GUO_PRESSURE_OUTLET
+    return fl
+
+
+@pytest.mark.parametrize("quirk", [0, _abi.LBM_QUIRK_GUO3D_OWN_MOMENTS])
+def test_uniform_flow_through_guo_inlet_and_outlet_is_a_fixed_point(quirk):
+    nx, ny, nz = 7, 4, 5
+    uin = 0.05
+    c = cfg3(nx, ny, nz, omega=1.3, periodic=(0, 1, 1), quirks=quirk)
+    c.u_inlet = uin
+    o = O.Oracle3D(c, 64)
+    o.set_flags(guo_flags(nx, ny, nz, False))
+    u = np.zeros((nz, ny, nx, 3))
+    u[..., 0] = np.float32(uin)
+    o.init_from_macros(np.ones((nz, ny, nx)), u)
+    f0 = o.get_pops()
+    o.step(25)
+    assert np.abs(o.get_pops() - f0).max() < 1e-14
+
+
+def test_flow_from_rest_between_guo_inlet_and_outlet_converges_to_the_inlet_velocity():
+    """Fluid at rest, inlet u = 0.04 at x = 0, pressure outlet (rho = 1) at x = NX-1, periodic y
+    and z: the only steady state both boundaries admit is rho = 1, u = (0.04, 0, 0) everywhere,
+    and the flow converges to it."""
+    nx, ny, nz = 8, 3, 3
+    c = cfg3(nx, ny, nz, omega=1.0, periodic=(0, 1, 1))
+    c.u_inlet = 0.04
+    o = O.Oracle3D(c, 64)
+    o.set_flags(guo_flags(nx, ny, nz, False))
+    o.init_from_macros(np.ones((nz, ny, nx)), np.zeros((nz, ny, nx, 3)))
+    o.step(3000)
+    rho, u = o.get_macros()
+    assert np.abs(u[..., 0] - np.float32(0.04)).max() < 1e-9
+    assert np.abs(rho - 1).max() < 1e-9 and np.abs(u[..., 1:]).max() < 1e-15
+
+
+def test_guo_own_moments_quirk_changes_the_result():
+    nx, ny, nz = 6, 4, 4
+    rng = np.random.default_rng(9)
+    out = []
+    for quirk in (0, _abi.LBM_QUIRK_GUO3D_OWN_MOMENTS):
+        c = cfg3(nx, ny, nz, omega=1.3, periodic=(0, 1, 1), quirks=quirk)
+        c.u_inlet = 0.03
+        o = O.Oracle3D(c, 64)
+        o.set_flags(guo_flags(nx, ny, nz, False))
+        rr = np.random.default_rng(9)
+        o.init_from_macros(1 + 0.02 * rr.standard_normal((nz, ny, nx)), 0.02 * rr.standard_normal((nz, ny, nx, 3)))
+        o.step(3)
+        out.append(o.get_pops())
+    assert np.abs(out[0] - out[1]).max() > 1e-6
