@@ -32,7 +32,7 @@ size_t elem_bytes(int storage) { return storage == LBM_F32 ? 4 : 2; }
 // workgroups (neighbouring row segments, which share cache lines at their ends) run on one
 // XCD. The last nb % 8 workgroups keep their index. Eight wavefronts per SIMD (at most 64
 // VGPRs, no scratch) measured 1-2% faster than the compiler's own 70-VGPR allocation.
-template <int COLL, int STORE, bool BUF>
+template <int COLL, int STORE, bool BUF, bool GUO>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void step3_kernel(Step3Params p) {
     const long long nb = (long long)p.nbx * p.nby * p.nz;
     const long long b = blockIdx.x;
@@ -41,21 +41,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void s
     int x, y, z;
     thread_node(p, L, int(threadIdx.x), x, y, z);
     if (x >= p.nx || y >= p.ny) return;
-    if (p.skip_x_edges && (x == 0 || x == p.nx - 1)) return;
-    step3_node<COLL, STORE, BUF>(p, x, y, z);
+    if (GUO && guo_ring(p, x, y, z)) return;
+    step3_node<COLL, STORE, BUF, GUO>(p, x, y, z);
 }
 
-// The x = 0 and x = NX-1 planes when they carry Guo inlet/outlet flags: thread i covers node
-// (side ? NX-1 : 0, y, z) of the 2 * NY * NZ plane nodes. Runs before step3_kernel, because
-// the Guo functors read their x +- 1 neighbours' stale slots, which that launch rewrites.
+// The ring nodes of the x = 0 and x = NX-1 planes (guo_ring) when Guo inlet/outlet flags are set:
+// thread i looks at node (side ? NX-1 : 0, y, z) of the 2 * NY * NZ plane nodes. Runs before
+// step3_kernel, because these nodes read their x +- 1 neighbours' stale slots, which that
+// launch rewrites.
 template <int COLL, int STORE, bool BUF>
-__global__ __launch_bounds__(256) void step3_planes_kernel(Step3Params p) {
+__global__ __launch_bounds__(256) void step3_ring_kernel(Step3Params p) {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     const long long per = (long long)p.ny * p.nz;
     if (i >= 2 * per) return;
     const int side = int(i / per);
     const long long r = i - side * per;
-    step3_node<COLL, STORE, BUF, true>(p, side ? p.nx - 1 : 0, int(r % p.ny), int(r / p.ny));
+    const int x = side ? p.nx - 1 : 0, y = int(r % p.ny), z = int(r / p.ny);
+    if (!guo_ring(p, x, y, z)) return;
+    step3_node<COLL, STORE, BUF, true>(p, x, y, z);
 }
 
 // init_kernel<3> + init_node (init.cuh:10-43): both buffers = f_eq(rho, u), u AoS [N][3].
@@ -259,7 +262,7 @@ int lbm3d_ctx::one_step(bool write_macros) {
     p.mrt.literal = (cfg.quirks & LBM_QUIRK_MRT3_LITERAL) ? 1 : 0;
     p.u_inlet = cfg.u_inlet;
     p.guo_own = (cfg.quirks & LBM_QUIRK_GUO3D_OWN_MOMENTS) ? 1 : 0;
-    p.skip_x_edges = has_guo ? 1 : 0;
+    p.guo = has_guo ? 1 : 0;
     p.stale_rest = (cfg.quirks & LBM_QUIRK_STALE_REST) ? 1 : 0;
     p.wx = wave_cols(cfg.nx);
     p.nbx = (cfg.nx + 64 * p.wx - 1) / (64 * p.wx);
@@ -272,11 +275,18 @@ int lbm3d_ctx::one_step(bool write_macros) {
     const dim3 pgrid{unsigned((nplane + 255) / 256)};
 #define LBM3_LAUNCH(C, S)                                                                       \
     if (has_guo) {                                                                              \
-        if (fits) hipLaunchKernelGGL((step3_planes_kernel<C, S, true>), pgrid, block, 0, stream, p);  \
-        else hipLaunchKernelGGL((step3_planes_kernel<C, S, false>), pgrid, block, 0, stream, p);      \
-    }                                                                                           \
-    if (fits) hipLaunchKernelGGL((step3_kernel<C, S, true>), grid, block, 0, stream, p);        \
-    else hipLaunchKernelGGL((step3_kernel<C, S, false>), grid, block, 0, stream, p);
+        if (fits) {                                                                             \
+            hipLaunchKernelGGL((step3_ring_kernel<C, S, true>), pgrid, block, 0, stream, p);    \
+            hipLaunchKernelGGL((step3_kernel<C, S, true, true>), grid, block, 0, stream, p);    \
+        } else {                                                                                \
+            hipLaunchKernelGGL((step3_ring_kernel<C, S, false>), pgrid, block, 0, stream, p);   \
+            hipLaunchKernelGGL((step3_kernel<C, S, false, true>), grid, block, 0, stream, p);   \
+        }                                                                                       \
+    } else if (fits) {                                                                          \
+        hipLaunchKernelGGL((step3_kernel<C, S, true, false>), grid, block, 0, stream, p);       \
+    } else {                                                                                    \
+        hipLaunchKernelGGL((step3_kernel<C, S, false, false>), grid, block, 0, stream, p);      \
+    }
 #define LBM3_LAUNCH_S(C)                                                                        \
     switch (cfg.storage) {                                                                      \
         case LBM_F32: LBM3_LAUNCH(C, LBM_F32) break;                                            \

@@ -38,7 +38,8 @@ struct Step3Params {
     Mrt3Rates mrt;         // MRT<3> rates S[4..8] and the literal flag
     float u_inlet;         // GUO_VELOCITY_INLET velocity
     int guo_own;           // LBM_QUIRK_GUO3D_OWN_MOMENTS
-    int skip_x_edges;      // the x = 0 and x = NX-1 planes run in their own launch (Guo flags)
+    int guo;               // Guo inlet/outlet flags present: their ring nodes run in a launch of
+                           // their own before the main launch, which skips them
     int stale_rest;        // LBM_QUIRK_STALE_REST
     // Workgroup shape: 256 threads = wx wavefronts along x (64*wx consecutive nodes of a row)
     // times 4/wx rows; nbx, nby workgroups along x and y, nz along z.
@@ -201,9 +202,18 @@ LBM3_HD void apply_guo(const Step3Params& p, const lbm::PopBuf<typename Store3<S
     bc_guo3<INLET>(g, gu, B);
 }
 
+// Guo plane nodes whose x +- 1 neighbour has upstream nodes outside a non-periodic y or z
+// boundary: the neighbour's pull then reads the neighbour's own stale slots in the destination
+// buffer, which the main launch rewrites, so these nodes run in a launch before it. Every other
+// Guo node reads its neighbour from the source buffer only.
+LBM3_HD bool guo_ring(const Step3Params& p, int x, int y, int z) {
+    if (x != 0 && x != p.nx - 1) return false;
+    return (!p.py && (y == 0 || y == p.ny - 1)) || (!p.pz && (z == 0 || z == p.nz - 1));
+}
+
 // One node of one time step: pull, boundary functor, macroscopics, Guo-forced BGK, MRT or CM,
 // store. Requires 0 <= x < nx, 0 <= y < ny, 0 <= z < nz and nx*ny*nz < 2^31. GUO: the node may
-// carry a Guo inlet/outlet flag (the launch over the x = 0 and x = NX-1 planes).
+// carry a Guo inlet/outlet flag.
 template <int COLL, int STORE, bool BUF, bool GUO = false>
 LBM3_HD void step3_node(const Step3Params& p, int x, int y, int z) {
     using St = Store3<STORE>;

@@ -56,7 +56,7 @@ int run3(const lbm3d_config& c, const float* f_in, const uint8_t* flags, const f
     bool guo = false;
     if (flags)
         for (long long n = 0; n < N; n++) guo = guo || flags[n] == LBM_GUO_VELOCITY_INLET || flags[n] == LBM_GUO_PRESSURE_OUTLET;
-    p.skip_x_edges = guo ? 1 : 0;
+    p.guo = guo ? 1 : 0;
     p.stale_rest = (c.quirks & LBM_QUIRK_STALE_REST) ? 1 : 0;
     p.wx = wave_cols(c.nx);
     p.nbx = (c.nx + 64 * p.wx - 1) / (64 * p.wx);
@@ -73,12 +73,13 @@ int run3(const lbm3d_config& c, const float* f_in, const uint8_t* flags, const f
         // lanes return; each node must be visited exactly once
         const long long nb = (long long)p.nbx * p.nby * p.nz, per = nb / 8;
         std::vector<uint8_t> seen(size_t(N), 0);
-        if (guo) {  // the planes launch first, as on the GPU
+        if (guo) {  // the ring launch first, as on the GPU
             const long long pn = (long long)p.ny * p.nz;
             for (long long i = 0; i < 2 * pn; i++) {
                 const int side = int(i / pn);
                 const long long r = i - side * pn;
                 const int x = side ? p.nx - 1 : 0, y = int(r % p.ny), z = int(r / p.ny);
+                if (!guo_ring(p, x, y, z)) continue;
                 if (seen[size_t((z * p.ny + y) * p.nx + x)]++) std::abort();
                 step3_node<COLL, STORE, false, true>(p, x, y, z);
             }
@@ -89,10 +90,11 @@ int run3(const lbm3d_config& c, const float* f_in, const uint8_t* flags, const f
                 int x, y, z;
                 thread_node(p, L, t, x, y, z);
                 if (x >= p.nx || y >= p.ny) continue;
-                if (p.skip_x_edges && (x == 0 || x == p.nx - 1)) continue;
+                if (guo && guo_ring(p, x, y, z)) continue;
                 if (z < 0 || z >= p.nz || x < 0 || y < 0) std::abort();
                 if (seen[size_t((z * p.ny + y) * p.nx + x)]++) std::abort();
-                step3_node<COLL, STORE, false>(p, x, y, z);
+                if (guo) step3_node<COLL, STORE, false, true>(p, x, y, z);
+                else step3_node<COLL, STORE, false>(p, x, y, z);
             }
         }
         for (long long n = 0; n < N; n++)
