@@ -45,10 +45,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void s
     step3_node<COLL, STORE, BUF, GUO>(p, x, y, z);
 }
 
-// The ring nodes of the x = 0 and x = NX-1 planes (guo_ring) when Guo inlet/outlet flags are set:
+// The outlet's ring nodes (guo_ring) when they do not share a wavefront with their neighbour:
 // thread i looks at node (side ? NX-1 : 0, y, z) of the 2 * NY * NZ plane nodes. Runs before
-// step3_kernel, because these nodes read their x +- 1 neighbours' stale slots, which that
-// launch rewrites.
+// step3_kernel, because these nodes read their x - 1 neighbours' stale slots, which that launch
+// rewrites.
 template <int COLL, int STORE, bool BUF>
 __global__ __launch_bounds__(256) void step3_ring_kernel(Step3Params p) {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -263,6 +263,7 @@ int lbm3d_ctx::one_step(bool write_macros) {
     p.u_inlet = cfg.u_inlet;
     p.guo_own = (cfg.quirks & LBM_QUIRK_GUO3D_OWN_MOMENTS) ? 1 : 0;
     p.guo = has_guo ? 1 : 0;
+    p.guo_ring_outlet = (has_guo && (cfg.nx - 1) % 64 == 0) ? 1 : 0;
     p.stale_rest = (cfg.quirks & LBM_QUIRK_STALE_REST) ? 1 : 0;
     p.wx = wave_cols(cfg.nx);
     p.nbx = (cfg.nx + 64 * p.wx - 1) / (64 * p.wx);
@@ -276,10 +277,12 @@ int lbm3d_ctx::one_step(bool write_macros) {
 #define LBM3_LAUNCH(C, S)                                                                       \
     if (has_guo) {                                                                              \
         if (fits) {                                                                             \
-            hipLaunchKernelGGL((step3_ring_kernel<C, S, true>), pgrid, block, 0, stream, p);    \
+            if (p.guo_ring_outlet)                                                              \
+                hipLaunchKernelGGL((step3_ring_kernel<C, S, true>), pgrid, block, 0, stream, p); \
             hipLaunchKernelGGL((step3_kernel<C, S, true, true>), grid, block, 0, stream, p);    \
         } else {                                                                                \
-            hipLaunchKernelGGL((step3_ring_kernel<C, S, false>), pgrid, block, 0, stream, p);   \
+            if (p.guo_ring_outlet)                                                              \
+                hipLaunchKernelGGL((step3_ring_kernel<C, S, false>), pgrid, block, 0, stream, p); \
             hipLaunchKernelGGL((step3_kernel<C, S, false, true>), grid, block, 0, stream, p);   \
         }                                                                                       \
     } else if (fits) {                                                                          \

@@ -38,8 +38,9 @@ struct Step3Params {
     Mrt3Rates mrt;         // MRT<3> rates S[4..8] and the literal flag
     float u_inlet;         // GUO_VELOCITY_INLET velocity
     int guo_own;           // LBM_QUIRK_GUO3D_OWN_MOMENTS
-    int guo;               // Guo inlet/outlet flags present: their ring nodes run in a launch of
-                           // their own before the main launch, which skips them
+    int guo;               // Guo inlet/outlet flags present
+    int guo_ring_outlet;   // (NX-1) % 64 == 0: the outlet's ring nodes run in a launch of their
+                           // own before the main launch, which skips them
     int stale_rest;        // LBM_QUIRK_STALE_REST
     // Workgroup shape: 256 threads = wx wavefronts along x (64*wx consecutive nodes of a row)
     // times 4/wx rows; nbx, nby workgroups along x and y, nz along z.
@@ -202,12 +203,15 @@ LBM3_HD void apply_guo(const Step3Params& p, const lbm::PopBuf<typename Store3<S
     bc_guo3<INLET>(g, gu, B);
 }
 
-// Guo plane nodes whose x +- 1 neighbour has upstream nodes outside a non-periodic y or z
-// boundary: the neighbour's pull then reads the neighbour's own stale slots in the destination
-// buffer, which the main launch rewrites, so these nodes run in a launch before it. Every other
-// Guo node reads its neighbour from the source buffer only.
+// A Guo node pulls its x +- 1 neighbour; on the ring of the plane (y or z at a non-periodic
+// edge) that pull reads the neighbour's own stale slots in the destination buffer, which the
+// neighbour's update rewrites. When both nodes are lanes of one wavefront that is safe: the
+// wavefront issues its stores only after the loads they depend on, and both lanes store through
+// the same instructions. The inlet (x = 0, neighbour x = 1) always shares a wavefront with its
+// neighbour; the outlet (x = NX-1) does unless (NX-1) % 64 == 0, and then its ring nodes run in a
+// launch of their own before the main launch.
 LBM3_HD bool guo_ring(const Step3Params& p, int x, int y, int z) {
-    if (x != 0 && x != p.nx - 1) return false;
+    if (!p.guo_ring_outlet || x != p.nx - 1) return false;
     return (!p.py && (y == 0 || y == p.ny - 1)) || (!p.pz && (z == 0 || z == p.nz - 1));
 }
 

@@ -57,6 +57,7 @@ int run3(const lbm3d_config& c, const float* f_in, const uint8_t* flags, const f
     if (flags)
         for (long long n = 0; n < N; n++) guo = guo || flags[n] == LBM_GUO_VELOCITY_INLET || flags[n] == LBM_GUO_PRESSURE_OUTLET;
     p.guo = guo ? 1 : 0;
+    p.guo_ring_outlet = (guo && (c.nx - 1) % 64 == 0) ? 1 : 0;
     p.stale_rest = (c.quirks & LBM_QUIRK_STALE_REST) ? 1 : 0;
     p.wx = wave_cols(c.nx);
     p.nbx = (c.nx + 64 * p.wx - 1) / (64 * p.wx);
@@ -86,16 +87,23 @@ int run3(const lbm3d_config& c, const float* f_in, const uint8_t* flags, const f
         }
         for (long long b = 0; b < nb; b++) {
             const long long L = b < per * 8 ? (b % 8) * per + b / 8 : b;
-            for (int t = 0; t < 256; t++) {
-                int x, y, z;
-                thread_node(p, L, t, x, y, z);
-                if (x >= p.nx || y >= p.ny) continue;
-                if (guo && guo_ring(p, x, y, z)) continue;
-                if (z < 0 || z >= p.nz || x < 0 || y < 0) std::abort();
-                if (seen[size_t((z * p.ny + y) * p.nx + x)]++) std::abort();
-                if (guo) step3_node<COLL, STORE, false, true>(p, x, y, z);
-                else step3_node<COLL, STORE, false>(p, x, y, z);
-            }
+            // a wavefront's loads precede its stores: within each wavefront the Guo lanes (which
+            // read a neighbouring lane's stale slots) run before the other lanes
+            for (int wv = 0; wv < 4; wv++)
+                for (int pass = 0; pass < 2; pass++)
+                    for (int t = 64 * wv; t < 64 * wv + 64; t++) {
+                        int x, y, z;
+                        thread_node(p, L, t, x, y, z);
+                        if (x >= p.nx || y >= p.ny) continue;
+                        if (guo && guo_ring(p, x, y, z)) continue;
+                        if (z < 0 || z >= p.nz || x < 0 || y < 0) std::abort();
+                        const uint8_t fl = flags ? flags[(z * p.ny + y) * p.nx + x] : uint8_t(LBM_FLUID);
+                        const bool gl = fl == LBM_GUO_VELOCITY_INLET || fl == LBM_GUO_PRESSURE_OUTLET;
+                        if ((pass == 0) != gl) continue;
+                        if (seen[size_t((z * p.ny + y) * p.nx + x)]++) std::abort();
+                        if (guo) step3_node<COLL, STORE, false, true>(p, x, y, z);
+                        else step3_node<COLL, STORE, false>(p, x, y, z);
+                    }
         }
         for (long long n = 0; n < N; n++)
             if (seen[size_t(n)] != 1) std::abort();
@@ -149,7 +157,8 @@ int emu3d_run(const lbm3d_config* c, const float* f_in, const uint8_t* flags, co
 // shapes below, at and above one wavefront row, every periodicity pattern, bounce-back on
 // all faces, a force field, every storage and both quirks.
 int main() {
-    const int shapes[][3] = {{3, 3, 3}, {5, 4, 3}, {64, 3, 4}, {65, 5, 3}, {130, 6, 4}, {7, 9, 11}, {193, 3, 5}, {260, 5, 3}};
+    const int shapes[][3] = {{3, 3, 3}, {5, 4, 3}, {64, 3, 4}, {65, 5, 3}, {130, 6, 4}, {7, 9, 11}, {193, 3, 5}, {260, 5, 3},
+                             {129, 4, 3}};
     int cases = 0;
     for (const auto& s : shapes)
         for (int per = 0; per < 8; per++)

@@ -205,6 +205,8 @@ GUO_CASES = [
     ((70, 5, 4), (0, 1), _abi.LBM_MRT, True, 0),
     ((8, 5, 5), (0, 0), _abi.LBM_CM, True, _abi.LBM_QUIRK_GUO3D_OWN_MOMENTS),
     ((8, 5, 5), (0, 0), _abi.LBM_BGK, True, _abi.LBM_QUIRK_EQ3D_LINEAR),
+    ((65, 5, 4), (0, 0), _abi.LBM_CM, True, 0),   # (NX-1) % 64 == 0: the outlet ring launch
+    ((129, 4, 3), (0, 1), _abi.LBM_BGK, True, 0),
 ]
 
 
