@@ -111,22 +111,28 @@ LBM_HD void collide_bgk(float* g, const Macro& m, float fx, float fy, float omeg
 // basis (lbm_constants.cuh:33-55) written out sparsely and m_eq in closed form.
 // Forcing moments per MRT.cu:4-24; `literal_rows` keeps its rows 4/5 swap (quirk Q10).
 // `forced` (uniform) = 0 skips the forcing moments, which are then all zero.
-LBM_HD void collide_mrt(float* g, const Macro& m, float fx, float fy, const float* S, bool literal_rows, bool forced) {
+// CONS: S[0] = S[3] = S[5] = 0 (density and momentum conserved, as DEFAULT_MRT_S_MATRIX and
+// every MRT scenario set them): those three moment changes are their forcing terms alone
+// (-0 (m - m_eq) + F = F exactly), so their m and m_eq are not formed.
+template <bool CONS>
+LBM_HD void collide_mrt_t(float* g, const Macro& m, float fx, float fy, const float* S, bool literal_rows, bool forced) {
     const float rho = m.rho, ux = m.ux, uy = m.uy;
     float a = g[1] + g[2] + g[3] + g[4];
     float d = g[5] + g[6] + g[7] + g[8];
     float px = g[1] - g[3], dxs = g[5] - g[6] - g[7] + g[8];
     float py = g[2] - g[4], dys = g[5] + g[6] - g[7] - g[8];
     float mm[Q];
-    mm[0] = g[0] + a + d;
     mm[1] = -4.0f * g[0] - a + 2.0f * d;
     mm[2] = 4.0f * g[0] - 2.0f * a + d;
-    mm[3] = px + dxs;
     mm[4] = -2.0f * px + dxs;
-    mm[5] = py + dys;
     mm[6] = -2.0f * py + dys;
     mm[7] = g[1] - g[2] + g[3] - g[4];
     mm[8] = g[5] - g[6] + g[7] - g[8];
+    if (!CONS) {
+        mm[0] = g[0] + a + d;
+        mm[3] = px + dxs;
+        mm[5] = py + dys;
+    }
     float usq = ux * ux + uy * uy;
     float meq[Q] = {rho,
                     rho * (-2.0f + 3.0f * usq),
@@ -139,7 +145,10 @@ LBM_HD void collide_mrt(float* g, const Macro& m, float fx, float fy, const floa
                     rho * ux * uy};
     float dm[Q];
 #pragma unroll
-    for (int k = 0; k < Q; k++) dm[k] = -S[k] * (mm[k] - meq[k]);
+    for (int k = 0; k < Q; k++) {
+        if (CONS && (k == 0 || k == 3 || k == 5)) dm[k] = 0.0f;
+        else dm[k] = -S[k] * (mm[k] - meq[k]);
+    }
     if (forced) {
         float fu = fx * ux + fy * uy;
         float F[Q] = {0.0f, 6.0f * fu, -6.0f * fu, fx, -fx, fy, -fy, 2.0f * (fx * ux - fy * uy), fx * uy + fy * ux};
@@ -148,11 +157,14 @@ LBM_HD void collide_mrt(float* g, const Macro& m, float fx, float fy, const floa
             F[5] = -fx;
         }
 #pragma unroll
-        for (int k = 0; k < Q; k++) dm[k] += (1.0f - 0.5f * S[k]) * F[k];
+        for (int k = 0; k < Q; k++) {
+            if (CONS && (k == 0 || k == 3 || k == 5)) dm[k] = F[k];
+            else dm[k] += (1.0f - 0.5f * S[k]) * F[k];
+        }
     }
     // M^-1 rows (lbm_constants.cuh:45-55)
     const float r9 = 1.0f / 9.0f, r18 = 1.0f / 18.0f, r36 = 1.0f / 36.0f, r6 = 1.0f / 6.0f, r12 = 1.0f / 12.0f;
-    float c0 = dm[0] * r9;
+    float c0 = CONS ? 0.0f : dm[0] * r9;
     float b1 = c0 - dm[1] * r36 - dm[2] * r18;
     float b2 = c0 + dm[1] * r18 + dm[2] * r36;
     float X = (dm[3] - dm[4]) * r6;
@@ -170,6 +182,10 @@ LBM_HD void collide_mrt(float* g, const Macro& m, float fx, float fy, const floa
     g[6] += b2 - XD + YD - Rr;
     g[7] += b2 - XD - YD + Rr;
     g[8] += b2 + XD - YD - Rr;
+}
+LBM_HD void collide_mrt(float* g, const Macro& m, float fx, float fy, const float* S, bool literal_rows, bool forced) {
+    if (S[0] == 0.0f && S[3] == 0.0f && S[5] == 0.0f) collide_mrt_t<true>(g, m, fx, fy, S, literal_rows, forced);
+    else collide_mrt_t<false>(g, m, fx, fy, S, literal_rows, forced);
 }
 
 // OptimalAdapter::compute_higher_order_relaxation (adapters.cuh:48-111), no printf.
