@@ -263,7 +263,7 @@ LBM3_HD void step3_node(const Step3Params& p, int x, int y, int z) {
     else if (COLL == LBM_CM) collide_cm3(g, m, fx, fy, fz, p.omega);
     else collide_bgk3(g, m, fx, fy, fz, p.omega, p.lin_ec);
 #pragma unroll
-    for (int q = 0; q < Q; q++) dst.st(node, (long long)q * p.plane, St::enc(g[q], q, p.rho0));
+    for (int q = 0; q < Q; q++) dst.st_nt(node, (long long)q * p.plane, St::enc(g[q], q, p.rho0));
 }
 
 }  // namespace lbm3

@@ -340,6 +340,23 @@ struct PopBuf {
 #endif
         __builtin_memcpy(b + (off + (long long)node) * (long long)sizeof(T), &v, sizeof(T));
     }
+    // st() with the non-temporal cache policy (aux bit 1: the line is not kept in L2 after
+    // the write), for populations that no later access of the same launch reads
+    LBM_HD void st_nt(uint32_t node, long long off, T v) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+        if constexpr (BUF) {
+            const int vo = int(node * uint32_t(sizeof(T))), so = int(uint32_t(off) * uint32_t(sizeof(T)));
+            if constexpr (sizeof(T) == 4)
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, vo, so, 2);
+            else
+                __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, v), r, vo, so, 2);
+            return;
+        }
+        __builtin_nontemporal_store(v, reinterpret_cast<T*>(b) + (off + (long long)node));
+        return;
+#endif
+        __builtin_memcpy(b + (off + (long long)node) * (long long)sizeof(T), &v, sizeof(T));
+    }
 };
 
 // Moment partial sums for the OptimalAdapter averages (FP64).
