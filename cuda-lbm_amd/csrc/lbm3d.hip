@@ -431,7 +431,7 @@ int lbm3d_set_flags(lbm3d_ctx* ctx, const uint8_t* flags, int on_device) {
         const int x = int(n % size_t(nx));
         if (h[n] == LBM_FLUID) continue;
         any = true;
-        if (h[n] == LBM_BOUNCE_BACK) continue;
+        if (h[n] == LBM_BOUNCE_BACK || h[n] == LBM_CORNER_EDGE_BOUNCE_BACK) continue;
         if ((h[n] == LBM_GUO_VELOCITY_INLET && x == 0) || (h[n] == LBM_GUO_PRESSURE_OUTLET && x == nx - 1)) {
             if (ctx->cfg.periodic_x) return ctx->fail(LBM_E_INVALID, "Guo inlet/outlet flags need a non-periodic x axis");
             guo = true;
@@ -439,7 +439,7 @@ int lbm3d_set_flags(lbm3d_ctx* ctx, const uint8_t* flags, int on_device) {
         }
         if (h[n] == LBM_GUO_VELOCITY_INLET || h[n] == LBM_GUO_PRESSURE_OUTLET)
             return ctx->fail(LBM_E_UNSUPPORTED, "flag %d at node %zu: Guo inlets are built at x = 0 and outlets at x = NX-1", int(h[n]), n);
-        return ctx->fail(LBM_E_UNSUPPORTED, "flag %d at node %zu: not built for D3Q27 (FLUID, BOUNCE_BACK, GUO_VELOCITY_INLET, GUO_PRESSURE_OUTLET are)", int(h[n]), n);
+        return ctx->fail(LBM_E_UNSUPPORTED, "flag %d at node %zu: not built for D3Q27 (FLUID, BOUNCE_BACK, CORNER_EDGE_BOUNCE_BACK, GUO_VELOCITY_INLET, GUO_PRESSURE_OUTLET are)", int(h[n]), n);
     }
     if (!any) return lbm3d_set_flags(ctx, nullptr, 0);
     ctx->has_guo = guo;

@@ -416,4 +416,22 @@ LBM3_HD void bc_bounce_back3(float* g, int x, int y, int z, const Box3& B) {
     bb_all(g, x, y, z, B, std::make_integer_sequence<int, Q - 1>{});
 }
 
+// EdgeCornerBounceBack::apply (edgeCornerBounceBack.cuh:10-49): on a node with at least two
+// domain faces (x, y, z at 0 or N-1, periodic or not), every direction with no negative and at
+// least one positive component copies g_i into g_opp(i). The functor computes the inward
+// normals only to count them; which populations it reflects does not depend on them. The 7
+// source directions and their 7 opposites are disjoint, so the loop order does not matter.
+template <int i>
+LBM3_HD void ceb_dir(float* g) {
+    if constexpr (CX[i] >= 0 && CY[i] >= 0 && CZ[i] >= 0 && (CX[i] + CY[i] + CZ[i]) > 0) g[OPP[i]] = g[i];
+}
+template <int... is>
+LBM3_HD void ceb_all(float* g, std::integer_sequence<int, is...>) {
+    (ceb_dir<is>(g), ...);
+}
+LBM3_HD void bc_corner_edge_bounce_back3(float* g, int x, int y, int z, const Box3& B) {
+    const int faces = (x == 0) + (x == B.nx - 1) + (y == 0) + (y == B.ny - 1) + (z == 0) + (z == B.nz - 1);
+    if (faces >= 2) ceb_all(g, std::make_integer_sequence<int, Q>{});
+}
+
 }  // namespace lbm3

@@ -26,7 +26,7 @@ namespace lbm3 {
 struct Step3Params {
     const void* src;       // post-collision populations of step t-1, [27][plane]
     void* dst;             // receives step t; holds step t-2 (stale slots)
-    const uint8_t* flags;  // per-node flag (FLUID / BOUNCE_BACK), or null (all FLUID)
+    const uint8_t* flags;  // per-node flag (FLUID / BOUNCE_BACK / CORNER_EDGE_BOUNCE_BACK / Guo), or null (all FLUID)
     const float* force;    // SoA [3][plane] force field, or null (uniform fx, fy, fz)
     float* out_rho;        // macroscopic outputs, rho [plane] and u SoA [3][plane], or null
     float* out_u;
@@ -241,6 +241,9 @@ LBM3_HD void step3_node(const Step3Params& p, int x, int y, int z) {
     if (fl == LBM_BOUNCE_BACK) {
         const Box3 B{p.nx, p.ny, p.nz, p.px, p.py, p.pz};
         bc_bounce_back3(g, x, y, z, B);
+    } else if (fl == LBM_CORNER_EDGE_BOUNCE_BACK) {
+        const Box3 B{p.nx, p.ny, p.nz, p.px, p.py, p.pz};
+        bc_corner_edge_bounce_back3(g, x, y, z, B);
     }
     if (GUO) {
         if (fl == LBM_GUO_VELOCITY_INLET) apply_guo<STORE, BUF, true>(p, src, g, x, y, z);

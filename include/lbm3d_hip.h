@@ -12,9 +12,10 @@
  *
  * Built: BGK<3> with Guo forcing (src/core/collision/BGK/BGK.cuh:55-96), MRT<3> in the De Rosis
  * raw-moment basis (src/core/collision/MRT/MRT.cu:79-456, basis src/core/lbm_constants.cuh:218-246),
- * CM<3, NoAdapter> (src/core/collision/CM/CM_D3Q27.cuh:1-105), the FLUID and BOUNCE_BACK flags
- * (src/core/boundaries/boundaries.cuh:89-138, BounceBack<3>
- * src/functors/boundaryConditions/bbDomainBoundary.cuh:52-90), GUO_VELOCITY_INLET at x = 0 and
+ * CM<3, NoAdapter> (src/core/collision/CM/CM_D3Q27.cuh:1-105), the FLUID, BOUNCE_BACK and
+ * CORNER_EDGE_BOUNCE_BACK flags (src/core/boundaries/boundaries.cuh:89-138, BounceBack<3>
+ * src/functors/boundaryConditions/bbDomainBoundary.cuh:52-90, EdgeCornerBounceBack
+ * src/functors/boundaryConditions/edgeCornerBounceBack.cuh:10-49), GUO_VELOCITY_INLET at x = 0 and
  * GUO_PRESSURE_OUTLET at x = NX-1 (guoInlet3D.cuh, guoOutlet3D.cuh), runtime periodic axes, FP32 or
  * 16-bit shifted population storage. The CM<3> adapters (ACM_D3Q27.cuh, E_ACM_D3Q27.cuh) and the
  * other 3-D boundary functors are rejected with LBM_E_UNSUPPORTED.
@@ -94,7 +95,8 @@ int lbm3d_destroy(lbm3d_ctx* ctx);
 int lbm3d_set_stream(lbm3d_ctx* ctx, void* hip_stream);
 
 /* Replaces setup_boundary_flags (boundaries.cuh:170-197) for 3-D: one flag per node, LBM_FLUID,
- * LBM_BOUNCE_BACK, LBM_GUO_VELOCITY_INLET (x = 0 only) or LBM_GUO_PRESSURE_OUTLET (x = NX-1 only,
+ * LBM_BOUNCE_BACK, LBM_CORNER_EDGE_BOUNCE_BACK (acts on nodes with two or more domain faces, like
+ * FLUID elsewhere), LBM_GUO_VELOCITY_INLET (x = 0 only) or LBM_GUO_PRESSURE_OUTLET (x = NX-1 only,
  * both need a non-periodic x axis); other flags give LBM_E_UNSUPPORTED. NULL = all FLUID. With
  * Guo flags a step is two launches: the inlet/outlet planes first (they read their neighbours'
  * post-stream populations, including slots the neighbours' own update overwrites), then the

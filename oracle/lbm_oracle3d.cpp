@@ -107,6 +107,16 @@ int oracle3d_stream_only(void* p) {
     });
 }
 
+// Streaming then the boundary functors (main.cu:98-99), no collision: the post-boundary
+// populations are read back with oracle3d_get_pops.
+int oracle3d_stream_boundaries(void* p) {
+    return dispatch3(static_cast<Handle3*>(p), [&](auto& s) {
+        s.stream();
+        std::swap(s.f, s.f_back);
+        s.apply_boundaries();
+    });
+}
+
 const char* oracle3d_last_error(void* p) { return static_cast<Handle3*>(p)->err.c_str(); }
 
 // ---- node-local building blocks, for the unit tests -------------------------------------

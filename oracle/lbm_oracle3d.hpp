@@ -420,7 +420,31 @@ struct Solver3 {
         }
     }
 
-    // Flags restated here: FLUID, BOUNCE_BACK, GUO_VELOCITY_INLET (x = 0) and
+    // EdgeCornerBounceBack::apply (edgeCornerBounceBack.cuh:10-49), loop as written: the node's
+    // inward normals are counted (x, y, z at 0 or N-1) and the functor returns below two; then
+    // for i = 0..26, i_out = (NX*cx > 0) || (NY*cy > 0) || (NZ*cz > 0) and o_out the same for
+    // opp(i), and i_out && !o_out copies f_i into f_opp(i).
+    void bc_corner_edge_bounce_back(size_t n, int x, int y, int z) {
+        int normals[3] = {0, 0, 0};
+        if (x == 0) normals[0] = 1;
+        else if (x == nx - 1) normals[0] = -1;
+        if (y == 0) normals[1] = 1;
+        else if (y == ny - 1) normals[1] = -1;
+        if (z == 0) normals[2] = 1;
+        else if (z == nz - 1) normals[2] = -1;
+        int num = 0;
+        for (int d = 0; d < 3; d++)
+            if (normals[d] != 0) num++;
+        if (num < 2) return;
+        for (int i = 0; i < Q; i++) {
+            const int o = OPP[i];
+            const bool i_out = (nx * CX[i] > 0) || (ny * CY[i] > 0) || (nz * CZ[i] > 0);
+            const bool o_out = (nx * CX[o] > 0) || (ny * CY[o] > 0) || (nz * CZ[o] > 0);
+            if (i_out && !o_out) f[o * N + n] = f[i * N + n];
+        }
+    }
+
+    // Flags restated here: FLUID, BOUNCE_BACK, CORNER_EDGE_BOUNCE_BACK, GUO_VELOCITY_INLET (x = 0) and
     // GUO_PRESSURE_OUTLET (x = NX-1), the placement of the reference's wind tunnel
     // (windTestScenario.cuh:13-28).
     bool has_guo = false;
@@ -430,7 +454,7 @@ struct Solver3 {
             for (int y = 0; y < ny; y++)
                 for (int x = 0; x < nx; x++) {
                     const uint8_t v = fl[node(x, y, z)];
-                    if (v == LBM_FLUID || v == LBM_BOUNCE_BACK) continue;
+                    if (v == LBM_FLUID || v == LBM_BOUNCE_BACK || v == LBM_CORNER_EDGE_BOUNCE_BACK) continue;
                     if ((v == LBM_GUO_VELOCITY_INLET && x == 0) || (v == LBM_GUO_PRESSURE_OUTLET && x == nx - 1)) {
                         if (cfg.periodic_x) throw std::runtime_error("Guo inlet/outlet need a non-periodic x axis");
                         has_guo = true;
@@ -501,6 +525,7 @@ struct Solver3 {
                 for (int x = 0; x < nx; x++) {
                     const size_t n = node(x, y, z);
                     if (flags[n] == LBM_BOUNCE_BACK) bc_bounce_back(n, x, y, z);
+                    else if (flags[n] == LBM_CORNER_EDGE_BOUNCE_BACK) bc_corner_edge_bounce_back(n, x, y, z);
                     else if (flags[n] == LBM_GUO_VELOCITY_INLET) bc_guo(n, x, y, z, true, snap);
                     else if (flags[n] == LBM_GUO_PRESSURE_OUTLET) bc_guo(n, x, y, z, false, snap);
                 }

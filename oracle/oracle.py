@@ -74,6 +74,7 @@ def lib():
             "oracle3d_get_pops": (ctypes.c_int, [_P, _D]),
             "oracle3d_set_pops": (ctypes.c_int, [_P, _D]),
             "oracle3d_stream_only": (ctypes.c_int, [_P]),
+            "oracle3d_stream_boundaries": (ctypes.c_int, [_P]),
             "oracle3d_last_error": (ctypes.c_char_p, [_P]),
             "oracle3d_tables": (None, [_D, _P, _P, _P, _P]),
             "oracle3d_equilibrium": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
@@ -365,6 +366,10 @@ class Oracle3D:
 
     def stream_only(self):
         self._ck(self.L.oracle3d_stream_only(self.h))
+
+    def stream_boundaries(self):
+        """Streaming and the boundary functors, no collision."""
+        self._ck(self.L.oracle3d_stream_boundaries(self.h))
 
     def get_macros(self):
         r = np.zeros(self.N)

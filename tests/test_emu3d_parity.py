@@ -81,6 +81,13 @@ def box_flags(shape, faces):
         fl[:, 0, :] = fl[:, -1, :] = _abi.BOUNCE_BACK
     if "z" in faces:
         fl[0] = fl[-1] = _abi.BOUNCE_BACK
+    if "c" in faces:
+        # CORNER_EDGE_BOUNCE_BACK on every node with two or more domain faces (the placement of
+        # flowPastSquareCylinderScenario.cuh:12-16), and on the x = 1 plane, where it is a no-op
+        z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+        nf = (x == 0) + (x == nx - 1) + (y == 0) + (y == ny - 1) + (z == 0) + (z == nz - 1)
+        fl[nf >= 2] = _abi.CORNER_EDGE_BOUNCE_BACK
+        fl[:, :, 1][nf[:, :, 1] < 2] = _abi.CORNER_EDGE_BOUNCE_BACK
     return fl
 
 
@@ -94,6 +101,8 @@ CASES = [
     ((5, 4, 5), (0, 0, 0), 1.2, "", False, 0),          # open edges: stale slots only
     ((6, 5, 4), (1, 0, 0), 1.1, "yz", False, _abi.LBM_QUIRK_STALE_REST),
     ((6, 5, 4), (1, 1, 1), 1.4, "", True, _abi.LBM_QUIRK_EQ3D_LINEAR),
+    ((5, 6, 4), (0, 0, 0), 1.3, "xyzc", False, 0),      # bounce-back faces, corner-edge edges
+    ((67, 5, 4), (1, 0, 0), 1.1, "c", True, 0),         # corner-edge only, periodic x
 ]
 
 

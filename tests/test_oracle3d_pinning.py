@@ -153,6 +153,42 @@ def test_rest_state_is_a_fixed_point_of_a_closed_bounce_back_box():
     assert np.abs(f - W[:, None, None, None]).max() < 1e-15
 
 
+def test_corner_edge_bounce_back_copies_the_nonnegative_directions_on_edges_only():
+    """EdgeCornerBounceBack (edgeCornerBounceBack.cuh:10-49) after streaming: on a node with two or
+    more domain faces, the 7 directions with no negative and some positive component are copied
+    into their opposites; on faces and inside, the flag changes nothing. At the (NX-1, NY-1, NZ-1)
+    corner those 7 opposite slots also receive what BounceBack<3> writes there; at the other
+    corners they do not (the functor ignores the normals it computes)."""
+    nx, ny, nz = 5, 4, 6
+    rng = np.random.default_rng(3)
+    f0 = W[:, None, None, None] * (1 + 0.1 * rng.random((27, nz, ny, nx)))
+
+    def run(fl):
+        o = O.Oracle3D(cfg3(nx, ny, nz, periodic=(0, 0, 0)), 64)
+        o.set_pops(f0)
+        if fl is not None:
+            o.set_flags(fl)
+        o.stream_boundaries()
+        return o.get_pops()
+
+    ce = np.full((nz, ny, nx), _abi.CORNER_EDGE_BOUNCE_BACK, np.uint8)
+    fa, fb = run(ce), run(None)
+    P = [i for i in range(27) if (C[i] >= 0).all() and C[i].any()]
+    assert len(P) == 7 and not set(P) & set(OPP[P])
+    expected = fb.copy()
+    for z, y, x in itertools.product(range(nz), range(ny), range(nx)):
+        if (x in (0, nx - 1)) + (y in (0, ny - 1)) + (z in (0, nz - 1)) >= 2:
+            for i in P:
+                expected[OPP[i], z, y, x] = fb[i, z, y, x]
+    assert np.array_equal(fa, expected)
+    assert not np.array_equal(fa, fb)
+    bb = np.zeros((nz, ny, nx), np.uint8)
+    bb[-1, -1, -1] = bb[0, 0, 0] = _abi.BOUNCE_BACK
+    fc = run(bb)
+    assert np.array_equal(fc[OPP[P], -1, -1, -1], fa[OPP[P], -1, -1, -1])
+    assert not np.array_equal(fc[OPP[P], 0, 0, 0], fa[OPP[P], 0, 0, 0])
+
+
 def test_force_driven_channel_has_curvature_minus_f_over_nu():
     """Walls at y = 0 and y = NY-1 (BOUNCE_BACK), periodic x and z, body force along x: the
     steady profile satisfies nu u'' = -F in the bulk and is symmetric about the centre."""
