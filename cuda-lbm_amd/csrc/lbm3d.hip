@@ -161,7 +161,7 @@ struct lbm3d_ctx {
     void* buf[2] = {nullptr, nullptr};
     int cur = 0;                       // buffer holding the latest post-collision state
     uint8_t* d_flags = nullptr;        // null while every node is FLUID
-    bool has_guo = false;              // Guo inlet/outlet flags present (two launches per step)
+    bool has_guo = false;              // Guo inlet/outlet or x-face ZG flags present (GUO kernel)
     float* d_force = nullptr;          // SoA [3][plane] or null
     float* d_rho = nullptr;            // [plane]
     float* d_u = nullptr;              // SoA [3][plane]
@@ -437,9 +437,16 @@ int lbm3d_set_flags(lbm3d_ctx* ctx, const uint8_t* flags, int on_device) {
             guo = true;
             continue;
         }
+        if (h[n] == LBM_ZG_OUTFLOW && (x == 0 || x == nx - 1)) {
+            if (ctx->cfg.periodic_x) return ctx->fail(LBM_E_INVALID, "ZG_OUTFLOW flags need a non-periodic x axis");
+            guo = true;
+            continue;
+        }
         if (h[n] == LBM_GUO_VELOCITY_INLET || h[n] == LBM_GUO_PRESSURE_OUTLET)
             return ctx->fail(LBM_E_UNSUPPORTED, "flag %d at node %zu: Guo inlets are built at x = 0 and outlets at x = NX-1", int(h[n]), n);
-        return ctx->fail(LBM_E_UNSUPPORTED, "flag %d at node %zu: not built for D3Q27 (FLUID, BOUNCE_BACK, CORNER_EDGE_BOUNCE_BACK, GUO_VELOCITY_INLET, GUO_PRESSURE_OUTLET are)", int(h[n]), n);
+        if (h[n] == LBM_ZG_OUTFLOW)
+            return ctx->fail(LBM_E_UNSUPPORTED, "ZG_OUTFLOW at node %zu: built on the x = 0 and x = NX-1 faces only", n);
+        return ctx->fail(LBM_E_UNSUPPORTED, "flag %d at node %zu: not built for D3Q27 (FLUID, BOUNCE_BACK, CORNER_EDGE_BOUNCE_BACK, GUO_VELOCITY_INLET, GUO_PRESSURE_OUTLET, ZG_OUTFLOW are)", int(h[n]), n);
     }
     if (!any) return lbm3d_set_flags(ctx, nullptr, 0);
     ctx->has_guo = guo;

@@ -38,7 +38,7 @@ struct Step3Params {
     Mrt3Rates mrt;         // MRT<3> rates S[4..8] and the literal flag
     float u_inlet;         // GUO_VELOCITY_INLET velocity
     int guo_own;           // LBM_QUIRK_GUO3D_OWN_MOMENTS
-    int guo;               // Guo inlet/outlet flags present
+    int guo;               // Guo inlet/outlet or x-face ZG outflow flags present
     int guo_ring_outlet;   // (NX-1) % 64 == 0: the outlet's ring nodes run in a launch of their
                            // own before the main launch, which skips them
     int stale_rest;        // LBM_QUIRK_STALE_REST
@@ -203,6 +203,26 @@ LBM3_HD void apply_guo(const Step3Params& p, const lbm::PopBuf<typename Store3<S
     bc_guo3<INLET>(g, gu, B);
 }
 
+// ZG_OutflowBoundary<3>::apply (zeroGradientOutflow.cuh:8-59) on an x face: the populations
+// entering the domain (c . n > 0, n the inward normal: c_x > 0 at x = 0, c_x < 0 at x = NX-1)
+// take the interior neighbour's post-stream values; the others keep the node's own. The
+// neighbour is pulled as a Guo node pulls it, with the same wavefront ordering.
+template <int STORE, bool BUF, bool LEFT, int q>
+LBM3_HD void zg_dir(const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src, float* g, int xn, int y,
+                    int z) {
+    if constexpr (LEFT ? CX[q] > 0 : CX[q] < 0) g[q] = pull_at<STORE, BUF, q>(p, src, xn, y, z);
+}
+template <int STORE, bool BUF, bool LEFT, int... qs>
+LBM3_HD void zg_all(const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src, float* g, int xn, int y,
+                    int z, std::integer_sequence<int, qs...>) {
+    (zg_dir<STORE, BUF, LEFT, qs>(p, src, g, xn, y, z), ...);
+}
+template <int STORE, bool BUF, bool LEFT>
+LBM3_HD void apply_zg(const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src, float* g, int x, int y,
+                      int z) {
+    zg_all<STORE, BUF, LEFT>(p, src, g, LEFT ? x + 1 : x - 1, y, z, std::make_integer_sequence<int, Q>{});
+}
+
 // A Guo node pulls its x +- 1 neighbour; on the ring of the plane (y or z at a non-periodic
 // edge) that pull reads the neighbour's own stale slots in the destination buffer, which the
 // neighbour's update rewrites. When both nodes are lanes of one wavefront that is safe: the
@@ -217,7 +237,7 @@ LBM3_HD bool guo_ring(const Step3Params& p, int x, int y, int z) {
 
 // One node of one time step: pull, boundary functor, macroscopics, Guo-forced BGK, MRT or CM,
 // store. Requires 0 <= x < nx, 0 <= y < ny, 0 <= z < nz and nx*ny*nz < 2^31. GUO: the node may
-// carry a Guo inlet/outlet flag.
+// carry a flag that reads its x +- 1 neighbour (Guo inlet/outlet, ZG outflow on an x face).
 template <int COLL, int STORE, bool BUF, bool GUO = false>
 LBM3_HD void step3_node(const Step3Params& p, int x, int y, int z) {
     using St = Store3<STORE>;
@@ -248,6 +268,10 @@ LBM3_HD void step3_node(const Step3Params& p, int x, int y, int z) {
     if (GUO) {
         if (fl == LBM_GUO_VELOCITY_INLET) apply_guo<STORE, BUF, true>(p, src, g, x, y, z);
         else if (fl == LBM_GUO_PRESSURE_OUTLET) apply_guo<STORE, BUF, false>(p, src, g, x, y, z);
+        else if (fl == LBM_ZG_OUTFLOW) {
+            if (x == 0) apply_zg<STORE, BUF, true>(p, src, g, x, y, z);
+            else apply_zg<STORE, BUF, false>(p, src, g, x, y, z);
+        }
     }
     float fx = p.fx, fy = p.fy, fz = p.fz;
     if (p.force) {

@@ -15,7 +15,8 @@
  * CM<3, NoAdapter> (src/core/collision/CM/CM_D3Q27.cuh:1-105), the FLUID, BOUNCE_BACK and
  * CORNER_EDGE_BOUNCE_BACK flags (src/core/boundaries/boundaries.cuh:89-138, BounceBack<3>
  * src/functors/boundaryConditions/bbDomainBoundary.cuh:52-90, EdgeCornerBounceBack
- * src/functors/boundaryConditions/edgeCornerBounceBack.cuh:10-49), GUO_VELOCITY_INLET at x = 0 and
+ * src/functors/boundaryConditions/edgeCornerBounceBack.cuh:10-49), ZG_OUTFLOW on the x = 0 and
+ * x = NX-1 faces (ZG_OutflowBoundary<3>, zeroGradientOutflow.cuh:8-59), GUO_VELOCITY_INLET at x = 0 and
  * GUO_PRESSURE_OUTLET at x = NX-1 (guoInlet3D.cuh, guoOutlet3D.cuh), runtime periodic axes, FP32 or
  * 16-bit shifted population storage. The CM<3> adapters (ACM_D3Q27.cuh, E_ACM_D3Q27.cuh) and the
  * other 3-D boundary functors are rejected with LBM_E_UNSUPPORTED.
@@ -96,11 +97,12 @@ int lbm3d_set_stream(lbm3d_ctx* ctx, void* hip_stream);
 
 /* Replaces setup_boundary_flags (boundaries.cuh:170-197) for 3-D: one flag per node, LBM_FLUID,
  * LBM_BOUNCE_BACK, LBM_CORNER_EDGE_BOUNCE_BACK (acts on nodes with two or more domain faces, like
- * FLUID elsewhere), LBM_GUO_VELOCITY_INLET (x = 0 only) or LBM_GUO_PRESSURE_OUTLET (x = NX-1 only,
- * both need a non-periodic x axis); other flags give LBM_E_UNSUPPORTED. NULL = all FLUID. With
- * Guo flags a step is two launches: the inlet/outlet planes first (they read their neighbours'
- * post-stream populations, including slots the neighbours' own update overwrites), then the
- * rest. */
+ * FLUID elsewhere), LBM_ZG_OUTFLOW (x = 0 and x = NX-1 only), LBM_GUO_VELOCITY_INLET (x = 0 only)
+ * or LBM_GUO_PRESSURE_OUTLET (x = NX-1 only); ZG and Guo flags need a non-periodic x axis; other
+ * flags give LBM_E_UNSUPPORTED. NULL = all FLUID. ZG and Guo nodes read their x +- 1 neighbour's
+ * post-stream populations, including slots the neighbour's own update overwrites: the neighbour
+ * is pulled in the same wavefront before any store, and when NX-1 is a multiple of 64 the
+ * x = NX-1 ring nodes (y or z on a non-periodic edge) run in a launch of their own first. */
 int lbm3d_set_flags(lbm3d_ctx* ctx, const uint8_t* flags, int on_device);
 
 /* Replaces init_kernel<3> + init_node (src/core/init/init.cuh:10-43): f = f_back = f_eq.

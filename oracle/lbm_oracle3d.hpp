@@ -444,7 +444,29 @@ struct Solver3 {
         }
     }
 
-    // Flags restated here: FLUID, BOUNCE_BACK, CORNER_EDGE_BOUNCE_BACK, GUO_VELOCITY_INLET (x = 0) and
+    // ZG_OutflowBoundary<3>::apply (zeroGradientOutflow.cuh:8-59): the first matching face in the
+    // order x = 0, x = NX-1, y = 0, y = NY-1, z = 0, z = NZ-1 gives the inward normal n and the
+    // interior node one step along it (an interior node returns); for i = 0..26, c_i . n > 0
+    // takes the interior node's value. The interior node is read from `snap`, the post-stream
+    // state before any functor (boundaries_kernel_3D runs all functors concurrently).
+    void bc_zg(size_t n, int x, int y, int z, const std::vector<R>& snap) {
+        int nrm[3] = {0, 0, 0};
+        int in[3] = {x, y, z};
+        if (x == 0) { nrm[0] = 1; in[0] = 1; }
+        else if (x == nx - 1) { nrm[0] = -1; in[0] = nx - 2; }
+        else if (y == 0) { nrm[1] = 1; in[1] = 1; }
+        else if (y == ny - 1) { nrm[1] = -1; in[1] = ny - 2; }
+        else if (z == 0) { nrm[2] = 1; in[2] = 1; }
+        else if (z == nz - 1) { nrm[2] = -1; in[2] = nz - 2; }
+        else return;
+        const size_t ni = node(in[0], in[1], in[2]);
+        for (int i = 0; i < Q; i++) {
+            const int cn = CX[i] * nrm[0] + CY[i] * nrm[1] + CZ[i] * nrm[2];
+            if (cn > 0) f[i * N + n] = snap[i * N + ni];
+        }
+    }
+
+    // Flags restated here: FLUID, BOUNCE_BACK, CORNER_EDGE_BOUNCE_BACK, ZG_OUTFLOW, GUO_VELOCITY_INLET (x = 0) and
     // GUO_PRESSURE_OUTLET (x = NX-1), the placement of the reference's wind tunnel
     // (windTestScenario.cuh:13-28).
     bool has_guo = false;
@@ -455,6 +477,10 @@ struct Solver3 {
                 for (int x = 0; x < nx; x++) {
                     const uint8_t v = fl[node(x, y, z)];
                     if (v == LBM_FLUID || v == LBM_BOUNCE_BACK || v == LBM_CORNER_EDGE_BOUNCE_BACK) continue;
+                    if (v == LBM_ZG_OUTFLOW) {
+                        has_guo = true;  // reads a neighbour: needs the snapshot
+                        continue;
+                    }
                     if ((v == LBM_GUO_VELOCITY_INLET && x == 0) || (v == LBM_GUO_PRESSURE_OUTLET && x == nx - 1)) {
                         if (cfg.periodic_x) throw std::runtime_error("Guo inlet/outlet need a non-periodic x axis");
                         has_guo = true;
@@ -526,6 +552,7 @@ struct Solver3 {
                     const size_t n = node(x, y, z);
                     if (flags[n] == LBM_BOUNCE_BACK) bc_bounce_back(n, x, y, z);
                     else if (flags[n] == LBM_CORNER_EDGE_BOUNCE_BACK) bc_corner_edge_bounce_back(n, x, y, z);
+                    else if (flags[n] == LBM_ZG_OUTFLOW) bc_zg(n, x, y, z, snap);
                     else if (flags[n] == LBM_GUO_VELOCITY_INLET) bc_guo(n, x, y, z, true, snap);
                     else if (flags[n] == LBM_GUO_PRESSURE_OUTLET) bc_guo(n, x, y, z, false, snap);
                 }

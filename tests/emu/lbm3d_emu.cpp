@@ -55,7 +55,7 @@ int run3(const lbm3d_config& c, const float* f_in, const uint8_t* flags, const f
     p.guo_own = (c.quirks & LBM_QUIRK_GUO3D_OWN_MOMENTS) ? 1 : 0;
     bool guo = false;
     if (flags)
-        for (long long n = 0; n < N; n++) guo = guo || flags[n] == LBM_GUO_VELOCITY_INLET || flags[n] == LBM_GUO_PRESSURE_OUTLET;
+        for (long long n = 0; n < N; n++) guo = guo || flags[n] == LBM_GUO_VELOCITY_INLET || flags[n] == LBM_GUO_PRESSURE_OUTLET || flags[n] == LBM_ZG_OUTFLOW;
     p.guo = guo ? 1 : 0;
     p.guo_ring_outlet = (guo && (c.nx - 1) % 64 == 0) ? 1 : 0;
     p.stale_rest = (c.quirks & LBM_QUIRK_STALE_REST) ? 1 : 0;
@@ -87,7 +87,7 @@ int run3(const lbm3d_config& c, const float* f_in, const uint8_t* flags, const f
         }
         for (long long b = 0; b < nb; b++) {
             const long long L = b < per * 8 ? (b % 8) * per + b / 8 : b;
-            // a wavefront's loads precede its stores: within each wavefront the Guo lanes (which
+            // a wavefront's loads precede its stores: within each wavefront the Guo and ZG lanes (which
             // read a neighbouring lane's stale slots) run before the other lanes
             for (int wv = 0; wv < 4; wv++)
                 for (int pass = 0; pass < 2; pass++)
@@ -98,7 +98,7 @@ int run3(const lbm3d_config& c, const float* f_in, const uint8_t* flags, const f
                         if (guo && guo_ring(p, x, y, z)) continue;
                         if (z < 0 || z >= p.nz || x < 0 || y < 0) std::abort();
                         const uint8_t fl = flags ? flags[(z * p.ny + y) * p.nx + x] : uint8_t(LBM_FLUID);
-                        const bool gl = fl == LBM_GUO_VELOCITY_INLET || fl == LBM_GUO_PRESSURE_OUTLET;
+                        const bool gl = fl == LBM_GUO_VELOCITY_INLET || fl == LBM_GUO_PRESSURE_OUTLET || fl == LBM_ZG_OUTFLOW;
                         if ((pass == 0) != gl) continue;
                         if (seen[size_t((z * p.ny + y) * p.nx + x)]++) std::abort();
                         if (guo) step3_node<COLL, STORE, false, true>(p, x, y, z);

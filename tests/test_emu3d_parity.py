@@ -236,6 +236,46 @@ def guo_case(case):
     return c, f0, fl
 
 
+ZG_CASES = [
+    # shape, periodic (y, z), collision, walls on y/z, inlet at x = 0 ("guo" or "zg")
+    ((9, 6, 5), (0, 0), _abi.LBM_BGK, True, "guo"),
+    ((65, 5, 4), (0, 0), _abi.LBM_CM, True, "guo"),    # (NX-1) % 64 == 0: the outlet ring launch
+    ((8, 5, 5), (0, 1), _abi.LBM_MRT, True, "zg"),
+    ((70, 5, 4), (1, 1), _abi.LBM_BGK, False, "zg"),
+]
+
+
+def zg_case(case):
+    """ZG_OUTFLOW (zeroGradientOutflow.cuh) on the x = NX-1 face, and on x = 0 or a Guo inlet there."""
+    shape, (py, pz), coll, walls, inlet = ZG_CASES[case]
+    nx, ny, nz = shape
+    c = make_cfg(shape, (0, py, pz), 1.2, collision=coll, S48=(1.2,) * 5)
+    c.u_inlet = 0.04
+    f0 = initial_state(shape, 130 + case)
+    fl = np.zeros((nz, ny, nx), np.uint8)
+    if walls:
+        if not py:
+            fl[:, 0, :] = fl[:, -1, :] = _abi.BOUNCE_BACK
+        if not pz:
+            fl[0], fl[-1] = _abi.BOUNCE_BACK, _abi.BOUNCE_BACK
+    fl[:, :, 0] = _abi.GUO_VELOCITY_INLET if inlet == "guo" else _abi.ZG_OUTFLOW
+    fl[:, :, -1] = _abi.ZG_OUTFLOW
+    return c, f0, fl
+
+
+@pytest.mark.parametrize("case", range(len(ZG_CASES)))
+def test_emulated_zg_outflow_matches_oracle(case):
+    """x-face ZG outflow (neighbour pulled by the streaming rule, Guo-lane ordering, ring launch)
+    against the oracle's functor on the post-stream snapshot."""
+    c, f0, fl = zg_case(case)
+    o = oracle_run(c, f0, fl, None, 6)
+    g, rho, u = emu_run(c, f0, fl, None, 6)
+    assert np.abs(g - o.get_pops()).max() < 1e-5
+    r_rho, r_u = o.get_macros()
+    assert np.abs(rho - r_rho).max() < 1e-5
+    assert np.abs(u - r_u).max() < 1e-6
+
+
 @pytest.mark.parametrize("case", range(len(GUO_CASES)))
 def test_emulated_guo_inlet_outlet_matches_oracle(case):
     """Guo inlet/outlet planes (their own launch, neighbours pulled by the streaming rule) against

@@ -7,8 +7,8 @@ import oracle as O
 from lbm_amd import _abi as A
 from lbm_amd.solver import LbmError
 from lbm_amd.solver3d import Config3D, Solver3D
-from test_emu3d_parity import (CASES, CM_CASES, GUO_CASES, MRT_CASES, box_flags, cm_case, guo_case, initial_state,
-                               make_cfg, mrt_case, oracle_run)
+from test_emu3d_parity import (CASES, CM_CASES, GUO_CASES, MRT_CASES, ZG_CASES, box_flags, cm_case, guo_case,
+                               initial_state, make_cfg, mrt_case, oracle_run, zg_case)
 
 pytestmark = pytest.mark.gpu
 W = O.tables3d()[0]
@@ -91,6 +91,23 @@ def test_gpu_mrt3_matches_dense_oracle(gpu_available, case):
         s.set_pops(f0)
         s.set_flags(flags)
         s.set_force_field(None if force is None else force.astype(np.float32))
+        s.step(8)
+        g = s.get_pops()
+        rho, u = s.get_macros()
+    assert np.abs(g - o.get_pops()).max() < 1e-5
+    r_rho, r_u = o.get_macros()
+    assert np.abs(rho - r_rho).max() < 1e-5
+    assert np.abs(u - r_u).max() < 1e-6
+
+
+@pytest.mark.parametrize("case", range(len(ZG_CASES)))
+def test_gpu_zg_outflow_matches_oracle(gpu_available, case):
+    c, f0, fl = zg_case(case)
+    f0 = f0.astype(np.float32)
+    o = oracle_run(c, f0, fl, None, 8)
+    with Solver3D(gpu_config(c)) as s:
+        s.set_pops(f0)
+        s.set_flags(fl)
         s.step(8)
         g = s.get_pops()
         rho, u = s.get_macros()
@@ -278,6 +295,12 @@ def test_gpu_d3q27_errors_are_reported(gpu_available):
         with pytest.raises(LbmError) as e:
             s.step(1)
         assert e.value.code == A.LBM_E_STATE
+        fl = np.zeros((8, 8, 8), np.uint8)
+        fl[:, 3, 0] = A.ZG_OUTFLOW
+        fl[:, 0, 4] = A.ZG_OUTFLOW  # ZG is built on the x faces only
+        with pytest.raises(LbmError) as e:
+            s.set_flags(fl)
+        assert e.value.code == A.LBM_E_UNSUPPORTED
         fl = np.zeros((8, 8, 8), np.uint8)
         fl[0, 0, 0] = A.ZOU_HE_LEFT
         with pytest.raises(LbmError) as e:

@@ -189,6 +189,40 @@ def test_corner_edge_bounce_back_copies_the_nonnegative_directions_on_edges_only
     assert not np.array_equal(fc[OPP[P], 0, 0, 0], fa[OPP[P], 0, 0, 0])
 
 
+def test_zg_outflow_copies_the_entering_populations_from_the_interior_neighbour():
+    """ZG_OutflowBoundary<3> (zeroGradientOutflow.cuh:8-59) after streaming: on a face node, the
+    populations with c . n > 0 (n the inward normal) take the interior neighbour's post-stream
+    values and the others keep the node's own; the first face in the order x, y, z decides n at
+    edges and corners; inside the domain the flag changes nothing."""
+    nx, ny, nz = 5, 4, 6
+    rng = np.random.default_rng(4)
+    f0 = W[:, None, None, None] * (1 + 0.1 * rng.random((27, nz, ny, nx)))
+
+    def run(fl):
+        o = O.Oracle3D(cfg3(nx, ny, nz, periodic=(0, 0, 0)), 64)
+        o.set_pops(f0)
+        if fl is not None:
+            o.set_flags(fl)
+        o.stream_boundaries()
+        return o.get_pops()
+
+    zg = np.full((nz, ny, nx), _abi.ZG_OUTFLOW, np.uint8)
+    fa, fb = run(zg), run(None)
+    expected = fb.copy()
+    for z, y, x in itertools.product(range(nz), range(ny), range(nx)):
+        for axis, (p, n) in enumerate(((x, nx), (y, ny), (z, nz))):
+            if p in (0, n - 1):
+                nrm = np.zeros(3, int)
+                nrm[axis] = 1 if p == 0 else -1
+                zi, yi, xi = np.array([z, y, x]) + nrm[::-1]
+                enter = C @ nrm > 0
+                assert enter.sum() == 9
+                expected[enter, z, y, x] = fb[enter, zi, yi, xi]
+                break
+    assert np.array_equal(fa, expected)
+    assert not np.array_equal(fa, fb)
+
+
 def test_force_driven_channel_has_curvature_minus_f_over_nu():
     """Walls at y = 0 and y = NY-1 (BOUNCE_BACK), periodic x and z, body force along x: the
     steady profile satisfies nu u'' = -F in the bulk and is symmetric about the centre."""
