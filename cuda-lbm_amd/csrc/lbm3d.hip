@@ -32,7 +32,7 @@ size_t elem_bytes(int storage) { return storage == LBM_F32 ? 4 : 2; }
 // workgroups (neighbouring row segments, which share cache lines at their ends) run on one
 // XCD. The last nb % 8 workgroups keep their index. Eight wavefronts per SIMD (at most 64
 // VGPRs, no scratch) measured 1-2% faster than the compiler's own 70-VGPR allocation.
-template <int COLL, int STORE, bool BUF, bool GUO>
+template <int COLL, int STORE, bool BUF, int GUO>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void step3_kernel(Step3Params p) {
     const long long nb = (long long)p.nbx * p.nby * p.nz;
     const long long b = blockIdx.x;
@@ -49,7 +49,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void s
 // thread i looks at node (side ? NX-1 : 0, y, z) of the 2 * NY * NZ plane nodes. Runs before
 // step3_kernel, because these nodes read their x - 1 neighbours' stale slots, which that launch
 // rewrites.
-template <int COLL, int STORE, bool BUF>
+template <int COLL, int STORE, bool BUF, int GUO>
 __global__ __launch_bounds__(256) void step3_ring_kernel(Step3Params p) {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     const long long per = (long long)p.ny * p.nz;
@@ -58,7 +58,7 @@ __global__ __launch_bounds__(256) void step3_ring_kernel(Step3Params p) {
     const long long r = i - side * per;
     const int x = side ? p.nx - 1 : 0, y = int(r % p.ny), z = int(r / p.ny);
     if (!guo_ring(p, x, y, z)) return;
-    step3_node<COLL, STORE, BUF, true>(p, x, y, z);
+    step3_node<COLL, STORE, BUF, GUO>(p, x, y, z);
 }
 
 // init_kernel<3> + init_node (init.cuh:10-43): both buffers = f_eq(rho, u), u AoS [N][3].
@@ -161,7 +161,8 @@ struct lbm3d_ctx {
     void* buf[2] = {nullptr, nullptr};
     int cur = 0;                       // buffer holding the latest post-collision state
     uint8_t* d_flags = nullptr;        // null while every node is FLUID
-    bool has_guo = false;              // Guo inlet/outlet or x-face ZG flags present (GUO kernel)
+    bool has_guo = false;              // Guo inlet/outlet, x-face ZG or regularized flags present
+    bool has_reg = false;              // regularized flags present (the GUO = 2 kernel variant)
     float* d_force = nullptr;          // SoA [3][plane] or null
     float* d_rho = nullptr;            // [plane]
     float* d_u = nullptr;              // SoA [3][plane]
@@ -274,21 +275,25 @@ int lbm3d_ctx::one_step(bool write_macros) {
     const dim3 grid{unsigned(nblk)}, block{256};
     const long long nplane = 2LL * cfg.ny * cfg.nz;
     const dim3 pgrid{unsigned((nplane + 255) / 256)};
-#define LBM3_LAUNCH(C, S)                                                                       \
-    if (has_guo) {                                                                              \
-        if (fits) {                                                                             \
-            if (p.guo_ring_outlet)                                                              \
-                hipLaunchKernelGGL((step3_ring_kernel<C, S, true>), pgrid, block, 0, stream, p); \
-            hipLaunchKernelGGL((step3_kernel<C, S, true, true>), grid, block, 0, stream, p);    \
-        } else {                                                                                \
-            if (p.guo_ring_outlet)                                                              \
-                hipLaunchKernelGGL((step3_ring_kernel<C, S, false>), pgrid, block, 0, stream, p); \
-            hipLaunchKernelGGL((step3_kernel<C, S, false, true>), grid, block, 0, stream, p);   \
-        }                                                                                       \
-    } else if (fits) {                                                                          \
-        hipLaunchKernelGGL((step3_kernel<C, S, true, false>), grid, block, 0, stream, p);       \
+#define LBM3_LAUNCH_G(C, S, G)                                                                  \
+    if (fits) {                                                                                 \
+        if (p.guo_ring_outlet)                                                                  \
+            hipLaunchKernelGGL((step3_ring_kernel<C, S, true, G>), pgrid, block, 0, stream, p); \
+        hipLaunchKernelGGL((step3_kernel<C, S, true, G>), grid, block, 0, stream, p);           \
     } else {                                                                                    \
-        hipLaunchKernelGGL((step3_kernel<C, S, false, false>), grid, block, 0, stream, p);      \
+        if (p.guo_ring_outlet)                                                                  \
+            hipLaunchKernelGGL((step3_ring_kernel<C, S, false, G>), pgrid, block, 0, stream, p); \
+        hipLaunchKernelGGL((step3_kernel<C, S, false, G>), grid, block, 0, stream, p);          \
+    }
+#define LBM3_LAUNCH(C, S)                                                                       \
+    if (has_reg) {                                                                              \
+        LBM3_LAUNCH_G(C, S, 2)                                                                  \
+    } else if (has_guo) {                                                                       \
+        LBM3_LAUNCH_G(C, S, 1)                                                                  \
+    } else if (fits) {                                                                          \
+        hipLaunchKernelGGL((step3_kernel<C, S, true, 0>), grid, block, 0, stream, p);       \
+    } else {                                                                                    \
+        hipLaunchKernelGGL((step3_kernel<C, S, false, 0>), grid, block, 0, stream, p);      \
     }
 #define LBM3_LAUNCH_S(C)                                                                        \
     switch (cfg.storage) {                                                                      \
@@ -415,6 +420,7 @@ int lbm3d_set_flags(lbm3d_ctx* ctx, const uint8_t* flags, int on_device) {
             ctx->d_flags = nullptr;
         }
         ctx->has_guo = false;
+        ctx->has_reg = false;
         return LBM_OK;
     }
     std::vector<uint8_t> h(size_t(ctx->N));
@@ -425,7 +431,7 @@ int lbm3d_set_flags(lbm3d_ctx* ctx, const uint8_t* flags, int on_device) {
     } else {
         std::memcpy(h.data(), flags, h.size());
     }
-    bool any = false, guo = false;
+    bool any = false, guo = false, reg = false;
     const int nx = ctx->cfg.nx;
     for (size_t n = 0; n < h.size(); n++) {
         const int x = int(n % size_t(nx));
@@ -437,6 +443,10 @@ int lbm3d_set_flags(lbm3d_ctx* ctx, const uint8_t* flags, int on_device) {
             guo = true;
             continue;
         }
+        if (h[n] == LBM_REGULARIZED_BOUNCE_BACK || h[n] == LBM_REGULARIZED_INLET_LEFT) {
+            guo = reg = true;  // the functor is compiled into the GUO = 2 kernel variant only
+            continue;
+        }
         if (h[n] == LBM_ZG_OUTFLOW && (x == 0 || x == nx - 1)) {
             if (ctx->cfg.periodic_x) return ctx->fail(LBM_E_INVALID, "ZG_OUTFLOW flags need a non-periodic x axis");
             guo = true;
@@ -446,10 +456,11 @@ int lbm3d_set_flags(lbm3d_ctx* ctx, const uint8_t* flags, int on_device) {
             return ctx->fail(LBM_E_UNSUPPORTED, "flag %d at node %zu: Guo inlets are built at x = 0 and outlets at x = NX-1", int(h[n]), n);
         if (h[n] == LBM_ZG_OUTFLOW)
             return ctx->fail(LBM_E_UNSUPPORTED, "ZG_OUTFLOW at node %zu: built on the x = 0 and x = NX-1 faces only", n);
-        return ctx->fail(LBM_E_UNSUPPORTED, "flag %d at node %zu: not built for D3Q27 (FLUID, BOUNCE_BACK, CORNER_EDGE_BOUNCE_BACK, GUO_VELOCITY_INLET, GUO_PRESSURE_OUTLET, ZG_OUTFLOW are)", int(h[n]), n);
+        return ctx->fail(LBM_E_UNSUPPORTED, "flag %d at node %zu: not built for D3Q27 (FLUID, BOUNCE_BACK, CORNER_EDGE_BOUNCE_BACK, REGULARIZED_BOUNCE_BACK, REGULARIZED_INLET_LEFT, GUO_VELOCITY_INLET, GUO_PRESSURE_OUTLET, ZG_OUTFLOW are)", int(h[n]), n);
     }
     if (!any) return lbm3d_set_flags(ctx, nullptr, 0);
     ctx->has_guo = guo;
+    ctx->has_reg = reg;
     if (!ctx->d_flags) {
         rc = ctx->dev_alloc(reinterpret_cast<void**>(&ctx->d_flags), size_t(ctx->plane), "flags");
         if (rc) return rc;

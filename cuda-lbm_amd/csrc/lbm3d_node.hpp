@@ -416,6 +416,84 @@ LBM3_HD void bc_bounce_back3(float* g, int x, int y, int z, const Box3& B) {
     bb_all(g, x, y, z, B, std::make_integer_sequence<int, Q - 1>{});
 }
 
+// RegularizedBoundary::apply (regularizedBoundary3D.cuh:10-137) with the imposed velocity u:
+//  - the face: the last match in the order x = 0, x = NX-1, y = 0, y = NY-1, z = 0, z = NZ-1
+//    gives the axis d and the inward sign s (an interior node returns);
+//  - rho = (sum_{c_d s = 0} g + 2 sum_{c_d s < 0} g) / (1 - s u_d);
+//  - the unknown populations (c_d s > 0) become f_eq + (g_opp - f_eq_opp);
+//  - Pi = sum c c g - rho/3 I - rho u u, then every population becomes f_eq + 4.5 w Q : Pi with
+//    Q = c c - I/3 (the w / (2 cs^4) projection).
+// The functor's own equilibrium (de_rosis_eq) is the solver's (lin_ec 1 = the reference's form).
+template <int D, int S, int q>
+constexpr int reg_cn() {
+    return (D == 0 ? CX[q] : (D == 1 ? CY[q] : CZ[q])) * S;
+}
+template <int D, int S, int q>
+LBM3_HD void reg_known(const float* g, float& par, float& inc) {
+    if constexpr (reg_cn<D, S, q>() == 0) par += g[q];
+    else if constexpr (reg_cn<D, S, q>() < 0) inc += g[q];
+}
+template <int D, int S, int q>
+LBM3_HD void reg_unknown(float* g, float rho, float ux, float uy, float uz, float base, float lin_ec) {
+    if constexpr (reg_cn<D, S, q>() > 0)
+        g[q] = feq3<q>(rho, ux, uy, uz, base, lin_ec) + (g[OPP[q]] - feq3<OPP[q]>(rho, ux, uy, uz, base, lin_ec));
+}
+template <int q>
+LBM3_HD void reg_pi(const float* g, float* P) {
+    if constexpr (CX[q] != 0) P[0] += g[q];
+    if constexpr (CY[q] != 0) P[1] += g[q];
+    if constexpr (CZ[q] != 0) P[2] += g[q];
+    if constexpr (CX[q] * CY[q] != 0) P[3] += float(CX[q] * CY[q]) * g[q];
+    if constexpr (CX[q] * CZ[q] != 0) P[4] += float(CX[q] * CZ[q]) * g[q];
+    if constexpr (CY[q] * CZ[q] != 0) P[5] += float(CY[q] * CZ[q]) * g[q];
+}
+template <int q>
+LBM3_HD void reg_rebuild(float* g, const float* P, float rho, float ux, float uy, float uz, float base, float lin_ec) {
+    constexpr float t = 1.0f / 3.0f;
+    const float qp = (float(CX[q] * CX[q]) - t) * P[0] + (float(CY[q] * CY[q]) - t) * P[1] +
+                     (float(CZ[q] * CZ[q]) - t) * P[2] +
+                     2.0f * (float(CX[q] * CY[q]) * P[3] + float(CX[q] * CZ[q]) * P[4] + float(CY[q] * CZ[q]) * P[5]);
+    g[q] = feq3<q>(rho, ux, uy, uz, base, lin_ec) + 4.5f * w3(q) * qp;
+}
+template <int D, int S, int... qs>
+LBM3_HD void bc_regularized3_face(float* g, float ux, float uy, float uz, float lin_ec, std::integer_sequence<int, qs...>) {
+    float par = 0.0f, inc = 0.0f;
+    (reg_known<D, S, qs>(g, par, inc), ...);
+    const float un = D == 0 ? ux : (D == 1 ? uy : uz);
+    const float rho = (par + 2.0f * inc) / (1.0f - float(S) * un);
+    const float base = rho * (1.0f - 1.5f * (ux * ux + uy * uy + uz * uz));
+    (reg_unknown<D, S, qs>(g, rho, ux, uy, uz, base, lin_ec), ...);
+    float P[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    (reg_pi<qs>(g, P), ...);
+    P[0] -= rho / 3.0f + rho * ux * ux;
+    P[1] -= rho / 3.0f + rho * uy * uy;
+    P[2] -= rho / 3.0f + rho * uz * uz;
+    P[3] -= rho * ux * uy;
+    P[4] -= rho * ux * uz;
+    P[5] -= rho * uy * uz;
+    (reg_rebuild<qs>(g, P, rho, ux, uy, uz, base, lin_ec), ...);
+}
+// One instantiation per face (axis and inward sign known at compile time).
+LBM3_HD void bc_regularized3(float* g, int x, int y, int z, const Box3& B, float ux, float uy, float uz, float lin_ec) {
+    int face = -1;
+    if (x == 0) face = 0;
+    if (x == B.nx - 1) face = 1;
+    if (y == 0) face = 2;
+    if (y == B.ny - 1) face = 3;
+    if (z == 0) face = 4;
+    if (z == B.nz - 1) face = 5;
+    constexpr auto qs = std::make_integer_sequence<int, Q>{};
+    switch (face) {
+        case 0: bc_regularized3_face<0, 1>(g, ux, uy, uz, lin_ec, qs); break;
+        case 1: bc_regularized3_face<0, -1>(g, ux, uy, uz, lin_ec, qs); break;
+        case 2: bc_regularized3_face<1, 1>(g, ux, uy, uz, lin_ec, qs); break;
+        case 3: bc_regularized3_face<1, -1>(g, ux, uy, uz, lin_ec, qs); break;
+        case 4: bc_regularized3_face<2, 1>(g, ux, uy, uz, lin_ec, qs); break;
+        case 5: bc_regularized3_face<2, -1>(g, ux, uy, uz, lin_ec, qs); break;
+        default: break;
+    }
+}
+
 // EdgeCornerBounceBack::apply (edgeCornerBounceBack.cuh:10-49): on a node with at least two
 // domain faces (x, y, z at 0 or N-1, periodic or not), every direction with no negative and at
 // least one positive component copies g_i into g_opp(i). The functor computes the inward

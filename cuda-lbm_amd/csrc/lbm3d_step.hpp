@@ -36,7 +36,7 @@ struct Step3Params {
     float omega, rho0, fx, fy, fz;
     float lin_ec;          // linear equilibrium coefficient of edge/corner directions (3 or 1)
     Mrt3Rates mrt;         // MRT<3> rates S[4..8] and the literal flag
-    float u_inlet;         // GUO_VELOCITY_INLET velocity
+    float u_inlet;         // GUO_VELOCITY_INLET and REGULARIZED_INLET_LEFT velocity
     int guo_own;           // LBM_QUIRK_GUO3D_OWN_MOMENTS
     int guo;               // Guo inlet/outlet or x-face ZG outflow flags present
     int guo_ring_outlet;   // (NX-1) % 64 == 0: the outlet's ring nodes run in a launch of their
@@ -236,9 +236,10 @@ LBM3_HD bool guo_ring(const Step3Params& p, int x, int y, int z) {
 }
 
 // One node of one time step: pull, boundary functor, macroscopics, Guo-forced BGK, MRT or CM,
-// store. Requires 0 <= x < nx, 0 <= y < ny, 0 <= z < nz and nx*ny*nz < 2^31. GUO: the node may
-// carry a flag that reads its x +- 1 neighbour (Guo inlet/outlet, ZG outflow on an x face).
-template <int COLL, int STORE, bool BUF, bool GUO = false>
+// store. Requires 0 <= x < nx, 0 <= y < ny, 0 <= z < nz and nx*ny*nz < 2^31. GUO 1: the node may
+// carry a flag that reads its x +- 1 neighbour (Guo inlet/outlet, ZG outflow on an x face); GUO 2:
+// also a regularized flag (a variant of its own: its 27 equilibria and Pi spill registers).
+template <int COLL, int STORE, bool BUF, int GUO = 0>
 LBM3_HD void step3_node(const Step3Params& p, int x, int y, int z) {
     using St = Store3<STORE>;
     using T = typename St::T;
@@ -265,12 +266,16 @@ LBM3_HD void step3_node(const Step3Params& p, int x, int y, int z) {
         const Box3 B{p.nx, p.ny, p.nz, p.px, p.py, p.pz};
         bc_corner_edge_bounce_back3(g, x, y, z, B);
     }
-    if (GUO) {
+    if constexpr (GUO != 0) {
         if (fl == LBM_GUO_VELOCITY_INLET) apply_guo<STORE, BUF, true>(p, src, g, x, y, z);
         else if (fl == LBM_GUO_PRESSURE_OUTLET) apply_guo<STORE, BUF, false>(p, src, g, x, y, z);
         else if (fl == LBM_ZG_OUTFLOW) {
             if (x == 0) apply_zg<STORE, BUF, true>(p, src, g, x, y, z);
             else apply_zg<STORE, BUF, false>(p, src, g, x, y, z);
+        } else if (GUO == 2 && (fl == LBM_REGULARIZED_BOUNCE_BACK || fl == LBM_REGULARIZED_INLET_LEFT)) {
+            const Box3 B{p.nx, p.ny, p.nz, p.px, p.py, p.pz};
+            const float ui = fl == LBM_REGULARIZED_INLET_LEFT ? p.u_inlet : 0.0f;
+            bc_regularized3(g, x, y, z, B, ui, 0.0f, 0.0f, p.lin_ec);
         }
     }
     float fx = p.fx, fy = p.fy, fz = p.fz;

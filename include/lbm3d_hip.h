@@ -16,7 +16,8 @@
  * CORNER_EDGE_BOUNCE_BACK flags (src/core/boundaries/boundaries.cuh:89-138, BounceBack<3>
  * src/functors/boundaryConditions/bbDomainBoundary.cuh:52-90, EdgeCornerBounceBack
  * src/functors/boundaryConditions/edgeCornerBounceBack.cuh:10-49), ZG_OUTFLOW on the x = 0 and
- * x = NX-1 faces (ZG_OutflowBoundary<3>, zeroGradientOutflow.cuh:8-59), GUO_VELOCITY_INLET at x = 0 and
+ * x = NX-1 faces (ZG_OutflowBoundary<3>, zeroGradientOutflow.cuh:8-59), REGULARIZED_BOUNCE_BACK and
+ * REGULARIZED_INLET_LEFT (RegularizedBoundary, regularizedBoundary3D.cuh:10-137), GUO_VELOCITY_INLET at x = 0 and
  * GUO_PRESSURE_OUTLET at x = NX-1 (guoInlet3D.cuh, guoOutlet3D.cuh), runtime periodic axes, FP32 or
  * 16-bit shifted population storage. The CM<3> adapters (ACM_D3Q27.cuh, E_ACM_D3Q27.cuh) and the
  * other 3-D boundary functors are rejected with LBM_E_UNSUPPORTED.
@@ -76,7 +77,8 @@ typedef struct lbm3d_config {
     int32_t  periodic_y;     /*   src/core/streaming/streaming.cu:45-53, never defined there) */
     int32_t  periodic_z;
     float    force[3];       /* uniform Guo body force, used where no force field is set      */
-    float    u_inlet;        /* Scenario::u_max of GUO_VELOCITY_INLET (boundaries.cuh:123)     */
+    float    u_inlet;        /* Scenario::u_max of GUO_VELOCITY_INLET and REGULARIZED_INLET_LEFT
+                                (boundaries.cuh:110-125)                                        */
     uint32_t quirks;         /* LBM_QUIRK_STALE_REST | LBM_QUIRK_EQ3D_LINEAR | LBM_QUIRK_MRT3_LITERAL
                                 | LBM_QUIRK_GUO3D_OWN_MOMENTS                                    */
     int32_t  reserved[8];
@@ -97,7 +99,9 @@ int lbm3d_set_stream(lbm3d_ctx* ctx, void* hip_stream);
 
 /* Replaces setup_boundary_flags (boundaries.cuh:170-197) for 3-D: one flag per node, LBM_FLUID,
  * LBM_BOUNCE_BACK, LBM_CORNER_EDGE_BOUNCE_BACK (acts on nodes with two or more domain faces, like
- * FLUID elsewhere), LBM_ZG_OUTFLOW (x = 0 and x = NX-1 only), LBM_GUO_VELOCITY_INLET (x = 0 only)
+ * FLUID elsewhere), LBM_REGULARIZED_BOUNCE_BACK (u = 0) and LBM_REGULARIZED_INLET_LEFT
+ * (u = (u_inlet, 0, 0)) on any face node (like FLUID inside), LBM_ZG_OUTFLOW (x = 0 and x = NX-1
+ * only), LBM_GUO_VELOCITY_INLET (x = 0 only)
  * or LBM_GUO_PRESSURE_OUTLET (x = NX-1 only); ZG and Guo flags need a non-periodic x axis; other
  * flags give LBM_E_UNSUPPORTED. NULL = all FLUID. ZG and Guo nodes read their x +- 1 neighbour's
  * post-stream populations, including slots the neighbour's own update overwrites: the neighbour

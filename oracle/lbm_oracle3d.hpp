@@ -444,6 +444,63 @@ struct Solver3 {
         }
     }
 
+    // RegularizedBoundary::apply (regularizedBoundary3D.cuh:10-137), loops as written, with the
+    // imposed velocity (ux, uy, uz): the face is the last match of the x, y, z tests; rho from
+    // the known populations (c[d] * s <= 0: parallel once, incoming twice) over (1 - s u_d); its
+    // private de_rosis_eq is the solver's equilibrium in its literal form (eq_literal selects it,
+    // as for every other equilibrium here); the unknown populations take f_eq + f_neq of their
+    // opposite; then Pi(1) from the updated populations and f = f_eq + w / (2 cs^4) Q : Pi.
+    void bc_regularized(size_t n, int x, int y, int z, R ux, R uy, R uz) {
+        int nd = -1, ns = 0;
+        if (x == 0) { nd = 0; ns = +1; }
+        if (x == nx - 1) { nd = 0; ns = -1; }
+        if (y == 0) { nd = 1; ns = +1; }
+        if (y == ny - 1) { nd = 1; ns = -1; }
+        if (z == 0) { nd = 2; ns = +1; }
+        if (z == nz - 1) { nd = 2; ns = -1; }
+        if (nd == -1) return;
+        const int* Cd = nd == 0 ? CX : (nd == 1 ? CY : CZ);
+        R rho_in = 0, rho_par = 0;
+        for (int i = 0; i < Q; i++) {
+            if (Cd[i] * ns <= 0) {
+                const R fi = f[i * N + n];
+                if (Cd[i] * ns == 0) rho_par += fi;
+                else rho_in += fi;
+            }
+        }
+        R rho = rho_par + R(2) * rho_in;
+        const R un = nd == 0 ? ux : (nd == 1 ? uy : uz);
+        rho /= (R(1) - R(ns) * un);
+        R feq[Q];
+        equilibrium<R>(rho, ux, uy, uz, feq, eq_literal);
+        for (int q = 0; q < Q; q++)
+            if (Cd[q] * ns > 0) f[q * N + n] = feq[q] + (f[OPP[q] * N + n] - feq[OPP[q]]);
+        const R cs2 = R(1) / R(3);
+        R Pxx = 0, Pyy = 0, Pzz = 0, Pxy = 0, Pxz = 0, Pyz = 0;
+        for (int q = 0; q < Q; q++) {
+            const R fi = f[q * N + n];
+            Pxx += R(CX[q] * CX[q]) * fi;
+            Pyy += R(CY[q] * CY[q]) * fi;
+            Pzz += R(CZ[q] * CZ[q]) * fi;
+            Pxy += R(CX[q] * CY[q]) * fi;
+            Pxz += R(CX[q] * CZ[q]) * fi;
+            Pyz += R(CY[q] * CZ[q]) * fi;
+        }
+        Pxx -= cs2 * rho + rho * ux * ux;
+        Pyy -= cs2 * rho + rho * uy * uy;
+        Pzz -= cs2 * rho + rho * uz * uz;
+        Pxy -= rho * ux * uy;
+        Pxz -= rho * ux * uz;
+        Pyz -= rho * uy * uz;
+        for (int q = 0; q < Q; q++) {
+            const R cx = R(CX[q]), cy = R(CY[q]), cz = R(CZ[q]);
+            const R fneq = weight<R>(q) / (R(2) * cs2 * cs2) *
+                           ((cx * cx - cs2) * Pxx + (cy * cy - cs2) * Pyy + (cz * cz - cs2) * Pzz +
+                            R(2) * (cx * cy * Pxy + cx * cz * Pxz + cy * cz * Pyz));
+            f[q * N + n] = feq[q] + fneq;
+        }
+    }
+
     // ZG_OutflowBoundary<3>::apply (zeroGradientOutflow.cuh:8-59): the first matching face in the
     // order x = 0, x = NX-1, y = 0, y = NY-1, z = 0, z = NZ-1 gives the inward normal n and the
     // interior node one step along it (an interior node returns); for i = 0..26, c_i . n > 0
@@ -481,6 +538,7 @@ struct Solver3 {
                         has_guo = true;  // reads a neighbour: needs the snapshot
                         continue;
                     }
+                    if (v == LBM_REGULARIZED_BOUNCE_BACK || v == LBM_REGULARIZED_INLET_LEFT) continue;
                     if ((v == LBM_GUO_VELOCITY_INLET && x == 0) || (v == LBM_GUO_PRESSURE_OUTLET && x == nx - 1)) {
                         if (cfg.periodic_x) throw std::runtime_error("Guo inlet/outlet need a non-periodic x axis");
                         has_guo = true;
@@ -553,6 +611,8 @@ struct Solver3 {
                     if (flags[n] == LBM_BOUNCE_BACK) bc_bounce_back(n, x, y, z);
                     else if (flags[n] == LBM_CORNER_EDGE_BOUNCE_BACK) bc_corner_edge_bounce_back(n, x, y, z);
                     else if (flags[n] == LBM_ZG_OUTFLOW) bc_zg(n, x, y, z, snap);
+                    else if (flags[n] == LBM_REGULARIZED_BOUNCE_BACK) bc_regularized(n, x, y, z, R(0), R(0), R(0));
+                    else if (flags[n] == LBM_REGULARIZED_INLET_LEFT) bc_regularized(n, x, y, z, R(cfg.u_inlet), R(0), R(0));
                     else if (flags[n] == LBM_GUO_VELOCITY_INLET) bc_guo(n, x, y, z, true, snap);
                     else if (flags[n] == LBM_GUO_PRESSURE_OUTLET) bc_guo(n, x, y, z, false, snap);
                 }

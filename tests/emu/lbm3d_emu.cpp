@@ -55,7 +55,8 @@ int run3(const lbm3d_config& c, const float* f_in, const uint8_t* flags, const f
     p.guo_own = (c.quirks & LBM_QUIRK_GUO3D_OWN_MOMENTS) ? 1 : 0;
     bool guo = false;
     if (flags)
-        for (long long n = 0; n < N; n++) guo = guo || flags[n] == LBM_GUO_VELOCITY_INLET || flags[n] == LBM_GUO_PRESSURE_OUTLET || flags[n] == LBM_ZG_OUTFLOW;
+        for (long long n = 0; n < N; n++) guo = guo || flags[n] == LBM_GUO_VELOCITY_INLET || flags[n] == LBM_GUO_PRESSURE_OUTLET || flags[n] == LBM_ZG_OUTFLOW ||
+                           flags[n] == LBM_REGULARIZED_BOUNCE_BACK || flags[n] == LBM_REGULARIZED_INLET_LEFT;
     p.guo = guo ? 1 : 0;
     p.guo_ring_outlet = (guo && (c.nx - 1) % 64 == 0) ? 1 : 0;
     p.stale_rest = (c.quirks & LBM_QUIRK_STALE_REST) ? 1 : 0;
@@ -82,7 +83,7 @@ int run3(const lbm3d_config& c, const float* f_in, const uint8_t* flags, const f
                 const int x = side ? p.nx - 1 : 0, y = int(r % p.ny), z = int(r / p.ny);
                 if (!guo_ring(p, x, y, z)) continue;
                 if (seen[size_t((z * p.ny + y) * p.nx + x)]++) std::abort();
-                step3_node<COLL, STORE, false, true>(p, x, y, z);
+                step3_node<COLL, STORE, false, 2>(p, x, y, z);
             }
         }
         for (long long b = 0; b < nb; b++) {
@@ -101,7 +102,7 @@ int run3(const lbm3d_config& c, const float* f_in, const uint8_t* flags, const f
                         const bool gl = fl == LBM_GUO_VELOCITY_INLET || fl == LBM_GUO_PRESSURE_OUTLET || fl == LBM_ZG_OUTFLOW;
                         if ((pass == 0) != gl) continue;
                         if (seen[size_t((z * p.ny + y) * p.nx + x)]++) std::abort();
-                        if (guo) step3_node<COLL, STORE, false, true>(p, x, y, z);
+                        if (guo) step3_node<COLL, STORE, false, 2>(p, x, y, z);
                         else step3_node<COLL, STORE, false>(p, x, y, z);
                     }
         }

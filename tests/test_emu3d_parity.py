@@ -263,6 +263,52 @@ def zg_case(case):
     return c, f0, fl
 
 
+REG_CASES = [
+    # shape, periodic (y, z), collision, layout, quirks
+    ((9, 6, 5), (0, 0), _abi.LBM_BGK, "square", 0),
+    ((8, 6, 5), (0, 0), _abi.LBM_CM, "closed", 0),
+    ((65, 5, 4), (0, 1), _abi.LBM_MRT, "square", 0),   # (NX-1) % 64 == 0: the ring launch
+    ((7, 5, 6), (0, 0), _abi.LBM_BGK, "closed", _abi.LBM_QUIRK_EQ3D_LINEAR),
+]
+
+
+def reg_case(case):
+    """Regularized boundaries (regularizedBoundary3D.cuh). "square": the flags of
+    flowPastSquareCylinderScenario.cuh:10-31 without the body (corner-edge bounce-back on edges,
+    regularized inlet at x = 0, Guo outlet at x = NX-1, bounce-back on y and z); "closed":
+    REGULARIZED_BOUNCE_BACK on every face, the lid x = 0 a regularized inlet."""
+    shape, (py, pz), coll, layout, quirks = REG_CASES[case]
+    nx, ny, nz = shape
+    c = make_cfg(shape, (0, py, pz), 1.25, collision=coll, S48=(1.25,) * 5, quirks=quirks)
+    c.u_inlet = 0.04
+    f0 = initial_state(shape, 170 + case)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    wall_y = ~np.bool_(py) & ((y == 0) | (y == ny - 1))
+    wall_z = ~np.bool_(pz) & ((z == 0) | (z == nz - 1))
+    fl = np.zeros((nz, ny, nx), np.uint8)
+    if layout == "square":
+        fl[wall_y | wall_z] = _abi.BOUNCE_BACK
+        fl[x == nx - 1] = _abi.GUO_PRESSURE_OUTLET
+        fl[x == 0] = _abi.REGULARIZED_INLET_LEFT
+        edges = (x == 0) + (x == nx - 1) + (y == 0) + (y == ny - 1) + (z == 0) + (z == nz - 1)
+        fl[edges >= 2] = _abi.CORNER_EDGE_BOUNCE_BACK
+    else:
+        fl[(x == nx - 1) | wall_y | wall_z] = _abi.REGULARIZED_BOUNCE_BACK
+        fl[x == 0] = _abi.REGULARIZED_INLET_LEFT
+    return c, f0, fl
+
+
+@pytest.mark.parametrize("case", range(len(REG_CASES)))
+def test_emulated_regularized_boundaries_match_oracle(case):
+    c, f0, fl = reg_case(case)
+    o = oracle_run(c, f0, fl, None, 6)
+    g, rho, u = emu_run(c, f0, fl, None, 6)
+    assert np.abs(g - o.get_pops()).max() < 1e-5
+    r_rho, r_u = o.get_macros()
+    assert np.abs(rho - r_rho).max() < 1e-5
+    assert np.abs(u - r_u).max() < 1e-6
+
+
 @pytest.mark.parametrize("case", range(len(ZG_CASES)))
 def test_emulated_zg_outflow_matches_oracle(case):
     """x-face ZG outflow (neighbour pulled by the streaming rule, Guo-lane ordering, ring launch)

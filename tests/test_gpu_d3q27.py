@@ -7,8 +7,8 @@ import oracle as O
 from lbm_amd import _abi as A
 from lbm_amd.solver import LbmError
 from lbm_amd.solver3d import Config3D, Solver3D
-from test_emu3d_parity import (CASES, CM_CASES, GUO_CASES, MRT_CASES, ZG_CASES, box_flags, cm_case, guo_case,
-                               initial_state, make_cfg, mrt_case, oracle_run, zg_case)
+from test_emu3d_parity import (CASES, CM_CASES, GUO_CASES, MRT_CASES, REG_CASES, ZG_CASES, box_flags, cm_case, guo_case,
+                               initial_state, make_cfg, mrt_case, oracle_run, reg_case, zg_case)
 
 pytestmark = pytest.mark.gpu
 W = O.tables3d()[0]
@@ -91,6 +91,23 @@ def test_gpu_mrt3_matches_dense_oracle(gpu_available, case):
         s.set_pops(f0)
         s.set_flags(flags)
         s.set_force_field(None if force is None else force.astype(np.float32))
+        s.step(8)
+        g = s.get_pops()
+        rho, u = s.get_macros()
+    assert np.abs(g - o.get_pops()).max() < 1e-5
+    r_rho, r_u = o.get_macros()
+    assert np.abs(rho - r_rho).max() < 1e-5
+    assert np.abs(u - r_u).max() < 1e-6
+
+
+@pytest.mark.parametrize("case", range(len(REG_CASES)))
+def test_gpu_regularized_boundaries_match_oracle(gpu_available, case):
+    c, f0, fl = reg_case(case)
+    f0 = f0.astype(np.float32)
+    o = oracle_run(c, f0, fl, None, 8)
+    with Solver3D(gpu_config(c)) as s:
+        s.set_pops(f0)
+        s.set_flags(fl)
         s.step(8)
         g = s.get_pops()
         rho, u = s.get_macros()

@@ -223,6 +223,52 @@ def test_zg_outflow_copies_the_entering_populations_from_the_interior_neighbour(
     assert not np.array_equal(fa, fb)
 
 
+def test_regularized_boundary_imposes_zou_he_density_and_the_wall_velocity():
+    """RegularizedBoundary (regularizedBoundary3D.cuh:10-137) after streaming: on a face node the
+    populations are rebuilt so that their density is the Zou-He density of the known post-stream
+    populations, rho = (sum_{c.n = 0} f + 2 sum_{c.n < 0} f) / (1 - u.n) with n the inward normal
+    of the LAST matching face in the order x, y, z, and their momentum is rho * u_imposed (the
+    second-order equilibrium carries rho u and the Q : Pi projection carries none); interior nodes
+    are unchanged."""
+    nx, ny, nz = 5, 4, 6
+    u_in = float(np.float32(0.04))  # lbm3d_config.u_inlet is float
+    rng = np.random.default_rng(5)
+    f0 = W[:, None, None, None] * (1 + 0.1 * rng.random((27, nz, ny, nx)))
+
+    def run(fl):
+        c = cfg3(nx, ny, nz, periodic=(0, 0, 0))
+        c.u_inlet = u_in
+        o = O.Oracle3D(c, 64)
+        o.set_pops(f0)
+        if fl is not None:
+            o.set_flags(fl)
+        o.stream_boundaries()
+        return o.get_pops()
+
+    fl = np.full((nz, ny, nx), _abi.REGULARIZED_BOUNCE_BACK, np.uint8)
+    fl[:, :, 0] = _abi.REGULARIZED_INLET_LEFT
+    fa, fb = run(fl), run(None)
+    checked = 0
+    for z, y, x in itertools.product(range(nz), range(ny), range(nx)):
+        nrm = None
+        for axis, (p, n) in enumerate(((x, nx), (y, ny), (z, nz))):
+            if p in (0, n - 1):
+                nrm = np.zeros(3)
+                nrm[axis] = 1 if p == 0 else -1
+        if nrm is None:
+            assert np.array_equal(fa[:, z, y, x], fb[:, z, y, x])
+            continue
+        u = np.array([u_in if x == 0 else 0.0, 0.0, 0.0])
+        cn = C @ nrm
+        g = fb[:, z, y, x]
+        rho = (g[cn == 0].sum() + 2 * g[cn < 0].sum()) / (1 - u @ nrm)
+        h = fa[:, z, y, x]
+        assert abs(h.sum() - rho) < 1e-13
+        assert np.allclose(C.T @ h, rho * u, rtol=0, atol=1e-13)
+        checked += 1
+    assert checked == nx * ny * nz - (nx - 2) * (ny - 2) * (nz - 2)
+
+
 def test_force_driven_channel_has_curvature_minus_f_over_nu():
     """Walls at y = 0 and y = NY-1 (BOUNCE_BACK), periodic x and z, body force along x: the
     steady profile satisfies nu u'' = -F in the bulk and is symmetric about the centre."""
