@@ -270,9 +270,21 @@ def run_d3q27(torch, steps=200, warmup=20):
     u0[..., 0] = 0.05
     wcfg = Config3D(nx, ny, nz, collision=A.LBM_CM, omega=1.0 / (3.0 * 0.05 * ny / 1000.0 + 0.5), u_inlet=0.05)
     res["wind_tunnel_256x128x96_cm_f32"] = one(wcfg, (np.ones((nz, ny, nx), np.float32), u0), wt)
+    # the flags of flowPastSquareCylinderScenario.cuh:10-31 without its body: corner-edge
+    # bounce-back on edges and corners, regularized inlet at x = 0, Guo pressure outlet at
+    # x = NX-1, bounce-back on y and z (the regularized kernel variant)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij", sparse=True)
+    faces = (x == 0) * 1 + (x == nx - 1) + (y == 0) + (y == ny - 1) + (z == 0) + (z == nz - 1)
+    sq = np.zeros((nz, ny, nx), np.uint8)
+    sq[np.broadcast_to((y == 0) | (y == ny - 1) | (z == 0) | (z == nz - 1), sq.shape)] = A.BOUNCE_BACK
+    sq[:, :, -1] = A.GUO_PRESSURE_OUTLET
+    sq[:, :, 0] = A.REGULARIZED_INLET_LEFT
+    sq[faces >= 2] = A.CORNER_EDGE_BOUNCE_BACK
+    res["square_duct_256x128x96_cm_f32"] = one(wcfg, (np.ones((nz, ny, nx), np.float32), u0), sq)
     res["workload"] = ("D3Q27 BGK with Guo forcing (and MRT<3>, CM<3> at 256^3), FP32 collision; Taylor-Green 3-D "
                        "(u_max 0.05) periodic, a force-driven channel with bounce-back walls on y and z, and the wind "
-                       "tunnel (CM<3>, Guo inlet/outlet, walls; planes launch + main launch per step); 216 B "
+                       "tunnel (CM<3>, Guo inlet/outlet, walls) and the square-cylinder duct's flags (corner-edge "
+                       "bounce-back, regularized inlet, Guo outlet, walls) without the body; 216 B "
                        "(FP32) / 108 B (FP16) per node update")
     return res
 
