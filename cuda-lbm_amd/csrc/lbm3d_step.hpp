@@ -42,6 +42,8 @@ struct Step3Params {
     int guo_ring_outlet;   // (NX-1) % 64 == 0: the outlet's ring nodes run in a launch of their
                            // own before the main launch, which skips them
     int stale_rest;        // LBM_QUIRK_STALE_REST
+    int nt;                // store with the non-temporal cache policy (buffers beyond the
+                           // Infinity Cache)
     // Workgroup shape: 256 threads = wx wavefronts along x (64*wx consecutive nodes of a row)
     // times 4/wx rows; nbx, nby workgroups along x and y, nz along z.
     int wx;
@@ -294,8 +296,13 @@ LBM3_HD void step3_node(const Step3Params& p, int x, int y, int z) {
     if (COLL == LBM_MRT) collide_mrt3(g, m, fx, fy, fz, p.mrt);
     else if (COLL == LBM_CM) collide_cm3(g, m, fx, fy, fz, p.omega);
     else collide_bgk3(g, m, fx, fy, fz, p.omega, p.lin_ec);
+    if (p.nt) {
 #pragma unroll
-    for (int q = 0; q < Q; q++) dst.st_nt(node, (long long)q * p.plane, St::enc(g[q], q, p.rho0));
+        for (int q = 0; q < Q; q++) dst.st_nt(node, (long long)q * p.plane, St::enc(g[q], q, p.rho0));
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; q++) dst.st(node, (long long)q * p.plane, St::enc(g[q], q, p.rho0));
+    }
 }
 
 }  // namespace lbm3
