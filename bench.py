@@ -35,31 +35,41 @@ def bytes_per_node(storage):
     return 2 * Q * (4 if storage == A.LBM_F32 else 2)
 
 
-def time_steps(solver, steps, warmup, torch):
-    """Warm up, then time `steps` steps with HIP events on the solver's stream."""
+BATCHES = 5
+
+
+def time_steps(solver, steps, warmup, torch, batches=BATCHES):
+    """Warm up, then time `batches` batches of exactly `steps` steps, each with HIP events on the
+    solver's stream and a device synchronisation on both sides. Returns (median seconds,
+    per-batch seconds)."""
     stream = torch.cuda.Stream()
     solver.set_stream(stream.cuda_stream)
     solver.step(warmup)
     torch.cuda.synchronize()
-    start = torch.cuda.Event(enable_timing=True)
-    end = torch.cuda.Event(enable_timing=True)
-    with torch.cuda.stream(stream):
-        start.record(stream)
-        solver.step(steps)
-        end.record(stream)
-    torch.cuda.synchronize()
-    return start.elapsed_time(end) / 1000.0  # seconds
+    secs = []
+    for _ in range(batches):
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(stream):
+            start.record(stream)
+            solver.step(steps)
+            end.record(stream)
+        torch.cuda.synchronize()
+        secs.append(start.elapsed_time(end) / 1000.0)
+    return sorted(secs)[len(secs) // 2], secs
 
 
-def run_case(case, steps, warmup, torch):
-    """MLUPS, and the bandwidth of the step kernel: a launch advances `steps_per_launch`
-    steps and moves every population once in and once out (2*Q*s bytes per node), so the
-    algorithmic bytes per node update are 2*Q*s / steps_per_launch."""
+def run_case(case, steps, warmup, torch, batches=BATCHES):
+    """MLUPS (median of `batches` batches of `steps` steps), and the bandwidth of the step
+    kernel: a launch advances `steps_per_launch` steps and moves every population once in and
+    once out (2*Q*s bytes per node), so the algorithmic bytes per node update are
+    2*Q*s / steps_per_launch."""
     from lbm_amd import Solver
 
     s = Solver(case.config)
     case.apply(s)
-    secs = time_steps(s, steps, warmup, torch)
+    secs, all_secs = time_steps(s, steps, warmup, torch, batches)
     spl = s.steps_per_launch
     N = case.config.N
     mlups = N * steps / secs / 1e6
@@ -69,7 +79,8 @@ def run_case(case, steps, warmup, torch):
     return s, {"mlups": round(mlups, 1), "ms_per_step": round(1000.0 * secs / steps, 5),
                "steps_per_launch": spl, "us_per_launch": round(1e6 * launch_s, 3),
                "GB_s": round(gbs, 1), "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
-               "bytes_per_node_per_launch": B, "bytes_per_node_update": B / spl}
+               "bytes_per_node_per_launch": B, "bytes_per_node_update": B / spl,
+               "batches": len(all_secs), "batch_mlups": [round(N * steps / t / 1e6, 1) for t in all_secs]}
 
 
 def run_adjoint(torch, steps=2000, every=100):
@@ -139,6 +150,57 @@ def run_adjoint(torch, steps=2000, every=100):
                                  "intermediate state stored) + adjoint step; cost ratio against the default forward "
                                  "(two steps per launch)",
             "gradient_finite": ok}
+
+
+def run_inverse_problem(torch, steps=2000, every=100, directions=16, iters=200, lr=1e-6):
+    """The differentiable inverse problem as configured: BGK 512^2 periodic, nu = 0.02, FP32,
+    2000 steps, checkpoint every 100; the unknown is the 16-mode Fourier body force (amplitude
+    1e-5, seed 0xF0F0); the loss is the L2 misfit of the final velocity. Reports the adjoint
+    gradient at F = 0 against central finite differences along 16 seeded random directions
+    (seed 0xD1D1), and Adam's recovery of the field from F = 0 over 200 iterations."""
+    import numpy as np
+
+    from lbm_amd import scenarios as sc
+    from lbm_amd.autograd import Simulation, recover_force
+
+    case = sc.inverse_problem(512, steps=steps)
+    dev = torch.device("cuda")
+    sim = Simulation(case.config, nsteps=steps, checkpoint_every=every,
+                     rho0=torch.as_tensor(case.rho, device=dev))
+    u0 = torch.as_tensor(case.u, device=dev)
+    F_true = torch.as_tensor(case.force_field, device=dev)
+    with torch.no_grad():
+        u_target = sim(F_true, u0).clone()
+
+    def loss_of(F):
+        with torch.no_grad():
+            return float(torch.sum((sim(F, u0).double() - u_target.double()) ** 2))
+
+    F0 = torch.zeros_like(F_true, requires_grad=True)
+    loss = torch.sum((sim(F0, u0) - u_target) ** 2)
+    loss.backward()
+    g = F0.grad.detach().double()
+    rng = np.random.default_rng(0xD1D1)
+    errs = []
+    for _ in range(directions):
+        d = torch.as_tensor(rng.standard_normal(tuple(F_true.shape)), device=dev)
+        d *= 1e-6 / float(d.abs().max())
+        fd = (loss_of((F0.detach() + d).float()) - loss_of((F0.detach() - d).float())) / 2.0
+        an = float(torch.sum(g * d))
+        errs.append(abs(fd - an) / max(abs(an), 1e-300))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    F_rec, losses = recover_force(sim, u_target, u0, iters=iters, lr=lr)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    sim.close()
+    ferr = float(torch.linalg.vector_norm(F_rec - F_true) / torch.linalg.vector_norm(F_true))
+    return {"workload": "inverse problem BGK 512x512 periodic nu=0.02 FP32, 2000 steps, checkpoint every 100, "
+                        "16-mode Fourier force amplitude 1e-5",
+            "fd_directions": directions, "fd_max_rel_err": float(max(errs)), "fd_median_rel_err": float(np.median(errs)),
+            "adam_iters": iters, "adam_lr": lr, "loss_first": losses[0], "loss_last": losses[-1],
+            "loss_ratio": losses[-1] / losses[0], "force_rel_l2_err": ferr,
+            "seconds_per_iteration": round(dt / iters, 4)}
 
 
 def run_accuracy(torch):
@@ -237,7 +299,7 @@ def run_d3q27(torch, steps=200, warmup=20):
             s.init_from_macros(*init)
             if flags is not None:
                 s.set_flags(flags)
-            secs = time_steps(s, n_steps, warmup, torch)
+            secs, _ = time_steps(s, n_steps, warmup, torch, batches=3)
             B = 2 * 27 * (4 if cfg.storage == A.LBM_F32 else 2)
             gbs = B * cfg.N / (secs / n_steps) / 1e9
             return {"mlups": round(cfg.N * n_steps / secs / 1e6, 1), "us_per_step": round(1e6 * secs / n_steps, 2),
@@ -320,6 +382,19 @@ def copy_calibration(torch, mb):
     return {"read": 4 * n, "write": 4 * n}
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(sample_steps=800):
     """The FP32 oracle (OpenMP over rows) on the host cores: a bounded sample of the
     same cavity (1024^2 MRT, `sample_steps` steps)."""
@@ -335,9 +410,139 @@ def cpu_baseline(sample_steps=800):
     o.step(sample_steps)
     dt = time.perf_counter() - t0
     return {"value": round(case.config.N * sample_steps / dt / 1e6, 2), "unit": "MLUPS",
-            "cores": orc.num_threads(), "kind": "port",
+            "cores": orc.num_threads(), "cpu_model": cpu_model(), "kind": "port",
             "sample": f"lid-driven cavity 1024x1024 MRT FP32, {sample_steps} steps after 1 warm-up step "
                       f"({dt:.1f} s)"}
+
+
+# Step counts of the FP64 host runs the configs' precision is stated against (bounded so the
+# host part of the bench stays near 20 s on 16 cores).
+REF_STEPS = {"ldc": 200, "poiseuille": 60, "ibm": 40, "tg": 1000}
+
+
+def cpu_fp64_references():
+    """FP64 oracle runs of the configs, the "FP64 run" each config's precision is stated
+    against: the cavity, the channel and the IBM cylinder over REF_STEPS steps, Taylor-Green
+    over its full 1000 steps. Host only; the GPU runs the same cases in run_config_precision."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import numpy as np
+
+    import oracle as orc
+    from lbm_amd import scenarios as sc
+
+    t0 = time.perf_counter()
+    out = {}
+    c = sc.lid_driven_cavity(1024)
+    o = orc.Oracle(c.config.to_c(), precision=64)
+    c.apply_oracle(o)
+    o.step(REF_STEPS["ldc"])
+    out["ldc_u"] = o.get_macros()[1]
+    del o
+    c = sc.poiseuille(4096, 1024, adapter=0)
+    o = orc.Oracle(c.config.to_c(), precision=64)
+    c.apply_oracle(o)
+    o.step(REF_STEPS["poiseuille"])
+    out["poiseuille_u"] = o.get_macros()[1]
+    del o
+    c = sc.ibm_cylinder()
+    o = orc.Oracle(c.config.to_c(), precision=64)
+    c.apply_oracle(o)
+    series = []
+    for _ in range(REF_STEPS["ibm"]):
+        o.step(1)
+        series.append(o.ibm_get_forces()[1])
+    out["ibm_series"] = np.array(series)
+    del o
+    c = sc.taylor_green(128, nu=0.01, u0=0.04)
+    o = orc.Oracle(c.config.to_c(), precision=64)
+    c.apply_oracle(o)
+    o.step(REF_STEPS["tg"])
+    out["tg_error"] = sc.tg_error(o.get_macros()[1], 0.01, 0.04, REF_STEPS["tg"])
+    out["seconds"] = round(time.perf_counter() - t0, 1)
+    return out
+
+
+def run_config_precision(torch, refs):
+    """Each config's stated precision, on the GPU:
+    - cavity 1024^2 MRT, Re = 1000: FP32 and FP16 storage against the FP64 host run after
+      REF_STEPS steps, and the Ghia centre-line NRMSE at the configured 50k steps;
+    - Poiseuille 4096x1024 (CM without the adapter, which diverges on this channel): FP32, BF16
+      and FP16 after the configured 20k steps, as the reference's RMS % of u_max against its
+      steady parabola (the config's analytic) and against the start-up solution at that step,
+      and against the FP64 host run after REF_STEPS steps;
+    - IBM cylinder: the drag and lift series over REF_STEPS steps against the FP64 host run;
+    - Taylor-Green: the FP64 host run's own analytic error beside the GPU's (run_accuracy).
+    Fields compared are velocities; "rel_l2" = ||u - u_ref|| / ||u_ref||."""
+    import numpy as np
+
+    from lbm_amd import Solver
+    from lbm_amd import _abi as A
+    from lbm_amd import scenarios as sc
+
+    def run(case, steps):
+        s = Solver(case.config)
+        case.apply(s)
+        s.step(steps)
+        _, u = s.get_macros()
+        s.close()
+        return u
+
+    out = {}
+    ldc = {"re": 1000, "ref_steps": REF_STEPS["ldc"], "steps": 50000}
+    for name, st in (("f32", A.LBM_F32), ("f16", A.LBM_F16_SHIFTED)):
+        r = {}
+        if refs:
+            r["u_rel_l2_vs_fp64"] = rel_l2(run(sc.lid_driven_cavity(1024, storage=st), REF_STEPS["ldc"]), refs["ldc_u"])
+        u = run(sc.lid_driven_cavity(1024, storage=st), 50000)
+        r["ghia_nrmse_pct"] = round(sc.ghia_error(u, 0.1, 1000), 3)
+        ldc[name] = r
+    out["lid_driven_cavity_1024_mrt"] = ldc
+    nu, umax, T = 0.05, 0.05, 20000
+    startup = sc.poiseuille_startup_profile(1024, nu, umax, T)
+    poi = {"adapter": "none (CM<OptimalAdapter> diverges on this channel, in the FP64 oracle too)", "steps": T,
+           "ref_steps": REF_STEPS["poiseuille"]}
+    for name, st in (("f32", A.LBM_F32), ("bf16", A.LBM_BF16_SHIFTED), ("f16", A.LBM_F16_SHIFTED)):
+        u = run(sc.poiseuille(4096, 1024, storage=st, adapter=0), T)
+        prof = u[..., 0].astype(np.float64).mean(axis=1)
+        r = {"rms_pct_umax_vs_parabola": round(sc.poiseuille_error(prof, nu, umax), 4),
+             "rms_pct_umax_vs_startup": round(sc.poiseuille_error(prof, nu, umax, profile=startup), 4),
+             "centre_u": float(prof[512])}
+        if refs:
+            r["u_rel_l2_vs_fp64"] = rel_l2(run(sc.poiseuille(4096, 1024, storage=st, adapter=0),
+                                               REF_STEPS["poiseuille"]), refs["poiseuille_u"])
+        poi[name] = r
+    poi["startup_centre_u"] = float(startup[512])
+    out["poiseuille_4096x1024_cm"] = poi
+    if refs:
+        c = sc.ibm_cylinder()
+        s = Solver(c.config)
+        c.apply(s)
+        series = []
+        for _ in range(REF_STEPS["ibm"]):
+            s.step(1)
+            series.append(s.ibm_get_forces()[1])
+        s.close()
+        g, h = np.array(series), refs["ibm_series"]
+        out["ibm_cylinder_drag_lift"] = {
+            "steps": REF_STEPS["ibm"],
+            "drag_max_rel_err": float(np.max(np.abs(g[:, 0] - h[:, 0])) / np.max(np.abs(h[:, 0]))),
+            "lift_max_err_over_max_drag": float(np.max(np.abs(g[:, 1] - h[:, 1])) / np.max(np.abs(h[:, 0]))),
+            "drag_last": float(g[-1, 0]), "lift_last": float(g[-1, 1])}
+        out["taylor_green_fp64_error_pct"] = round(refs["tg_error"], 5)
+    out["storage_validity"] = {
+        "f32": "all configs",
+        "f16": "Taylor-Green and the cavity; not the force-driven channel from rest, whose per-step "
+               "increments fall below half an FP16 ulp of the stored deviations",
+        "bf16": "reported only (8-bit mantissa): Taylor-Green error 8x the FP64 run's"}
+    return out
+
+
+_T0 = time.perf_counter()
+
+
+def progress(what):
+    """One line per finished section on stderr (stdout carries the single JSON line)."""
+    print(f"bench: {what} ({time.perf_counter() - _T0:.1f} s)", file=sys.stderr, flush=True)
 
 
 def main():
@@ -347,7 +552,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-extras", action="store_true", help="only the headline workload")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["hbm_grid", "adjoint", "poiseuille", "ibm_cylinder", "d3q27"], default=None,
+    ap.add_argument("--only", choices=["hbm_grid", "adjoint", "poiseuille", "ibm_cylinder", "d3q27", "inverse",
+                                       "precision"], default=None,
                     help="run one extra section alone (for profiling its kernels); prints that section")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="also run one plain device copy of MB megabytes (for scaling profiler byte counters)")
@@ -381,6 +587,13 @@ def main():
     if args.only == "poiseuille":
         print(json.dumps({"poiseuille": run_poiseuille(torch)}))
         return
+    if args.only == "inverse":
+        print(json.dumps({"inverse_problem": run_inverse_problem(torch)}))
+        return
+    if args.only == "precision":
+        refs = None if args.no_cpu else cpu_fp64_references()
+        print(json.dumps({"config_precision": run_config_precision(torch, refs)}))
+        return
     if args.only == "ibm_cylinder":
         print(json.dumps({"ibm_cylinder": run_ibm_cylinder(torch)}))
         return
@@ -396,6 +609,7 @@ def main():
     s, head = run_case(case, args.steps, args.warmup, torch)
     _, u32 = s.get_macros()
     s.close()
+    progress("headline cavity")
     out = {
         "metric": METRIC,
         "value": head["mlups"],
@@ -443,15 +657,30 @@ def main():
         grid["speedup_f16_vs_f32"] = round(grid["f16"]["mlups"] / grid["f32"]["mlups"], 3)
         grid["workload"] = "periodic BGK 8192x8192 (2 x 2.4 GB FP32 population buffers)"
         out["hbm_grid"] = grid
+        progress("hbm grid")
         out["ibm_cylinder"] = run_ibm_cylinder(torch)
+        progress("ibm cylinder")
         out["poiseuille"] = run_poiseuille(torch)
+        progress("poiseuille")
         out["adjoint"] = run_adjoint(torch)
+        progress("adjoint")
+        out["inverse_problem"] = run_inverse_problem(torch)
+        progress("inverse problem")
         out["accuracy"] = run_accuracy(torch)
         out["d3q27"] = run_d3q27(torch, steps=max(20, args.steps // 10))
+        progress("d3q27")
     if args.calibrate_copy:
         out["copy_calibration_bytes"] = copy_calibration(torch, args.calibrate_copy)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
+        progress("cpu baseline")
+        if not args.no_extras:
+            refs = cpu_fp64_references()
+            progress("fp64 host runs")
+            out["config_precision"] = run_config_precision(torch, refs)
+            out["config_precision"]["fp64_host_runs_s"] = refs["seconds"]
+    elif not args.no_extras:
+        out["config_precision"] = run_config_precision(torch, None)
     print(json.dumps(out))
 
 

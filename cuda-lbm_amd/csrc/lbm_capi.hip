@@ -396,6 +396,7 @@ int lbm_set_stream(lbm_ctx* ctx, void* s) {
 
 int lbm_set_flags(lbm_ctx* ctx, const uint8_t* flags, int on_device) {
     if (!ctx || !flags) return LBM_E_INVALID;
+    ctx->adj.ready = false;  // the adjoint's recorded forward no longer matches the state
     int rc = ctx->activate();
     if (rc) return rc;
     const int nx = ctx->cfg.nx, ny = ctx->cfg.ny;
@@ -475,6 +476,7 @@ int lbm_set_flags(lbm_ctx* ctx, const uint8_t* flags, int on_device) {
 
 int lbm_init_from_macros(lbm_ctx* ctx, const float* rho, const float* u, int on_device) {
     if (!ctx || !rho || !u) return LBM_E_INVALID;
+    ctx->adj.ready = false;  // the adjoint's recorded forward no longer matches the state
     int rc = ctx->activate();
     if (rc) return rc;
     const float *drho = rho, *du = u;
@@ -512,6 +514,7 @@ int lbm_init_from_macros(lbm_ctx* ctx, const float* rho, const float* u, int on_
 
 int lbm_set_force_field(lbm_ctx* ctx, const float* force, int on_device) {
     if (!ctx) return LBM_E_INVALID;
+    ctx->adj.ready = false;  // the adjoint's recorded forward no longer matches the state
     int rc = ctx->activate();
     if (rc) return rc;
     ctx->ibm.box_valid = false;  // the IBM box's external force changes with the field
@@ -532,6 +535,7 @@ int lbm_set_force_field(lbm_ctx* ctx, const float* force, int on_device) {
 
 int lbm_step(lbm_ctx* ctx, int32_t nsteps) {
     if (!ctx || nsteps < 0) return LBM_E_INVALID;
+    ctx->adj.ready = false;  // the adjoint's recorded forward no longer matches the state
     if (!ctx->initialized) return ctx->fail(LBM_E_STATE, "lbm_step before lbm_init_from_macros / lbm_set_pops");
     int rc = ctx->activate();
     if (rc) return rc;
@@ -627,6 +631,7 @@ int lbm_get_pops(lbm_ctx* ctx, float* f, int on_device) {
 
 int lbm_set_pops(lbm_ctx* ctx, const float* f, int on_device) {
     if (!ctx || !f) return LBM_E_INVALID;
+    ctx->adj.ready = false;  // the adjoint's recorded forward no longer matches the state
     int rc = ctx->activate();
     if (rc) return rc;
     const float* in = f;

@@ -13,10 +13,11 @@
 // which changes only with the force field: they are written on the first pass after the
 // markers or the field change, and later passes write the band alone.
 #include "lbm_ctx.hpp"
+#include "lbm_ibm_tables.hpp"
 
 #include <algorithm>
 #include <cmath>
-#include <map>
+#include <mutex>
 #include <vector>
 
 using namespace lbm;
@@ -202,14 +203,6 @@ __global__ __launch_bounds__(kIbmBlock) void ibm_kernel(IbmParams p) {
     }
 }
 
-// delta4 (IBMUtils.cuh:23-33), FP32 as in the reference.
-float delta4f(float r) {
-    float rabs = std::fabs(r);
-    if (rabs < 1.0f) return 0.125f * (3.0f - 2.0f * rabs + std::sqrt(1.0f + 4.0f * rabs - 4.0f * r * r));
-    if (rabs < 2.0f) return 0.125f * (5.0f - 2.0f * rabs - std::sqrt(-7.0f + 12.0f * rabs - 4.0f * r * r));
-    return 0.0f;
-}
-
 template <class T>
 int put(lbm_ctx* c, T** dst, const std::vector<T>& v, const char* what) {
     size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
@@ -217,6 +210,28 @@ int put(lbm_ctx* c, T** dst, const std::vector<T>& v, const char* what) {
     if (rc) return rc;
     if (v.empty()) return LBM_OK;
     return c->upload(*dst, v.data(), v.size() * sizeof(T), 0);
+}
+
+// Raise the dynamic-LDS limit of the six IBM kernel instances to the device's per-workgroup
+// maximum, once per process (a process-wide attribute of each kernel: set it once, with one
+// value, rather than per marker set). Returns whether the limit is in place.
+bool allow_dynamic_lds(int max_bytes) {
+    static std::once_flag once;
+    static bool ok = false;
+    std::call_once(once, [max_bytes] {
+        const void* k[] = {reinterpret_cast<const void*>(&ibm_kernel<LBM_F32, true>),
+                           reinterpret_cast<const void*>(&ibm_kernel<LBM_F16_SHIFTED, true>),
+                           reinterpret_cast<const void*>(&ibm_kernel<LBM_BF16_SHIFTED, true>),
+                           reinterpret_cast<const void*>(&ibm_kernel<LBM_F32, false>),
+                           reinterpret_cast<const void*>(&ibm_kernel<LBM_F16_SHIFTED, false>),
+                           reinterpret_cast<const void*>(&ibm_kernel<LBM_BF16_SHIFTED, false>)};
+        bool all = true;
+        for (const void* f : k)
+            if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_bytes) != hipSuccess) all = false;
+        (void)hipGetLastError();
+        ok = all;
+    });
+    return ok;
 }
 
 }  // namespace
@@ -284,86 +299,12 @@ int lbm_ibm_set_markers(lbm_ctx* ctx, const float* xy, const float* ds, int32_t 
     (void)hipStreamSynchronize(ctx->stream);
     ctx->ibm_free();
     if (n == 0) return LBM_OK;
+    IbmTables T;
+    const char* why = ibm_build_tables(ctx->cfg.nx, ctx->cfg.ny, (ctx->cfg.quirks & LBM_QUIRK_IBM_2X2) != 0,
+                                       (ctx->cfg.quirks & LBM_QUIRK_IBM_NO_DS) != 0, xy, ds, n, T);
+    if (why) return ctx->fail(LBM_E_INVALID, "%s", why);
+    const int nb = int(T.band_node.size());
     const int nx = ctx->cfg.nx, ny = ctx->cfg.ny;
-    const bool two = (ctx->cfg.quirks & LBM_QUIRK_IBM_2X2) != 0;
-    const bool no_ds = (ctx->cfg.quirks & LBM_QUIRK_IBM_NO_DS) != 0;
-    const int W = two ? 2 : 4, off = two ? 0 : -1;
-    for (int l = 0; l < n; l++)
-        if (!std::isfinite(xy[2 * l]) || !std::isfinite(xy[2 * l + 1]) || (ds && !(ds[l] > 0.0f)))
-            return ctx->fail(LBM_E_INVALID, "marker %d: coordinates must be finite and ds positive", l);
-    // stencils in the reference's loop order (x offset outer, y inner), out-of-domain skipped
-    struct Ent { int node; float k; };
-    std::vector<std::vector<Ent>> st(static_cast<size_t>(n));
-    std::map<int, int> band;
-    for (int l = 0; l < n; l++) {
-        float px = xy[2 * l], py = xy[2 * l + 1];
-        int gx = int(std::floor(px)), gy = int(std::floor(py));
-        for (int i = 0; i < W; i++)
-            for (int j = 0; j < W; j++) {
-                int xx = gx + off + i, yy = gy + off + j;
-                if (xx >= nx || xx < 0 || yy >= ny || yy < 0) continue;
-                float k = delta4f(px - float(xx)) * delta4f(py - float(yy));
-                int node = yy * nx + xx;
-                st[l].push_back({node, k});
-                band.emplace(node, 0);
-            }
-    }
-    if (band.empty()) return ctx->fail(LBM_E_INVALID, "no marker stencil touches the grid");
-    std::vector<int> band_node;
-    int xmin = nx, xmax = -1, ymin = ny, ymax = -1;
-    for (auto& kv : band) {
-        kv.second = int(band_node.size());
-        band_node.push_back(kv.first);
-        int y = kv.first / nx, x = kv.first % nx;
-        xmin = std::min(xmin, x);
-        xmax = std::max(xmax, x);
-        ymin = std::min(ymin, y);
-        ymax = std::max(ymax, y);
-    }
-    const int nb = int(band_node.size());
-    std::vector<int> mk_ptr(size_t(n) + 1, 0), mk_band;
-    std::vector<float> mk_w;
-    std::vector<std::vector<std::pair<int, float>>> per_node(static_cast<size_t>(nb));
-    for (int l = 0; l < n; l++) {
-        float d = (ds && !no_ds) ? ds[l] : 1.0f;
-        for (const Ent& e : st[l]) {
-            int b = band[e.node];
-            mk_band.push_back(b);
-            mk_w.push_back(e.k);
-            per_node[size_t(b)].push_back({l, e.k * d});
-        }
-        mk_ptr[l + 1] = int(mk_band.size());
-    }
-    std::vector<int> bn_ptr(size_t(nb) + 1, 0), bn_mk;
-    std::vector<float> bn_w;
-    for (int b = 0; b < nb; b++) {
-        for (auto& pr : per_node[size_t(b)]) {
-            bn_mk.push_back(pr.first);
-            bn_w.push_back(pr.second);
-        }
-        bn_ptr[b + 1] = int(bn_mk.size());
-    }
-    // the box's x range in whole 4-node segments, so that one-step launches can split at it
-    xmin = xmin / 4 * 4;
-    xmax = std::min(nx, (xmax + 1 + 3) / 4 * 4) - 1;
-    const int bw = xmax - xmin + 1, bh = ymax - ymin + 1;
-    std::vector<int> box_map(size_t(bw) * bh, -1);
-    for (int b = 0; b < nb; b++) {
-        int y = band_node[b] / nx, x = band_node[b] % nx;
-        box_map[size_t(y - ymin) * bw + (x - xmin)] = b;
-    }
-    // Every index the kernel forms comes from these tables: check them all here.
-    for (int v : band_node)
-        if (v < 0 || size_t(v) >= ctx->N) return ctx->fail(LBM_E_INVALID, "IBM band node out of range");
-    for (int v : mk_band)
-        if (v < 0 || v >= nb) return ctx->fail(LBM_E_INVALID, "IBM stencil entry out of range");
-    for (int v : bn_mk)
-        if (v < 0 || v >= n) return ctx->fail(LBM_E_INVALID, "IBM spread entry out of range");
-    if (mk_ptr.back() != int(mk_band.size()) || bn_ptr.back() != int(bn_mk.size()))
-        return ctx->fail(LBM_E_INVALID, "IBM table sizes inconsistent");
-    for (int v : box_map)
-        if (v < -1 || v >= nb) return ctx->fail(LBM_E_INVALID, "IBM box map out of range");
-    if (xmin < 0 || ymin < 0 || xmax >= nx || ymax >= ny) return ctx->fail(LBM_E_INVALID, "IBM box outside the grid");
     size_t shmem = (size_t(5) * nb + size_t(4) * n) * sizeof(float) + size_t(nb) * sizeof(int);
     int maxsh = 0;
     (void)hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
@@ -376,14 +317,14 @@ int lbm_ibm_set_markers(lbm_ctx* ctx, const float* xy, const float* ds, int32_t 
         for (int l = 0; l < n; l++) dsv[size_t(l)] = ds[l];
     rc = put(ctx, &I.d_xy, xyv, "IBM markers");
     rc = rc ? rc : put(ctx, &I.d_ds, dsv, "IBM ds");
-    rc = rc ? rc : put(ctx, &I.d_band_node, band_node, "IBM band");
-    rc = rc ? rc : put(ctx, &I.d_mk_ptr, mk_ptr, "IBM stencils");
-    rc = rc ? rc : put(ctx, &I.d_mk_band, mk_band, "IBM stencils");
-    rc = rc ? rc : put(ctx, &I.d_mk_w, mk_w, "IBM weights");
-    rc = rc ? rc : put(ctx, &I.d_bn_ptr, bn_ptr, "IBM spread lists");
-    rc = rc ? rc : put(ctx, &I.d_bn_mk, bn_mk, "IBM spread lists");
-    rc = rc ? rc : put(ctx, &I.d_bn_w, bn_w, "IBM spread weights");
-    rc = rc ? rc : put(ctx, &I.d_box_map, box_map, "IBM box map");
+    rc = rc ? rc : put(ctx, &I.d_band_node, T.band_node, "IBM band");
+    rc = rc ? rc : put(ctx, &I.d_mk_ptr, T.mk_ptr, "IBM stencils");
+    rc = rc ? rc : put(ctx, &I.d_mk_band, T.mk_band, "IBM stencils");
+    rc = rc ? rc : put(ctx, &I.d_mk_w, T.mk_w, "IBM weights");
+    rc = rc ? rc : put(ctx, &I.d_bn_ptr, T.bn_ptr, "IBM spread lists");
+    rc = rc ? rc : put(ctx, &I.d_bn_mk, T.bn_mk, "IBM spread lists");
+    rc = rc ? rc : put(ctx, &I.d_bn_w, T.bn_w, "IBM spread weights");
+    rc = rc ? rc : put(ctx, &I.d_box_map, T.box_map, "IBM box map");
     rc = rc ? rc : ctx->dev_alloc(reinterpret_cast<void**>(&I.d_force_total), ctx->N * 8, "IBM force field");
     rc = rc ? rc : ctx->dev_alloc(reinterpret_cast<void**>(&I.d_lag_force), size_t(n) * 8, "IBM Lagrangian forces");
     if (rc) {
@@ -395,31 +336,16 @@ int lbm_ibm_set_markers(lbm_ctx* ctx, const float* xy, const float* ds, int32_t 
     I.n_markers = n;
     I.n_band = nb;
     I.iters = iters;
-    I.x0 = xmin;
-    I.x1 = xmax + 1;
-    I.y0 = ymin;
-    I.y1 = ymax + 1;
+    I.x0 = T.x0;
+    I.x1 = T.x1;
+    I.y0 = T.y0;
+    I.y1 = T.y1;
     I.shmem = shmem;
     I.split_ok = (ctx->cfg.periodic_x || (I.x0 > 0 && I.x1 < nx)) && (ctx->cfg.periodic_y || (I.y0 > 0 && I.y1 < ny));
-    I.n_ent = int(mk_band.size());
+    I.n_ent = int(T.mk_band.size());
     // the tables beside the band data: mk_ptr, bn_ptr, mk_band, bn_mk (ints), mk_w, bn_w
-    const size_t tables = (size_t(n) + 1 + size_t(nb) + 1 + 4 * mk_band.size()) * 4;
-    I.shmem_tables = shmem + tables <= size_t(maxsh) ? shmem + tables : 0;
-    if (I.shmem_tables > 0) {
-        const int want = int(I.shmem_tables);
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&ibm_kernel<LBM_F32, true>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, want);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&ibm_kernel<LBM_F16_SHIFTED, true>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, want);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&ibm_kernel<LBM_BF16_SHIFTED, true>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, want);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            I.shmem_tables = 0;  // read the tables from memory instead
-        }
-    }
+    const size_t tables = (size_t(n) + 1 + size_t(nb) + 1 + 4 * T.mk_band.size()) * 4;
+    I.shmem_tables = shmem + tables <= size_t(maxsh) && allow_dynamic_lds(maxsh) ? shmem + tables : 0;
     I.h_ds = dsv;
     I.active = true;
     return LBM_OK;

@@ -1,0 +1,135 @@
+// lbm_ibm_tables.hpp — host-side construction of the IBM band tables (no HIP calls).
+//
+// IBMManager<2> (IBMManager.cuh:31-51, 222-252) touches, per marker, the nodes of the
+// kernel stencil around it: 4x4 nodes at floor(p) - 1 .. floor(p) + 2 with the Peskin
+// 4-point kernel (IBMUtils.cuh:23-43), or the reference's 2x2 stencil at floor(p) .. floor(p)+1
+// (IBM_impl.cu:5-51, quirk bit LBM_QUIRK_IBM_2X2). The band is the union of those stencils.
+// The tables built here drive the single-workgroup IBM pass:
+//   - band_node[b]: global node index of band entry b (sorted by node index);
+//   - mk_ptr / mk_band / mk_w: CSR marker -> (band entry, phi(dx) phi(dy)) in the reference's
+//     loop order (x offset outer, y inner), out-of-domain nodes skipped;
+//   - bn_ptr / bn_mk / bn_w: CSR band entry -> (marker, phi phi ds) in marker order, the
+//     spreading gather whose sum order equals the reference's scatter order;
+//   - box_map: band entry or -1 per cell of the band's bounding box, whose x range is widened
+//     to whole 4-node segments (so one-step launches can split at it).
+// Every index is checked before the function returns success; the device code forms no
+// index that does not come from these tables.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <vector>
+
+namespace lbm {
+
+// delta4 (IBMUtils.cuh:23-33), FP32 as in the reference.
+inline float delta4f(float r) {
+    float rabs = std::fabs(r);
+    if (rabs < 1.0f) return 0.125f * (3.0f - 2.0f * rabs + std::sqrt(1.0f + 4.0f * rabs - 4.0f * r * r));
+    if (rabs < 2.0f) return 0.125f * (5.0f - 2.0f * rabs - std::sqrt(-7.0f + 12.0f * rabs - 4.0f * r * r));
+    return 0.0f;
+}
+
+struct IbmTables {
+    std::vector<int> band_node, mk_ptr, mk_band, bn_ptr, bn_mk, box_map;
+    std::vector<float> mk_w, bn_w;
+    int x0 = 0, x1 = 0, y0 = 0, y1 = 0;  // bounding box of the band, exclusive ends
+};
+
+// Builds the tables for n markers at xy (AoS [n][2]) with arc lengths ds (or null: 1) on an
+// nx x ny grid. Returns null on success, else the reason the markers are rejected.
+inline const char* ibm_build_tables(int nx, int ny, bool two_by_two, bool no_ds, const float* xy, const float* ds,
+                                    int n, IbmTables& T) {
+    T = IbmTables();
+    if (nx < 1 || ny < 1 || n < 1 || !xy) return "IBM tables need a grid and at least one marker";
+    for (int l = 0; l < n; l++) {
+        if (!std::isfinite(xy[2 * l]) || !std::isfinite(xy[2 * l + 1])) return "marker coordinates must be finite";
+        if (ds && !(ds[l] > 0.0f)) return "marker arc lengths must be positive";
+    }
+    const int W = two_by_two ? 2 : 4, off = two_by_two ? 0 : -1;
+    struct Ent {
+        int node;
+        float k;
+    };
+    std::vector<std::vector<Ent>> st(static_cast<size_t>(n));
+    std::map<int, int> band;
+    for (int l = 0; l < n; l++) {
+        const float px = xy[2 * l], py = xy[2 * l + 1];
+        // a marker more than a stencil width outside the grid touches no node (and its floor
+        // need not fit an int)
+        if (px < -4.0f || py < -4.0f || px > float(nx) + 4.0f || py > float(ny) + 4.0f) continue;
+        const int gx = int(std::floor(px)), gy = int(std::floor(py));
+        for (int i = 0; i < W; i++)
+            for (int j = 0; j < W; j++) {
+                const int xx = gx + off + i, yy = gy + off + j;
+                if (xx >= nx || xx < 0 || yy >= ny || yy < 0) continue;
+                const float k = delta4f(px - float(xx)) * delta4f(py - float(yy));
+                const int node = yy * nx + xx;
+                st[size_t(l)].push_back({node, k});
+                band.emplace(node, 0);
+            }
+    }
+    if (band.empty()) return "no marker stencil touches the grid";
+    int xmin = nx, xmax = -1, ymin = ny, ymax = -1;
+    for (auto& kv : band) {
+        kv.second = int(T.band_node.size());
+        T.band_node.push_back(kv.first);
+        const int y = kv.first / nx, x = kv.first % nx;
+        xmin = std::min(xmin, x);
+        xmax = std::max(xmax, x);
+        ymin = std::min(ymin, y);
+        ymax = std::max(ymax, y);
+    }
+    const int nb = int(T.band_node.size());
+    T.mk_ptr.assign(size_t(n) + 1, 0);
+    std::vector<std::vector<std::pair<int, float>>> per_node(static_cast<size_t>(nb));
+    for (int l = 0; l < n; l++) {
+        const float d = (ds && !no_ds) ? ds[l] : 1.0f;
+        for (const Ent& e : st[size_t(l)]) {
+            const int b = band.at(e.node);
+            T.mk_band.push_back(b);
+            T.mk_w.push_back(e.k);
+            per_node[size_t(b)].push_back({l, e.k * d});
+        }
+        T.mk_ptr[size_t(l) + 1] = int(T.mk_band.size());
+    }
+    T.bn_ptr.assign(size_t(nb) + 1, 0);
+    for (int b = 0; b < nb; b++) {
+        for (const auto& pr : per_node[size_t(b)]) {
+            T.bn_mk.push_back(pr.first);
+            T.bn_w.push_back(pr.second);
+        }
+        T.bn_ptr[size_t(b) + 1] = int(T.bn_mk.size());
+    }
+    xmin = xmin / 4 * 4;
+    xmax = std::min(nx, (xmax + 1 + 3) / 4 * 4) - 1;
+    const int bw = xmax - xmin + 1, bh = ymax - ymin + 1;
+    T.box_map.assign(size_t(bw) * size_t(bh), -1);
+    for (int b = 0; b < nb; b++) {
+        const int y = T.band_node[size_t(b)] / nx, x = T.band_node[size_t(b)] % nx;
+        T.box_map[size_t(y - ymin) * size_t(bw) + size_t(x - xmin)] = b;
+    }
+    T.x0 = xmin;
+    T.x1 = xmax + 1;
+    T.y0 = ymin;
+    T.y1 = ymax + 1;
+    // Every index the kernel forms comes from these tables: check them all.
+    const long long N = (long long)nx * ny;
+    for (int v : T.band_node)
+        if (v < 0 || v >= N) return "IBM band node out of range";
+    for (int v : T.mk_band)
+        if (v < 0 || v >= nb) return "IBM stencil entry out of range";
+    for (int v : T.bn_mk)
+        if (v < 0 || v >= n) return "IBM spread entry out of range";
+    if (T.mk_ptr.back() != int(T.mk_band.size()) || T.bn_ptr.back() != int(T.bn_mk.size()) ||
+        T.mk_band.size() != T.mk_w.size() || T.bn_mk.size() != T.bn_w.size() || T.mk_band.size() != T.bn_mk.size())
+        return "IBM table sizes inconsistent";
+    for (int v : T.box_map)
+        if (v < -1 || v >= nb) return "IBM box map out of range";
+    if (T.x0 < 0 || T.y0 < 0 || T.x1 > nx || T.y1 > ny || T.x0 >= T.x1 || T.y0 >= T.y1)
+        return "IBM box outside the grid";
+    return nullptr;
+}
+
+}  // namespace lbm

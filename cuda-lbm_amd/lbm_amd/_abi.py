@@ -170,6 +170,37 @@ PROTOTYPES = [
 _lib = None
 
 
+def _bind_torch_runtime():
+    """Load PyTorch's HIP runtime first when PyTorch is installed.
+
+    The library needs `libamdhip64.so.7`. PyTorch-ROCm ships its own copy under that SONAME,
+    but its modules ask for it as `libamdhip64.so`, a name the dynamic loader does not match
+    against a copy loaded from /opt/rocm. Loaded first, the library therefore pulls in
+    /opt/rocm's HIP and HSA runtimes, and a later `import torch` adds PyTorch's: two runtimes
+    that each drive the GPU from one process, and tensor pointers of one passed to the other.
+    Importing torch first makes the library bind to the copy already loaded."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
+def hip_runtimes_loaded() -> list:
+    """Paths of the HIP runtime libraries mapped into this process (Linux)."""
+    try:
+        with open("/proc/self/maps") as fh:
+            return sorted({ln.split()[-1] for ln in fh if "libamdhip64" in ln and "/" in ln})
+    except OSError:
+        return []
+
+
+def _check_one_hip_runtime():
+    libs = hip_runtimes_loaded()
+    if len(libs) > 1:
+        raise RuntimeError("more than one HIP runtime is loaded into this process (" + ", ".join(libs) +
+                           "); load liblbm_hip.so before any other HIP library, or import torch first")
+
+
 def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
     """Load liblbm_hip.so (built by `make -C cuda-lbm_amd`) and declare its prototypes."""
     global _lib
@@ -178,7 +209,9 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
     p = Path(path) if path else LIB_PATH
     if not p.exists():
         raise RuntimeError(f"liblbm_hip.so not found at {p}; build it with `make -C cuda-lbm_amd`")
+    _bind_torch_runtime()
     lib = ctypes.CDLL(str(p))
+    _check_one_hip_runtime()
     for name, res, args in PROTOTYPES:
         fn = getattr(lib, name)
         fn.restype = res

@@ -13,22 +13,37 @@ import torch
 from .solver import Config, Solver
 
 
+def _run_forward(sim, force, u0):
+    """Upload the inputs and run the checkpointing forward on PyTorch's current stream."""
+    s = sim.solver
+    s.set_stream(torch.cuda.current_stream().cuda_stream)
+    s.set_force_field(force.detach().contiguous())
+    s.init_from_macros(sim.rho0, u0.detach().contiguous())
+    s.adjoint_forward(sim.nsteps, sim.checkpoint_every)
+    sim._run += 1
+    return sim._run
+
+
 class _LbmRun(torch.autograd.Function):
     @staticmethod
     def forward(ctx, force, u0, sim):
-        s = sim.solver
-        s.set_stream(torch.cuda.current_stream().cuda_stream)
-        s.set_force_field(force.detach().contiguous())
-        s.init_from_macros(sim.rho0, u0.detach().contiguous())
-        s.adjoint_forward(sim.nsteps, sim.checkpoint_every)
+        run = _run_forward(sim, force, u0)
         uT = torch.empty_like(u0)
-        s.get_macros_device(u=uT)
+        sim.solver.get_macros_device(u=uT)
         ctx.sim = sim
+        ctx.run = run
+        ctx.save_for_backward(force, u0)
         return uT
 
     @staticmethod
     def backward(ctx, grad_u):
-        s = ctx.sim.solver
+        sim = ctx.sim
+        if sim._run != ctx.run:
+            # The solver's checkpoints and force field belong to a later call of `sim` (for
+            # example a validation forward): run this node's forward again from its own inputs.
+            force, u0 = ctx.saved_tensors
+            ctx.run = _run_forward(sim, force, u0)
+        s = sim.solver
         s.set_stream(torch.cuda.current_stream().cuda_stream)
         gF = torch.empty_like(grad_u)
         gU = torch.empty_like(grad_u)
@@ -44,6 +59,7 @@ class Simulation:
         self.nsteps = int(nsteps)
         self.checkpoint_every = int(checkpoint_every)
         self.solver = Solver(cfg)
+        self._run = 0  # forward runs so far; the solver holds the checkpoints of the last one
         dev = torch.device("cuda", torch.cuda.current_device())
         self.rho0 = (torch.ones(cfg.ny, cfg.nx, dtype=torch.float32, device=dev) if rho0 is None
                      else torch.as_tensor(rho0, dtype=torch.float32, device=dev).contiguous())

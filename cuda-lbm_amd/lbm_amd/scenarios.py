@@ -226,9 +226,26 @@ def poiseuille_profile(ny: int, nu: float, u_max: float) -> np.ndarray:
     return (F / (2.0 * nu)) * y * (ny - y)
 
 
-def poiseuille_error(row_mean_ux: np.ndarray, nu: float, u_max: float) -> float:
+def poiseuille_startup_profile(ny: int, nu: float, u_max: float, t: float, terms: int = 4001) -> np.ndarray:
+    """The channel flow from rest under the same force, with the validator's walls (u = 0 at
+    y = 0 and y = ny): u_t = nu u_yy + F, u(y, 0) = 0, whose solution is the parabola minus its
+    decaying sine series,
+        u(y, t) = F/(2 nu) y (ny - y) - sum_{n odd} 4 F ny^2 / (nu n^3 pi^3) sin(n pi y / ny) exp(-nu (n pi / ny)^2 t).
+    It tends to poiseuille_profile as t grows, so it measures the start-up at any step count."""
+    F = 8.0 * nu * u_max / (ny * ny)
+    y = np.arange(ny, dtype=np.float64)
+    u = (F / (2.0 * nu)) * y * (ny - y)
+    for n in range(1, terms + 1, 2):
+        a = 4.0 * F * ny * ny / (nu * n ** 3 * math.pi ** 3) * math.exp(-nu * (n * math.pi / ny) ** 2 * t)
+        if a < 1e-30:
+            break
+        u -= a * np.sin(n * math.pi * y / ny)
+    return u
+
+
+def poiseuille_error(row_mean_ux: np.ndarray, nu: float, u_max: float, profile: np.ndarray | None = None) -> float:
     """PoiseuilleScenario::compute_error: RMS over y of (x-mean u_x - profile), % of u_max."""
-    prof = poiseuille_profile(len(row_mean_ux), nu, u_max)
+    prof = poiseuille_profile(len(row_mean_ux), nu, u_max) if profile is None else profile
     d = np.asarray(row_mean_ux, np.float64) - prof
     return math.sqrt(float(np.mean(d * d))) * 100.0 / u_max
 

@@ -3,6 +3,7 @@ or liblbm_hip.so on the GPU, and compare."""
 from __future__ import annotations
 
 import ctypes
+import math
 from pathlib import Path
 
 import numpy as np
@@ -139,3 +140,43 @@ def layout_case(layout: str, nx: int, ny: int, collision: int, storage: int = A.
     rho = (1.0 + 0.02 * rng.uniform(-1, 1, size=(ny, nx))).astype(np.float32)
     u = (0.04 * rng.uniform(-1, 1, size=(ny, nx, 2))).astype(np.float32)
     return sc.Case(layout, cfg, rho, u, flags=None if not flags.any() else flags)
+
+
+# ---- closed forms from the lattice definition (no oracle code) -----------------------------
+
+D2Q9_C = np.array([(0, 0), (1, 0), (0, 1), (-1, 0), (0, -1), (1, 1), (-1, 1), (-1, -1), (1, -1)], np.float64)
+D2Q9_W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+THETA = (0.0003, -0.00775, 0.00016, 0.0087)  # OptimalAdapter (adapters.cuh:63)
+
+
+def feq2(rho, ux, uy):
+    """Second-order equilibrium w rho [1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u^2] (equilibrium.cu:5-39)."""
+    cu = D2Q9_C[:, 0] * ux + D2Q9_C[:, 1] * uy
+    return D2Q9_W * rho * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * (ux * ux + uy * uy))
+
+
+def central_moment_matrix(ux, uy):
+    """T(u): rows 1, x, y, x^2+y^2, x^2-y^2, xy, x^2 y, x y^2, x^2 y^2 of (c - u) (CM.cuh:53-68)."""
+    x = D2Q9_C[:, 0] - ux
+    y = D2Q9_C[:, 1] - uy
+    return np.stack([np.ones(9), x, y, x * x + y * y, x * x - y * y, x * y, x * x * y, x * y * y, x * x * y * y])
+
+
+def momentum_flux_magnitude(f):
+    """|Pi| = sqrt(Pxx^2 + 2 Pxy^2 + Pyy^2), Pi = sum f c c (macroscopics.cu:18-33)."""
+    pxx = np.sum(f * D2Q9_C[:, 0] ** 2)
+    pxy = np.sum(f * D2Q9_C[:, 0] * D2Q9_C[:, 1])
+    pyy = np.sum(f * D2Q9_C[:, 1] ** 2)
+    return math.sqrt(pxx * pxx + 2 * pxy * pxy + pyy * pyy)
+
+
+def cm_collide_uniform(rho, ux, uy, S, high_rate):
+    """Post-collision populations of the uniform equilibrium state f = feq2(rho, u) under CM
+    with rates S[0..5] and `high_rate` for moments 6..8 (CM.cuh:104-115, no force), inverted
+    by solving T(u) f = k* (independent of the closed-form T^-1)."""
+    T = central_moment_matrix(ux, uy)
+    k = T @ feq2(rho, ux, uy)
+    cs2 = 1.0 / 3.0
+    keq = np.array([rho, 0, 0, 2 * rho * cs2, 0, 0, 0, 0, rho * cs2 * cs2])
+    s = np.array(list(S[:6]) + [high_rate] * 3, np.float64)
+    return np.linalg.solve(T, k - s * (k - keq))

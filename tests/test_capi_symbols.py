@@ -31,7 +31,9 @@ def test_header_declares_the_boundary():
 def test_library_exports_every_declared_symbol():
     if not LIB.exists():
         pytest.fail("liblbm_hip.so is not built (run __graft_entry__.build())")
-    lib = ctypes.CDLL(str(LIB))
+    from lbm_amd import _abi
+
+    lib = _abi.load_library()  # binds to PyTorch's HIP runtime, as every user of the library does
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
     assert not missing, missing
 
@@ -73,3 +75,24 @@ def test_3d_defaults_without_gpu():
     assert (c.nx, c.ny, c.nz) == (16, 8, 4) and c.omega == 1.0 and c.rho0 == 1.0
     assert c.collision == _abi.LBM_BGK and c.storage == _abi.LBM_F32 and c.quirks == 0
     assert list(c.S) == [1.0] * 27
+
+
+def test_one_hip_runtime_per_process():
+    """The test-infrastructure libraries and liblbm_hip.so, loaded in any order with torch, leave
+    exactly one HIP runtime mapped (PyTorch's): the library binds to the runtime PyTorch loads."""
+    import subprocess
+    import sys
+
+    code = (
+        "import ctypes, sys\n"
+        f"sys.path.insert(0, {str(ROOT / 'cuda-lbm_amd')!r})\n"
+        f"ctypes.CDLL({str(ROOT / 'tests' / 'emu' / 'liblbm_emu.so')!r})\n"
+        f"ctypes.CDLL({str(ROOT / 'tests' / 'emu' / 'liblbm3d_emu.so')!r})\n"
+        "from lbm_amd import _abi\n"
+        "_abi.load_library()\n"
+        "import torch\n"
+        "libs = _abi.hip_runtimes_loaded()\n"
+        "assert len(libs) == 1 and 'torch' in libs[0], libs\n"
+        "print('ok')\n")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stdout + r.stderr

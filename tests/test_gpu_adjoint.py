@@ -114,3 +114,42 @@ def test_torch_autograd_function_and_inverse_problem(gpu_available):
     _, losses = recover_force(sim, u_target, u0, iters=40, lr=2e-6)
     assert losses[-1] < 0.05 * losses[0], (losses[0], losses[-1])
     sim.close()
+
+
+def test_autograd_backward_after_a_later_forward(gpu_available):
+    """A second call of the same Simulation (here a no-grad validation forward with another
+    force) before backward must not leak into the first node's gradient."""
+    import torch
+
+    from lbm_amd.autograd import Simulation
+    from lbm_amd.solver import LbmError
+
+    cfg, rho, u, _ = periodic_case(A.LBM_BGK, 32, 24)
+    dev = "cuda"
+    u0 = torch.as_tensor(u, device=dev)
+    FA = torch.as_tensor(sc.fourier_force(32, 24, modes=3, amp=1e-5, seed=2), device=dev)
+    FB = torch.as_tensor(sc.fourier_force(32, 24, modes=3, amp=3e-5, seed=5), device=dev)
+    target = torch.zeros_like(u0)
+
+    def grad_of(sim, F, between=None):
+        Fv = F.clone().requires_grad_(True)
+        loss = torch.sum((sim(Fv, u0) - target) ** 2)
+        if between is not None:
+            with torch.no_grad():
+                sim(between, u0)
+        loss.backward()
+        return Fv.grad.cpu().numpy()
+
+    sim = Simulation(cfg, nsteps=30, checkpoint_every=10, rho0=rho)
+    g_clean = grad_of(sim, FA)
+    g_after = grad_of(sim, FA, between=FB)
+    g_b = grad_of(sim, FB)
+    assert rel(g_after, g_clean) < 1e-6
+    assert rel(g_b, g_clean) > 1e-3  # the two forces give different gradients
+    # the C side refuses a backward whose forward state was changed underneath it
+    s = sim.solver
+    s.adjoint_forward(30, 10)
+    s.set_force_field(FB)
+    with pytest.raises(LbmError):
+        s.adjoint_backward(np.zeros((24, 32, 2), np.float32))
+    sim.close()

@@ -181,3 +181,53 @@ def test_gpu_two_step_strips_on_hbm_sized_grid(gpu_available, layout):
     assert max_rel(f2, f1) <= 1e-5 and float(np.max(np.abs(u2 - u1))) <= 1e-6
     fo, *_ = run_oracle(case, 5, 32)
     assert max_rel(f2, fo) <= 1e-5
+
+
+@pytest.mark.parametrize("storage", [A.LBM_F32, A.LBM_F16_SHIFTED])
+@pytest.mark.parametrize("mode", [A.LBM_ADAPTER_SAME_STEP, A.LBM_ADAPTER_LAGGED])
+def test_gpu_optimal_adapter_on_a_uniform_equilibrium(gpu_available, mode, storage):
+    """The adapter path on the GPU against its closed form on a uniform equilibrium state
+    (periodic, no force): LBM_REDUCE_MOMENT_AVG = (rho0, rho0 |u0|, |Pi(rho0, u0)|), and the
+    step relaxes moments 6..8 with s = 1 / (3 x 0.00141 + 0.5) (adapters.cuh:48-111)."""
+    import math
+
+    from helpers import THETA, cm_collide_uniform
+    from lbm_amd.solver import Config, Solver
+
+    rho0, ux, uy = 1.03, 0.04, -0.025
+    S = (0.0, 0.0, 0.0, 1.0, 1.7, 1.7, 1.0, 1.0, 1.0)
+    ny, nx = 40, 70
+    cfg = Config(nx=nx, ny=ny, collision=A.LBM_CM, adapter=A.LBM_OPTIMAL_ADAPTER, adapter_mode=mode, omega=1.7, S=S,
+                 periodic_x=True, periodic_y=True, storage=storage)
+    print(f"case adapter uniform mode={mode} storage={storage}", flush=True)
+    s = Solver(cfg)
+    s.init_from_macros(np.full((ny, nx), rho0, np.float32), np.tile(np.float32([ux, uy]), (ny, nx, 1)))
+    s.step(1)
+    pi_cf = math.sqrt((rho0 / 3 + rho0 * ux * ux) ** 2 + 2 * (rho0 * ux * uy) ** 2 + (rho0 / 3 + rho0 * uy * uy) ** 2)
+    avg = s.reduce(A.LBM_REDUCE_MOMENT_AVG)
+    q = 1e-6 if storage == A.LBM_F32 else 4 * 2.0 ** -11 * 0.05  # FP16 deviations: a few quanta
+    np.testing.assert_allclose(avg, [rho0, rho0 * math.hypot(ux, uy), pi_cf], rtol=1e-6 if storage == A.LBM_F32 else 1e-3)
+    want = cm_collide_uniform(rho0, ux, uy, S, 1.0 / (3.0 * sum(THETA) + 0.5))
+    f = s.get_pops()
+    assert float(np.max(np.abs(f - want[:, None, None]))) <= q
+    s.close()
+
+
+def test_gpu_reductions_against_host_sums(gpu_available):
+    """LBM_REDUCE_MASS / MOMENTUM / L2_VS_REF / ROW_MEAN_UX (FP64 device reductions of the
+    stored fields) against FP64 sums of the same fields read back to the host."""
+    print("case reductions 300x77 channel", flush=True)
+    case = layout_case("channel", 300, 77, A.LBM_MRT)
+    _, rho, u, s = run_gpu(case, 9)
+    r64, u64 = rho.astype(np.float64), u.astype(np.float64)
+    np.testing.assert_allclose(s.reduce(A.LBM_REDUCE_MASS)[0], r64.sum(), rtol=1e-12)
+    mom = s.reduce(A.LBM_REDUCE_MOMENTUM)[:2]
+    np.testing.assert_allclose(mom, [(r64 * u64[..., 0]).sum(), (r64 * u64[..., 1]).sum()], rtol=1e-9, atol=1e-12)
+    rng = np.random.default_rng(11)
+    ref = (u + 0.01 * rng.standard_normal(u.shape)).astype(np.float32)
+    l2 = s.reduce(A.LBM_REDUCE_L2_VS_REF, ref=ref)[:2]
+    ref64 = ref.astype(np.float64)
+    np.testing.assert_allclose(l2, [((u64 - ref64) ** 2).sum(), (ref64 ** 2).sum()], rtol=1e-9)
+    rows = s.reduce(A.LBM_REDUCE_ROW_MEAN_UX)
+    np.testing.assert_allclose(rows, u64[..., 0].mean(axis=1), rtol=1e-9, atol=1e-15)
+    s.close()

@@ -173,6 +173,39 @@ def test_optimal_adapter_values():
     assert abs(orc.adapter_rate(1e4, 0.0, 0.0, [1.0, 1.0, 1.0]) - 1 / (3 * 1.5 + 0.5)) < 1e-15
 
 
+
+@pytest.mark.parametrize("mode", [A.LBM_ADAPTER_SAME_STEP, A.LBM_ADAPTER_LAGGED])
+def test_optimal_adapter_on_a_uniform_equilibrium(mode):
+    """Closed form of the whole adapter path on a uniform equilibrium state (periodic, no force):
+    the domain means are <rho> = rho0, <rho|u|> = rho0 |u0|, <|Pi|> = |Pi(rho0, u0)| with
+    Pi = rho cs^2 I + rho u u; every node's regression inputs are then 1, so
+    tau* = 0.0003 - 0.00775 + 0.00016 + 0.0087 = 0.00141 and s = 1 / (3 tau* + 0.5) = 1.98322,
+    which relaxes moments 6..8 of the step (adapters.cuh:48-111, macroscopics.cuh:50-120)."""
+    from helpers import THETA, cm_collide_uniform, feq2, momentum_flux_magnitude
+
+    rho0, ux, uy = 1.03, 0.04, -0.025
+    S = (0.0, 0.0, 0.0, 1.0, 1.7, 1.7, 1.0, 1.0, 1.0)
+    c = Config(nx=12, ny=10, collision=A.LBM_CM, adapter=A.LBM_OPTIMAL_ADAPTER, adapter_mode=mode, omega=1.7, S=S,
+               periodic_x=True, periodic_y=True).to_c()
+    o = orc.Oracle(c, precision=64)
+    o.init_from_macros(np.full((10, 12), rho0), np.tile([ux, uy], (10, 12, 1)))
+    o.step(1)
+    pi_cf = math.sqrt((rho0 / 3 + rho0 * ux * ux) ** 2 + 2 * (rho0 * ux * uy) ** 2 + (rho0 / 3 + rho0 * uy * uy) ** 2)
+    assert abs(momentum_flux_magnitude(feq2(rho0, ux, uy)) - pi_cf) < 1e-15
+    np.testing.assert_allclose(o.get_avg(), [rho0, rho0 * math.hypot(ux, uy), pi_cf], rtol=1e-13)
+    tau = sum(THETA)
+    assert abs(tau - 0.00141) < 1e-15
+    s_high = 1.0 / (3.0 * tau + 0.5)
+    assert abs(s_high - 1.98322) < 1e-5
+    want = cm_collide_uniform(rho0, ux, uy, S, s_high)
+    f = o.get_pops()
+    # theta is FP32 in the reference: the rate carries its rounding (~1e-8 relative)
+    np.testing.assert_allclose(f, np.broadcast_to(want[:, None, None], f.shape), rtol=1e-7, atol=1e-12)
+    # and a wrong rate (the NoAdapter's 1.0, or S[6..8]) would be far off
+    wrong = cm_collide_uniform(rho0, ux, uy, S, 1.0)
+    assert np.max(np.abs(want - wrong)) > 1e-7
+
+
 # ---- streaming and boundaries ----------------------------------------------------------
 
 def run(c, flags=None, rho=None, u=None, steps=1, precision=64, F=None, f=None):
@@ -460,6 +493,40 @@ def test_ghia_error_formula_is_zero_on_the_table():
     assert sc.ghia_closest(999) == 1000 and sc.ghia_closest(4000) == 3200
 
 
+@pytest.mark.parametrize("n", [129, 257, 513])
+def test_ghia_error_of_scaled_and_shifted_centre_lines(n):
+    """NRMSE values known by hand (lidDrivenCavityScenario.cuh:89-157): centre lines at
+    (1 + a) x Ghia u and (1 + b) x Ghia v give 100 (|a| + |b|) / 2; a constant shift d of u
+    alone gives 100 |d| sqrt(17 / sum u_ghia^2) / 2. Any other normalisation, a lid velocity
+    not divided out, or samples taken from the wrong rows or columns change these numbers."""
+    u_lid = 0.1
+    a, b = 0.1, -0.3
+    # grids on which the 17 sample rows and columns are distinct (at 100^2 two share a column)
+    assert len({int(math.floor(v * (n - 1) + 0.5)) for v in sc.GHIA_X}) == 17
+    u = np.zeros((n, n, 2))
+    for i in range(len(sc.GHIA_Y)):
+        y = int(math.floor(sc.GHIA_Y[i] * (n - 1) + 0.5))
+        u[y, n // 2, 0] = (1 + a) * sc.GHIA_U[1000][i] * u_lid
+        x = int(math.floor(sc.GHIA_X[i] * (n - 1) + 0.5))
+        u[n // 2, x, 1] = (1 + b) * sc.GHIA_V[1000][i] * u_lid
+    assert abs(sc.ghia_error(u, u_lid, 1000) - 100 * (abs(a) + abs(b)) / 2) < 1e-9
+    d = 0.02
+    u2 = np.zeros((n, n, 2))
+    for i in range(len(sc.GHIA_Y)):
+        y = int(math.floor(sc.GHIA_Y[i] * (n - 1) + 0.5))
+        u2[y, n // 2, 0] = (sc.GHIA_U[1000][i] + d) * u_lid
+        x = int(math.floor(sc.GHIA_X[i] * (n - 1) + 0.5))
+        u2[n // 2, x, 1] = sc.GHIA_V[1000][i] * u_lid
+    assert len(sc.GHIA_Y) == 17
+    want = 100 * d * math.sqrt(17 / sum(v * v for v in sc.GHIA_U[1000])) / 2
+    assert abs(sc.ghia_error(u2, u_lid, 1000) - want) < 1e-9
+    # samples are taken at the centre column / row only: a field elsewhere does not count
+    u3 = u.copy()
+    u3[:, n // 2 + 1] = 7.0
+    u3[n // 2 + 1, :] = 7.0
+    assert abs(sc.ghia_error(u3, u_lid, 1000) - 100 * (abs(a) + abs(b)) / 2) < 1e-9
+
+
 # ---- storage rounding ------------------------------------------------------------------
 
 def test_half_and_bfloat16_rounding_match_numpy_and_torch():
@@ -521,3 +588,25 @@ def test_uniform_flow_through_zou_he_inlet_and_pressure_outlet_is_steady():
     o2 = run(c, flags=flags, u=(u + 0.01 * rng.uniform(-1, 1, u.shape)).reshape(-1), steps=3)
     r, _ = o2.get_macros()
     assert np.allclose(r[:, -1], 1.0, atol=1e-12)
+
+
+def test_poiseuille_startup_series():
+    """The start-up closed form: zero at t = 0, the validator's parabola for t -> infinity, the
+    bulk's free acceleration F t while the walls are far, and the diffusion equation
+    u_t = nu u_yy + F checked by finite differences of the closed form itself."""
+    ny, nu, umax = 64, 0.05, 0.05
+    F = 8 * nu * umax / ny ** 2
+    assert np.max(np.abs(sc.poiseuille_startup_profile(ny, nu, umax, 0.0))) < 1e-3 * umax
+    np.testing.assert_allclose(sc.poiseuille_startup_profile(ny, nu, umax, 1e7), sc.poiseuille_profile(ny, nu, umax),
+                               atol=1e-12)
+    t = 50.0  # sqrt(nu t) = 1.6 cells: the centre does not feel the walls yet
+    assert abs(sc.poiseuille_startup_profile(ny, nu, umax, t)[ny // 2] - F * t) < 1e-9 * F * t + 1e-15
+    t, dt = 3000.0, 1.0
+    u0 = sc.poiseuille_startup_profile(ny, nu, umax, t - dt)
+    u1 = sc.poiseuille_startup_profile(ny, nu, umax, t + dt)
+    uc = sc.poiseuille_startup_profile(ny, nu, umax, t)
+    ut = (u1 - u0) / (2 * dt)
+    uyy = uc[:-2] - 2 * uc[1:-1] + uc[2:]
+    np.testing.assert_allclose(ut[1:-1], nu * uyy + F, rtol=1e-3, atol=1e-4 * F)
+    # and the validator's metric against it is the reference's RMS % of u_max
+    assert sc.poiseuille_error(uc, nu, umax, profile=uc) == 0.0
