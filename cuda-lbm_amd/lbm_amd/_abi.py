@@ -143,6 +143,7 @@ PROTOTYPES = [
     ("lbm_get_pops", _I, [_P, _P, _I]),
     ("lbm_set_pops", _I, [_P, _P, _I]),
     ("lbm_reduce", _I, [_P, _I, _P, _D]),
+    ("lbm_reduce_noalloc", _I, [_P, _I, _P, _D, _I]),
     ("lbm_ibm_set_markers", _I, [_P, _P, _P, _I, _I]),
     ("lbm_ibm_get_forces", _I, [_P, _P, _D]),
     ("lbm_adjoint_forward", _I, [_P, _I, _I]),

@@ -196,6 +196,24 @@ class Solver:
         self._check(self.lib.lbm_reduce(self.h, kind, p, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
 
+    def reduce_noalloc(self, kind: int, ref=None, out=None):
+        """lbm_reduce_noalloc: the same reductions without any device allocation. `ref` (for
+        LBM_REDUCE_L2_VS_REF) must be a contiguous device tensor, AoS [N][2]. With `out` a
+        float64 device tensor the result is written there asynchronously; else returned."""
+        n = self.cfg.ny if kind == A.LBM_REDUCE_ROW_MEAN_UX else 3
+        pr = None
+        if ref is not None:
+            if not _is_torch(ref) or not ref.is_cuda or not ref.is_contiguous():
+                raise ValueError("reduce_noalloc needs the reference field as a contiguous device tensor")
+            pr = ctypes.c_void_p(ref.data_ptr())
+        if out is not None:
+            self._check(self.lib.lbm_reduce_noalloc(self.h, kind, pr, ctypes.cast(ctypes.c_void_p(out.data_ptr()),
+                                                                                 ctypes.POINTER(ctypes.c_double)), 1))
+            return out
+        h = np.zeros(n, np.float64)
+        self._check(self.lib.lbm_reduce_noalloc(self.h, kind, pr, h.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 0))
+        return h
+
     # -- immersed boundary --------------------------------------------------------------
     def ibm_set_markers(self, xy, ds=None, iters: int = 3):
         xy = np.ascontiguousarray(xy, dtype=np.float32).reshape(-1, 2)

@@ -170,6 +170,17 @@ int lbm_set_pops(lbm_ctx* ctx, const float* f, int on_device);
 #define LBM_REDUCE_MOMENT_AVG  4 /* out[0..2] = <rho>, <rho|u|>, <|Pi|> (adapter inputs)   */
 /* Runs on the last stored macroscopic fields; `ref` is host memory. Synchronises. */
 int lbm_reduce(lbm_ctx* ctx, int32_t kind, const float* ref, double* out);
+/* lbm_reduce without any allocation: no counterpart in the reference. For callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc). Same kernels and results as lbm_reduce; the partial sums live in a fixed
+ * per-GPU array of LBM_NOALLOC_SCRATCH_BYTES, held until the process ends and shared by every
+ * call on that GPU: calls that overlap on one GPU race. `ref_device` (L2_VS_REF) is device
+ * memory. Limits: N <= 715,784,192 nodes (43,688 partial sums), NY <= 131,072 rows
+ * (ROW_MEAN_UX); larger grids return LBM_E_UNSUPPORTED. out_on_device = 1: `out` is device
+ * memory, written asynchronously on the context's stream (no synchronisation); 0: host
+ * memory, synchronises. */
+#define LBM_NOALLOC_SCRATCH_BYTES (1u << 20)
+int lbm_reduce_noalloc(lbm_ctx* ctx, int32_t kind, const float* ref_device, double* out, int out_on_device);
 
 /* ---- immersed boundary (src/IBM, multi-direct forcing IBMManager.cuh:222-252) ------
  * Static markers in lattice units, AoS [n][2]; ds: per-marker arc length (NULL = 1).
