@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -234,6 +235,59 @@ int lbm_ctx::split_ibm_step(bool write_macros) {
     return LBM_OK;
 }
 
+void lbm_ctx::ibm_graph_free() {
+    if (ibm_graph) (void)hipGraphExecDestroy(ibm_graph);
+    ibm_graph = nullptr;
+    ibm_graph_cur = -1;
+    ibm_graph_stream = nullptr;
+}
+
+// Graph replay applies once a direct split step has run (the box's external force written,
+// the internal stream created) and on a stream that can be captured (not the null stream).
+// LBM_HIP_GRAPHS=0 in the environment turns it off.
+bool lbm_ctx::ibm_graph_ok() const {
+    const char* v = std::getenv("LBM_HIP_GRAPHS");
+    const bool enabled = !(v && v[0] == '0');
+    const bool adapt = cfg.collision == LBM_CM && cfg.adapter == LBM_OPTIMAL_ADAPTER;
+    return enabled && ibm.active && ibm.split_ok && !adapt && ibm.box_valid && aux_stream && stream &&
+           !edge_in_store;
+}
+
+// Two split IBM steps through the graph, capturing it first when there is none for this
+// buffer parity and stream. Leaves cur as it was (two steps) and advances timestep by 2.
+int lbm_ctx::ibm_graph_pair() {
+    if (!ibm_graph || ibm_graph_cur != cur || ibm_graph_stream != stream) {
+        ibm_graph_free();
+        const int c0 = cur;
+        const int64_t t0 = timestep;
+        hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed);
+        if (e != hipSuccess) return hip_fail(e, "IBM step capture");
+        int rc = split_ibm_step(false);
+        if (!rc) rc = split_ibm_step(false);
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(stream, &g);
+        cur = c0;  // the capture recorded the two steps without running them
+        timestep = t0;
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            return rc ? rc : hip_fail(e, "IBM step capture");
+        }
+        e = hipGraphInstantiate(&ibm_graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            ibm_graph = nullptr;
+            return hip_fail(e, "IBM graph instantiate");
+        }
+        ibm_graph_cur = c0;
+        ibm_graph_stream = stream;
+    }
+    hipError_t e = hipGraphLaunch(ibm_graph, stream);
+    if (e != hipSuccess) return hip_fail(e, "IBM graph launch");
+    timestep += 2;
+    return LBM_OK;
+}
+
 // One step from buf[cur] into buf[1-cur].
 int lbm_ctx::one_step(bool write_macros) {
     int rc = restore_edges();
@@ -278,6 +332,7 @@ lbm_ctx::~lbm_ctx() {
                     d_uy,    d_avg,  d_avg64, partials,    tmp,           d_edge[0], d_edge[1]};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
+    ibm_graph_free();
     ibm_free();
     adjoint_free();
     if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -387,6 +442,7 @@ int lbm_destroy(lbm_ctx* ctx) {
 
 int lbm_set_stream(lbm_ctx* ctx, void* s) {
     if (!ctx) return LBM_E_INVALID;
+    if (static_cast<hipStream_t>(s) != ctx->stream) ctx->ibm_graph_free();
     // NULL is the HIP null stream (PyTorch's default stream reports handle 0), not a request
     // for the context's own stream: mapping it to the own non-blocking stream would let the
     // library race with the caller's work.
@@ -396,6 +452,7 @@ int lbm_set_stream(lbm_ctx* ctx, void* s) {
 
 int lbm_set_flags(lbm_ctx* ctx, const uint8_t* flags, int on_device) {
     if (!ctx || !flags) return LBM_E_INVALID;
+    ctx->ibm_graph_free();  // the captured IBM steps read the flags and the force field
     ctx->adj.ready = false;  // the adjoint's recorded forward no longer matches the state
     int rc = ctx->activate();
     if (rc) return rc;
@@ -514,6 +571,7 @@ int lbm_init_from_macros(lbm_ctx* ctx, const float* rho, const float* u, int on_
 
 int lbm_set_force_field(lbm_ctx* ctx, const float* force, int on_device) {
     if (!ctx) return LBM_E_INVALID;
+    ctx->ibm_graph_free();  // the captured IBM steps read the flags and the force field
     ctx->adj.ready = false;  // the adjoint's recorded forward no longer matches the state
     int rc = ctx->activate();
     if (rc) return rc;
@@ -546,6 +604,13 @@ int lbm_step(lbm_ctx* ctx, int32_t nsteps) {
             if (rc) return rc;
         }
     for (; i < nsteps; i++) {
+        // IBM split steps: pairs by graph replay, the last step direct (it stores the fields)
+        if (i + 3 <= nsteps && ctx->ibm_graph_ok()) {
+            rc = ctx->ibm_graph_pair();
+            if (rc) return rc;
+            i++;
+            continue;
+        }
         rc = ctx->one_step(i == nsteps - 1);
         if (rc) return rc;
     }

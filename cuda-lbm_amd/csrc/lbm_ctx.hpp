@@ -66,6 +66,12 @@ struct lbm_ctx {
     // pass and the box's step on `stream` (fork and join through the two events)
     hipStream_t aux_stream = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // Two split IBM steps (buf[ibm_graph_cur] -> other -> back) captured once as a HIP graph
+    // and replayed: the seven runtime calls of a split step cost more host time than the GPU
+    // spends on the step. Dropped whenever anything the steps read changes.
+    hipGraphExec_t ibm_graph = nullptr;
+    int ibm_graph_cur = -1;
+    hipStream_t ibm_graph_stream = nullptr;
     size_t N = 0;
     long long plane = 0;
     void* buf[2] = {nullptr, nullptr};
@@ -106,6 +112,9 @@ struct lbm_ctx {
     int adapter_pass(int src_idx);
     int one_step(bool write_macros);
     int split_ibm_step(bool write_macros);
+    bool ibm_graph_ok() const;
+    int ibm_graph_pair();
+    void ibm_graph_free();
     bool two_step_ok() const;
     int two_step(bool write_macros);
     int restore_edges();

@@ -237,6 +237,7 @@ bool allow_dynamic_lds(int max_bytes) {
 }  // namespace
 
 void lbm_ctx::ibm_free() {
+    ibm_graph_free();
     void* ptrs[] = {ibm.d_force_total, ibm.d_xy, ibm.d_ds, ibm.d_band_node, ibm.d_mk_ptr, ibm.d_mk_band, ibm.d_mk_w,
                     ibm.d_bn_ptr, ibm.d_bn_mk, ibm.d_bn_w, ibm.d_lag_force, ibm.d_force_sum, ibm.d_box_map};
     for (void* q : ptrs)
