@@ -42,8 +42,8 @@ int main(int argc, char** argv) {
         if (out_dir) {
             lbm.save_macroscopics(lbm.timestep, out_dir);
             lbm.save_vtk(lbm.timestep, out_dir);
-            std::printf("wrote %s/density/density_%d.bin, velocity_%d.bin, lbm_%d.vti\n", out_dir, lbm.timestep,
-                        lbm.timestep, lbm.timestep);
+            std::printf("wrote %s/density/density_%d.bin, velocity_%d.bin, vtk/sim_data_%06d.vti\n", out_dir,
+                        lbm.timestep, lbm.timestep, lbm.timestep);
         }
     } catch (const std::exception& e) {
         std::fprintf(stderr, "lbm_run: %s\n", e.what());

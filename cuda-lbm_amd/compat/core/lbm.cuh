@@ -195,45 +195,18 @@ public:
     // 2N values: the "aos" layout of the reference's graphics/visualize.py reader).
     void save_macroscopics(int t, const std::string& dir = "output") {
         update_macroscopics();
-        namespace fs = std::filesystem;
-        fs::create_directories(dir + "/density");
-        fs::create_directories(dir + "/velocity");
-        const size_t n = size_t(NX) * NY;
-        std::ostringstream rn, vn;
-        rn << dir << "/density/density_" << t << ".bin";
-        vn << dir << "/velocity/velocity_" << t << ".bin";
-        std::ofstream rf(rn.str(), std::ios::binary);
-        std::ofstream vf(vn.str(), std::ios::binary);
-        if (!rf || !vf) throw std::runtime_error("cannot open " + rn.str() + " / " + vn.str());
-        rf.write(reinterpret_cast<const char*>(h_rho.data()), std::streamsize(n * sizeof(float)));
-        vf.write(reinterpret_cast<const char*>(h_u.data()), std::streamsize(2 * n * sizeof(float)));
+        if (lbm_write_macros(dir.c_str(), t, (int64_t)NX * NY, 2, h_rho.data(), h_u.data()) != LBM_OK)
+            throw std::runtime_error("save_macroscopics: cannot write the dumps under " + dir);
     }
 
-    // save_vtk (lbm.cuh:269-343): VTK ImageData with appended raw Float32 "Density" and
-    // 3-component "Velocity".
-    void save_vtk(int t, const std::string& dir = "output") {
+    // save_vtk (lbm.cuh:269-343): <dir>/vtk/sim_data_%06d.vti, VTK ImageData (UInt64
+    // headers) with appended raw Float32 "Density" and 3-component "Velocity". vtk_flags =
+    // LBM_VTI_REFERENCE_OFFSET writes the reference's Velocity offset (without the density
+    // block's length word).
+    void save_vtk(int t, const std::string& dir = "output", uint32_t vtk_flags = 0) {
         update_macroscopics();
-        std::filesystem::create_directories(dir);
-        std::ostringstream name;
-        name << dir << "/lbm_" << t << ".vti";
-        std::ofstream out(name.str(), std::ios::binary);
-        if (!out) throw std::runtime_error("cannot open " + name.str());
-        const size_t n = size_t(NX) * NY;
-        const uint32_t rho_bytes = uint32_t(n * 4), vel_bytes = uint32_t(n * 12);
-        out << "<?xml version=\"1.0\"?>\n<VTKFile type=\"ImageData\" version=\"0.1\" byte_order=\"LittleEndian\">\n"
-            << "  <ImageData WholeExtent=\"0 " << NX - 1 << " 0 " << NY - 1 << " 0 0\" Origin=\"0 0 0\" Spacing=\"1 1 1\">\n"
-            << "    <Piece Extent=\"0 " << NX - 1 << " 0 " << NY - 1 << " 0 0\">\n      <PointData Scalars=\"Density\" Vectors=\"Velocity\">\n"
-            << "        <DataArray type=\"Float32\" Name=\"Density\" format=\"appended\" offset=\"0\"/>\n"
-            << "        <DataArray type=\"Float32\" Name=\"Velocity\" NumberOfComponents=\"3\" format=\"appended\" offset=\""
-            << (4 + rho_bytes) << "\"/>\n      </PointData>\n    </Piece>\n  </ImageData>\n  <AppendedData encoding=\"raw\">\n_";
-        out.write(reinterpret_cast<const char*>(&rho_bytes), 4);
-        out.write(reinterpret_cast<const char*>(h_rho.data()), rho_bytes);
-        out.write(reinterpret_cast<const char*>(&vel_bytes), 4);
-        for (size_t i = 0; i < n; i++) {
-            float v3[3] = {h_u[2 * i], h_u[2 * i + 1], 0.0f};
-            out.write(reinterpret_cast<const char*>(v3), 12);
-        }
-        out << "\n  </AppendedData>\n</VTKFile>\n";
+        if (lbm_write_vti(dir.c_str(), t, NX, NY, 1, 2, h_rho.data(), h_u.data(), vtk_flags) != LBM_OK)
+            throw std::runtime_error("save_vtk: cannot write " + dir + "/vtk");
     }
 };
 

@@ -24,6 +24,7 @@ LBM_BGK, LBM_MRT, LBM_CM = 0, 1, 2
 LBM_NO_ADAPTER, LBM_OPTIMAL_ADAPTER = 0, 1
 LBM_ADAPTER_SAME_STEP, LBM_ADAPTER_LAGGED = 0, 1
 LBM_F32, LBM_F16_SHIFTED, LBM_BF16_SHIFTED = 0, 1, 2
+LBM_VTI_REFERENCE_OFFSET = 1
 
 # boundary flags (src/core/lbm_constants.cuh:377-397 order, + MOVING_WALL_BOUNCE_BACK)
 FLUID = 0
@@ -148,6 +149,9 @@ PROTOTYPES = [
     ("lbm_ibm_get_forces", _I, [_P, _P, _D]),
     ("lbm_adjoint_forward", _I, [_P, _I, _I]),
     ("lbm_adjoint_backward", _I, [_P, _P, _P, _P, _I]),
+    ("lbm_write_vti", _I, [ctypes.c_char_p, ctypes.c_int64, _I, _I, _I, _I, _P, _P, ctypes.c_uint32]),
+    ("lbm_write_macros", _I, [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int64, _I, _P, _P]),
+    ("lbm_write_midplane", _I, [ctypes.c_char_p, ctypes.c_int64, _I, _I, _I, _P, _P]),
     ("lbm_last_error", ctypes.c_char_p, [_P]),
     ("lbm_version", ctypes.c_char_p, []),
     # include/lbm3d_hip.h

@@ -214,6 +214,19 @@ class Solver:
         self._check(self.lib.lbm_reduce_noalloc(self.h, kind, pr, h.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 0))
         return h
 
+    # -- output files (src/core/lbm.cuh:173-204, 269-343) -----------------------------------
+    def save_vtk(self, directory, reference_offset: bool = False):
+        from . import output
+
+        rho, u = self.get_macros()
+        return output.save_vtk(directory, self.timestep, rho, u, reference_offset)
+
+    def save_macroscopics(self, directory):
+        from . import output
+
+        rho, u = self.get_macros()
+        output.save_macroscopics(directory, self.timestep, rho, u)
+
     # -- immersed boundary --------------------------------------------------------------
     def ibm_set_markers(self, xy, ds=None, iters: int = 3):
         xy = np.ascontiguousarray(xy, dtype=np.float32).reshape(-1, 2)

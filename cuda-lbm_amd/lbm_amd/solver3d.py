@@ -159,6 +159,27 @@ class Solver3D:
         self._check(self.lib.lbm3d_reduce(self.h, kind, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out if kind == A.LBM3D_REDUCE_MOMENTUM else out[0]
 
+    # -- output files (src/core/lbm.cuh:173-343) -------------------------------------------
+    def save_vtk(self, directory, reference_offset: bool = False):
+        from . import output
+
+        rho, u = self.get_macros()
+        return output.save_vtk(directory, self.timestep, rho, u, reference_offset)
+
+    def save_macroscopics(self, directory):
+        """Full 3-D dumps (N densities, 3N velocities). The reference's LBM<3> writes only the
+        first NX*NY densities and 2*NX*NY velocity floats here (lbm.cuh:174, 200)."""
+        from . import output
+
+        rho, u = self.get_macros()
+        output.save_macroscopics(directory, self.timestep, rho, u)
+
+    def save_midplane_slice(self, directory):
+        from . import output
+
+        rho, u = self.get_macros()
+        output.save_midplane_slice(directory, self.timestep, rho, u)
+
     def kinetic_energy(self, normalise_u: float | None = 0.05) -> float:
         """calculate_kinetic_energy (kineticEnergy.cuh:44-67): sum 0.5 rho |u|^2 over the last
         stored fields, divided by N * normalise_u^2 (the reference divides by N * 0.05^2);

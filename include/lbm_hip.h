@@ -203,6 +203,25 @@ int lbm_adjoint_forward(lbm_ctx* ctx, int32_t nsteps, int32_t checkpoint_every);
 int lbm_adjoint_backward(lbm_ctx* ctx, const float* dL_du, float* dL_dforce, float* dL_du0,
                          int on_device);
 
+/* ---- output files (host arrays; no device work, no context) --------------------------
+ * Replace LBM::save_vtk (lbm.cuh:269-343): <dir>/vtk/sim_data_%06d.vti, VTK XML ImageData
+ * version 1.0 with UInt64 headers, appended raw Float32 Density [N] and Velocity [N][3]
+ * (dim = 2: u AoS [N][2], written with u_z = 0 and nz = 1; dim = 3: u AoS [N][3]). The
+ * Velocity offset is the one the appended layout implies (8 + 4N); flags =
+ * LBM_VTI_REFERENCE_OFFSET writes the reference's 4N, which omits the density block's
+ * length word. */
+#define LBM_VTI_REFERENCE_OFFSET (1u << 0)
+int lbm_write_vti(const char* dir, int64_t timestep, int32_t nx, int32_t ny, int32_t nz, int32_t dim, const float* rho,
+                  const float* u, uint32_t flags);
+/* Replaces LBM::save_macroscopics (lbm.cuh:173-204): <dir>/density/density_<t>.bin (N floats)
+ * and <dir>/velocity/velocity_<t>.bin (N x dim floats, AoS). */
+int lbm_write_macros(const char* dir, int64_t timestep, int64_t n, int32_t dim, const float* rho, const float* u);
+/* Replaces LBM<3>::save_midplane_slice (lbm.cuh:206-267): the z = nz/2 plane of rho [nz][ny][nx]
+ * and u AoS [nz][ny][nx][3] into <dir>/midplane_density/density_<t>.bin and
+ * <dir>/midplane_velocity/velocity_<t>.bin. */
+int lbm_write_midplane(const char* dir, int64_t timestep, int32_t nx, int32_t ny, int32_t nz, const float* rho,
+                       const float* u);
+
 const char* lbm_last_error(const lbm_ctx* ctx);
 const char* lbm_version(void);
 

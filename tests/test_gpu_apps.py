@@ -26,16 +26,12 @@ def run_app(name, steps, out):
 
 
 def read_vti_velocity(path, n):
-    """Velocity array of the appended-raw .vti written by save_vtk (3 components per node)."""
-    data = path.read_bytes()
-    start = data.index(b"<AppendedData encoding=\"raw\">") + len(b"<AppendedData encoding=\"raw\">")
-    start = data.index(b"_", start) + 1
-    nb = int(np.frombuffer(data[start:start + 4], np.uint32)[0])
-    rho = np.frombuffer(data[start + 4:start + 4 + nb], np.float32)
-    off = start + 4 + nb
-    nv = int(np.frombuffer(data[off:off + 4], np.uint32)[0])
-    vel = np.frombuffer(data[off + 4:off + 4 + nv], np.float32).reshape(n, 3)
-    return rho, vel
+    """(Density, Velocity [n][3]) of the appended-raw .vti, read as VTK does: each DataArray
+    at its declared offset after the '_' marker, a UInt64 byte count, then the values."""
+    from test_output_files import read_vti
+
+    arrays = read_vti(path)
+    return arrays["Density"], arrays["Velocity"].reshape(n, 3)
 
 
 def test_taylor_green_driver_validates_and_writes_outputs(gpu_available, tmp_path):
@@ -49,7 +45,7 @@ def test_taylor_green_driver_validates_and_writes_outputs(gpu_available, tmp_pat
     # the driver's TaylorGreenScenario::compute_error and the same formula on the dump agree
     assert abs(sc.tg_error(u, 0.01, 0.04, 1000) - err) <= 1e-4 * max(err, 1e-3)
     assert err < 1.0
-    vrho, vel = read_vti_velocity(tmp_path / "lbm_1000.vti", n)
+    vrho, vel = read_vti_velocity(tmp_path / "vtk" / "sim_data_001000.vti", n)
     assert np.array_equal(vrho, rho) and np.array_equal(vel[:, :2], u.reshape(n, 2)) and not vel[:, 2].any()
 
 
