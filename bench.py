@@ -152,7 +152,19 @@ def run_adjoint(torch, steps=2000, every=100):
             "gradient_finite": ok}
 
 
-def run_inverse_problem(torch, steps=2000, every=100, directions=16, iters=200, lr=1e-6):
+def low_pass(F, kmax):
+    """The Fourier modes of a periodic field [ny, nx, 2] with |kx|, |ky| <= kmax cycles."""
+    import torch
+
+    G = torch.fft.fft2(F.double(), dim=(0, 1))
+    ny, nx = F.shape[:2]
+    ky = torch.fft.fftfreq(ny, 1.0 / ny, device=F.device).abs()[:, None]
+    kx = torch.fft.fftfreq(nx, 1.0 / nx, device=F.device).abs()[None, :]
+    mask = ((kx <= kmax) & (ky <= kmax)).to(G.dtype)[..., None]
+    return torch.fft.ifft2(G * mask, dim=(0, 1)).real
+
+
+def run_inverse_problem(torch, steps=2000, every=100, directions=16, iters=200, lr=1e-6, fd_scale=1e-5):
     """The differentiable inverse problem as configured: BGK 512^2 periodic, nu = 0.02, FP32,
     2000 steps, checkpoint every 100; the unknown is the 16-mode Fourier body force (amplitude
     1e-5, seed 0xF0F0); the loss is the L2 misfit of the final velocity. Reports the adjoint
@@ -184,7 +196,7 @@ def run_inverse_problem(torch, steps=2000, every=100, directions=16, iters=200, 
     errs = []
     for _ in range(directions):
         d = torch.as_tensor(rng.standard_normal(tuple(F_true.shape)), device=dev)
-        d *= 1e-6 / float(d.abs().max())
+        d *= fd_scale / float(d.abs().max())  # well above the FP32 forward's round-off in the loss
         fd = (loss_of((F0.detach() + d).float()) - loss_of((F0.detach() - d).float())) / 2.0
         an = float(torch.sum(g * d))
         errs.append(abs(fd - an) / max(abs(an), 1e-300))
@@ -195,11 +207,15 @@ def run_inverse_problem(torch, steps=2000, every=100, directions=16, iters=200, 
     dt = time.perf_counter() - t0
     sim.close()
     ferr = float(torch.linalg.vector_norm(F_rec - F_true) / torch.linalg.vector_norm(F_true))
+    # the unknown's own modes (|k| <= 4 cycles per period): the part the final snapshot determines
+    lp = low_pass(F_rec, 4)
+    ferr_lp = float(torch.linalg.vector_norm(lp - F_true.double()) / torch.linalg.vector_norm(F_true.double()))
     return {"workload": "inverse problem BGK 512x512 periodic nu=0.02 FP32, 2000 steps, checkpoint every 100, "
                         "16-mode Fourier force amplitude 1e-5",
             "fd_directions": directions, "fd_max_rel_err": float(max(errs)), "fd_median_rel_err": float(np.median(errs)),
             "adam_iters": iters, "adam_lr": lr, "loss_first": losses[0], "loss_last": losses[-1],
-            "loss_ratio": losses[-1] / losses[0], "force_rel_l2_err": ferr,
+            "loss_ratio": losses[-1] / losses[0], "force_rel_l2_err": ferr, "force_rel_l2_err_k_le_4": ferr_lp,
+            "fd_scale": fd_scale,
             "seconds_per_iteration": round(dt / iters, 4)}
 
 
@@ -261,7 +277,8 @@ def run_poiseuille(torch, steps=400, warmup=20, adapter_steps=100):
         s.close()
         adapt[name] = dict(r, finite=bool(np.isfinite(ua).all()))
     adapt["workload"] = (f"CM<OptimalAdapter> (same-step means), first {warmup + adapter_steps} steps; per step: "
-                         f"moments pass + FP64 means + fused step")
+                         f"one fused step that also sums the next step's means (look-ahead) + an FP64 finalize; "
+                         f"the F32/BF16 runs diverge here as the reference's adapter does on this channel")
     return {"cm_no_adapter": plain, "cm_optimal_adapter": adapt,
             "workload": "Poiseuille channel 4096x1024, nu=0.05, Guo force for u_max=0.05, periodic x, bounce-back walls"}
 

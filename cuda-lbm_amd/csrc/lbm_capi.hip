@@ -152,6 +152,61 @@ int lbm_ctx::adapter_pass(int src_idx) {
     return LBM_OK;
 }
 
+// CM<OptimalAdapter> with same-step averages (the reference's order: the averages of a
+// step's post-stream state feed that step's collision) in one launch per step: the
+// look-ahead strip kernel collides step t+1 with averages computed by the launch before and
+// sums the moments of step t+2 on the way. Needs the strip machinery's conditions: local
+// boundary functors, no stale-rest quirk, no IBM body.
+bool lbm_ctx::ahead_ok() const {
+    const char* v = std::getenv("LBM_ADAPTER_LOOKAHEAD");
+    if (v && v[0] == '0') return false;
+    return cfg.collision == LBM_CM && cfg.adapter == LBM_OPTIMAL_ADAPTER && cfg.adapter_mode == LBM_ADAPTER_SAME_STEP &&
+           flags_local && !ibm.active && !(cfg.quirks & LBM_QUIRK_STALE_REST) && !edge_in_store;
+}
+
+int lbm_ctx::ahead_step(bool write_macros) {
+    hipError_t e;
+    int rc;
+    if (!ahead_valid) {  // this step's averages from a moments pass over its post-stream state
+        StepParams p = step_params(cur, false);
+        LaunchPolicy lp = step_policy(plane, cfg.storage);
+        long long nb = step_grid_blocks(p, lp);
+        rc = ensure_partials(nb);
+        if (rc) return rc;
+        e = launch_step(cfg.collision, cfg.storage, 1, p, lp, partials, stream);
+        if (e != hipSuccess) return hip_fail(e, "moments pass launch");
+        e = launch_finalize_avg(partials, nb, (long long)N, d_ahead, d_ahead64, stream);
+        if (e != hipSuccess) return hip_fail(e, "finalize launch");
+    }
+    if (n_edge > 0 && !ahead_edges) {  // state t-1 at the edge nodes: buf[1-cur] -> store
+        e = launch_edge_copy(cfg.storage, buf[1 - cur], d_edge[edge_cur], plane, cfg.nx, cfg.ny, cfg.periodic_x,
+                             cfg.periodic_y, 1, stream);
+        if (e != hipSuccess) return hip_fail(e, "edge save launch");
+    }
+    StepParams p = step_params(cur, write_macros);
+    p.mid = p.dst;  // phase A's own nodes are the launch's output
+    p.avg = d_ahead;
+    p.edge_prev = d_edge[edge_cur];
+    p.edge_next = d_edge[1 - edge_cur];
+    p.n_edge = n_edge;
+    const Step2Policy lp = ahead_policy(plane, cfg.storage);
+    const long long np = ahead_partials_needed(p, lp);
+    rc = ensure_partials(np);
+    if (rc) return rc;
+    e = launch_step_ahead(cfg.collision, cfg.storage, p, lp, partials, stream);
+    if (e != hipSuccess) return hip_fail(e, "look-ahead step launch");
+    e = launch_finalize_ahead(partials, np, (long long)N, d_ahead, d_ahead64, d_avg, d_avg64, stream);
+    if (e != hipSuccess) return hip_fail(e, "finalize launch");
+    if (n_edge > 0) edge_cur = 1 - edge_cur;
+    ahead_edges = n_edge > 0;
+    ahead_valid = true;
+    avg_valid = true;
+    cur = 1 - cur;
+    timestep++;
+    if (write_macros) macros_valid = true;
+    return LBM_OK;
+}
+
 // After a two-step launch, put the edge nodes' previous state back into buf[1-cur], where
 // one-step launches (and the IBM gather) read unstreamed slots.
 int lbm_ctx::restore_edges() {
@@ -172,6 +227,7 @@ bool lbm_ctx::two_step_ok() const {
 // one-step launches).
 int lbm_ctx::two_step(bool write_macros) {
     hipError_t e;
+    ahead_valid = ahead_edges = false;
     if (n_edge > 0 && !edge_in_store) {  // state t-1 at the edge nodes: buf[1-cur] -> store
         e = launch_edge_copy(cfg.storage, buf[1 - cur], d_edge[edge_cur], plane, cfg.nx, cfg.ny, cfg.periodic_x,
                              cfg.periodic_y, 1, stream);
@@ -292,6 +348,8 @@ int lbm_ctx::ibm_graph_pair() {
 int lbm_ctx::one_step(bool write_macros) {
     int rc = restore_edges();
     if (rc) return rc;
+    if (ahead_ok()) return ahead_step(write_macros);
+    ahead_valid = ahead_edges = false;
     bool adapt = cfg.collision == LBM_CM && cfg.adapter == LBM_OPTIMAL_ADAPTER;
     if (ibm.active && ibm.split_ok && !adapt) return split_ibm_step(write_macros);
     if (ibm.active) {
@@ -328,8 +386,8 @@ lbm_ctx::~lbm_ctx() {
     if (own_stream) (void)hipStreamSynchronize(own_stream);
     if (stream != own_stream) (void)hipStreamSynchronize(stream);
     if (aux_stream) (void)hipStreamSynchronize(aux_stream);
-    void* ptrs[] = {buf[0],  buf[1], d_flags, d_row_class, d_force_field, d_rho,     d_ux,
-                    d_uy,    d_avg,  d_avg64, partials,    tmp,           d_edge[0], d_edge[1]};
+    void* ptrs[] = {buf[0],  buf[1], d_flags, d_row_class, d_force_field, d_rho,     d_ux,      d_uy,
+                    d_avg,   d_avg64, d_ahead, d_ahead64, partials,     tmp,       d_edge[0], d_edge[1]};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     ibm_graph_free();
@@ -420,6 +478,8 @@ int lbm_create(const lbm_config* cfg, lbm_ctx** out) {
     rc = rc ? rc : c->dev_alloc(reinterpret_cast<void**>(&c->d_uy), c->N * 4, "uy");
     rc = rc ? rc : c->dev_alloc(reinterpret_cast<void**>(&c->d_avg), 4 * sizeof(float), "adapter averages");
     rc = rc ? rc : c->dev_alloc(reinterpret_cast<void**>(&c->d_avg64), 4 * sizeof(double), "adapter averages");
+    rc = rc ? rc : c->dev_alloc(reinterpret_cast<void**>(&c->d_ahead), 4 * sizeof(float), "adapter averages");
+    rc = rc ? rc : c->dev_alloc(reinterpret_cast<void**>(&c->d_ahead64), 4 * sizeof(double), "adapter averages");
     c->n_edge = edge_count(k.nx, k.ny, k.periodic_x, k.periodic_y);
     if (c->n_edge > 0) {
         const size_t eb = size_t(c->n_edge) * 9 * storage_bytes(k.storage);
@@ -454,6 +514,7 @@ int lbm_set_flags(lbm_ctx* ctx, const uint8_t* flags, int on_device) {
     if (!ctx || !flags) return LBM_E_INVALID;
     ctx->ibm_graph_free();  // the captured IBM steps read the flags and the force field
     ctx->adj.ready = false;  // the adjoint's recorded forward no longer matches the state
+    ctx->ahead_valid = ctx->ahead_edges = false;
     int rc = ctx->activate();
     if (rc) return rc;
     const int nx = ctx->cfg.nx, ny = ctx->cfg.ny;
@@ -534,6 +595,7 @@ int lbm_set_flags(lbm_ctx* ctx, const uint8_t* flags, int on_device) {
 int lbm_init_from_macros(lbm_ctx* ctx, const float* rho, const float* u, int on_device) {
     if (!ctx || !rho || !u) return LBM_E_INVALID;
     ctx->adj.ready = false;  // the adjoint's recorded forward no longer matches the state
+    ctx->ahead_valid = ctx->ahead_edges = false;
     int rc = ctx->activate();
     if (rc) return rc;
     const float *drho = rho, *du = u;
@@ -573,6 +635,7 @@ int lbm_set_force_field(lbm_ctx* ctx, const float* force, int on_device) {
     if (!ctx) return LBM_E_INVALID;
     ctx->ibm_graph_free();  // the captured IBM steps read the flags and the force field
     ctx->adj.ready = false;  // the adjoint's recorded forward no longer matches the state
+    ctx->ahead_valid = ctx->ahead_edges = false;
     int rc = ctx->activate();
     if (rc) return rc;
     ctx->ibm.box_valid = false;  // the IBM box's external force changes with the field
@@ -697,6 +760,7 @@ int lbm_get_pops(lbm_ctx* ctx, float* f, int on_device) {
 int lbm_set_pops(lbm_ctx* ctx, const float* f, int on_device) {
     if (!ctx || !f) return LBM_E_INVALID;
     ctx->adj.ready = false;  // the adjoint's recorded forward no longer matches the state
+    ctx->ahead_valid = ctx->ahead_edges = false;
     int rc = ctx->activate();
     if (rc) return rc;
     const float* in = f;

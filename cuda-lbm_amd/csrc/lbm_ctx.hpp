@@ -90,6 +90,12 @@ struct lbm_ctx {
     float *d_rho = nullptr, *d_ux = nullptr, *d_uy = nullptr;
     float* d_avg = nullptr;
     double* d_avg64 = nullptr;
+    // CM<OptimalAdapter>, same-step averages: the averages of the next step, computed by the
+    // previous look-ahead launch (valid while nothing else changed the state)
+    float* d_ahead = nullptr;
+    double* d_ahead64 = nullptr;
+    bool ahead_valid = false;
+    bool ahead_edges = false;  // d_edge[edge_cur] holds the edge nodes' state one step back
     double* partials = nullptr;
     long long partials_n = 0;
     void* tmp = nullptr;
@@ -110,6 +116,8 @@ struct lbm_ctx {
     int ensure_partials(long long n);
     lbm::StepParams step_params(int src_idx, bool write_macros) const;
     int adapter_pass(int src_idx);
+    bool ahead_ok() const;
+    int ahead_step(bool write_macros);
     int one_step(bool write_macros);
     int split_ibm_step(bool write_macros);
     bool ibm_graph_ok() const;

@@ -135,6 +135,62 @@ void fused_step2s_kernel(StepParams p, long long nb_padded, int remap, int strip
     }
 }
 
+// One step per launch with the OptimalAdapter's averages of the next step computed on the
+// way (look-ahead): the marching-strip machinery of fused_step2s_kernel with phase A as the
+// step itself (own nodes stored to p.mid = the destination) and phase B reduced to the
+// moment sums of the following step's post-stream state. Each workgroup writes its sums to
+// partials[logical block]; finalize_ahead_kernel turns them into the next step's averages.
+template <int COLL, int STORE, int W, bool BUF>
+__global__ __launch_bounds__(kT2L * W) __attribute__((amdgpu_waves_per_eu(kStripWavesPerEu<STORE>)))
+void fused_step_ahead_kernel(StepParams p, long long nb_padded, int remap, int strips_x, long long n_strips, int H,
+                             double* partials) {
+    using T = typename Elem<STORE>::T;
+    constexpr int R = W + 2;  // ring rows
+    __shared__ T ring[Q * R * kT2L];
+    const long long lb = logical_block(nb_padded, remap);
+    if (lb >= n_strips) return;  // whole workgroup: remap padding
+    const int sy = int(lb / strips_x), sx = int(lb - (long long)sy * strips_x);
+    const int x0 = sx * kT2W, y0 = sy * H;
+    const int hs = p.ny - y0 < H ? p.ny - y0 : H;
+    const int l = threadIdx.x & (kT2L - 1), w = threadIdx.x / kT2L;
+    const int K = (hs + 1) / W + 1;
+    const bool interior_x = interior_cols(p, x0);
+    int i = w, gx = 0, gy = 0;
+    bool va = i <= hs + 1 && strip_node(p, x0, y0, l, i, gx, gy);
+    MomAcc acc{0.0, 0.0, 0.0};
+    StepParams pl = p;
+    for (int k = 0; k < K; k++) {
+        asm volatile("" : "+s"(pl.plane), "+s"(pl.nx), "+s"(pl.n_edge), "+s"(pl.src), "+s"(pl.dst), "+s"(pl.mid),
+                     "+s"(pl.edge_prev), "+s"(pl.edge_next));
+        asm volatile("" : "+s"(pl.px), "+s"(pl.py), "+s"(pl.ny), "+s"(pl.flags), "+s"(pl.forced), "+s"(pl.force_field),
+                     "+s"(pl.ibm_force), "+s"(pl.out_rho), "+s"(pl.quirks));
+        if (va) {
+            float g[Q];
+            step2_gather<STORE, BUF>(pl, gx, gy, g, interior_x && interior_row(pl, y0 - 1 + i));
+            step2s_phase_a<COLL, STORE, BUF, true>(pl, ring, R, i % R, l, i, x0, y0, hs, gx, gy, g);
+        } else if (i <= hs + 1) {
+            step2_fill_halo<STORE, BUF>(pl, ring + (i % R) * kT2L + l, R * kT2L, x0 - 1 + l, y0 - 1 + i);
+        }
+        __syncthreads();
+        const int in = (k + 1) * W + w;
+        int nxg = 0, nyg = 0;
+        const bool vn = k + 1 < K && in <= hs + 1 && strip_node(p, x0, y0, l, in, nxg, nyg);
+        const int j = k * W - 2 + w;
+        if (j >= 0 && j < hs) step2s_phase_b_moments<STORE, BUF>(pl, ring, R, l, j, x0, y0, acc);
+        __syncthreads();
+        i = in;
+        va = vn;
+        gx = nxg;
+        gy = nyg;
+    }
+    block_sum3(acc.rho, acc.jmag, acc.pimag);
+    if (threadIdx.x == 0) {
+        partials[3 * lb + 0] = acc.rho;
+        partials[3 * lb + 1] = acc.jmag;
+        partials[3 * lb + 2] = acc.pimag;
+    }
+}
+
 // Copy the populations of the non-periodic edge nodes between a population buffer and an
 // edge store ([9][n_edge], edge_slot order). to_store = 1: buffer -> store (every slot);
 // 0: store -> buffer (each node from its first slot only).
@@ -178,6 +234,31 @@ __global__ __launch_bounds__(kBlock) void finalize_avg_kernel(const double* part
             avg64[1] = b * inv_nodes;
             avg64[2] = c * inv_nodes;
         }
+    }
+}
+
+// After a look-ahead launch: the averages that launch collided with (`ahead`) become the
+// reported ones (`avg`), and the launch's partial sums become the next step's `ahead`.
+__global__ __launch_bounds__(kBlock) void finalize_ahead_kernel(const double* partials, long long n, double inv_nodes,
+                                                              float* ahead, double* ahead64, float* avg, double* avg64) {
+    if (threadIdx.x < 3) {
+        avg[threadIdx.x] = ahead[threadIdx.x];
+        avg64[threadIdx.x] = ahead64[threadIdx.x];
+    }
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (long long i = threadIdx.x; i < n; i += kBlock) {
+        a += partials[3 * i];
+        b += partials[3 * i + 1];
+        c += partials[3 * i + 2];
+    }
+    block_sum3(a, b, c);  // its barrier orders the copies above before the stores below
+    if (threadIdx.x == 0) {
+        ahead[0] = float(a * inv_nodes);
+        ahead[1] = float(b * inv_nodes);
+        ahead[2] = float(c * inv_nodes);
+        ahead64[0] = a * inv_nodes;
+        ahead64[1] = b * inv_nodes;
+        ahead64[2] = c * inv_nodes;
     }
 }
 
@@ -498,6 +579,55 @@ hipError_t launch_edge_copy(int storage, void* buf, void* store, long long plane
         hipLaunchKernelGGL(edge_copy_kernel<LBM_BF16_SHIFTED>, dim3(nb), dim3(kBlock), 0, s, buf, store, plane, nx, ny, px, py, n_edge, to_store);
     else
         hipLaunchKernelGGL(edge_copy_kernel<LBM_F32>, dim3(nb), dim3(kBlock), 0, s, buf, store, plane, nx, ny, px, py, n_edge, to_store);
+    return hipGetLastError();
+}
+
+template <int COLL, int STORE, bool BUF>
+static hipError_t launch_ahead_t(const StepParams& p, const Step2Policy& lp, double* partials, hipStream_t s) {
+    const int strips_x = (p.nx + kT2W - 1) / kT2W;
+    const long long n_strips = (long long)strips_x * ((p.ny + lp.h - 1) / lp.h);
+    const long long nb = step2_grid_blocks(p, lp);
+    hipLaunchKernelGGL((fused_step_ahead_kernel<COLL, STORE, 4, BUF>), dim3(unsigned(nb)), dim3(kT2L * 4), 0, s, p, nb,
+                       lp.remap, strips_x, n_strips, lp.h, partials);
+    return hipGetLastError();
+}
+
+template <int STORE>
+static hipError_t launch_ahead_s(int coll, const StepParams& p, const Step2Policy& lp, double* partials, hipStream_t s) {
+    using T = typename Elem<STORE>::T;
+    const bool fits = buf_fits<T>(Q, p.plane);
+    (void)coll;  // the OptimalAdapter applies to CM only
+    return fits ? launch_ahead_t<LBM_CM, STORE, true>(p, lp, partials, s) : launch_ahead_t<LBM_CM, STORE, false>(p, lp, partials, s);
+}
+
+long long ahead_partials_needed(const StepParams& p, const Step2Policy& lp) {
+    return (long long)((p.nx + kT2W - 1) / kT2W) * ((p.ny + lp.h - 1) / lp.h);
+}
+
+Step2Policy ahead_policy(long long plane, int storage) {
+    Step2Policy lp = step2_policy(plane, storage, LBM_CM);
+    lp.strip = 1;
+    lp.w = 4;
+    if (lp.h < 1 || lp.h > 64) lp.h = 16;
+    if (storage == LBM_F32 && lp.h > 16) lp.h = 16;
+    return lp;
+}
+
+hipError_t launch_step_ahead(int coll, int storage, const StepParams& p, const Step2Policy& lp, double* partials,
+                             hipStream_t s) {
+    if (coll != LBM_CM || !lp.strip || lp.w != 4 || lp.h < 1 || !partials || !p.avg || p.mid != p.dst)
+        return hipErrorInvalidValue;
+    if (p.n_edge != edge_count(p.nx, p.ny, p.px, p.py)) return hipErrorInvalidValue;
+    if (p.n_edge > 0 && (!p.edge_prev || !p.edge_next)) return hipErrorInvalidValue;
+    if (storage == LBM_F16_SHIFTED) return launch_ahead_s<LBM_F16_SHIFTED>(coll, p, lp, partials, s);
+    if (storage == LBM_BF16_SHIFTED) return launch_ahead_s<LBM_BF16_SHIFTED>(coll, p, lp, partials, s);
+    return launch_ahead_s<LBM_F32>(coll, p, lp, partials, s);
+}
+
+hipError_t launch_finalize_ahead(const double* partials, long long n, long long N, float* ahead, double* ahead64, float* avg,
+                                 double* avg64, hipStream_t s) {
+    hipLaunchKernelGGL(finalize_ahead_kernel, dim3(1), dim3(kBlock), 0, s, partials, n, 1.0 / double(N), ahead, ahead64, avg,
+                       avg64);
     return hipGetLastError();
 }
 

@@ -48,6 +48,16 @@ hipError_t launch_step2(int coll, int storage, const StepParams& p, const Step2P
 hipError_t launch_edge_copy(int storage, void* buf, void* store, long long plane, int nx, int ny, int px, int py,
                             int to_store, hipStream_t s);
 hipError_t launch_finalize_avg(const double* partials, long long n, long long N, float* avg, double* avg64, hipStream_t s);
+// One CM<OptimalAdapter> step per launch (marching strips of 4 wavefronts) that also sums the
+// next step's adapter moments into partials (one triple per strip): p.avg holds the averages
+// of this step, p.mid == p.dst, edge stores as for launch_step2 (the next store receives the
+// source state of the edge nodes). Then launch_finalize_ahead: avg <- ahead, ahead <- means.
+Step2Policy ahead_policy(long long plane, int storage);
+long long ahead_partials_needed(const StepParams& p, const Step2Policy& lp);
+hipError_t launch_step_ahead(int coll, int storage, const StepParams& p, const Step2Policy& lp, double* partials,
+                             hipStream_t s);
+hipError_t launch_finalize_ahead(const double* partials, long long n, long long N, float* ahead, double* ahead64, float* avg,
+                                 double* avg64, hipStream_t s);
 hipError_t launch_init(int storage, const InitParams& p, hipStream_t s);
 hipError_t launch_pops_to_f32(int storage, const void* buf, long long plane, long long N, float rho0, float* out, hipStream_t s);
 hipError_t launch_pops_from_f32(int storage, void* b0, void* b1, long long plane, long long N, float rho0, const float* in, hipStream_t s);

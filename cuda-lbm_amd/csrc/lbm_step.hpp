@@ -887,7 +887,11 @@ LBM_HD bool strip_node(const StepParams& p, int x0, int y0, int l, int i, int& g
 
 // Phase A of strip node (l, i) from its gathered sources g: collide, store into the ring
 // row `slot` and, for the strip's own edge nodes, into the next edge store.
-template <int COLL, int STORE, bool BUF>
+// AHEAD (one step per launch, the look-ahead form below): the launch's output is phase A
+// itself, so the host points p.mid at the destination; own nodes also store the step's
+// fields, and the next edge store receives their current (source) state, which is what
+// the next launch's unstreamed slots hold.
+template <int COLL, int STORE, bool BUF, bool AHEAD = false>
 LBM_HD void step2s_phase_a(const StepParams& p, typename Elem<STORE>::T* ring, int ring_rows, int slot, int l, int i,
                            int x0, int y0, int hs, int gx, int gy, float* g) {
     using St = Elem<STORE>;
@@ -910,8 +914,14 @@ LBM_HD void step2s_phase_a(const StepParams& p, typename Elem<STORE>::T* ring, i
 #pragma unroll
         for (int q = 0; q < Q; q++) md.st(node, q * p.plane, St::enc1(g[q], q, p.rho0));
     }
+    if (AHEAD && own && p.out_rho) {
+        p.out_rho[node] = m.rho;
+        p.out_ux[node] = m.ux;
+        p.out_uy[node] = m.uy;
+    }
     if (own && (ex || ey)) {
         const PopBuf<T, BUF> en(p.edge_next);
+        const PopBuf<T, BUF> src(p.src);
         const int base = p.py ? 0 : 2 * p.nx;
         int es[2] = {-1, -1};
         if (ey) es[0] = gy == 0 ? gx : p.nx + gx;
@@ -920,9 +930,40 @@ LBM_HD void step2s_phase_a(const StepParams& p, typename Elem<STORE>::T* ring, i
         for (int k = 0; k < 2; k++) {
             if (es[k] < 0) continue;
 #pragma unroll
-            for (int q = 0; q < Q; q++) en.st(uint32_t(es[k]), q * (long long)p.n_edge, St::enc1(g[q], q, p.rho0));
+            for (int q = 0; q < Q; q++)
+                en.st(uint32_t(es[k]), q * (long long)p.n_edge,
+                      AHEAD ? src.ld(node, q * p.plane) : St::enc1(g[q], q, p.rho0));
         }
     }
+}
+
+// Look-ahead phase B of strip row j, lane l (AHEAD launches): the post-stream, boundary-
+// applied state of the next step from the ring, reduced to the OptimalAdapter's moment sums
+// (macroscopics.cuh:50-120) in acc; nothing is collided or stored. The adapter's averages
+// of step t+2 are thus ready when the launch of step t+2 starts, without a separate pass
+// over the populations.
+template <int STORE, bool BUF>
+LBM_HD void step2s_phase_b_moments(const StepParams& p, const typename Elem<STORE>::T* ring, int ring_rows, int l,
+                                   int j, int x0, int y0, MomAcc& acc) {
+    using St = Elem<STORE>;
+    if (l < 1 || l > kT2W) return;
+    const int x = x0 - 1 + l, y = y0 + j;
+    if (x >= p.nx) return;
+    const uint32_t node = uint32_t(y) * uint32_t(p.nx) + uint32_t(x);
+    const int stride = ring_rows * kT2L;
+    const int s_lo = j % ring_rows, s_mid = (j + 1) % ring_rows, s_hi = (j + 2) % ring_rows;
+    float g[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int srow = CY[q] > 0 ? s_lo : (CY[q] < 0 ? s_hi : s_mid);
+        g[q] = St::load1(ring + q * stride + srow * kT2L + (l - CX[q]), q, p.rho0);
+    }
+    const uint8_t fl = p.flags ? p.flags[node] : uint8_t(LBM_FLUID);
+    if (fl != LBM_FLUID) apply_local_bc(g, fl, x, y, node_args(p));
+    float fx, fy;
+    node_force(p, x, y, node, fx, fy);
+    Macro m;
+    node_update<LBM_BGK, 1>(p, g, fx, fy, m, acc);
 }
 
 // Phase B of strip row j (grid row y0 + j), lane l: pulls from ring rows j, j+1, j+2 (the

@@ -94,6 +94,52 @@ void launch2s_emu(const StepParams& p, int H, int reverse) {
     }
 }
 
+// One look-ahead launch (CM<OptimalAdapter>, same-step averages): marching strips of W
+// wavefronts and H rows whose phase A is the step (own nodes into p.mid = p.dst) and whose
+// phase B sums the next step's moments, one triple per strip into partials (logical order).
+template <int COLL, int STORE, int W>
+void launch_ahead_emu(const StepParams& p, int H, int reverse, std::vector<double>& partials) {
+    using T = typename Elem<STORE>::T;
+    constexpr int R = W + 2;
+    const int strips_x = (p.nx + kT2W - 1) / kT2W, strips_y = (p.ny + H - 1) / H;
+    const long long ns = (long long)strips_x * strips_y;
+    partials.assign(size_t(3 * ns), 0.0);
+    std::vector<T> ring(size_t(Q) * R * kT2L);
+    for (long long k0 = 0; k0 < ns; k0++) {
+        const long long lb = reverse ? ns - 1 - k0 : k0;
+        const int sy = int(lb / strips_x), sx = int(lb - (long long)sy * strips_x);
+        const int x0 = sx * kT2W, y0 = sy * H;
+        const int hs = p.ny - y0 < H ? p.ny - y0 : H;
+        const int K = (hs + 1) / W + 1;
+        std::fill(ring.begin(), ring.end(), poison<T>());
+        const bool interior_x = interior_cols(p, x0);
+        MomAcc acc{0, 0, 0};
+        for (int k = 0; k < K; k++) {
+            for (int w = 0; w < W; w++)
+                for (int l = 0; l < kT2L; l++) {
+                    const int i = k * W + w;
+                    int gx, gy;
+                    if (i > hs + 1) continue;
+                    if (!strip_node(p, x0, y0, l, i, gx, gy)) {
+                        step2_fill_halo<STORE, true>(p, ring.data() + (i % R) * kT2L + l, R * kT2L, x0 - 1 + l, y0 - 1 + i);
+                        continue;
+                    }
+                    float g[Q];
+                    step2_gather<STORE, true>(p, gx, gy, g, interior_x && interior_row(p, y0 - 1 + i));
+                    step2s_phase_a<COLL, STORE, true, true>(p, ring.data(), R, i % R, l, i, x0, y0, hs, gx, gy, g);
+                }
+            for (int w = 0; w < W; w++)
+                for (int l = 0; l < kT2L; l++) {
+                    const int j = k * W - 2 + w;
+                    if (j >= 0 && j < hs) step2s_phase_b_moments<STORE, true>(p, ring.data(), R, l, j, x0, y0, acc);
+                }
+        }
+        partials[size_t(3 * lb)] = acc.rho;
+        partials[size_t(3 * lb + 1)] = acc.jmag;
+        partials[size_t(3 * lb + 2)] = acc.pimag;
+    }
+}
+
 template <int COLL, int STORE, int SEG>
 int run_t(const lbm_config& c, const uint8_t* flags, const float* rho, const float* u, const float* force,
           int nsteps, float* f_out, float* rho_out, float* u_out, double* avg_out, int two_ty, int reverse) {
@@ -141,6 +187,13 @@ int run_t(const lbm_config& c, const uint8_t* flags, const float* rho, const flo
         for (long long n = 0; n < N; n++)
             local = local && !flag_reads_neighbour(fl[n]);
     const bool two = two_ty != 0 && !adapt && !(c.quirks & LBM_QUIRK_STALE_REST) && local;
+    // CM<OptimalAdapter> with same-step averages: look-ahead launches where the library uses them
+    const bool ahead = two_ty < 0 && adapt && c.adapter_mode == LBM_ADAPTER_SAME_STEP &&
+                       !(c.quirks & LBM_QUIRK_STALE_REST) && local;
+    bool ahead_valid = false, ahead_edges = false;
+    float ahead_avg[3] = {0.f, 0.f, 0.f};
+    double ahead64[3] = {0, 0, 0};
+    std::vector<double> partials;
     const int n_edge = edge_count(nx, ny, c.periodic_x, c.periodic_y);
     std::vector<T> edge[2] = {std::vector<T>(size_t(9) * n_edge), std::vector<T>(size_t(9) * n_edge)};
     int edge_cur = 0;
@@ -213,6 +266,52 @@ int run_t(const lbm_config& c, const uint8_t* flags, const float* rho, const flo
         }
         p.ntx = (nx + SEG - 1) / SEG;
         const long long nseg = (long long)p.ntx * ny;
+        if (ahead) {
+            if (!ahead_valid) {  // this step's averages from a moments pass
+                MomAcc tot{0, 0, 0};
+                for (long long sg = 0; sg < nseg; sg++) step_segment<LBM_BGK, STORE, 1, SEG>(p, sg, tot);
+                ahead64[0] = tot.rho / double(N);
+                ahead64[1] = tot.jmag / double(N);
+                ahead64[2] = tot.pimag / double(N);
+                for (int k = 0; k < 3; k++) ahead_avg[k] = float(ahead64[k]);
+            }
+            if (n_edge > 0 && !ahead_edges) {  // state t-1 at the edge nodes -> store
+                for (int e = 0; e < n_edge; e++) {
+                    int x, y;
+                    edge_node(e, nx, ny, c.periodic_y, x, y);
+                    for (int q = 0; q < Q; q++)
+                        edge[edge_cur][size_t(q) * n_edge + e] = buf[1 - cur][q * N + (long long)y * nx + x];
+                }
+            }
+            p.mid = p.dst;
+            p.avg = ahead_avg;
+            p.edge_prev = n_edge ? edge[edge_cur].data() : nullptr;
+            p.edge_next = n_edge ? edge[1 - edge_cur].data() : nullptr;
+            p.n_edge = n_edge;
+            const int H = (-two_ty) % 1000;
+            launch_ahead_emu<COLL, STORE, 4>(p, H, reverse, partials);
+            // finalize: the averages used become the reported ones, the sums the next ones
+            for (int k = 0; k < 3; k++) {
+                avg[k] = ahead_avg[k];
+                avg64[k] = ahead64[k];
+            }
+            double a = 0, b = 0, cc = 0;
+            for (size_t i = 0; i < partials.size(); i += 3) {
+                a += partials[i];
+                b += partials[i + 1];
+                cc += partials[i + 2];
+            }
+            ahead64[0] = a / double(N);
+            ahead64[1] = b / double(N);
+            ahead64[2] = cc / double(N);
+            for (int k = 0; k < 3; k++) ahead_avg[k] = float(ahead64[k]);
+            if (n_edge > 0) edge_cur = 1 - edge_cur;
+            ahead_edges = n_edge > 0;
+            ahead_valid = true;
+            cur = 1 - cur;
+            s += 1;
+            continue;
+        }
         if (adapt) {
             MomAcc tot{0, 0, 0};
             for (long long sg = 0; sg < nseg; sg++) step_segment<LBM_BGK, STORE, 1, SEG>(p, sg, tot);
