@@ -6,6 +6,10 @@ from pathlib import Path
 import pytest
 
 ROOT = Path(__file__).resolve().parent.parent
+# Under pytest-xdist every worker would start an OpenMP team of all cores for the oracle, and
+# spinning teams starve each other; one thread per worker (set before libgomp loads).
+if os.environ.get("PYTEST_XDIST_WORKER"):
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
 for p in (ROOT / "cuda-lbm_amd", ROOT / "oracle", ROOT / "tests"):
     if str(p) not in sys.path:
         sys.path.insert(0, str(p))
