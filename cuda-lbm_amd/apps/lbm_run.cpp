@@ -1,6 +1,7 @@
 // lbm_run — C++ driver for a compile-time scenario (the role of src/main.cu): allocate,
 // init, run the fused steps through liblbm_hip.so and print the scenario's validation.
-// Build with -DRUN_TAYLOR_GREEN or -DRUN_LID_DRIVEN and -DNX=.. -DNY=.. .
+// Build with -DRUN_TAYLOR_GREEN or -DRUN_LID_DRIVEN, or -DSCENARIO_HEADER='"path"'
+// -DSCENARIO_TYPE=Name, and -DNX=.. -DNY=.. .
 // Usage: lbm_run [steps [output_dir]]; with an output directory the final fields are also
 // written as .bin dumps and a .vti file (save_macroscopics / save_vtk).
 #include <chrono>
@@ -15,6 +16,10 @@ using Scenario = TaylorGreenPeriodic;
 #elif defined(RUN_LID_DRIVEN)
 #include "lidDrivenMovingWall.cuh"
 using Scenario = LidDrivenMovingWall;
+#elif defined(SCENARIO_HEADER)
+// any scenario header, included from where it lies (e.g. one of the reference's own)
+#include SCENARIO_HEADER
+using Scenario = SCENARIO_TYPE;
 #else
 #error "choose a scenario: -DRUN_TAYLOR_GREEN or -DRUN_LID_DRIVEN"
 #endif

@@ -57,3 +57,47 @@ def test_lid_driven_driver_runs_and_writes_outputs(gpu_available, tmp_path):
     assert np.all(np.isfinite(u)) and np.all(np.isfinite(rho))
     # the lid drags the fluid below it in +x; the bulk stays near rest early on
     assert u[-2, 128, 0] > 0.01 and abs(u[128, 128, 0]) < 0.05
+
+
+def test_unchanged_reference_taylor_green_header_against_the_oracle(gpu_available, tmp_path):
+    """The reference's own src/scenarios/taylorGreen/taylorGreenScenario.cuh, included unchanged
+    from where it lies (built here into apps/lbm_run_ref_tg128 next to the library), run through
+    compat/ on the GPU: BGK, nu = 1/6 (omega = 1), u_max = 0.04, FLUID everywhere on a
+    non-periodic 128^2 grid (PERIODIC_X/Y are never defined there). Its initial fields (the dump
+    at step 0) seed an FP32 oracle run of the same configuration, whose fields after 100 steps
+    the driver's dumps match, and TaylorGreenScenario::compute_error as printed by the driver
+    equals the validator's formula on the dumped field."""
+    import math
+
+    import oracle as orc
+    from lbm_amd import _abi as A
+    from lbm_amd.solver import Config
+
+    exe = APPS / "lbm_run_ref_tg128"
+    if not exe.exists():
+        pytest.skip("apps/lbm_run_ref_tg128 is built only where the reference tree is present")
+    n = 128
+    out0 = run_app("lbm_run_ref_tg128", 0, tmp_path / "t0")
+    assert "TaylorGreen" in out0
+    rho0 = np.fromfile(tmp_path / "t0" / "density" / "density_0.bin", np.float32).reshape(n, n)
+    u0 = np.fromfile(tmp_path / "t0" / "velocity" / "velocity_0.bin", np.float32).reshape(n, n, 2)
+    # the reference's init: rho = 1 - rho0 u0^2 / (4 cs2) (cos 2kx + cos 2ky), u = u0 (cos kx sin ky, -sin kx cos ky)
+    assert abs(float(np.abs(u0).max()) - 0.04) < 1e-3
+    steps = 100
+    out = run_app("lbm_run_ref_tg128", steps, tmp_path / "t1")
+    err = float(re.search(r"validation error ([0-9.eE+-]+) %", out).group(1))
+    rho = np.fromfile(tmp_path / "t1" / "density" / f"density_{steps}.bin", np.float32).reshape(n, n)
+    u = np.fromfile(tmp_path / "t1" / "velocity" / f"velocity_{steps}.bin", np.float32).reshape(n, n, 2)
+    tau = 3.0 * np.float32(1.0 / 6.0) + 0.5
+    cfg = Config(nx=n, ny=n, collision=A.LBM_BGK, omega=float(np.float32(1.0) / np.float32(tau)))
+    o = orc.Oracle(cfg.to_c(), precision=32)
+    o.init_from_macros(rho0, u0)
+    o.step(steps)
+    r_o, u_o = o.get_macros()
+    assert float(np.max(np.abs(rho - r_o))) < 1e-5
+    assert float(np.max(np.abs(u - u_o))) < 1e-6
+    # compute_error (taylorGreenScenario.cuh:59-88) on the dumped field, from the validator's analytic
+    # decay at nu = 1/6 over the same steps
+    want = sc.tg_error(u, 1.0 / 6.0, 0.04, steps)
+    assert abs(want - err) <= 1e-3 * max(err, 1e-3), (want, err)
+    assert math.isfinite(err)
