@@ -7,15 +7,18 @@
 // against ~330-370 VALU (BGK, MRT and CM without 27x27 matrices).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/lbm3d_hip.h"
-#include "lbm3d_step.hpp"
+#include "lbm3d_ibm.hpp"
+#include "lbm_ibm_tables.hpp"
 
 using namespace lbm3;
 
@@ -41,23 +44,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void s
     int x, y, z;
     thread_node(p, L, int(threadIdx.x), x, y, z);
     if (x >= p.nx || y >= p.ny) return;
-    if (GUO && guo_ring(p, x, y, z)) return;
+    if (GUO && p.flags && (p.flags[((long long)z * p.ny + y) * p.nx + x] & kPreLaunch)) return;
     step3_node<COLL, STORE, BUF, GUO>(p, x, y, z);
 }
 
-// The outlet's ring nodes (guo_ring) when they do not share a wavefront with their neighbour:
-// thread i looks at node (side ? NX-1 : 0, y, z) of the 2 * NY * NZ plane nodes. Runs before
-// step3_kernel, because these nodes read their x - 1 neighbours' stale slots, which that launch
-// rewrites.
+// The nodes marked kPreLaunch (needs_pre_launch: they read a neighbour's stale slots from
+// another wavefront), listed in `nodes`. Runs before step3_kernel, which rewrites those slots
+// and skips these nodes.
 template <int COLL, int STORE, bool BUF, int GUO>
-__global__ __launch_bounds__(256) void step3_ring_kernel(Step3Params p) {
-    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    const long long per = (long long)p.ny * p.nz;
-    if (i >= 2 * per) return;
-    const int side = int(i / per);
-    const long long r = i - side * per;
-    const int x = side ? p.nx - 1 : 0, y = int(r % p.ny), z = int(r / p.ny);
-    if (!guo_ring(p, x, y, z)) return;
+__global__ __launch_bounds__(256) void step3_pre_kernel(Step3Params p, const int* nodes, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int node = nodes[i];
+    const int x = node % p.nx, y = (node / p.nx) % p.ny, z = node / (p.nx * p.ny);
     step3_node<COLL, STORE, BUF, GUO>(p, x, y, z);
 }
 
@@ -114,6 +113,29 @@ __global__ void soa_to_aos3_kernel(const float* in, long long N, long long plane
     for (int c = 0; c < 3; c++) out[3 * n + c] = in[c * plane + n];
 }
 
+// IBM pass kernels (lbm3d_ibm.hpp): one thread per band node or marker.
+template <int STORE, bool BUF, int GUO>
+__global__ __launch_bounds__(256) void ibm3_gather_kernel(Ibm3Params p) {
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b < p.n_band) ibm3_gather<STORE, BUF, GUO>(p, b);
+}
+__global__ __launch_bounds__(256) void ibm3_interpolate_kernel(Ibm3Params p) {
+    const int l = blockIdx.x * 256 + threadIdx.x;
+    if (l < p.n_markers) ibm3_interpolate(p, l);
+}
+__global__ __launch_bounds__(256) void ibm3_spread_kernel(Ibm3Params p) {
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b < p.n_band) ibm3_spread(p, b);
+}
+
+// The step's force array outside the band: the external field (SoA) or the uniform force.
+__global__ void force3_fill_kernel(float* out, const float* ext, long long N, long long plane, float fx, float fy, float fz) {
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const float f[3] = {fx, fy, fz};
+    for (int c = 0; c < 3; c++) out[c * plane + n] = ext ? ext[c * plane + n] : f[c];
+}
+
 // Deterministic FP64 reduction: block b sums nodes b*256 + t + k*256*nb in a fixed order and
 // a fixed LDS tree; the host adds the per-block partials in block order.
 constexpr int kRedBlocks = 1024;
@@ -161,9 +183,23 @@ struct lbm3d_ctx {
     void* buf[2] = {nullptr, nullptr};
     int cur = 0;                       // buffer holding the latest post-collision state
     uint8_t* d_flags = nullptr;        // null while every node is FLUID
-    bool has_guo = false;              // Guo inlet/outlet, x-face ZG or regularized flags present
-    bool has_reg = false;              // regularized flags present (the GUO = 2 kernel variant)
+    bool has_guo = false;              // flags that read a neighbour (or regularized flags) present
+    bool has_reg = false;              // regularized or extrapolation flags (the GUO = 2 kernel variant)
+    int* d_pre = nullptr;              // nodes of the pre-launches, in launch order (pre_launch_levels)
+    int n_pre = 0;
+    std::vector<int> pre_level_end;    // end of each pre-launch's nodes in d_pre
     float* d_force = nullptr;          // SoA [3][plane] or null
+    struct Ibm3 {                      // immersed boundary (lbm3d_ibm_set_markers)
+        bool active = false;
+        bool force_valid = false;      // d_total holds the external force outside the band
+        int n_markers = 0, n_band = 0, iters = 3;
+        int box[6] = {0, 0, 0, 0, 0, 0};  // the band's bounding box [x0, x1) x [y0, y1) x [z0, z1)
+        int *band_node = nullptr, *mk_ptr = nullptr, *mk_band = nullptr, *bn_ptr = nullptr, *bn_mk = nullptr;
+        float *mk_w = nullptr, *bn_w = nullptr;
+        float *brho = nullptr, *bu = nullptr, *bf = nullptr, *lag = nullptr, *lag_sum = nullptr;
+        float* total = nullptr;        // the step's force array SoA [3][plane]
+        std::vector<float> h_ds;
+    } ibm;
     float* d_rho = nullptr;            // [plane]
     float* d_u = nullptr;              // SoA [3][plane]
     double* d_partials = nullptr;      // [kRedBlocks][3]
@@ -175,7 +211,9 @@ struct lbm3d_ctx {
 
     ~lbm3d_ctx() {
         if (stream) (void)hipStreamSynchronize(stream);
-        for (void* p : {buf[0], buf[1], (void*)d_flags, (void*)d_force, (void*)d_rho, (void*)d_u, (void*)d_partials, tmp})
+        ibm_free();
+        for (void* p : {buf[0], buf[1], (void*)d_flags, (void*)d_force, (void*)d_rho, (void*)d_u, (void*)d_partials, tmp,
+                        (void*)d_pre})
             if (p) (void)hipFree(p);
         if (own_stream) (void)hipStreamDestroy(own_stream);
     }
@@ -235,8 +273,72 @@ struct lbm3d_ctx {
         return LBM_OK;
     }
     float lin_ec() const { return (cfg.quirks & LBM_QUIRK_EQ3D_LINEAR) ? 1.0f : 3.0f; }
+    void ibm_free() {
+        for (void* q : {(void*)ibm.band_node, (void*)ibm.mk_ptr, (void*)ibm.mk_band, (void*)ibm.bn_ptr, (void*)ibm.bn_mk,
+                        (void*)ibm.mk_w, (void*)ibm.bn_w, (void*)ibm.brho, (void*)ibm.bu, (void*)ibm.bf, (void*)ibm.lag,
+                        (void*)ibm.lag_sum, (void*)ibm.total})
+            if (q) (void)hipFree(q);
+        ibm = Ibm3();
+    }
+    int ibm_pass(const Step3Params& sp, bool fits);
     int one_step(bool write_macros);
 };
+
+// The IBM pass of one step, ahead of the step launch on the same stream: it reads the step's
+// source buffer and the stale slots of its destination before the step rewrites them.
+int lbm3d_ctx::ibm_pass(const Step3Params& sp, bool fits) {
+    if (!ibm.force_valid) {
+        hipLaunchKernelGGL(force3_fill_kernel, dim3(grid1(N)), dim3(256), 0, stream, ibm.total, d_force, N, plane,
+                           cfg.force[0], cfg.force[1], cfg.force[2]);
+        ibm.force_valid = true;
+    }
+    Ibm3Params q{};
+    q.sp = sp;
+    q.n_markers = ibm.n_markers;
+    q.n_band = ibm.n_band;
+    q.clamp = (cfg.quirks & LBM_QUIRK_IBM_CLAMP) ? 1 : 0;
+    q.band_node = ibm.band_node;
+    q.mk_ptr = ibm.mk_ptr;
+    q.mk_band = ibm.mk_band;
+    q.mk_w = ibm.mk_w;
+    q.bn_ptr = ibm.bn_ptr;
+    q.bn_mk = ibm.bn_mk;
+    q.bn_w = ibm.bn_w;
+    q.ext = d_force;
+    q.brho = ibm.brho;
+    q.bu = ibm.bu;
+    q.bf = ibm.bf;
+    q.lag = ibm.lag;
+    q.lag_sum = ibm.lag_sum;
+    q.force_total = ibm.total;
+    const dim3 gb(grid1(ibm.n_band)), gm(grid1(ibm.n_markers)), blk(256);
+    const int G = has_reg ? 2 : (has_guo ? 1 : 0);
+#define LBM3_GATHER(S, B)                                                                                    \
+    if (G == 2) hipLaunchKernelGGL((ibm3_gather_kernel<S, B, 2>), gb, blk, 0, stream, q);                   \
+    else if (G == 1) hipLaunchKernelGGL((ibm3_gather_kernel<S, B, 1>), gb, blk, 0, stream, q);              \
+    else hipLaunchKernelGGL((ibm3_gather_kernel<S, B, 0>), gb, blk, 0, stream, q);
+#define LBM3_GATHER_B(S)   \
+    if (fits) {            \
+        LBM3_GATHER(S, true)  \
+    } else {               \
+        LBM3_GATHER(S, false) \
+    }
+    switch (cfg.storage) {
+        case LBM_F32: LBM3_GATHER_B(LBM_F32) break;
+        case LBM_F16_SHIFTED: LBM3_GATHER_B(LBM_F16_SHIFTED) break;
+        default: LBM3_GATHER_B(LBM_BF16_SHIFTED) break;
+    }
+#undef LBM3_GATHER_B
+#undef LBM3_GATHER
+    for (int it = 0; it < ibm.iters; it++) {
+        q.first = it == 0;
+        q.last = it == ibm.iters - 1;
+        hipLaunchKernelGGL(ibm3_interpolate_kernel, gm, blk, 0, stream, q);
+        hipLaunchKernelGGL(ibm3_spread_kernel, gb, blk, 0, stream, q);
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? LBM_OK : hip_fail(e, "IBM launch");
+}
 
 int lbm3d_ctx::one_step(bool write_macros) {
     Step3Params p{};
@@ -244,6 +346,9 @@ int lbm3d_ctx::one_step(bool write_macros) {
     p.dst = buf[1 - cur];
     p.flags = d_flags;
     p.force = d_force;
+    p.fbox[1] = cfg.nx;
+    p.fbox[3] = cfg.ny;
+    p.fbox[5] = cfg.nz;
     p.out_rho = write_macros ? d_rho : nullptr;
     p.out_u = write_macros ? d_u : nullptr;
     p.nx = cfg.nx;
@@ -264,7 +369,7 @@ int lbm3d_ctx::one_step(bool write_macros) {
     p.u_inlet = cfg.u_inlet;
     p.guo_own = (cfg.quirks & LBM_QUIRK_GUO3D_OWN_MOMENTS) ? 1 : 0;
     p.guo = has_guo ? 1 : 0;
-    p.guo_ring_outlet = (has_guo && (cfg.nx - 1) % 64 == 0) ? 1 : 0;
+    p.corner_uz = (cfg.quirks & LBM_QUIRK_CORNER3D_UZ) ? 1 : 0;
     p.stale_rest = (cfg.quirks & LBM_QUIRK_STALE_REST) ? 1 : 0;
     // Non-temporal stores keep the written populations from evicting the pulls' shared lines
     // once the two buffers exceed the 256 MB Infinity Cache (+2% FP32, +4% FP16 at 256^3);
@@ -276,18 +381,28 @@ int lbm3d_ctx::one_step(bool write_macros) {
     const long long nblk = (long long)p.nbx * p.nby * cfg.nz;
     if (nblk > 0x7fffffffLL) return fail(LBM_E_INVALID, "grid needs too many workgroups");
     const bool fits = cfg.storage == LBM_F32 ? lbm::buf_fits<float>(Q, plane) : lbm::buf_fits<uint16_t>(Q, plane);
+    if (ibm.active) {
+        int rc = ibm_pass(p, fits);
+        if (rc) return rc;
+        p.force = ibm.total;
+        if (!d_force)  // the uniform force holds outside the band's box
+            for (int k = 0; k < 6; k++) p.fbox[k] = ibm.box[k];
+    }
     const dim3 grid{unsigned(nblk)}, block{256};
-    const long long nplane = 2LL * cfg.ny * cfg.nz;
-    const dim3 pgrid{unsigned((nplane + 255) / 256)};
-#define LBM3_LAUNCH_G(C, S, G)                                                                  \
-    if (fits) {                                                                                 \
-        if (p.guo_ring_outlet)                                                                  \
-            hipLaunchKernelGGL((step3_ring_kernel<C, S, true, G>), pgrid, block, 0, stream, p); \
-        hipLaunchKernelGGL((step3_kernel<C, S, true, G>), grid, block, 0, stream, p);           \
-    } else {                                                                                    \
-        if (p.guo_ring_outlet)                                                                  \
-            hipLaunchKernelGGL((step3_ring_kernel<C, S, false, G>), pgrid, block, 0, stream, p); \
-        hipLaunchKernelGGL((step3_kernel<C, S, false, G>), grid, block, 0, stream, p);          \
+#define LBM3_PRE(C, S, B, G)                                                                          \
+    for (size_t k = 0, beg = 0; k < pre_level_end.size(); beg = size_t(pre_level_end[k]), k++) {      \
+        const int cnt = pre_level_end[k] - int(beg);                                                  \
+        if (cnt > 0)                                                                                  \
+            hipLaunchKernelGGL((step3_pre_kernel<C, S, B, G>), dim3(unsigned((cnt + 255) / 256)), block, 0, \
+                               stream, p, d_pre + beg, cnt);                                          \
+    }
+#define LBM3_LAUNCH_G(C, S, G)                                                                        \
+    if (fits) {                                                                                       \
+        LBM3_PRE(C, S, true, G)                                                                       \
+        hipLaunchKernelGGL((step3_kernel<C, S, true, G>), grid, block, 0, stream, p);                 \
+    } else {                                                                                          \
+        LBM3_PRE(C, S, false, G)                                                                      \
+        hipLaunchKernelGGL((step3_kernel<C, S, false, G>), grid, block, 0, stream, p);                \
     }
 #define LBM3_LAUNCH(C, S)                                                                       \
     if (has_reg) {                                                                              \
@@ -314,6 +429,8 @@ int lbm3d_ctx::one_step(bool write_macros) {
     }
 #undef LBM3_LAUNCH_S
 #undef LBM3_LAUNCH
+#undef LBM3_LAUNCH_G
+#undef LBM3_PRE
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "step3 launch");
     cur = 1 - cur;
@@ -361,8 +478,11 @@ int lbm3d_create(const lbm3d_config* cfg, lbm3d_ctx** out) {
     if (!(k.rho0 > 0.0f)) return bad(LBM_E_INVALID, "rho0 must be positive");
     for (int c = 0; c < 3; c++)
         if (!std::isfinite(k.force[c])) return bad(LBM_E_INVALID, "force must be finite");
-    if (k.quirks & ~(LBM_QUIRK_STALE_REST | LBM_QUIRK_EQ3D_LINEAR | LBM_QUIRK_MRT3_LITERAL | LBM_QUIRK_GUO3D_OWN_MOMENTS))
-        return bad(LBM_E_UNSUPPORTED, "only the STALE_REST, EQ3D_LINEAR, MRT3_LITERAL and GUO3D_OWN_MOMENTS quirks apply to D3Q27");
+    if (k.quirks & ~(LBM_QUIRK_STALE_REST | LBM_QUIRK_EQ3D_LINEAR | LBM_QUIRK_MRT3_LITERAL | LBM_QUIRK_GUO3D_OWN_MOMENTS |
+                     LBM_QUIRK_CORNER3D_UZ | LBM_QUIRK_IBM_CLAMP | LBM_QUIRK_IBM_NO_DS))
+        return bad(LBM_E_UNSUPPORTED,
+                   "only the STALE_REST, EQ3D_LINEAR, MRT3_LITERAL, GUO3D_OWN_MOMENTS, CORNER3D_UZ, IBM_CLAMP and IBM_NO_DS "
+                   "quirks apply to D3Q27");
     if (!std::isfinite(k.u_inlet) || !(std::fabs(k.u_inlet) < 0.5f)) return bad(LBM_E_INVALID, "u_inlet must stay below 0.5");
 
     auto* c = new lbm3d_ctx();
@@ -425,6 +545,13 @@ int lbm3d_set_flags(lbm3d_ctx* ctx, const uint8_t* flags, int on_device) {
         }
         ctx->has_guo = false;
         ctx->has_reg = false;
+        if (ctx->d_pre) {
+            (void)hipStreamSynchronize(ctx->stream);
+            (void)hipFree(ctx->d_pre);
+            ctx->d_pre = nullptr;
+        }
+        ctx->n_pre = 0;
+        ctx->pre_level_end.clear();
         return LBM_OK;
     }
     std::vector<uint8_t> h(size_t(ctx->N));
@@ -442,29 +569,65 @@ int lbm3d_set_flags(lbm3d_ctx* ctx, const uint8_t* flags, int on_device) {
         if (h[n] == LBM_FLUID) continue;
         any = true;
         if (h[n] == LBM_BOUNCE_BACK || h[n] == LBM_CORNER_EDGE_BOUNCE_BACK) continue;
-        if ((h[n] == LBM_GUO_VELOCITY_INLET && x == 0) || (h[n] == LBM_GUO_PRESSURE_OUTLET && x == nx - 1)) {
-            if (ctx->cfg.periodic_x) return ctx->fail(LBM_E_INVALID, "Guo inlet/outlet flags need a non-periodic x axis");
+        if ((h[n] == LBM_GUO_VELOCITY_INLET && x == 0) ||
+            ((h[n] == LBM_GUO_PRESSURE_OUTLET || h[n] == LBM_REGULARIZED_OUTLET) && x == nx - 1)) {
+            if (ctx->cfg.periodic_x)
+                return ctx->fail(LBM_E_INVALID, "Guo inlet/outlet and regularized outlet flags need a non-periodic x axis");
+            guo = true;
+            if (h[n] == LBM_REGULARIZED_OUTLET) reg = true;  // the GUO = 2 kernel variant
+            continue;
+        }
+        if (h[n] == LBM_REGULARIZED_BOUNCE_BACK || h[n] == LBM_REGULARIZED_INLET_LEFT ||
+            h[n] == LBM_EXTRAPOLATED_CORNER_EDGE) {
+            guo = reg = true;  // the functors are compiled into the GUO = 2 kernel variant only
+            continue;
+        }
+        if (h[n] == LBM_ZG_OUTFLOW) {
             guo = true;
             continue;
         }
-        if (h[n] == LBM_REGULARIZED_BOUNCE_BACK || h[n] == LBM_REGULARIZED_INLET_LEFT) {
-            guo = reg = true;  // the functor is compiled into the GUO = 2 kernel variant only
-            continue;
-        }
-        if (h[n] == LBM_ZG_OUTFLOW && (x == 0 || x == nx - 1)) {
-            if (ctx->cfg.periodic_x) return ctx->fail(LBM_E_INVALID, "ZG_OUTFLOW flags need a non-periodic x axis");
-            guo = true;
-            continue;
-        }
-        if (h[n] == LBM_GUO_VELOCITY_INLET || h[n] == LBM_GUO_PRESSURE_OUTLET)
-            return ctx->fail(LBM_E_UNSUPPORTED, "flag %d at node %zu: Guo inlets are built at x = 0 and outlets at x = NX-1", int(h[n]), n);
-        if (h[n] == LBM_ZG_OUTFLOW)
-            return ctx->fail(LBM_E_UNSUPPORTED, "ZG_OUTFLOW at node %zu: built on the x = 0 and x = NX-1 faces only", n);
-        return ctx->fail(LBM_E_UNSUPPORTED, "flag %d at node %zu: not built for D3Q27 (FLUID, BOUNCE_BACK, CORNER_EDGE_BOUNCE_BACK, REGULARIZED_BOUNCE_BACK, REGULARIZED_INLET_LEFT, GUO_VELOCITY_INLET, GUO_PRESSURE_OUTLET, ZG_OUTFLOW are)", int(h[n]), n);
+        if (h[n] == LBM_GUO_VELOCITY_INLET || h[n] == LBM_GUO_PRESSURE_OUTLET || h[n] == LBM_REGULARIZED_OUTLET)
+            return ctx->fail(LBM_E_UNSUPPORTED,
+                             "flag %d at node %zu: Guo inlets are built at x = 0, Guo and regularized outlets at x = NX-1",
+                             int(h[n]), n);
+        return ctx->fail(LBM_E_UNSUPPORTED, "flag %d at node %zu: not built for D3Q27 (FLUID, BOUNCE_BACK, CORNER_EDGE_BOUNCE_BACK, REGULARIZED_BOUNCE_BACK, REGULARIZED_INLET_LEFT, REGULARIZED_OUTLET, EXTRAPOLATED_CORNER_EDGE, GUO_VELOCITY_INLET, GUO_PRESSURE_OUTLET, ZG_OUTFLOW are)", int(h[n]), n);
     }
     if (!any) return lbm3d_set_flags(ctx, nullptr, 0);
+    // nodes reading a neighbour's stale slots that another wavefront (or an earlier launch)
+    // rewrites run in pre-launches, in the order pre_launch_levels gives
+    Step3Params q{};
+    q.nx = ctx->cfg.nx;
+    q.ny = ctx->cfg.ny;
+    q.nz = ctx->cfg.nz;
+    q.px = ctx->cfg.periodic_x != 0;
+    q.py = ctx->cfg.periodic_y != 0;
+    q.pz = ctx->cfg.periodic_z != 0;
+    std::vector<std::vector<int>> levels;
+    if (const char* why = pre_launch_levels(q, h.data(), levels)) return ctx->fail(LBM_E_UNSUPPORTED, "%s", why);
+    std::vector<int> pre, level_end;
+    for (size_t k = levels.size(); k-- > 0;) {  // the launch order: the last level first
+        for (int n : levels[k]) {
+            pre.push_back(n);
+            h[size_t(n)] = uint8_t(h[size_t(n)] | kPreLaunch);
+        }
+        level_end.push_back(int(pre.size()));
+    }
     ctx->has_guo = guo;
     ctx->has_reg = reg;
+    if (ctx->d_pre) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(ctx->d_pre);
+        ctx->d_pre = nullptr;
+        ctx->n_pre = 0;
+    }
+    ctx->pre_level_end = level_end;
+    if (!pre.empty()) {
+        rc = ctx->dev_alloc(reinterpret_cast<void**>(&ctx->d_pre), pre.size() * sizeof(int), "pre-launch nodes");
+        if (rc) return rc;
+        rc = ctx->upload(ctx->d_pre, pre.data(), pre.size() * sizeof(int), 0);
+        if (rc) return rc;
+        ctx->n_pre = int(pre.size());
+    }
     if (!ctx->d_flags) {
         rc = ctx->dev_alloc(reinterpret_cast<void**>(&ctx->d_flags), size_t(ctx->plane), "flags");
         if (rc) return rc;
@@ -516,6 +679,7 @@ int lbm3d_set_force_field(lbm3d_ctx* ctx, const float* force, int on_device) {
     if (!ctx) return LBM_E_INVALID;
     int rc = ctx->activate();
     if (rc) return rc;
+    ctx->ibm.force_valid = false;
     if (!force) {
         if (ctx->d_force) {
             (void)hipStreamSynchronize(ctx->stream);
@@ -668,6 +832,90 @@ int lbm3d_reduce(lbm3d_ctx* ctx, int32_t kind, double* out) {
         out[1] = s[1];
         out[2] = s[2];
     }
+    return LBM_OK;
+}
+
+int lbm3d_ibm_set_markers(lbm3d_ctx* ctx, const float* xyz, const float* ds, int32_t n, int32_t iters) {
+    if (!ctx || n < 0 || (n > 0 && !xyz) || iters < 1) return LBM_E_INVALID;
+    int rc = ctx->activate();
+    if (rc) return rc;
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->ibm_free();
+    if (n == 0) return LBM_OK;
+    lbm::IbmTables3 T;
+    const char* why = lbm::ibm_build_tables3(ctx->cfg.nx, ctx->cfg.ny, ctx->cfg.nz, (ctx->cfg.quirks & LBM_QUIRK_IBM_NO_DS) != 0,
+                                             xyz, ds, n, T);
+    if (why) return ctx->fail(LBM_E_INVALID, "%s", why);
+    auto& I = ctx->ibm;
+    const int nb = int(T.band_node.size());
+    auto put = [&](auto** dst, const auto& v, const char* what) {
+        using E = typename std::decay_t<decltype(v)>::value_type;
+        int r = ctx->dev_alloc(reinterpret_cast<void**>(dst), std::max<size_t>(v.size(), 1) * sizeof(E), what);
+        return r ? r : (v.empty() ? LBM_OK : ctx->upload(*dst, v.data(), v.size() * sizeof(E), 0));
+    };
+    rc = put(&I.band_node, T.band_node, "IBM band");
+    rc = rc ? rc : put(&I.mk_ptr, T.mk_ptr, "IBM stencils");
+    rc = rc ? rc : put(&I.mk_band, T.mk_band, "IBM stencils");
+    rc = rc ? rc : put(&I.mk_w, T.mk_w, "IBM weights");
+    rc = rc ? rc : put(&I.bn_ptr, T.bn_ptr, "IBM spread lists");
+    rc = rc ? rc : put(&I.bn_mk, T.bn_mk, "IBM spread lists");
+    rc = rc ? rc : put(&I.bn_w, T.bn_w, "IBM spread weights");
+    rc = rc ? rc : ctx->dev_alloc(reinterpret_cast<void**>(&I.brho), size_t(nb) * 4, "IBM band rho");
+    rc = rc ? rc : ctx->dev_alloc(reinterpret_cast<void**>(&I.bu), size_t(nb) * 12, "IBM band velocity");
+    rc = rc ? rc : ctx->dev_alloc(reinterpret_cast<void**>(&I.bf), size_t(nb) * 12, "IBM band force");
+    rc = rc ? rc : ctx->dev_alloc(reinterpret_cast<void**>(&I.lag), size_t(n) * 12, "IBM Lagrangian forces");
+    rc = rc ? rc : ctx->dev_alloc(reinterpret_cast<void**>(&I.lag_sum), size_t(n) * 12, "IBM Lagrangian forces");
+    rc = rc ? rc : ctx->dev_alloc(reinterpret_cast<void**>(&I.total), size_t(ctx->plane) * 12, "IBM force field");
+    if (!rc) {
+        hipError_t e = hipMemsetAsync(I.lag_sum, 0, size_t(n) * 12, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = ctx->hip_fail(e, "IBM reset");
+    }
+    if (rc) {
+        ctx->ibm_free();
+        return rc;
+    }
+    const int nx = ctx->cfg.nx, nxy = nx * ctx->cfg.ny;
+    int box[6] = {nx, 0, ctx->cfg.ny, 0, ctx->cfg.nz, 0};
+    for (int v : T.band_node) {
+        const int z = v / nxy, y = (v - z * nxy) / nx, x = v - z * nxy - y * nx;
+        box[0] = std::min(box[0], x);
+        box[1] = std::max(box[1], x + 1);
+        box[2] = std::min(box[2], y);
+        box[3] = std::max(box[3], y + 1);
+        box[4] = std::min(box[4], z);
+        box[5] = std::max(box[5], z + 1);
+    }
+    std::copy(box, box + 6, I.box);
+    I.n_markers = n;
+    I.n_band = nb;
+    I.iters = iters;
+    I.h_ds.assign(size_t(n), 1.0f);
+    if (ds) std::copy(ds, ds + n, I.h_ds.begin());
+    I.force_valid = false;
+    I.active = true;
+    return LBM_OK;
+}
+
+int lbm3d_ibm_get_forces(lbm3d_ctx* ctx, float* lag_force, double* force_sum) {
+    if (!ctx) return LBM_E_INVALID;
+    if (!ctx->ibm.active) return ctx->fail(LBM_E_STATE, "no IBM markers set");
+    int rc = ctx->activate();
+    if (rc) return rc;
+    const int n = ctx->ibm.n_markers;
+    std::vector<float> h(3 * size_t(n));
+    rc = ctx->download(h.data(), ctx->ibm.lag_sum, h.size() * sizeof(float), 0);
+    if (rc) return rc;
+    const bool no_ds = (ctx->cfg.quirks & LBM_QUIRK_IBM_NO_DS) != 0;
+    double t[3] = {0.0, 0.0, 0.0};
+    for (int l = 0; l < n; l++)
+        for (int d = 0; d < 3; d++) {
+            const float v = h[size_t(d) * n + l];
+            if (lag_force) lag_force[3 * size_t(l) + d] = v;
+            t[d] += double(v) * (no_ds ? 1.0 : double(ctx->ibm.h_ds[size_t(l)]));
+        }
+    if (force_sum)
+        for (int d = 0; d < 3; d++) force_sum[d] = t[d];
     return LBM_OK;
 }
 

@@ -494,6 +494,47 @@ LBM3_HD void bc_regularized3(float* g, int x, int y, int z, const Box3& B, float
     }
 }
 
+// RegularizedOutlet::apply (regularizedOutlet.cuh:9-95) at x = NX-1, given the density and
+// velocity (rho, u) of the x - 1 neighbour's post-stream populations: the unknown populations
+// (c_x < 0) become f_eq + (g_opp - f_eq_opp) with f_eq(rho, u) (the functor's own de_rosis_eq
+// is the solver's equilibrium: lin_ec as everywhere else); then Pi = sum c c g - rho/3 I -
+// rho u u of the completed node, and every population f_eq + 4.5 w Q : Pi.
+template <int... qs>
+LBM3_HD void bc_regularized_outlet3_all(float* g, float rho, float ux, float uy, float uz, float lin_ec,
+                                        std::integer_sequence<int, qs...>) {
+    const float base = rho * (1.0f - 1.5f * (ux * ux + uy * uy + uz * uz));
+    (reg_unknown<0, -1, qs>(g, rho, ux, uy, uz, base, lin_ec), ...);
+    float P[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    (reg_pi<qs>(g, P), ...);
+    P[0] -= rho / 3.0f + rho * ux * ux;
+    P[1] -= rho / 3.0f + rho * uy * uy;
+    P[2] -= rho / 3.0f + rho * uz * uz;
+    P[3] -= rho * ux * uy;
+    P[4] -= rho * ux * uz;
+    P[5] -= rho * uy * uz;
+    (reg_rebuild<qs>(g, P, rho, ux, uy, uz, base, lin_ec), ...);
+}
+LBM3_HD void bc_regularized_outlet3(float* g, float rho, float ux, float uy, float uz, float lin_ec) {
+    bc_regularized_outlet3_all(g, rho, ux, uy, uz, lin_ec, std::make_integer_sequence<int, Q>{});
+}
+
+// ExtrapolatedCornerEdgeBoundary<3>::apply (cornerBoundaries.cuh:6-36): every population of the
+// node becomes w rho (1 + c.u/cs2 + (c.u)^2/(2 cs4) - u.u/(2 cs2)) of the inward node's
+// post-stream density and velocity: the second-order equilibrium whatever lin_ec (the functor
+// writes it out rather than calling the solver's equilibrium).
+template <int... qs>
+LBM3_HD void bc_extrapolated3_all(float* g, float rho, float ux, float uy, float uz, std::integer_sequence<int, qs...>) {
+    const float base = rho * (1.0f - 1.5f * (ux * ux + uy * uy + uz * uz));
+    ((g[qs] = feq3<qs>(rho, ux, uy, uz, base, 3.0f)), ...);
+}
+LBM3_HD void bc_extrapolated3(float* g, float rho, float ux, float uy, float uz) {
+    bc_extrapolated3_all(g, rho, ux, uy, uz, std::make_integer_sequence<int, Q>{});
+}
+
+// Component k % 3 of direction k / 3 of the velocity table read as a flat [27][3] array: the
+// value cornerBoundaries.cuh:23 multiplies f_i by for u_z (k = 6 + i) under LBM_QUIRK_CORNER3D_UZ.
+LBM3_HD constexpr int flat_c(int k) { return k % 3 == 0 ? CX[k / 3] : (k % 3 == 1 ? CY[k / 3] : CZ[k / 3]); }
+
 // EdgeCornerBounceBack::apply (edgeCornerBounceBack.cuh:10-49): on a node with at least two
 // domain faces (x, y, z at 0 or N-1, periodic or not), every direction with no negative and at
 // least one positive component copies g_i into g_opp(i). The functor computes the inward

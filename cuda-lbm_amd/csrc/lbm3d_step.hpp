@@ -20,14 +20,18 @@
 #include "lbm3d_node.hpp"
 #include "lbm_step.hpp"
 
+#include <vector>
+
 namespace lbm3 {
 
 // Kernel arguments (passed by value).
 struct Step3Params {
     const void* src;       // post-collision populations of step t-1, [27][plane]
     void* dst;             // receives step t; holds step t-2 (stale slots)
-    const uint8_t* flags;  // per-node flag (FLUID / BOUNCE_BACK / CORNER_EDGE_BOUNCE_BACK / Guo), or null (all FLUID)
+    const uint8_t* flags;  // per-node flag, or null (all FLUID); bit 7 (kPreLaunch) marks the
+                           // nodes the pre-launch updates (see needs_pre_launch)
     const float* force;    // SoA [3][plane] force field, or null (uniform fx, fy, fz)
+    int fbox[6];           // force is read inside [x0, x1) x [y0, y1) x [z0, z1) only
     float* out_rho;        // macroscopic outputs, rho [plane] and u SoA [3][plane], or null
     float* out_u;
     int nx, ny, nz;
@@ -38,9 +42,8 @@ struct Step3Params {
     Mrt3Rates mrt;         // MRT<3> rates S[4..8] and the literal flag
     float u_inlet;         // GUO_VELOCITY_INLET and REGULARIZED_INLET_LEFT velocity
     int guo_own;           // LBM_QUIRK_GUO3D_OWN_MOMENTS
-    int guo;               // Guo inlet/outlet or x-face ZG outflow flags present
-    int guo_ring_outlet;   // (NX-1) % 64 == 0: the outlet's ring nodes run in a launch of their
-                           // own before the main launch, which skips them
+    int guo;               // flags that read a neighbour's populations present
+    int corner_uz;         // LBM_QUIRK_CORNER3D_UZ
     int stale_rest;        // LBM_QUIRK_STALE_REST
     int nt;                // store with the non-temporal cache policy (buffers beyond the
                            // Infinity Cache)
@@ -205,62 +208,197 @@ LBM3_HD void apply_guo(const Step3Params& p, const lbm::PopBuf<typename Store3<S
     bc_guo3<INLET>(g, gu, B);
 }
 
-// ZG_OutflowBoundary<3>::apply (zeroGradientOutflow.cuh:8-59) on an x face: the populations
-// entering the domain (c . n > 0, n the inward normal: c_x > 0 at x = 0, c_x < 0 at x = NX-1)
-// take the interior neighbour's post-stream values; the others keep the node's own. The
-// neighbour is pulled as a Guo node pulls it, with the same wavefront ordering.
-template <int STORE, bool BUF, bool LEFT, int q>
-LBM3_HD void zg_dir(const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src, float* g, int xn, int y,
-                    int z) {
-    if constexpr (LEFT ? CX[q] > 0 : CX[q] < 0) g[q] = pull_at<STORE, BUF, q>(p, src, xn, y, z);
+// ZG_OutflowBoundary<3>::apply (zeroGradientOutflow.cuh:8-59): the first face in the order
+// x = 0, x = NX-1, y = 0, y = NY-1, z = 0, z = NZ-1 gives the inward normal (axis D, sign S);
+// the populations entering the domain (c_D S > 0) take the post-stream values of the node one
+// step inward, the others keep the node's own; a node on no face is left as it is.
+template <int STORE, bool BUF, int D, int S, int q>
+LBM3_HD void zg_dir(const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src, float* g, int xn, int yn,
+                    int zn) {
+    constexpr int cn = (D == 0 ? CX[q] : (D == 1 ? CY[q] : CZ[q])) * S;
+    if constexpr (cn > 0) g[q] = pull_at<STORE, BUF, q>(p, src, xn, yn, zn);
 }
-template <int STORE, bool BUF, bool LEFT, int... qs>
-LBM3_HD void zg_all(const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src, float* g, int xn, int y,
+template <int STORE, bool BUF, int D, int S, int... qs>
+LBM3_HD void zg_all(const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src, float* g, int x, int y,
                     int z, std::integer_sequence<int, qs...>) {
-    (zg_dir<STORE, BUF, LEFT, qs>(p, src, g, xn, y, z), ...);
+    const int xn = x + (D == 0 ? S : 0), yn = y + (D == 1 ? S : 0), zn = z + (D == 2 ? S : 0);
+    (zg_dir<STORE, BUF, D, S, qs>(p, src, g, xn, yn, zn), ...);
 }
-template <int STORE, bool BUF, bool LEFT>
+// The face of a ZG node: 0..5 for x = 0, x = NX-1, y = 0, y = NY-1, z = 0, z = NZ-1, -1 inside.
+LBM3_HD int zg_face(const Step3Params& p, int x, int y, int z) {
+    if (x == 0) return 0;
+    if (x == p.nx - 1) return 1;
+    if (y == 0) return 2;
+    if (y == p.ny - 1) return 3;
+    if (z == 0) return 4;
+    if (z == p.nz - 1) return 5;
+    return -1;
+}
+template <int STORE, bool BUF>
 LBM3_HD void apply_zg(const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src, float* g, int x, int y,
                       int z) {
-    zg_all<STORE, BUF, LEFT>(p, src, g, LEFT ? x + 1 : x - 1, y, z, std::make_integer_sequence<int, Q>{});
+    constexpr auto qs = std::make_integer_sequence<int, Q>{};
+    switch (zg_face(p, x, y, z)) {
+        case 0: zg_all<STORE, BUF, 0, 1>(p, src, g, x, y, z, qs); break;
+        case 1: zg_all<STORE, BUF, 0, -1>(p, src, g, x, y, z, qs); break;
+        case 2: zg_all<STORE, BUF, 1, 1>(p, src, g, x, y, z, qs); break;
+        case 3: zg_all<STORE, BUF, 1, -1>(p, src, g, x, y, z, qs); break;
+        case 4: zg_all<STORE, BUF, 2, 1>(p, src, g, x, y, z, qs); break;
+        case 5: zg_all<STORE, BUF, 2, -1>(p, src, g, x, y, z, qs); break;
+        default: break;
+    }
 }
 
-// A Guo node pulls its x +- 1 neighbour; on the ring of the plane (y or z at a non-periodic
-// edge) that pull reads the neighbour's own stale slots in the destination buffer, which the
-// neighbour's update rewrites. When both nodes are lanes of one wavefront that is safe: the
-// wavefront issues its stores only after the loads they depend on, and both lanes store through
-// the same instructions. The inlet (x = 0, neighbour x = 1) always shares a wavefront with its
-// neighbour; the outlet (x = NX-1) does unless (NX-1) % 64 == 0, and then its ring nodes run in a
-// launch of their own before the main launch.
-LBM3_HD bool guo_ring(const Step3Params& p, int x, int y, int z) {
-    if (!p.guo_ring_outlet || x != p.nx - 1) return false;
-    return (!p.py && (y == 0 || y == p.ny - 1)) || (!p.pz && (z == 0 || z == p.nz - 1));
+// Density and velocity sums of the post-stream populations of node (xn, yn, zn); uzq the z
+// momentum read through the flat velocity table (ExtrapolatedCornerEdgeBoundary's quirk).
+template <int STORE, bool BUF, int q>
+LBM3_HD void mom_at_dir(const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src, int xn, int yn, int zn,
+                        float& r, float& jx, float& jy, float& jz, float& jzq) {
+    const float v = pull_at<STORE, BUF, q>(p, src, xn, yn, zn);
+    r += v;
+    if (CX[q] > 0) jx += v;
+    if (CX[q] < 0) jx -= v;
+    if (CY[q] > 0) jy += v;
+    if (CY[q] < 0) jy -= v;
+    if (CZ[q] > 0) jz += v;
+    if (CZ[q] < 0) jz -= v;
+    if constexpr (q + 6 < 3 * Q && flat_c(q + 6) != 0) jzq += float(flat_c(q + 6)) * v;
+}
+template <int STORE, bool BUF, int... qs>
+LBM3_HD void mom_at(const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src, int xn, int yn, int zn,
+                    float& r, float& jx, float& jy, float& jz, float& jzq, std::integer_sequence<int, qs...>) {
+    (mom_at_dir<STORE, BUF, qs>(p, src, xn, yn, zn, r, jx, jy, jz, jzq), ...);
 }
 
-// One node of one time step: pull, boundary functor, macroscopics, Guo-forced BGK, MRT or CM,
-// store. Requires 0 <= x < nx, 0 <= y < ny, 0 <= z < nz and nx*ny*nz < 2^31. GUO 1: the node may
-// carry a flag that reads its x +- 1 neighbour (Guo inlet/outlet, ZG outflow on an x face); GUO 2:
-// also a regularized flag (a variant of its own: its 27 equilibria and Pi spill registers).
-template <int COLL, int STORE, bool BUF, int GUO = 0>
-LBM3_HD void step3_node(const Step3Params& p, int x, int y, int z) {
-    using St = Store3<STORE>;
-    using T = typename St::T;
+// RegularizedOutlet (x = NX-1): the x - 1 neighbour's density and velocity, then the functor.
+template <int STORE, bool BUF>
+LBM3_HD void apply_regularized_outlet(const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src,
+                                      float* g, int x, int y, int z) {
+    float r = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f, jzq = 0.0f;
+    mom_at<STORE, BUF>(p, src, x - 1, y, z, r, jx, jy, jz, jzq, std::make_integer_sequence<int, Q>{});
+    const float inv = 1.0f / r;
+    bc_regularized_outlet3(g, r, jx * inv, jy * inv, jz * inv, p.lin_ec);
+}
+
+// The node one step inward on each axis where (x, y, z) is on a domain face (the reference's
+// test is x == 0 / x == NX-1 whatever the periodicity).
+LBM3_HD void inward_node(const Step3Params& p, int x, int y, int z, int& xi, int& yi, int& zi) {
+    xi = x == 0 ? 1 : (x == p.nx - 1 ? p.nx - 2 : x);
+    yi = y == 0 ? 1 : (y == p.ny - 1 ? p.ny - 2 : y);
+    zi = z == 0 ? 1 : (z == p.nz - 1 ? p.nz - 2 : z);
+}
+
+// ExtrapolatedCornerEdgeBoundary: the inward node's density and velocity, then the functor.
+template <int STORE, bool BUF>
+LBM3_HD void apply_extrapolated(const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src, float* g,
+                                int x, int y, int z) {
+    int xi, yi, zi;
+    inward_node(p, x, y, z, xi, yi, zi);
+    float r = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f, jzq = 0.0f;
+    mom_at<STORE, BUF>(p, src, xi, yi, zi, r, jx, jy, jz, jzq, std::make_integer_sequence<int, Q>{});
+    const float inv = 1.0f / r;
+    bc_extrapolated3(g, r, jx * inv, jy * inv, (p.corner_uz ? jzq : jz) * inv);
+}
+
+// A flag that reads a neighbour's post-stream populations pulls them by the streaming rule,
+// which reads the neighbour's stale slots (sources outside a non-periodic domain) from the
+// destination buffer; the neighbour's own update rewrites them. That is safe when both nodes are
+// lanes of one wavefront (a row's 64-column group): the wavefront issues its stores after the
+// loads they depend on. It is also safe when the neighbour has no stale slots (only the source
+// buffer is read). Otherwise the node runs in a pre-launch before the main launch, which skips
+// it (kPreLaunch in its flag). Neighbours read: Guo inlet x + 1, Guo and regularized outlets
+// x - 1, ZG the node one step inward on its face; the corner-edge extrapolation's inward node
+// has no stale slots (it is inside every non-periodic axis).
+constexpr uint8_t kPreLaunch = 0x80;
+
+LBM3_HD bool read_neighbour(const Step3Params& p, uint8_t fl, int x, int y, int z, int& xn, int& yn, int& zn) {
+    xn = x;
+    yn = y;
+    zn = z;
+    if (fl == LBM_GUO_VELOCITY_INLET) xn = x + 1;
+    else if (fl == LBM_GUO_PRESSURE_OUTLET || fl == LBM_REGULARIZED_OUTLET) xn = x - 1;
+    else if (fl == LBM_ZG_OUTFLOW) {
+        const int f = zg_face(p, x, y, z);
+        if (f < 0) return false;
+        if (f < 2) xn = f == 0 ? 1 : p.nx - 2;
+        else if (f < 4) yn = f == 2 ? 1 : p.ny - 2;
+        else zn = f == 4 ? 1 : p.nz - 2;
+    } else {
+        return false;
+    }
+    return true;
+}
+LBM3_HD bool has_stale_slots(const Step3Params& p, int x, int y, int z) {
+    return (!p.px && (x == 0 || x == p.nx - 1)) || (!p.py && (y == 0 || y == p.ny - 1)) ||
+           (!p.pz && (z == 0 || z == p.nz - 1));
+}
+LBM3_HD bool needs_pre_launch(const Step3Params& p, uint8_t fl, int x, int y, int z) {
+    int xn, yn, zn;
+    if (!read_neighbour(p, fl, x, y, z, xn, yn, zn)) return false;
+    if (!has_stale_slots(p, xn, yn, zn)) return false;
+    return !(yn == y && zn == z && xn / 64 == x / 64);
+}
+
+// The pre-launch order (host). A node that reads a neighbour's stale slots must run before the
+// neighbour's update: in a launch ahead of the main launch when the neighbour is in another
+// wavefront (needs_pre_launch), and ahead of the neighbour's own pre-launch when the neighbour
+// runs in one. levels[k] lists the nodes of level k + 1; the launches run from the last level to
+// the first, then the main launch (level 0) skips every listed node. Returns null, or the reason
+// no order exists (two nodes each reading the other's stale slots).
+inline const char* pre_launch_levels(const Step3Params& p, const uint8_t* flags, std::vector<std::vector<int>>& levels) {
+    levels.clear();
+    const long long N = (long long)p.nx * p.ny * p.nz;
+    std::vector<int> level(size_t(N), 0), reads(size_t(N), -1);
+    for (long long n = 0; n < N; n++) {
+        const int x = int(n % p.nx), y = int((n / p.nx) % p.ny), z = int(n / ((long long)p.nx * p.ny));
+        int xn, yn, zn;
+        const uint8_t fl = uint8_t(flags[n] & ~kPreLaunch);
+        if (!read_neighbour(p, fl, x, y, z, xn, yn, zn) || !has_stale_slots(p, xn, yn, zn)) continue;
+        reads[size_t(n)] = int((zn * (long long)p.ny + yn) * p.nx + xn);
+        if (needs_pre_launch(p, fl, x, y, z)) level[size_t(n)] = 1;
+    }
+    // a reader of a pre-launched node runs one level earlier (chains are a few nodes long)
+    for (int it = 0;; it++) {
+        bool changed = false;
+        for (long long n = 0; n < N; n++) {
+            const int m = reads[size_t(n)];
+            if (m < 0 || level[size_t(m)] == 0) continue;
+            if (level[size_t(n)] <= level[size_t(m)]) {
+                level[size_t(n)] = level[size_t(m)] + 1;
+                changed = true;
+            }
+        }
+        if (!changed) break;
+        if (it > 16) return "boundary nodes read each other's stale slots in a cycle";
+    }
+    int top = 0;
+    for (int v : level) top = v > top ? v : top;
+    levels.assign(size_t(top), {});
+    for (long long n = 0; n < N; n++)
+        if (level[size_t(n)] > 0) levels[size_t(level[size_t(n)] - 1)].push_back(int(n));
+    return nullptr;
+}
+
+// The post-stream populations of one node: pull, then the boundary functor of its flag.
+// Requires 0 <= x < nx, 0 <= y < ny, 0 <= z < nz and nx*ny*nz < 2^31. GUO 1: the node may carry
+// a flag that reads a neighbour (Guo inlet/outlet, ZG outflow); GUO 2: also a regularized or
+// corner-edge extrapolation flag (a variant of its own: its 27 equilibria and Pi spill registers).
+template <int STORE, bool BUF, int GUO>
+LBM3_HD void post_stream3(const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src, int x, int y,
+                          int z, uint32_t node, float* g) {
     const int nxy = p.nx * p.ny;
-    const uint32_t node = uint32_t(z * nxy + y * p.nx + x);
     int ox[3], oy[3], oz[3];
     bool vx[3], vy[3], vz[3];
     axis_pull(x, p.nx, p.px, 1, ox, vx);
     axis_pull(y, p.ny, p.py, p.nx, oy, vy);
     axis_pull(z, p.nz, p.pz, nxy, oz, vz);
-    const lbm::PopBuf<T, BUF> src(p.src), dst(p.dst);
-    float g[Q];
     const bool lane_in = vx[0] && vx[2] && vy[0] && vy[2] && vz[0] && vz[2];
     if (wave_all(lane_in))
         pull_all<STORE, BUF, true>(g, p, src, node, ox, vx, oy, vy, oz, vz, std::make_integer_sequence<int, Q>{});
     else
         pull_all<STORE, BUF, false>(g, p, src, node, ox, vx, oy, vy, oz, vz, std::make_integer_sequence<int, Q>{});
 
-    const uint8_t fl = p.flags ? p.flags[node] : uint8_t(LBM_FLUID);
+    const uint8_t fl = p.flags ? uint8_t(p.flags[node] & ~kPreLaunch) : uint8_t(LBM_FLUID);
     if (fl == LBM_BOUNCE_BACK) {
         const Box3 B{p.nx, p.ny, p.nz, p.px, p.py, p.pz};
         bc_bounce_back3(g, x, y, z, B);
@@ -271,21 +409,45 @@ LBM3_HD void step3_node(const Step3Params& p, int x, int y, int z) {
     if constexpr (GUO != 0) {
         if (fl == LBM_GUO_VELOCITY_INLET) apply_guo<STORE, BUF, true>(p, src, g, x, y, z);
         else if (fl == LBM_GUO_PRESSURE_OUTLET) apply_guo<STORE, BUF, false>(p, src, g, x, y, z);
-        else if (fl == LBM_ZG_OUTFLOW) {
-            if (x == 0) apply_zg<STORE, BUF, true>(p, src, g, x, y, z);
-            else apply_zg<STORE, BUF, false>(p, src, g, x, y, z);
-        } else if (GUO == 2 && (fl == LBM_REGULARIZED_BOUNCE_BACK || fl == LBM_REGULARIZED_INLET_LEFT)) {
+        else if (fl == LBM_ZG_OUTFLOW) apply_zg<STORE, BUF>(p, src, g, x, y, z);
+        else if (GUO == 2 && (fl == LBM_REGULARIZED_BOUNCE_BACK || fl == LBM_REGULARIZED_INLET_LEFT)) {
             const Box3 B{p.nx, p.ny, p.nz, p.px, p.py, p.pz};
             const float ui = fl == LBM_REGULARIZED_INLET_LEFT ? p.u_inlet : 0.0f;
             bc_regularized3(g, x, y, z, B, ui, 0.0f, 0.0f, p.lin_ec);
+        } else if (GUO == 2 && fl == LBM_REGULARIZED_OUTLET) {
+            apply_regularized_outlet<STORE, BUF>(p, src, g, x, y, z);
+        } else if (GUO == 2 && fl == LBM_EXTRAPOLATED_CORNER_EDGE) {
+            apply_extrapolated<STORE, BUF>(p, src, g, x, y, z);
         }
     }
-    float fx = p.fx, fy = p.fy, fz = p.fz;
-    if (p.force) {
+}
+
+// The force on a node: the array inside the box fbox = [x0, x1) x [y0, y1) x [z0, z1) when one
+// is set, the uniform force elsewhere.
+LBM3_HD void node_force(const Step3Params& p, int x, int y, int z, uint32_t node, float& fx, float& fy, float& fz) {
+    fx = p.fx;
+    fy = p.fy;
+    fz = p.fz;
+    if (p.force && x >= p.fbox[0] && x < p.fbox[1] && y >= p.fbox[2] && y < p.fbox[3] && z >= p.fbox[4] &&
+        z < p.fbox[5]) {
         fx = p.force[node];
         fy = p.force[p.plane + node];
         fz = p.force[2 * p.plane + node];
     }
+}
+
+// One node of one time step: post-stream populations, macroscopics, Guo-forced BGK, MRT or CM,
+// store.
+template <int COLL, int STORE, bool BUF, int GUO = 0>
+LBM3_HD void step3_node(const Step3Params& p, int x, int y, int z) {
+    using St = Store3<STORE>;
+    using T = typename St::T;
+    const uint32_t node = uint32_t(z * (p.nx * p.ny) + y * p.nx + x);
+    const lbm::PopBuf<T, BUF> src(p.src), dst(p.dst);
+    float g[Q];
+    post_stream3<STORE, BUF, GUO>(p, src, x, y, z, node, g);
+    float fx, fy, fz;
+    node_force(p, x, y, z, node, fx, fy, fz);
     const Macro3 m = macroscopics3(g, fx, fy, fz);
     if (p.out_rho) {
         p.out_rho[node] = m.rho;

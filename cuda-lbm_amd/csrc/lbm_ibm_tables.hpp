@@ -133,3 +133,88 @@ inline const char* ibm_build_tables(int nx, int ny, bool two_by_two, bool no_ds,
 }
 
 }  // namespace lbm
+
+namespace lbm {
+
+// The 3-D tables (IBMManager<3>; IBM_impl.cu:53-118, 156-196): the 4x4x4 stencil of each
+// marker at floor(p) - 1 .. floor(p) + 2 on every axis, in the reference's loop order (x outer,
+// then y, then z), nodes outside the grid skipped; weights phi(dx) phi(dy) phi(dz); the spread
+// weights carry the marker's area ds unless no_ds. Band nodes sorted by node index
+// ((z NY + y) NX + x).
+struct IbmTables3 {
+    std::vector<int> band_node, mk_ptr, mk_band, bn_ptr, bn_mk;
+    std::vector<float> mk_w, bn_w;
+};
+
+inline const char* ibm_build_tables3(int nx, int ny, int nz, bool no_ds, const float* xyz, const float* ds, int n,
+                                     IbmTables3& T) {
+    T = IbmTables3();
+    if (nx < 1 || ny < 1 || nz < 1 || n < 1 || !xyz) return "IBM tables need a grid and at least one marker";
+    for (int l = 0; l < n; l++) {
+        for (int d = 0; d < 3; d++)
+            if (!std::isfinite(xyz[3 * l + d])) return "marker coordinates must be finite";
+        if (ds && !(ds[l] > 0.0f)) return "marker areas must be positive";
+    }
+    struct Ent {
+        long long node;
+        float k;
+    };
+    std::vector<std::vector<Ent>> st(static_cast<size_t>(n));
+    std::map<long long, int> band;
+    const long long N = (long long)nx * ny * nz;
+    for (int l = 0; l < n; l++) {
+        const float px = xyz[3 * l], py = xyz[3 * l + 1], pz = xyz[3 * l + 2];
+        if (px < -4.0f || py < -4.0f || pz < -4.0f || px > float(nx) + 4.0f || py > float(ny) + 4.0f ||
+            pz > float(nz) + 4.0f)
+            continue;  // touches no node
+        const int gx = int(std::floor(px)), gy = int(std::floor(py)), gz = int(std::floor(pz));
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++)
+                for (int k = 0; k < 4; k++) {
+                    const int x = gx + i - 1, y = gy + j - 1, z = gz + k - 1;
+                    if (x >= nx || x < 0 || y >= ny || y < 0 || z >= nz || z < 0) continue;
+                    const float w = delta4f(px - float(x)) * delta4f(py - float(y)) * delta4f(pz - float(z));
+                    const long long node = ((long long)z * ny + y) * nx + x;
+                    st[size_t(l)].push_back({node, w});
+                    band.emplace(node, 0);
+                }
+    }
+    if (band.empty()) return "no marker stencil touches the grid";
+    for (auto& kv : band) {
+        kv.second = int(T.band_node.size());
+        T.band_node.push_back(int(kv.first));
+    }
+    const int nb = int(T.band_node.size());
+    T.mk_ptr.assign(size_t(n) + 1, 0);
+    std::vector<std::vector<std::pair<int, float>>> per_node(static_cast<size_t>(nb));
+    for (int l = 0; l < n; l++) {
+        const float d = (ds && !no_ds) ? ds[l] : 1.0f;
+        for (const Ent& e : st[size_t(l)]) {
+            const int b = band.at(e.node);
+            T.mk_band.push_back(b);
+            T.mk_w.push_back(e.k);
+            per_node[size_t(b)].push_back({l, e.k * d});
+        }
+        T.mk_ptr[size_t(l) + 1] = int(T.mk_band.size());
+    }
+    T.bn_ptr.assign(size_t(nb) + 1, 0);
+    for (int b = 0; b < nb; b++) {
+        for (const auto& pr : per_node[size_t(b)]) {
+            T.bn_mk.push_back(pr.first);
+            T.bn_w.push_back(pr.second);
+        }
+        T.bn_ptr[size_t(b) + 1] = int(T.bn_mk.size());
+    }
+    for (int v : T.band_node)
+        if (v < 0 || v >= N) return "IBM band node out of range";
+    for (int v : T.mk_band)
+        if (v < 0 || v >= nb) return "IBM stencil entry out of range";
+    for (int v : T.bn_mk)
+        if (v < 0 || v >= n) return "IBM spread entry out of range";
+    if (T.mk_ptr.back() != int(T.mk_band.size()) || T.bn_ptr.back() != int(T.bn_mk.size()) ||
+        T.mk_band.size() != T.bn_mk.size())
+        return "IBM table sizes inconsistent";
+    return nullptr;
+}
+
+}  // namespace lbm

@@ -58,6 +58,7 @@ LBM_QUIRK_IBM_NO_DS = 1 << 5
 LBM_QUIRK_EQ3D_LINEAR = 1 << 6   # include/lbm3d_hip.h
 LBM_QUIRK_MRT3_LITERAL = 1 << 7
 LBM_QUIRK_GUO3D_OWN_MOMENTS = 1 << 8
+LBM_QUIRK_CORNER3D_UZ = 1 << 9
 
 # reductions
 LBM_REDUCE_MASS = 0
@@ -169,6 +170,8 @@ PROTOTYPES = [
     ("lbm3d_get_pops", _I, [_P, _P, _I]),
     ("lbm3d_set_pops", _I, [_P, _P, _I]),
     ("lbm3d_reduce", _I, [_P, _I, _D]),
+    ("lbm3d_ibm_set_markers", _I, [_P, _P, _P, _I, _I]),
+    ("lbm3d_ibm_get_forces", _I, [_P, _P, _D]),
     ("lbm3d_last_error", ctypes.c_char_p, [_P]),
 ]
 

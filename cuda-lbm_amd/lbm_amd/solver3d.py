@@ -159,6 +159,26 @@ class Solver3D:
         self._check(self.lib.lbm3d_reduce(self.h, kind, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out if kind == A.LBM3D_REDUCE_MOMENTUM else out[0]
 
+    # -- immersed boundary (IBMManager<3>, IBMManager.cuh:54-110, 222-252) ----------------
+    def ibm_set_markers(self, xyz, ds=None, iters: int = 3):
+        """Static markers AoS [n][3] in lattice units, per-marker area ds (None = 1)."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+        n = xyz.shape[0]
+        dsa = None if ds is None else np.ascontiguousarray(ds, dtype=np.float32).reshape(n)
+        self._check(self.lib.lbm3d_ibm_set_markers(
+            self.h, xyz.ctypes.data_as(ctypes.c_void_p),
+            None if dsa is None else dsa.ctypes.data_as(ctypes.c_void_p), n, int(iters)))
+        self._ibm_n = n
+
+    def ibm_get_forces(self):
+        """(Lagrangian forces of the last step [n, 3], total force on the fluid [3] in FP64)."""
+        n = getattr(self, "_ibm_n", 0)
+        lag = np.empty(3 * max(n, 1), np.float32)
+        tot = np.zeros(3, np.float64)
+        self._check(self.lib.lbm3d_ibm_get_forces(self.h, lag.ctypes.data_as(ctypes.c_void_p),
+                                                  tot.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return lag[: 3 * n].reshape(n, 3), tot
+
     # -- output files (src/core/lbm.cuh:173-343) -------------------------------------------
     def save_vtk(self, directory, reference_offset: bool = False):
         from . import output

@@ -15,12 +15,14 @@
  * CM<3, NoAdapter> (src/core/collision/CM/CM_D3Q27.cuh:1-105), the FLUID, BOUNCE_BACK and
  * CORNER_EDGE_BOUNCE_BACK flags (src/core/boundaries/boundaries.cuh:89-138, BounceBack<3>
  * src/functors/boundaryConditions/bbDomainBoundary.cuh:52-90, EdgeCornerBounceBack
- * src/functors/boundaryConditions/edgeCornerBounceBack.cuh:10-49), ZG_OUTFLOW on the x = 0 and
- * x = NX-1 faces (ZG_OutflowBoundary<3>, zeroGradientOutflow.cuh:8-59), REGULARIZED_BOUNCE_BACK and
- * REGULARIZED_INLET_LEFT (RegularizedBoundary, regularizedBoundary3D.cuh:10-137), GUO_VELOCITY_INLET at x = 0 and
- * GUO_PRESSURE_OUTLET at x = NX-1 (guoInlet3D.cuh, guoOutlet3D.cuh), runtime periodic axes, FP32 or
- * 16-bit shifted population storage. The CM<3> adapters (ACM_D3Q27.cuh, E_ACM_D3Q27.cuh) and the
- * other 3-D boundary functors are rejected with LBM_E_UNSUPPORTED.
+ * src/functors/boundaryConditions/edgeCornerBounceBack.cuh:10-49), ZG_OUTFLOW on every domain
+ * face (ZG_OutflowBoundary<3>, zeroGradientOutflow.cuh:8-59), REGULARIZED_BOUNCE_BACK,
+ * REGULARIZED_INLET_LEFT (regularizedBoundary3D.cuh:10-137), REGULARIZED_OUTLET (RegularizedOutlet,
+ * regularizedOutlet.cuh), EXTRAPOLATED_CORNER_EDGE (ExtrapolatedCornerEdgeBoundary<3>,
+ * cornerBoundaries.cuh:6-36), GUO_VELOCITY_INLET at
+ * x = 0 and GUO_PRESSURE_OUTLET at x = NX-1 (guoInlet3D.cuh, guoOutlet3D.cuh), runtime periodic
+ * axes, FP32 or 16-bit shifted population storage. The CM<3> adapters (ACM_D3Q27.cuh,
+ * E_ACM_D3Q27.cuh) are rejected with LBM_E_UNSUPPORTED.
  */
 #ifndef LBM3D_HIP_H
 #define LBM3D_HIP_H
@@ -55,6 +57,11 @@ extern "C" {
  * u_f) reads the neighbour. Default (0): rho_f, u_f of the neighbour. */
 #define LBM_QUIRK_GUO3D_OWN_MOMENTS (1u << 8)
 
+/* Reference-literal velocity of ExtrapolatedCornerEdgeBoundary<3> (bit in lbm3d_config.quirks):
+ * src/functors/boundaryConditions/cornerBoundaries.cuh:23 sums u_z with C[3*2+i], the flat
+ * velocity table read at 6 + i, instead of C[3*i+2]. Default (0): the interior node's u_z. */
+#define LBM_QUIRK_CORNER3D_UZ (1u << 9)
+
 /* CM<3> adapters (src/core/collision/adapters.cuh): LBM_NO_ADAPTER is built. LBM_OPTIMAL_ADAPTER
  * (CM<3, OptimalAdapter>, src/core/collision/CM/ACM_D3Q27.cuh) and LBM3D_EMPIRICAL_ADAPTER
  * (E_ACM_D3Q27.cuh) are rejected with LBM_E_UNSUPPORTED: as shipped, both assign instead of
@@ -80,7 +87,7 @@ typedef struct lbm3d_config {
     float    u_inlet;        /* Scenario::u_max of GUO_VELOCITY_INLET and REGULARIZED_INLET_LEFT
                                 (boundaries.cuh:110-125)                                        */
     uint32_t quirks;         /* LBM_QUIRK_STALE_REST | LBM_QUIRK_EQ3D_LINEAR | LBM_QUIRK_MRT3_LITERAL
-                                | LBM_QUIRK_GUO3D_OWN_MOMENTS                                    */
+                                | LBM_QUIRK_GUO3D_OWN_MOMENTS | LBM_QUIRK_CORNER3D_UZ            */
     int32_t  reserved[8];
 } lbm3d_config;
 
@@ -100,13 +107,16 @@ int lbm3d_set_stream(lbm3d_ctx* ctx, void* hip_stream);
 /* Replaces setup_boundary_flags (boundaries.cuh:170-197) for 3-D: one flag per node, LBM_FLUID,
  * LBM_BOUNCE_BACK, LBM_CORNER_EDGE_BOUNCE_BACK (acts on nodes with two or more domain faces, like
  * FLUID elsewhere), LBM_REGULARIZED_BOUNCE_BACK (u = 0) and LBM_REGULARIZED_INLET_LEFT
- * (u = (u_inlet, 0, 0)) on any face node (like FLUID inside), LBM_ZG_OUTFLOW (x = 0 and x = NX-1
- * only), LBM_GUO_VELOCITY_INLET (x = 0 only)
- * or LBM_GUO_PRESSURE_OUTLET (x = NX-1 only); ZG and Guo flags need a non-periodic x axis; other
- * flags give LBM_E_UNSUPPORTED. NULL = all FLUID. ZG and Guo nodes read their x +- 1 neighbour's
- * post-stream populations, including slots the neighbour's own update overwrites: the neighbour
- * is pulled in the same wavefront before any store, and when NX-1 is a multiple of 64 the
- * x = NX-1 ring nodes (y or z on a non-periodic edge) run in a launch of their own first. */
+ * (u = (u_inlet, 0, 0)) on any face node (like FLUID inside), LBM_EXTRAPOLATED_CORNER_EDGE (nodes
+ * with two or more domain faces), LBM_ZG_OUTFLOW (any face node; the first face of x = 0,
+ * x = NX-1, y = 0, y = NY-1, z = 0, z = NZ-1 that holds the node names its inward neighbour),
+ * LBM_GUO_VELOCITY_INLET (x = 0 only), LBM_GUO_PRESSURE_OUTLET or LBM_REGULARIZED_OUTLET (x = NX-1
+ * only); Guo and regularized-outlet flags need a non-periodic x axis; other flags give
+ * LBM_E_UNSUPPORTED. NULL = all FLUID. ZG, outlet and extrapolated nodes read a neighbour's
+ * post-stream populations, including slots the neighbour's own update overwrites: a neighbour
+ * in the same wavefront is pulled before any store; other such nodes run in launches of their
+ * own first, ordered so that each reads its neighbour before that neighbour's launch. Flags that
+ * read each other in a cycle across wavefronts give LBM_E_UNSUPPORTED. */
 int lbm3d_set_flags(lbm3d_ctx* ctx, const uint8_t* flags, int on_device);
 
 /* Replaces init_kernel<3> + init_node (src/core/init/init.cuh:10-43): f = f_back = f_eq.
@@ -139,6 +149,22 @@ int lbm3d_set_pops(lbm3d_ctx* ctx, const float* f, int on_device);
                                          (src/scenarios/taylorGreen3D/postprocessing/
                                          kineticEnergy.cuh:7-42) before its /(N*0.05^2)  */
 int lbm3d_reduce(lbm3d_ctx* ctx, int32_t kind, double* out);
+
+/* ---- immersed boundary, 3-D (multi-direct forcing IBMManager<3>::multi_direct,
+ * IBMManager.cuh:222-252; interpolation and spreading IBM_impl.cu:53-118, 156-196) ----------
+ * Replaces IBMManager<3>::init_and_dispatch + send_to_gpu (IBMManager.cuh:54-110) for static
+ * markers in lattice units, AoS [n][3], in the caller's order (the reference Morton-sorts a
+ * body's markers first, IBMManager.cuh:57-66: pass them sorted to reproduce its order); ds:
+ * per-marker area (NULL = 1; ignored under LBM_QUIRK_IBM_NO_DS). iters = ITER_MAX
+ * (IBMManager.cuh:8, 3 in the reference); LBM_QUIRK_IBM_CLAMP applies as in 2-D. Each step
+ * interpolates the post-stream rho and u over the 4x4x4 Peskin stencils, iterates the
+ * forcing on the band of stencil nodes and adds the spread force to the external force there.
+ * n = 0 removes the body. */
+int lbm3d_ibm_set_markers(lbm3d_ctx* ctx, const float* xyz, const float* ds, int32_t n, int32_t iters);
+/* Lagrangian forces summed over the iterations of the last step, AoS [n][3] (may be NULL), and
+ * the total force the markers apply to the fluid, sum_l F_l ds_l in FP64 (force_sum[3], may be
+ * NULL). Synchronises. */
+int lbm3d_ibm_get_forces(lbm3d_ctx* ctx, float* lag_force, double* force_sum);
 
 const char* lbm3d_last_error(const lbm3d_ctx* ctx);
 

@@ -100,6 +100,16 @@ int oracle3d_set_pops(void* p, const double* f) {
 }
 
 // Only the streaming stage: f -> f_back by the push rule, then the swap (pops read back).
+// The second buffer alone (after oracle3d_set_pops): the values streaming leaves in the slots
+// whose source lies outside a non-periodic domain.
+int oracle3d_set_pops_back(void* p, const double* f) {
+    return dispatch3(static_cast<Handle3*>(p), [&](auto& s) {
+        using R = typename std::remove_reference_t<decltype(s.f)>::value_type;
+        for (size_t i = 0; i < size_t(Q) * s.N; i++) s.f_back[i] = R(f[i]);
+        s.quantize(s.f_back);
+    });
+}
+
 int oracle3d_stream_only(void* p) {
     return dispatch3(static_cast<Handle3*>(p), [&](auto& s) {
         s.stream();
@@ -114,6 +124,35 @@ int oracle3d_stream_boundaries(void* p) {
         s.stream();
         std::swap(s.f, s.f_back);
         s.apply_boundaries();
+    });
+}
+
+// Immersed-boundary markers (AoS [n][3], lattice units) with areas ds (or null: 1) and the
+// number of multi-direct iterations (IBMManager<3>, IBMManager.cuh:54-110, 222-252).
+int oracle3d_ibm_set_markers(void* p, const double* xyz, const double* ds, int n, int iters) {
+    return dispatch3(static_cast<Handle3*>(p), [&](auto& s) {
+        using R = typename std::remove_reference_t<decltype(s.f)>::value_type;
+        std::vector<R> a(xyz, xyz + 3 * size_t(n)), d;
+        if (ds) d.assign(ds, ds + n);
+        s.ibm_set_markers(a.data(), ds ? d.data() : nullptr, n, iters);
+    });
+}
+
+// Lagrangian forces summed over the iterations of the last step, AoS [n][3], and the total
+// force on the fluid, sum_l F_l ds_l (ds = 1 under LBM_QUIRK_IBM_NO_DS).
+int oracle3d_ibm_get_forces(void* p, double* lag, double* total) {
+    return dispatch3(static_cast<Handle3*>(p), [&](auto& s) {
+        const size_t n = s.lag_ds.size();
+        double t[3] = {0, 0, 0};
+        const bool no_ds = (s.cfg.quirks & LBM_QUIRK_IBM_NO_DS) != 0;
+        for (size_t l = 0; l < n; l++)
+            for (int d = 0; d < 3; d++) {
+                const double v = double(s.lag_force_sum[d * n + l]);
+                if (lag) lag[3 * l + d] = v;
+                t[d] += v * (no_ds ? 1.0 : double(s.lag_ds[l]));
+            }
+        if (total)
+            for (int d = 0; d < 3; d++) total[d] = t[d];
     });
 }
 

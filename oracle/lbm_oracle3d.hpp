@@ -523,9 +523,90 @@ struct Solver3 {
         }
     }
 
-    // Flags restated here: FLUID, BOUNCE_BACK, CORNER_EDGE_BOUNCE_BACK, ZG_OUTFLOW, GUO_VELOCITY_INLET (x = 0) and
-    // GUO_PRESSURE_OUTLET (x = NX-1), the placement of the reference's wind tunnel
-    // (windTestScenario.cuh:13-28).
+    // RegularizedOutlet::apply (regularizedOutlet.cuh:9-95) at x = NX-1: rho_n, u_n of the
+    // x - 1 neighbour's post-stream populations (read from `snap`, as for the Guo functors);
+    // f_eq(rho_n, u_n) in the solver's equilibrium (its private de_rosis_eq is the literal
+    // form, selected with the rest by eq_literal); the unknown populations (c_x < 0) take
+    // f_eq + (f_opp - f_eq_opp); then Pi(1) of the updated node minus that of the equilibrium,
+    // and every population f_eq + w / (2 cs^4) Q : Pi(1).
+    void bc_regularized_outlet(size_t n, int x, int y, int z, const std::vector<R>& snap) {
+        const size_t nb = node(x - 1, y, z);
+        R rho = 0, ux = 0, uy = 0, uz = 0;
+        for (int q = 0; q < Q; q++) {
+            const R v = snap[q * N + nb];
+            rho += v;
+            ux += v * R(CX[q]);
+            uy += v * R(CY[q]);
+            uz += v * R(CZ[q]);
+        }
+        ux /= rho;
+        uy /= rho;
+        uz /= rho;
+        R feq[Q];
+        equilibrium<R>(rho, ux, uy, uz, feq, eq_literal);
+        for (int q = 0; q < Q; q++)
+            if (CX[q] < 0) f[q * N + n] = feq[q] + (f[OPP[q] * N + n] - feq[OPP[q]]);
+        const R cs2 = R(1) / R(3);
+        R Pxx = 0, Pyy = 0, Pzz = 0, Pxy = 0, Pxz = 0, Pyz = 0;
+        for (int q = 0; q < Q; q++) {
+            const R fi = f[q * N + n];
+            Pxx += R(CX[q] * CX[q]) * fi;
+            Pyy += R(CY[q] * CY[q]) * fi;
+            Pzz += R(CZ[q] * CZ[q]) * fi;
+            Pxy += R(CX[q] * CY[q]) * fi;
+            Pxz += R(CX[q] * CZ[q]) * fi;
+            Pyz += R(CY[q] * CZ[q]) * fi;
+        }
+        Pxx -= cs2 * rho + rho * ux * ux;
+        Pyy -= cs2 * rho + rho * uy * uy;
+        Pzz -= cs2 * rho + rho * uz * uz;
+        Pxy -= rho * ux * uy;
+        Pxz -= rho * ux * uz;
+        Pyz -= rho * uy * uz;
+        for (int q = 0; q < Q; q++) {
+            const R cx = R(CX[q]), cy = R(CY[q]), cz = R(CZ[q]);
+            const R fneq = weight<R>(q) / (R(2) * cs2 * cs2) *
+                           ((cx * cx - cs2) * Pxx + (cy * cy - cs2) * Pyy + (cz * cz - cs2) * Pzz +
+                            R(2) * (cx * cy * Pxy + cx * cz * Pxz + cy * cz * Pyz));
+            f[q * N + n] = feq[q] + fneq;
+        }
+    }
+
+    // ExtrapolatedCornerEdgeBoundary<3>::apply (cornerBoundaries.cuh:6-36): every population of
+    // the node becomes the second-order equilibrium w rho (1 + c.u/cs2 + (c.u)^2/(2 cs4) -
+    // u.u/(2 cs2)) of the node one step inward on each axis where the node sits on a domain face
+    // (its post-stream state, from `snap`). LBM_QUIRK_CORNER3D_UZ sums u_z with the flat
+    // velocity table read at 6 + i, as :23 does.
+    void bc_extrapolated(size_t n, int x, int y, int z, const std::vector<R>& snap) {
+        const int ix = x == 0 ? 1 : (x == nx - 1 ? nx - 2 : x);
+        const int iy = y == 0 ? 1 : (y == ny - 1 ? ny - 2 : y);
+        const int iz = z == 0 ? 1 : (z == nz - 1 ? nz - 2 : z);
+        const size_t ni = node(ix, iy, iz);
+        const bool quirk = (cfg.quirks & LBM_QUIRK_CORNER3D_UZ) != 0;
+        R rho = 0, ux = 0, uy = 0, uz = 0;
+        for (int i = 0; i < Q; i++) {
+            const R v = snap[i * N + ni];
+            rho += v;
+            ux += v * R(CX[i]);
+            uy += v * R(CY[i]);
+            const int k = 6 + i;  // C[3*2+i] of the flat [27][3] table
+            uz += v * R(quirk ? (k % 3 == 0 ? CX[k / 3] : (k % 3 == 1 ? CY[k / 3] : CZ[k / 3])) : CZ[i]);
+        }
+        ux /= rho;
+        uy /= rho;
+        uz /= rho;
+        const R cs2 = R(1) / R(3);
+        const R uu = ux * ux + uy * uy + uz * uz;
+        for (int i = 0; i < Q; i++) {
+            const R cu = R(CX[i]) * ux + R(CY[i]) * uy + R(CZ[i]) * uz;
+            f[i * N + n] = weight<R>(i) * rho * (R(1) + cu / cs2 + (cu * cu) / (R(2) * cs2 * cs2) - uu / (R(2) * cs2));
+        }
+    }
+
+    // Flags restated here: FLUID, BOUNCE_BACK, CORNER_EDGE_BOUNCE_BACK, ZG_OUTFLOW, GUO_VELOCITY_INLET (x = 0),
+    // GUO_PRESSURE_OUTLET and REGULARIZED_OUTLET (x = NX-1), EXTRAPOLATED_CORNER_EDGE, the
+    // regularized wall and inlet; the placements of the reference's 3-D scenarios
+    // (windTestScenario.cuh:13-28, flowPastSquareCylinderScenario.cuh:10-31).
     bool has_guo = false;
     void set_flags(const uint8_t* fl) {
         has_guo = false;
@@ -539,7 +620,12 @@ struct Solver3 {
                         continue;
                     }
                     if (v == LBM_REGULARIZED_BOUNCE_BACK || v == LBM_REGULARIZED_INLET_LEFT) continue;
-                    if ((v == LBM_GUO_VELOCITY_INLET && x == 0) || (v == LBM_GUO_PRESSURE_OUTLET && x == nx - 1)) {
+                    if (v == LBM_EXTRAPOLATED_CORNER_EDGE) {
+                        has_guo = true;
+                        continue;
+                    }
+                    if ((v == LBM_GUO_VELOCITY_INLET && x == 0) ||
+                        ((v == LBM_GUO_PRESSURE_OUTLET || v == LBM_REGULARIZED_OUTLET) && x == nx - 1)) {
                         if (cfg.periodic_x) throw std::runtime_error("Guo inlet/outlet need a non-periodic x axis");
                         has_guo = true;
                         continue;
@@ -615,6 +701,8 @@ struct Solver3 {
                     else if (flags[n] == LBM_REGULARIZED_INLET_LEFT) bc_regularized(n, x, y, z, R(cfg.u_inlet), R(0), R(0));
                     else if (flags[n] == LBM_GUO_VELOCITY_INLET) bc_guo(n, x, y, z, true, snap);
                     else if (flags[n] == LBM_GUO_PRESSURE_OUTLET) bc_guo(n, x, y, z, false, snap);
+                    else if (flags[n] == LBM_REGULARIZED_OUTLET) bc_regularized_outlet(n, x, y, z, snap);
+                    else if (flags[n] == LBM_EXTRAPOLATED_CORNER_EDGE) bc_extrapolated(n, x, y, z, snap);
                 }
     }
 
@@ -694,6 +782,90 @@ struct Solver3 {
         }
     }
 
+    // ---- immersed boundary, 3-D (IBMManager<3>::multi_direct, IBMManager.cuh:222-252) ----
+    std::vector<R> lag_xyz, lag_ds;   // markers AoS [n][3], area per marker
+    std::vector<R> lag_force_sum;     // Lagrangian force summed over the iterations, SoA [3][n]
+    int ibm_iters = 3;
+
+    void ibm_set_markers(const R* xyz, const R* ds, int n, int iters) {
+        lag_xyz.assign(xyz, xyz + 3 * size_t(n));
+        lag_ds.assign(size_t(n), R(1));
+        if (ds)
+            for (int l = 0; l < n; l++) lag_ds[size_t(l)] = ds[l];
+        lag_force_sum.assign(3 * size_t(n), R(0));
+        ibm_iters = iters;
+    }
+
+    // delta4 (IBMUtils.cuh:23-33) in R.
+    static R delta4(R r) {
+        const R a = std::abs(r);
+        if (a < R(1)) return R(0.125) * (R(3) - R(2) * a + std::sqrt(R(1) + R(4) * a - R(4) * r * r));
+        if (a < R(2)) return R(0.125) * (R(5) - R(2) * a - std::sqrt(R(-7) + R(12) * a - R(4) * r * r));
+        return R(0);
+    }
+
+    // The 4x4x4 stencil of marker l (KERNEL_W = 4, OFFSET = -1; IBMUtils.cuh:12-13) in the
+    // reference's loop order (x outer, then y, then z), nodes outside the domain skipped:
+    // IBM_impl.cu:53-118 (interpolation) and :156-196 (spreading) visit the same nodes.
+    template <class F>
+    void stencil(int l, F&& fn) const {
+        const R px = lag_xyz[3 * size_t(l)], py = lag_xyz[3 * size_t(l) + 1], pz = lag_xyz[3 * size_t(l) + 2];
+        if (!(std::abs(px) < R(1e9) && std::abs(py) < R(1e9) && std::abs(pz) < R(1e9))) return;
+        const int gx = int(std::floor(px)), gy = int(std::floor(py)), gz = int(std::floor(pz));
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++)
+                for (int k = 0; k < 4; k++) {
+                    const int x = gx + i - 1, y = gy + j - 1, z = gz + k - 1;
+                    if (x >= nx || x < 0 || y >= ny || y < 0 || z >= nz || z < 0) continue;
+                    fn(node(x, y, z), delta4(px - R(x)) * delta4(py - R(y)) * delta4(pz - R(z)));
+                }
+    }
+
+    // multi_direct: u_prev <- u (uncorrected); ITER times: F_iter <- 0; interpolate rho, u_prev
+    // to the markers; F_L = 2 rho_L (0 - u_L) (> 1e-8 kept under LBM_QUIRK_IBM_CLAMP,
+    // IBM_impl.cuh:7-26); spread F_L phi (times the marker area unless LBM_QUIRK_IBM_NO_DS: the
+    // reference spreads without it); u_prev += F_iter / (2 rho) (clamped > 1e-8 under the clamp
+    // quirk, :28-47); force += F_iter (:49-68).
+    void multi_direct() {
+        const int n = int(lag_ds.size());
+        if (n == 0) return;
+        const bool clamp = (cfg.quirks & LBM_QUIRK_IBM_CLAMP) != 0, no_ds = (cfg.quirks & LBM_QUIRK_IBM_NO_DS) != 0;
+        std::vector<R> u_prev(u), f_iter(3 * N, R(0)), lag_f(3 * size_t(n), R(0));
+        std::fill(lag_force_sum.begin(), lag_force_sum.end(), R(0));
+        for (int it = 0; it < ibm_iters; it++) {
+            std::fill(f_iter.begin(), f_iter.end(), R(0));
+            for (int l = 0; l < n; l++) {
+                R r = 0, ux = 0, uy = 0, uz = 0;
+                stencil(l, [&](size_t m, R k) {
+                    r += k * rho[m];
+                    ux += k * u_prev[m];
+                    uy += k * u_prev[N + m];
+                    uz += k * u_prev[2 * N + m];
+                });
+                const R ul[3] = {ux, uy, uz};
+                for (int d = 0; d < 3; d++) {
+                    R fl = R(2) * r * (R(0) - ul[d]);
+                    if (clamp) fl = fl > R(1e-8) ? fl : R(0);
+                    lag_f[size_t(d) * n + l] = fl;
+                    lag_force_sum[size_t(d) * n + l] += fl;
+                }
+            }
+            for (int l = 0; l < n; l++) {
+                const R a = no_ds ? R(1) : lag_ds[size_t(l)];
+                stencil(l, [&](size_t m, R k) {
+                    for (int d = 0; d < 3; d++) f_iter[d * N + m] += k * lag_f[size_t(d) * n + l] * a;
+                });
+            }
+            for (size_t m = 0; m < N; m++)
+                for (int d = 0; d < 3; d++) {
+                    R c = u_prev[d * N + m] + f_iter[d * N + m] / (R(2) * rho[m]);
+                    if (clamp) c = c > R(1e-8) ? c : R(0);
+                    u_prev[d * N + m] = c;
+                    force[d * N + m] += f_iter[d * N + m];
+                }
+        }
+    }
+
     void step() {
         timestep++;
         stream();
@@ -701,6 +873,7 @@ struct Solver3 {
         apply_boundaries();
         uncorrected_macroscopics();
         reset_forces();
+        multi_direct();
         correct_macroscopics();
         collide();
         quantize(f);

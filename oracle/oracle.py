@@ -73,6 +73,9 @@ def lib():
             "oracle3d_get_macros": (ctypes.c_int, [_P, _D, _D]),
             "oracle3d_get_pops": (ctypes.c_int, [_P, _D]),
             "oracle3d_set_pops": (ctypes.c_int, [_P, _D]),
+            "oracle3d_set_pops_back": (ctypes.c_int, [_P, _D]),
+            "oracle3d_ibm_set_markers": (ctypes.c_int, [_P, _D, _D, ctypes.c_int, ctypes.c_int]),
+            "oracle3d_ibm_get_forces": (ctypes.c_int, [_P, _D, _D]),
             "oracle3d_stream_only": (ctypes.c_int, [_P]),
             "oracle3d_stream_boundaries": (ctypes.c_int, [_P]),
             "oracle3d_last_error": (ctypes.c_char_p, [_P]),
@@ -382,6 +385,24 @@ class Oracle3D:
         self._ck(self.L.oracle3d_get_pops(self.h, _dp(f)))
         return f.reshape(27, *self.shape)
 
-    def set_pops(self, f):
+    def ibm_set_markers(self, xyz, ds=None, iters=3):
+        a = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+        n = a.size // 3
+        d = None if ds is None else np.ascontiguousarray(ds, np.float64).reshape(n)
+        self._ck(self.L.oracle3d_ibm_set_markers(self.h, _dp(a), None if d is None else _dp(d), n, iters))
+        self._n = n
+
+    def ibm_get_forces(self):
+        lag = np.zeros(3 * self._n)
+        tot = np.zeros(3)
+        self._ck(self.L.oracle3d_ibm_get_forces(self.h, _dp(lag), _dp(tot)))
+        return lag.reshape(self._n, 3), tot
+
+    def set_pops(self, f, f_back=None):
+        """Both buffers = f; with f_back, the second buffer (the stale slots streaming leaves
+        at non-periodic edges) takes f_back."""
         a = np.ascontiguousarray(f, np.float64).reshape(-1)
         self._ck(self.L.oracle3d_set_pops(self.h, _dp(a)))
+        if f_back is not None:
+            b = np.ascontiguousarray(f_back, np.float64).reshape(-1)
+            self._ck(self.L.oracle3d_set_pops_back(self.h, _dp(b)))

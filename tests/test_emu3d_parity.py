@@ -352,3 +352,170 @@ def test_emulated_reduced_storage_matches_oracle(storage, eps):
     ref = o.get_pops()
     dev = np.abs(ref - W[:, None, None, None]).max()
     assert np.abs(g - ref).max() <= 4 * eps * dev
+
+
+NEW3_CASES = [
+    # shape, periodic (y, z), collision, layout, quirks
+    ((9, 6, 5), (0, 0), _abi.LBM_BGK, "reg_outlet", 0),
+    ((65, 5, 4), (0, 0), _abi.LBM_CM, "reg_outlet", 0),        # (NX-1) % 64 == 0: ring nodes pre-launched
+    ((8, 5, 6), (0, 0), _abi.LBM_MRT, "reg_outlet", _abi.LBM_QUIRK_EQ3D_LINEAR),
+    ((7, 6, 5), (0, 0), _abi.LBM_BGK, "extrapolated", 0),
+    ((7, 6, 5), (0, 0), _abi.LBM_CM, "extrapolated", _abi.LBM_QUIRK_CORNER3D_UZ),
+    ((70, 5, 4), (0, 1), _abi.LBM_BGK, "extrapolated", 0),
+    ((8, 6, 5), (0, 0), _abi.LBM_BGK, "zg_all", 0),            # y/z faces: edge neighbours pre-launched
+    ((9, 5, 6), (1, 0), _abi.LBM_CM, "zg_all", 0),
+    ((66, 6, 5), (0, 0), _abi.LBM_MRT, "zg_all", 0),
+]
+
+
+def new3_case(case):
+    """REGULARIZED_OUTLET (regularizedOutlet.cuh) at x = NX-1 with a regularized inlet and walls;
+    EXTRAPOLATED_CORNER_EDGE (cornerBoundaries.cuh) on every edge and corner with walls and a
+    Guo inlet/outlet; ZG_OUTFLOW on every face node (each takes the first face in x, y, z order)."""
+    shape, (py, pz), coll, layout, quirks = NEW3_CASES[case]
+    nx, ny, nz = shape
+    c = make_cfg(shape, (0, py, pz), 1.25, collision=coll, S48=(1.25,) * 5, quirks=quirks)
+    c.u_inlet = 0.04
+    f0 = initial_state(shape, 220 + case)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    wall_y = ~np.bool_(py) & ((y == 0) | (y == ny - 1))
+    wall_z = ~np.bool_(pz) & ((z == 0) | (z == nz - 1))
+    fl = np.zeros((nz, ny, nx), np.uint8)
+    if layout == "reg_outlet":
+        fl[wall_y | wall_z] = _abi.BOUNCE_BACK
+        fl[x == 0] = _abi.REGULARIZED_INLET_LEFT
+        fl[x == nx - 1] = _abi.REGULARIZED_OUTLET
+    elif layout == "extrapolated":
+        fl[wall_y | wall_z] = _abi.BOUNCE_BACK
+        fl[x == 0] = _abi.GUO_VELOCITY_INLET
+        fl[x == nx - 1] = _abi.GUO_PRESSURE_OUTLET
+        edges = (x == 0) + (x == nx - 1) + (y == 0) + (y == ny - 1) + (z == 0) + (z == nz - 1)
+        fl[edges >= 2] = _abi.EXTRAPOLATED_CORNER_EDGE
+    else:
+        face = (x == 0) | (x == nx - 1) | (y == 0) | (y == ny - 1) | (z == 0) | (z == nz - 1)
+        fl[face] = _abi.ZG_OUTFLOW
+    return c, f0, fl
+
+
+@pytest.mark.parametrize("case", range(len(NEW3_CASES)))
+def test_emulated_outlet_extrapolation_and_zg_faces_match_oracle(case):
+    """The kernel body (neighbour and inward-node pulls by the streaming rule, pre-launch of the
+    nodes reading stale slots of another wavefront, lane ordering) against the oracle's functors
+    on the post-stream snapshot."""
+    c, f0, fl = new3_case(case)
+    o = oracle_run(c, f0, fl, None, 6)
+    g, rho, u = emu_run(c, f0, fl, None, 6)
+    assert np.abs(g - o.get_pops()).max() < 1e-5
+    r_rho, r_u = o.get_macros()
+    assert np.abs(rho - r_rho).max() < 1e-5
+    assert np.abs(u - r_u).max() < 1e-6
+
+
+def sphere_markers(centre, radius, n):
+    """n markers on a sphere (golden-angle spiral), each carrying the area 4 pi r^2 / n."""
+    k = np.arange(n) + 0.5
+    phi = np.arccos(1 - 2 * k / n)
+    theta = np.pi * (1 + 5 ** 0.5) * k
+    xyz = np.stack([np.cos(theta) * np.sin(phi), np.sin(theta) * np.sin(phi), np.cos(phi)], 1) * radius
+    return (xyz + np.asarray(centre)).astype(np.float32), np.full(n, 4 * np.pi * radius ** 2 / n, np.float32)
+
+
+IBM3_CASES = [
+    # shape, periodic, collision, layout, quirks, iters, centre
+    ((16, 14, 12), (1, 1, 1), _abi.LBM_BGK, "none", 0, 3, (7.6, 6.8, 5.9)),
+    ((18, 12, 10), (1, 0, 1), _abi.LBM_CM, "walls_y", 0, 3, (8.3, 5.6, 4.7)),
+    ((14, 12, 12), (1, 1, 1), _abi.LBM_MRT, "field", _abi.LBM_QUIRK_IBM_CLAMP, 3, (6.9, 6.2, 5.4)),
+    ((70, 10, 9), (0, 0, 0), _abi.LBM_BGK, "guo", _abi.LBM_QUIRK_IBM_NO_DS, 2, (3.2, 2.1, 4.4)),  # band on the inlet/walls
+    ((16, 12, 10), (1, 1, 1), _abi.LBM_CM, "none", 0, 1, (15.4, 0.6, 9.3)),                      # clipped by the grid edge
+]
+
+
+def ibm3_case(case):
+    """A sphere of 60 markers (radius 2.5) in a disturbed flow: periodic box, walls with a
+    uniform force, a force field with the clamp quirk, a Guo inlet/outlet channel with walls
+    whose inlet and walls the band touches, and markers overhanging a corner of the grid."""
+    shape, per, coll, layout, quirks, iters, centre = IBM3_CASES[case]
+    nx, ny, nz = shape
+    c = make_cfg(shape, per, 1.3, collision=coll, S48=(1.3,) * 5, quirks=quirks,
+                 force=(2e-5, 0.0, 0.0) if layout == "walls_y" else (0.0, 0.0, 0.0))
+    c.u_inlet = 0.03
+    f0 = initial_state(shape, 300 + case)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    fl = None
+    if layout == "walls_y":
+        fl = np.zeros((nz, ny, nx), np.uint8)
+        fl[(y == 0) | (y == ny - 1)] = _abi.BOUNCE_BACK
+    elif layout == "guo":
+        fl = np.zeros((nz, ny, nx), np.uint8)
+        fl[(y == 0) | (y == ny - 1) | (z == 0) | (z == nz - 1)] = _abi.BOUNCE_BACK
+        fl[x == 0] = _abi.GUO_VELOCITY_INLET
+        fl[x == nx - 1] = _abi.GUO_PRESSURE_OUTLET
+    force = 1e-5 * np.random.default_rng(310 + case).standard_normal((nz, ny, nx, 3)) if layout == "field" else None
+    xyz, ds = sphere_markers(centre, 2.5, 60)
+    return c, f0, fl, force, xyz, ds, iters
+
+
+def emu_run_ibm(c, f0, flags, force, xyz, ds, iters, steps):
+    L = emu_lib()
+    P = ctypes.c_void_p
+    L.emu3d_run_ibm.restype = ctypes.c_int
+    L.emu3d_run_ibm.argtypes = [P, P, P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P, P]
+    N = c.nx * c.ny * c.nz
+    n = len(xyz)
+    f_in = np.ascontiguousarray(f0, np.float32).reshape(-1)
+    out = np.zeros(27 * N, np.float32)
+    rho = np.zeros(N, np.float32)
+    u = np.zeros(3 * N, np.float32)
+    lag = np.zeros(3 * n, np.float32)
+    fl = None if flags is None else np.ascontiguousarray(flags, np.uint8).reshape(-1)
+    F = None if force is None else np.ascontiguousarray(force, np.float32).reshape(-1)
+    m = np.ascontiguousarray(xyz, np.float32).reshape(-1)
+    d = None if ds is None else np.ascontiguousarray(ds, np.float32)
+    p = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+    assert L.emu3d_run_ibm(ctypes.byref(c), p(f_in), p(fl), p(F), p(m), p(d), n, iters, steps, p(out), p(rho), p(u),
+                           p(lag)) == 0
+    shp = (c.nz, c.ny, c.nx)
+    return out.reshape(27, *shp), rho.reshape(shp), u.reshape(*shp, 3), lag.reshape(n, 3)
+
+
+def oracle_run_ibm(c, f0, flags, force, xyz, ds, iters, steps):
+    o = O.Oracle3D(c, 32)
+    o.set_pops(f0.astype(np.float32))
+    if flags is not None:
+        o.set_flags(flags)
+    if force is not None:
+        o.set_force_field(force.astype(np.float32))
+    o.ibm_set_markers(xyz.astype(np.float64), None if ds is None else ds.astype(np.float64), iters)
+    o.step(steps)
+    return o
+
+
+@pytest.mark.parametrize("case", range(len(IBM3_CASES)))
+def test_emulated_ibm3_matches_oracle(case):
+    """The IBM pass (band gather of the post-stream populations, interpolation over the 4x4x4
+    stencils, spreading as a gather over the band tables, the force array read inside the band's
+    box) and the forced step against the oracle's whole-grid multi-direct forcing."""
+    c, f0, fl, force, xyz, ds, iters = ibm3_case(case)
+    steps = 4
+    o = oracle_run_ibm(c, f0, fl, force, xyz, ds, iters, steps)
+    g, rho, u, lag = emu_run_ibm(c, f0, fl, force, xyz, ds, iters, steps)
+    assert np.abs(g - o.get_pops()).max() < 2e-6
+    r_rho, r_u = o.get_macros()
+    assert np.abs(rho - r_rho).max() < 2e-6
+    assert np.abs(u - r_u).max() < 2e-7
+    o_lag, o_tot = o.ibm_get_forces()
+    assert np.abs(lag - o_lag).max() < 1e-6 * max(1.0, float(np.abs(o_lag).max()) * 100)
+    assert np.abs(o_lag).max() > 1e-3  # the markers do force the flow
+
+
+def test_asan_bounds_matrix_3d():
+    """Every access of the D3Q27 step body (pre-launches and wavefront ordering included) on
+    exact-size buffers under AddressSanitizer, over the matrix in lbm3d_emu.cpp."""
+    import subprocess
+
+    exe = ROOT / "tests" / "emu" / "lbm3d_emu_asan"
+    if not exe.exists():
+        pytest.skip("ASan emulator not built")
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "cases ok" in r.stdout

@@ -7,8 +7,9 @@ import oracle as O
 from lbm_amd import _abi as A
 from lbm_amd.solver import LbmError
 from lbm_amd.solver3d import Config3D, Solver3D
-from test_emu3d_parity import (CASES, CM_CASES, GUO_CASES, MRT_CASES, REG_CASES, ZG_CASES, box_flags, cm_case, guo_case,
-                               initial_state, make_cfg, mrt_case, oracle_run, reg_case, zg_case)
+from test_emu3d_parity import (CASES, CM_CASES, GUO_CASES, IBM3_CASES, MRT_CASES, NEW3_CASES, REG_CASES, ZG_CASES, box_flags,
+                               cm_case, guo_case, ibm3_case, initial_state, make_cfg, mrt_case, new3_case, oracle_run,
+                               oracle_run_ibm, reg_case, zg_case)
 
 pytestmark = pytest.mark.gpu
 W = O.tables3d()[0]
@@ -313,8 +314,7 @@ def test_gpu_d3q27_errors_are_reported(gpu_available):
             s.step(1)
         assert e.value.code == A.LBM_E_STATE
         fl = np.zeros((8, 8, 8), np.uint8)
-        fl[:, 3, 0] = A.ZG_OUTFLOW
-        fl[:, 0, 4] = A.ZG_OUTFLOW  # ZG is built on the x faces only
+        fl[:, 3, 4] = A.REGULARIZED_OUTLET  # regularized outlets are built at x = NX-1 only
         with pytest.raises(LbmError) as e:
             s.set_flags(fl)
         assert e.value.code == A.LBM_E_UNSUPPORTED
@@ -326,3 +326,134 @@ def test_gpu_d3q27_errors_are_reported(gpu_available):
         with pytest.raises(LbmError) as e:
             s.reduce(A.LBM3D_REDUCE_MASS)
         assert e.value.code == A.LBM_E_STATE
+
+
+@pytest.mark.parametrize("case", range(len(NEW3_CASES)))
+def test_gpu_outlet_extrapolation_and_zg_faces_match_oracle(gpu_available, case):
+    """REGULARIZED_OUTLET, EXTRAPOLATED_CORNER_EDGE (with and without its velocity quirk) and
+    ZG_OUTFLOW on every face (nodes whose neighbour's stale slots another wavefront rewrites run
+    in pre-launches) against the oracle's functors on the post-stream snapshot."""
+    c, f0, fl = new3_case(case)
+    print(f"case d3q27 new flags {case} {NEW3_CASES[case][0]} {NEW3_CASES[case][3]}", flush=True)
+    f0 = f0.astype(np.float32)
+    o = oracle_run(c, f0, fl, None, 8)
+    with Solver3D(gpu_config(c)) as s:
+        s.set_pops(f0)
+        s.set_flags(fl)
+        s.step(8)
+        g = s.get_pops()
+        rho, u = s.get_macros()
+    assert np.abs(g - o.get_pops()).max() < 1e-5
+    r_rho, r_u = o.get_macros()
+    assert np.abs(rho - r_rho).max() < 1e-5
+    assert np.abs(u - r_u).max() < 1e-6
+
+
+def test_gpu_flags_reading_each_other_in_a_cycle_are_rejected(gpu_available):
+    """Every node of a 3x3x3 box ZG: the centre plane's neighbours form no cycle, so the set-up
+    succeeds; a Guo outlet whose neighbour is a Guo inlet on a 3-wide axis reads and is read
+    within one wavefront, also fine. Unsupported flags still fail loudly."""
+    c = make_cfg((3, 3, 3), (0, 0, 0), 1.0)
+    with Solver3D(gpu_config(c)) as s:
+        s.set_flags(np.full((3, 3, 3), A.ZG_OUTFLOW, np.uint8))
+        with pytest.raises(LbmError):
+            s.set_flags(np.full((3, 3, 3), A.ZOU_HE_LEFT, np.uint8))
+
+
+def run_ibm3_gpu(c, f0, fl, force, xyz, ds, iters, steps, force_after=0):
+    """Solver3D with markers; with force_after > 0 the force field is set after that many steps."""
+    with Solver3D(gpu_config(c)) as s:
+        s.set_pops(f0)
+        if fl is not None:
+            s.set_flags(fl)
+        if force is not None and not force_after:
+            s.set_force_field(force.astype(np.float32))
+        s.ibm_set_markers(xyz, ds, iters)
+        if force_after:
+            s.step(force_after)
+            s.set_force_field(force.astype(np.float32))
+        s.step(steps - force_after)
+        g = s.get_pops()
+        rho, u = s.get_macros()
+        lag, tot = s.ibm_get_forces()
+    return g, rho, u, lag, tot
+
+
+@pytest.mark.parametrize("case", range(len(IBM3_CASES)))
+def test_gpu_ibm3_matches_oracle(gpu_available, case):
+    """3-D multi-direct forcing (IBMManager<3>::multi_direct) on the GPU: the band gather, the
+    interpolation and spreading passes and the force array read by the step, against the
+    oracle's whole-grid forcing; Lagrangian forces and their total as well."""
+    c, f0, fl, force, xyz, ds, iters = ibm3_case(case)
+    print(f"case d3q27 ibm {case} {IBM3_CASES[case][0]} {IBM3_CASES[case][3]}", flush=True)
+    f0 = f0.astype(np.float32)
+    steps = 6
+    o = oracle_run_ibm(c, f0, fl, force, xyz, ds, iters, steps)
+    g, rho, u, lag, tot = run_ibm3_gpu(c, f0, fl, force, xyz, ds, iters, steps)
+    assert np.abs(g - o.get_pops()).max() < 1e-5
+    r_rho, r_u = o.get_macros()
+    assert np.abs(rho - r_rho).max() < 1e-5
+    assert np.abs(u - r_u).max() < 1e-6
+    o_lag, o_tot = o.ibm_get_forces()
+    assert np.abs(o_lag).max() > 1e-3
+    assert np.abs(lag - o_lag).max() < 1e-5 * max(1.0, float(np.abs(o_lag).max()) * 10)
+    assert np.abs(tot - o_tot).max() < 1e-5 * max(1.0, float(np.abs(o_tot).max()))
+
+
+def test_gpu_ibm3_force_field_set_after_the_markers(gpu_available):
+    """A force field set after the markers and two steps replaces the uniform force outside the
+    band (the step's force array is refilled) and under it."""
+    c, f0, fl, force, xyz, ds, iters = ibm3_case(2)
+    f0 = f0.astype(np.float32)
+    o = O.Oracle3D(c, 32)
+    o.set_pops(f0)
+    o.ibm_set_markers(xyz.astype(np.float64), ds.astype(np.float64), iters)
+    o.step(2)
+    o.set_force_field(force.astype(np.float32))
+    o.step(4)
+    g, rho, u, lag, tot = run_ibm3_gpu(c, f0, fl, force, xyz, ds, iters, 6, force_after=2)
+    assert np.abs(g - o.get_pops()).max() < 1e-5
+    assert np.abs(u - o.get_macros()[1]).max() < 1e-6
+
+
+@pytest.mark.parametrize("storage,eps", [(A.LBM_F16_SHIFTED, 2.0 ** -11), (A.LBM_BF16_SHIFTED, 2.0 ** -8)])
+def test_gpu_ibm3_reduced_storage(gpu_available, storage, eps):
+    c, f0, fl, force, xyz, ds, iters = ibm3_case(1)
+    c.storage = storage
+    f0 = f0.astype(np.float32)
+    o = oracle_run_ibm(c, f0, fl, force, xyz, ds, iters, 6)
+    g, rho, u, lag, tot = run_ibm3_gpu(c, f0, fl, force, xyz, ds, iters, 6)
+    ref = o.get_pops()
+    dev = np.abs(ref - W[:, None, None, None]).max()
+    assert np.abs(g - ref).max() <= 4 * eps * dev
+    assert np.abs(lag - o.ibm_get_forces()[0]).max() <= 8 * eps * float(np.abs(o.ibm_get_forces()[0]).max())
+
+
+def test_gpu_ibm3_markers_removed_and_errors(gpu_available):
+    """n = 0 removes the body: the following steps equal a run that never had one. Forces
+    without markers, non-finite coordinates and a body no stencil of which touches the grid are
+    reported."""
+    c, f0, fl, force, xyz, ds, iters = ibm3_case(0)
+    f0 = f0.astype(np.float32)
+    with Solver3D(gpu_config(c)) as s, Solver3D(gpu_config(c)) as t:
+        with pytest.raises(LbmError) as e:
+            s.ibm_get_forces()
+        assert e.value.code == A.LBM_E_STATE
+        s.set_pops(f0)
+        t.set_pops(f0)
+        s.ibm_set_markers(xyz, ds, iters)
+        s.step(1)
+        s.ibm_set_markers(np.zeros((0, 3), np.float32))
+        g1 = s.get_pops()
+        t.set_pops(g1)
+        s.step(3)
+        t.step(3)
+        assert np.array_equal(s.get_pops(), t.get_pops())
+        bad = xyz.copy()
+        bad[3, 1] = np.nan
+        with pytest.raises(LbmError) as e:
+            s.ibm_set_markers(bad, ds)
+        assert e.value.code == A.LBM_E_INVALID
+        with pytest.raises(LbmError) as e:
+            s.ibm_set_markers(xyz + 100.0, ds)
+        assert e.value.code == A.LBM_E_INVALID

@@ -3,6 +3,7 @@ lattice isotropy, moment identities, closed forms, numpy.roll, conservation laws
 analytic flows. CPU only."""
 import ctypes
 import itertools
+import math
 
 import numpy as np
 import pytest
@@ -592,3 +593,158 @@ def test_guo_own_moments_quirk_changes_the_result():
         o.step(3)
         out.append(o.get_pops())
     assert np.abs(out[0] - out[1]).max() > 1e-6
+
+
+def hermite2(P):
+    """w_i / (2 cs^4) Q_i : P for a symmetric 3x3 P (the regularized non-equilibrium part), Q_i =
+    c_i c_i - cs^2 I: its density and momentum vanish and its second moment is P."""
+    cs2 = 1.0 / 3.0
+    Qt = np.einsum("ia,ib->iab", C, C) - cs2 * np.eye(3)[None]
+    return W / (2 * cs2 * cs2) * np.einsum("iab,ab->i", Qt, P)
+
+
+def test_hermite_part_has_the_stated_moments():
+    P = np.array([[2e-3, 5e-4, -3e-4], [5e-4, -1e-3, 7e-4], [-3e-4, 7e-4, 4e-4]])
+    h = hermite2(P)
+    assert abs(h.sum()) < 1e-15 and np.max(np.abs(C.T @ h)) < 1e-15
+    np.testing.assert_allclose(np.einsum("i,ia,ib->ab", h, C, C), P, atol=1e-15)
+
+
+@pytest.mark.parametrize("literal", [False, True])
+def test_regularized_outlet_rebuilds_a_regularized_state_from_its_known_populations(literal):
+    """RegularizedOutlet (regularizedOutlet.cuh:9-95) at x = NX-1 on a uniform regularized state
+    f_eq(rho0, u0) + h(P): its unknown populations (c_x < 0, the slots streaming leaves stale)
+    hold garbage, yet the functor returns f_eq + h exactly (non-equilibrium bounce of the knowns,
+    then Pi(1) projected), with the neighbour's density and momentum. Wrong unknown sets, the
+    node's own moments (which include the garbage) or a missing or mis-scaled Pi term all fail."""
+    nx, ny, nz = 6, 5, 4
+    rho0, u0 = 1.02, (0.03, -0.02, 0.01)
+    P = np.array([[2e-3, 5e-4, -3e-4], [5e-4, -1e-3, 7e-4], [-3e-4, 7e-4, 4e-4]])
+    f_unif = O.equilibrium3d(rho0, *u0, literal=literal) + hermite2(P)
+    f0 = np.broadcast_to(f_unif[:, None, None, None], (27, nz, ny, nx)).copy()
+    rng = np.random.default_rng(8)
+    back = f0.copy()
+    unknown = C[:, 0] < 0
+    back[unknown, :, :, -1] = rng.uniform(0.0, 0.2, size=(int(unknown.sum()), nz, ny))
+    c = cfg3(nx, ny, nz, periodic=(0, 1, 1))
+    if literal:
+        c.quirks = _abi.LBM_QUIRK_EQ3D_LINEAR
+    fl = np.zeros((nz, ny, nx), np.uint8)
+    fl[:, :, -1] = _abi.REGULARIZED_OUTLET
+    fl[:, :, 0] = _abi.BOUNCE_BACK  # the x = 0 face's stale slots are not part of this check
+    o = O.Oracle3D(c, 64)
+    o.set_pops(f0, back)
+    o.set_flags(fl)
+    o.stream_boundaries()
+    f = o.get_pops()
+    if not literal:
+        np.testing.assert_allclose(f[:, :, :, -1], f0[:, :, :, -1], atol=1e-14)
+    # density and momentum are the neighbour's; the literal equilibrium carries 17/27 of the
+    # momentum (its edge and corner directions' linear coefficient is 1 instead of 3)
+    nb = f[:, :, :, -2]
+    np.testing.assert_allclose(f[:, :, :, -1].sum(0), nb.sum(0), rtol=1e-14)
+    share = 17.0 / 27.0 if literal else 1.0
+    np.testing.assert_allclose(np.einsum("iab,ic->cab", f[:, :, :, -1], C),
+                               share * np.einsum("iab,ic->cab", nb, C), rtol=1e-12, atol=1e-16)
+    # without the functor the garbage stays
+    o2 = O.Oracle3D(c, 64)
+    o2.set_pops(f0, back)
+    fl2 = fl.copy()
+    fl2[:, :, -1] = _abi.FLUID
+    o2.set_flags(fl2)
+    o2.stream_boundaries()
+    assert np.max(np.abs(o2.get_pops()[:, :, :, -1] - f0[:, :, :, -1])) > 1e-3
+
+
+@pytest.mark.parametrize("quirk", [0, 1])
+def test_extrapolated_corner_edge_is_the_inward_node_equilibrium(quirk):
+    """ExtrapolatedCornerEdgeBoundary<3> (cornerBoundaries.cuh:6-36): every boundary node takes the
+    second-order equilibrium of the node one step inward on each of its boundary axes, so its
+    density and momentum are that node's post-stream ones (momentum z from the flat velocity
+    table at 6 + i under LBM_QUIRK_CORNER3D_UZ), and its populations are f_eq of them; a flagged
+    node off the faces is its own inward node and relaxes to its own equilibrium."""
+    nx, ny, nz = 5, 6, 4
+    rng = np.random.default_rng(9)
+    f0 = W[:, None, None, None] * (1 + 0.1 * rng.random((27, nz, ny, nx)))
+    c = cfg3(nx, ny, nz, periodic=(0, 0, 0))
+    c.quirks = _abi.LBM_QUIRK_CORNER3D_UZ if quirk else 0
+
+    def run(fl):
+        o = O.Oracle3D(c, 64)
+        o.set_pops(f0)
+        if fl is not None:
+            o.set_flags(fl)
+        o.stream_boundaries()
+        return o.get_pops()
+
+    fe = run(np.full((nz, ny, nx), _abi.EXTRAPOLATED_CORNER_EDGE, np.uint8))
+    fs = run(None)  # the post-stream state every functor reads
+    flat = np.stack([C[:, 0], C[:, 1], C[:, 2]], 1).reshape(-1)  # [27][3] row-major
+    for z, y, x in itertools.product(range(nz), range(ny), range(nx)):
+        xi = 1 if x == 0 else (nx - 2 if x == nx - 1 else x)
+        yi = 1 if y == 0 else (ny - 2 if y == ny - 1 else y)
+        zi = 1 if z == 0 else (nz - 2 if z == nz - 1 else z)
+        g = fs[:, zi, yi, xi]
+        r = g.sum()
+        u = C.T @ g / r
+        if quirk:
+            u[2] = float(g @ flat[6:33]) / r
+        want = O.equilibrium3d(r, *u)
+        np.testing.assert_allclose(fe[:, z, y, x], want, rtol=1e-13, atol=1e-16)
+        assert abs(fe[:, z, y, x].sum() - r) < 1e-13
+
+
+def delta4(r):
+    a = abs(r)
+    if a < 1:
+        return 0.125 * (3 - 2 * a + math.sqrt(1 + 4 * a - 4 * r * r))
+    if a < 2:
+        return 0.125 * (5 - 2 * a - math.sqrt(-7 + 12 * a - 4 * r * r))
+    return 0.0
+
+
+def test_peskin_kernel_square_sum_is_three_eighths():
+    """The 4-point kernel's sum of squares over its support is 3/8 at every offset (one of its
+    defining moment conditions), so a 4x4x4 stencil has sum phi^2 = (3/8)^3 = 27/512."""
+    for r in np.linspace(0, 1, 17):
+        assert abs(sum(delta4(r - j) ** 2 for j in range(-2, 3)) - 0.375) < 1e-12
+
+
+def ibm3_run(u0, ds, iters, quirks=0, pos=(6.3, 5.7, 6.9), precision=64):
+    n = 13
+    c = cfg3(n, n, n, omega=1.2, periodic=(1, 1, 1), quirks=quirks)
+    o = O.Oracle3D(c, precision)
+    o.init_from_macros(np.full((n, n, n), 1.02), np.broadcast_to(np.array(u0), (n, n, n, 3)).copy())
+    o.ibm_set_markers(np.array([pos]), None if ds is None else np.array([ds]), iters)
+    o.step(1)
+    return o.ibm_get_forces()
+
+
+@pytest.mark.parametrize("ds,iters", [(None, 1), (None, 3), (0.7, 3), (2.0, 2)])
+def test_single_marker_multi_direct_closed_form(ds, iters):
+    """IBMManager<3>::multi_direct (IBMManager.cuh:222-252) for one marker in a uniform flow u0
+    (periodic, rho0 = 1.02, no force): every interpolation is a partition of unity, so iteration k
+    sees u_L = u0 (1 - a)^(k-1) with a = ds * sum phi^2 = ds * 27/512, and the Lagrangian forces sum
+    to -2 rho0 u0 sum_k (1 - a)^(k-1); the force the fluid receives totals that times ds."""
+    u0 = np.array([0.05, -0.03, 0.02])
+    lag, tot = ibm3_run(u0, ds, iters)
+    a = (1.0 if ds is None else ds) * 27.0 / 512.0
+    series = sum((1 - a) ** k for k in range(iters))
+    want = -2 * 1.02 * u0 * series
+    np.testing.assert_allclose(lag[0], want, rtol=1e-12)
+    np.testing.assert_allclose(tot, want * (1.0 if ds is None else ds), rtol=1e-12)
+
+
+def test_single_marker_without_ds_and_with_the_clamp():
+    """LBM_QUIRK_IBM_NO_DS spreads without the area (a = 27/512 whatever ds); the clamp quirk keeps
+    only Lagrangian forces above 1e-8 and zeroes every corrected velocity component at or below
+    1e-8 over the whole domain (IBM_impl.cuh:7-47): with u0 = (0.05, -0.03, 0.02) the first
+    iteration keeps F_y = 0.06 rho0 only, and the corrected u_y < 0 becomes 0 everywhere, so the
+    later iterations add nothing."""
+    u0 = np.array([0.05, -0.03, 0.02])
+    lag, tot = ibm3_run(u0, 0.7, 3, quirks=_abi.LBM_QUIRK_IBM_NO_DS)
+    a = 27.0 / 512.0
+    np.testing.assert_allclose(lag[0], -2 * 1.02 * u0 * (1 + (1 - a) + (1 - a) ** 2), rtol=1e-12)
+    np.testing.assert_allclose(tot, lag[0], rtol=1e-12)
+    lag, tot = ibm3_run(u0, None, 3, quirks=_abi.LBM_QUIRK_IBM_CLAMP)
+    np.testing.assert_allclose(lag[0], [0.0, 2 * 1.02 * 0.03, 0.0], atol=1e-15)
