@@ -123,8 +123,26 @@ LBM3D_REDUCE_MOMENTUM = 1
 LBM3D_REDUCE_KINETIC_ENERGY = 2
 
 
-# (name, restype, argtypes) of every entry point include/lbm_hip.h and include/lbm3d_hip.h
-# declare.
+class LbmBodyConfig(ctypes.Structure):
+    """struct lbm_body_config (include/lbm_body.h)."""
+    _fields_ = [("obj_filepath", ctypes.c_char * 1024), ("save_transformed_obj_to", ctypes.c_char * 1024),
+                ("has", ctypes.c_uint32)] + [(k, ctypes.c_float) for k in (
+                    "scale_overall_size", "scale_x", "scale_y", "scale_z", "rotate_x_deg", "rotate_y_deg",
+                    "rotate_z_deg", "set_anchor_local_x", "set_anchor_local_y", "set_anchor_local_z",
+                    "position_anchor_x", "position_anchor_y", "position_anchor_z")] + [
+                ("sampling_type", ctypes.c_int32), ("target_points", ctypes.c_int32)]
+
+
+# body configuration (include/lbm_body.h)
+LBM_BODY_SAMPLING_DIRECT = 0
+LBM_BODY_SAMPLING_TARGET_POINTS = 1
+LBM_BODY_HAS = {k: 1 << i for i, k in enumerate((
+    "scale_overall_size", "scale_x", "scale_y", "scale_z", "rotate_x_deg", "rotate_y_deg", "rotate_z_deg",
+    "set_anchor_local_x", "set_anchor_local_y", "set_anchor_local_z", "position_anchor_x", "position_anchor_y",
+    "position_anchor_z", "target_points"))}
+
+# (name, restype, argtypes) of every entry point include/lbm_hip.h, include/lbm3d_hip.h and
+# include/lbm_body.h declare.
 _P = ctypes.c_void_p
 _F = ctypes.POINTER(ctypes.c_float)
 _D = ctypes.POINTER(ctypes.c_double)
@@ -172,6 +190,37 @@ PROTOTYPES = [
     ("lbm3d_reduce", _I, [_P, _I, _D]),
     ("lbm3d_ibm_set_markers", _I, [_P, _P, _P, _I, _I]),
     ("lbm3d_ibm_get_forces", _I, [_P, _P, _D]),
+    # include/lbm_body.h
+    ("lbm_body_load_obj", _I, [ctypes.c_char_p, ctypes.POINTER(_P)]),
+    ("lbm_body_from_mesh", _I, [_P, _I, _P, _I, ctypes.POINTER(_P)]),
+    ("lbm_body_destroy", _I, [_P]),
+    ("lbm_body_info", _I, [_P, _P, _P, _P]),
+    ("lbm_body_vertices", _I, [_P, _P]),
+    ("lbm_body_faces", _I, [_P, _P]),
+    ("lbm_body_markers", _I, [_P, _P]),
+    ("lbm_body_translate", _I, [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    ("lbm_body_scale", _I, [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    ("lbm_body_scale_to_overall_size", _I, [_P, ctypes.c_float]),
+    ("lbm_body_rotate", _I, [_P, _I, ctypes.c_float]),
+    ("lbm_body_set_anchor", _I, [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    ("lbm_body_set_anchor_to_centroid", _I, [_P]),
+    ("lbm_body_move_anchor_to_world", _I, [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    ("lbm_body_move_centroid_to_world", _I, [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    ("lbm_body_aabb", _I, [_P, _P, _P]),
+    ("lbm_body_anchor", _I, [_P, _P]),
+    ("lbm_body_fits_in_domain", _I, [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P]),
+    ("lbm_body_save_obj", _I, [_P, ctypes.c_char_p]),
+    ("lbm_body_mesh_volume", _I, [_P, _P]),
+    ("lbm_body_r_max", ctypes.c_float, [ctypes.c_float, _I]),
+    ("lbm_body_sample", _I, [_P, _I]),
+    ("lbm_body_markers_from_vertices", _I, [_P]),
+    ("lbm_body_morton_sort", _I, [_P, _I, _I]),
+    ("lbm_points_morton_sort", _I, [_P, _I, _I, _I, _I]),
+    ("lbm_points_save_csv", _I, [ctypes.c_char_p, _P, _I]),
+    ("lbm_points_load_csv", _I, [ctypes.c_char_p, _P, _I, _P]),
+    ("lbm_body_parse_config", _I, [ctypes.c_char_p, ctypes.POINTER(LbmBodyConfig)]),
+    ("lbm_body_from_config", _I, [ctypes.c_char_p, ctypes.c_char_p, _I, _I, _I, ctypes.POINTER(_P)]),
+    ("lbm_body_last_error", ctypes.c_char_p, [_P]),
     ("lbm3d_last_error", ctypes.c_char_p, [_P]),
 ]
 
