@@ -311,11 +311,13 @@ def run_d3q27(torch, steps=200, warmup=20):
                       -0.05 * np.cos(k * x) * np.sin(k * y) * np.cos(k * z), np.zeros_like(x)], -1)
         return np.ones((n, n, n), np.float32), u.astype(np.float32)
 
-    def one(cfg, init, flags=None, n_steps=steps):
+    def one(cfg, init, flags=None, n_steps=steps, markers=None):
         with Solver3D(cfg) as s:
             s.init_from_macros(*init)
             if flags is not None:
                 s.set_flags(flags)
+            if markers is not None:
+                s.ibm_set_markers(*markers)
             secs, _ = time_steps(s, n_steps, warmup, torch, batches=3)
             B = 2 * 27 * (4 if cfg.storage == A.LBM_F32 else 2)
             gbs = B * cfg.N / (secs / n_steps) / 1e9
@@ -349,6 +351,30 @@ def run_d3q27(torch, steps=200, warmup=20):
     u0[..., 0] = 0.05
     wcfg = Config3D(nx, ny, nz, collision=A.LBM_CM, omega=1.0 / (3.0 * 0.05 * ny / 1000.0 + 0.5), u_inlet=0.05)
     res["wind_tunnel_256x128x96_cm_f32"] = one(wcfg, (np.ones((nz, ny, nx), np.float32), u0), wt)
+    # ... and with its body (windTest/sphere_config.ini): a 64x64 UV-sphere mesh scaled to D = 16
+    # with its anchor at (128, 64, 48), 256 markers by weighted sample elimination in Morton
+    # order (the host pipeline of include/lbm_body.h), area pi D^2 / 256 each, 3 IBM iterations
+    import tempfile
+
+    from lbm_amd import bodies as Bd
+
+    with tempfile.TemporaryDirectory() as td:
+        v, f = Bd.uv_sphere(64, 64)
+        Bd.Body.from_mesh(v, f).collect_file(os.path.join(td, "sphere_unit_64x64.obj"))
+        ini = os.path.join(td, "sphere_config.ini")
+        with open(ini, "w") as fh:
+            fh.write("[General]\nobj_filepath = sphere_unit_64x64.obj\n[Transformations]\nscale_overall_size = 16.0\n"
+                     "position_anchor_x = 128.0\nposition_anchor_y = 64.0\nposition_anchor_z = 48.0\n"
+                     "[Sampling]\ntype = target_points\ntarget_points = 256\n")
+        t0 = time.perf_counter()
+        with Bd.Body.from_config(ini, (nx, ny, nz), base_dir=td) as body:
+            xyz = body.markers
+        prep = time.perf_counter() - t0
+    ds = np.full(len(xyz), np.pi * 16.0 ** 2 / len(xyz), np.float32)
+    r = one(wcfg, (np.ones((nz, ny, nx), np.float32), u0), wt, markers=(xyz, ds, 3))
+    r.update(markers=int(len(xyz)), body_preparation_s=round(prep, 3),
+             ibm_cost_vs_no_body=round(res["wind_tunnel_256x128x96_cm_f32"]["mlups"] / r["mlups"], 3))
+    res["wind_tunnel_sphere_256x128x96_cm_f32"] = r
     # the flags of flowPastSquareCylinderScenario.cuh:10-31 without its body: corner-edge
     # bounce-back on edges and corners, regularized inlet at x = 0, Guo pressure outlet at
     # x = NX-1, bounce-back on y and z (the regularized kernel variant)
@@ -362,8 +388,9 @@ def run_d3q27(torch, steps=200, warmup=20):
     res["square_duct_256x128x96_cm_f32"] = one(wcfg, (np.ones((nz, ny, nx), np.float32), u0), sq)
     res["workload"] = ("D3Q27 BGK with Guo forcing (and MRT<3>, CM<3> at 256^3), FP32 collision; Taylor-Green 3-D "
                        "(u_max 0.05) periodic, a force-driven channel with bounce-back walls on y and z, and the wind "
-                       "tunnel (CM<3>, Guo inlet/outlet, walls) and the square-cylinder duct's flags (corner-edge "
-                       "bounce-back, regularized inlet, Guo outlet, walls) without the body; 216 B "
+                       "tunnel (CM<3>, Guo inlet/outlet, walls) with and without its IBM sphere, and the "
+                       "square-cylinder duct's flags (corner-edge bounce-back, regularized inlet, Guo outlet, walls) "
+                       "without the body; 216 B "
                        "(FP32) / 108 B (FP16) per node update")
     return res
 

@@ -119,9 +119,17 @@ __global__ __launch_bounds__(256) void ibm3_gather_kernel(Ibm3Params p) {
     const int b = blockIdx.x * 256 + threadIdx.x;
     if (b < p.n_band) ibm3_gather<STORE, BUF, GUO>(p, b);
 }
+// Four markers per workgroup, one per wavefront.
 __global__ __launch_bounds__(256) void ibm3_interpolate_kernel(Ibm3Params p) {
-    const int l = blockIdx.x * 256 + threadIdx.x;
-    if (l < p.n_markers) ibm3_interpolate(p, l);
+    const int l = blockIdx.x * 4 + int(threadIdx.x >> 6), lane = int(threadIdx.x & 63);
+    if (l >= p.n_markers) return;  // uniform per wavefront
+    float s[4], t[4];
+    ibm3_interp_partial(p, l, lane, 64, s);
+    for (int off = 32; off > 0; off >>= 1) {
+        for (int c = 0; c < 4; c++) t[c] = __shfl_xor(s[c], off, 64);
+        ibm3_combine(s, t);
+    }
+    if (lane == 0) ibm3_interp_finish(p, l, s);
 }
 __global__ __launch_bounds__(256) void ibm3_spread_kernel(Ibm3Params p) {
     const int b = blockIdx.x * 256 + threadIdx.x;
@@ -311,7 +319,7 @@ int lbm3d_ctx::ibm_pass(const Step3Params& sp, bool fits) {
     q.lag = ibm.lag;
     q.lag_sum = ibm.lag_sum;
     q.force_total = ibm.total;
-    const dim3 gb(grid1(ibm.n_band)), gm(grid1(ibm.n_markers)), blk(256);
+    const dim3 gb(grid1(ibm.n_band)), gm(unsigned((ibm.n_markers + 3) / 4)), blk(256);
     const int G = has_reg ? 2 : (has_guo ? 1 : 0);
 #define LBM3_GATHER(S, B)                                                                                    \
     if (G == 2) hipLaunchKernelGGL((ibm3_gather_kernel<S, B, 2>), gb, blk, 0, stream, q);                   \

@@ -7,9 +7,10 @@
 // for a force that is non-zero on the band alone. Here one pass per step is
 //   gather  (one thread per band node): post-stream populations -> rho, uncorrected u, the
 //           external force;
-//   ITER x  interpolate (one thread per marker) over its stencil in the reference's order, then
-//           spread as a gather (one thread per band node) over a (marker, weight) list in marker
-//           order — deterministic, the reference's scatter order without atomics;
+//   ITER x  interpolate (one wavefront per marker, its 64 stencil entries summed by a fixed
+//           butterfly), then spread as a gather (one thread per band node) over a (marker,
+//           weight) list in marker order — deterministic, the reference's scatter order without
+//           atomics;
 // with the band's fields in three small SoA arrays that stay in L2. The last spread writes the
 // total force of each band node into the step's force array, whose other nodes keep the
 // external force.
@@ -76,20 +77,30 @@ LBM3_HD void ibm3_gather(const Ibm3Params& p, int b) {
 
 // interpolate_velocities_kernel<3> (IBM_impl.cu:53-118) and compute_lagrangian_kernel
 // (IBM_impl.cuh:7-26) for marker l: rho_L, u_L over the stencil, F_L = 2 rho_L (0 - u_L).
-LBM3_HD void ibm3_interpolate(const Ibm3Params& p, int l) {
-    const int nb = p.n_band, n = p.n_markers;
-    float r = 0.0f, ux = 0.0f, uy = 0.0f, uz = 0.0f;
-    for (int e = p.mk_ptr[l]; e < p.mk_ptr[l + 1]; e++) {
+// A wavefront per marker: lane `lane` of `nlanes` sums the stencil entries lane, lane + nlanes,
+// ... into s[4] (rho, u_x, u_y, u_z); the lanes' sums are then combined by the butterfly
+// ibm3_combine (the same fixed order on the GPU and in the host emulation) and lane 0
+// finishes the marker.
+LBM3_HD void ibm3_interp_partial(const Ibm3Params& p, int l, int lane, int nlanes, float* s) {
+    const int nb = p.n_band;
+    s[0] = s[1] = s[2] = s[3] = 0.0f;
+    for (int e = p.mk_ptr[l] + lane; e < p.mk_ptr[l + 1]; e += nlanes) {
         const float k = p.mk_w[e];
         const int b = p.mk_band[e];
-        r += k * p.brho[b];
-        ux += k * p.bu[b];
-        uy += k * p.bu[nb + b];
-        uz += k * p.bu[2 * nb + b];
+        s[0] += k * p.brho[b];
+        s[1] += k * p.bu[b];
+        s[2] += k * p.bu[nb + b];
+        s[3] += k * p.bu[2 * nb + b];
     }
-    const float ul[3] = {ux, uy, uz};
+}
+// One butterfly level: every lane adds the sums of lane ^ offset (offsets 32, 16, ..., 1).
+LBM3_HD void ibm3_combine(float* s, const float* partner) {
+    for (int c = 0; c < 4; c++) s[c] += partner[c];
+}
+LBM3_HD void ibm3_interp_finish(const Ibm3Params& p, int l, const float* s) {
+    const int n = p.n_markers;
     for (int d = 0; d < 3; d++) {
-        float fl = 2.0f * r * (0.0f - ul[d]);
+        float fl = 2.0f * s[0] * (0.0f - s[1 + d]);
         if (p.clamp) fl = fl > 1e-8f ? fl : 0.0f;
         p.lag[d * n + l] = fl;
         p.lag_sum[d * n + l] = p.first ? fl : p.lag_sum[d * n + l] + fl;

@@ -209,22 +209,12 @@ LBM3_HD void apply_guo(const Step3Params& p, const lbm::PopBuf<typename Store3<S
 }
 
 // ZG_OutflowBoundary<3>::apply (zeroGradientOutflow.cuh:8-59): the first face in the order
-// x = 0, x = NX-1, y = 0, y = NY-1, z = 0, z = NZ-1 gives the inward normal (axis D, sign S);
-// the populations entering the domain (c_D S > 0) take the post-stream values of the node one
-// step inward, the others keep the node's own; a node on no face is left as it is.
-template <int STORE, bool BUF, int D, int S, int q>
-LBM3_HD void zg_dir(const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src, float* g, int xn, int yn,
-                    int zn) {
-    constexpr int cn = (D == 0 ? CX[q] : (D == 1 ? CY[q] : CZ[q])) * S;
-    if constexpr (cn > 0) g[q] = pull_at<STORE, BUF, q>(p, src, xn, yn, zn);
-}
-template <int STORE, bool BUF, int D, int S, int... qs>
-LBM3_HD void zg_all(const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src, float* g, int x, int y,
-                    int z, std::integer_sequence<int, qs...>) {
-    const int xn = x + (D == 0 ? S : 0), yn = y + (D == 1 ? S : 0), zn = z + (D == 2 ? S : 0);
-    (zg_dir<STORE, BUF, D, S, qs>(p, src, g, xn, yn, zn), ...);
-}
-// The face of a ZG node: 0..5 for x = 0, x = NX-1, y = 0, y = NY-1, z = 0, z = NZ-1, -1 inside.
+// x = 0, x = NX-1, y = 0, y = NY-1, z = 0, z = NZ-1 gives the inward normal n (axis D, sign S);
+// the populations entering the domain (c . n > 0) take the post-stream values of the node one
+// step inward, the others keep the node's own; a node on no face is left as it is. One code
+// path for every face: direction q is replaced when its component along the face's axis has
+// the normal's sign (a per-direction conditional pull into a fixed element of g, so g stays in
+// registers).
 LBM3_HD int zg_face(const Step3Params& p, int x, int y, int z) {
     if (x == 0) return 0;
     if (x == p.nx - 1) return 1;
@@ -234,19 +224,25 @@ LBM3_HD int zg_face(const Step3Params& p, int x, int y, int z) {
     if (z == p.nz - 1) return 5;
     return -1;
 }
+template <int STORE, bool BUF, int q>
+LBM3_HD void zg_dir(const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src, float* g, int d, int s,
+                    int xn, int yn, int zn) {
+    const int c = d == 0 ? CX[q] : (d == 1 ? CY[q] : CZ[q]);
+    if (c * s > 0) g[q] = pull_at<STORE, BUF, q>(p, src, xn, yn, zn);
+}
+template <int STORE, bool BUF, int... qs>
+LBM3_HD void zg_all(const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src, float* g, int d, int s,
+                    int xn, int yn, int zn, std::integer_sequence<int, qs...>) {
+    (zg_dir<STORE, BUF, qs>(p, src, g, d, s, xn, yn, zn), ...);
+}
 template <int STORE, bool BUF>
 LBM3_HD void apply_zg(const Step3Params& p, const lbm::PopBuf<typename Store3<STORE>::T, BUF>& src, float* g, int x, int y,
                       int z) {
-    constexpr auto qs = std::make_integer_sequence<int, Q>{};
-    switch (zg_face(p, x, y, z)) {
-        case 0: zg_all<STORE, BUF, 0, 1>(p, src, g, x, y, z, qs); break;
-        case 1: zg_all<STORE, BUF, 0, -1>(p, src, g, x, y, z, qs); break;
-        case 2: zg_all<STORE, BUF, 1, 1>(p, src, g, x, y, z, qs); break;
-        case 3: zg_all<STORE, BUF, 1, -1>(p, src, g, x, y, z, qs); break;
-        case 4: zg_all<STORE, BUF, 2, 1>(p, src, g, x, y, z, qs); break;
-        case 5: zg_all<STORE, BUF, 2, -1>(p, src, g, x, y, z, qs); break;
-        default: break;
-    }
+    const int f = zg_face(p, x, y, z);
+    if (f < 0) return;
+    const int d = f >> 1, s = (f & 1) ? -1 : 1;
+    zg_all<STORE, BUF>(p, src, g, d, s, x + (d == 0 ? s : 0), y + (d == 1 ? s : 0), z + (d == 2 ? s : 0),
+                       std::make_integer_sequence<int, Q>{});
 }
 
 // Density and velocity sums of the post-stream populations of node (xn, yn, zn); uzq the z
@@ -445,7 +441,38 @@ LBM3_HD void step3_node(const Step3Params& p, int x, int y, int z) {
     const uint32_t node = uint32_t(z * (p.nx * p.ny) + y * p.nx + x);
     const lbm::PopBuf<T, BUF> src(p.src), dst(p.dst);
     float g[Q];
-    post_stream3<STORE, BUF, GUO>(p, src, x, y, z, node, g);
+    if constexpr (GUO >= 2) {
+        post_stream3<STORE, BUF, GUO>(p, src, x, y, z, node, g);
+    } else {
+        // post_stream3 for GUO <= 1, written out: through the call the compiler kept g in
+        // scratch (16 B plain, 112 B Guo variant) and the Guo kernel ran 1.7x slower; written
+        // out, both keep g in registers. The regularized variant is the other way round (40 B
+        // through the call, 160 B written out).
+        const int nxy = p.nx * p.ny;
+        int ox[3], oy[3], oz[3];
+        bool vx[3], vy[3], vz[3];
+        axis_pull(x, p.nx, p.px, 1, ox, vx);
+        axis_pull(y, p.ny, p.py, p.nx, oy, vy);
+        axis_pull(z, p.nz, p.pz, nxy, oz, vz);
+        const bool lane_in = vx[0] && vx[2] && vy[0] && vy[2] && vz[0] && vz[2];
+        if (wave_all(lane_in))
+            pull_all<STORE, BUF, true>(g, p, src, node, ox, vx, oy, vy, oz, vz, std::make_integer_sequence<int, Q>{});
+        else
+            pull_all<STORE, BUF, false>(g, p, src, node, ox, vx, oy, vy, oz, vz, std::make_integer_sequence<int, Q>{});
+        const uint8_t fl = p.flags ? uint8_t(p.flags[node] & ~kPreLaunch) : uint8_t(LBM_FLUID);
+        if (fl == LBM_BOUNCE_BACK) {
+            const Box3 B{p.nx, p.ny, p.nz, p.px, p.py, p.pz};
+            bc_bounce_back3(g, x, y, z, B);
+        } else if (fl == LBM_CORNER_EDGE_BOUNCE_BACK) {
+            const Box3 B{p.nx, p.ny, p.nz, p.px, p.py, p.pz};
+            bc_corner_edge_bounce_back3(g, x, y, z, B);
+        }
+        if constexpr (GUO == 1) {
+            if (fl == LBM_GUO_VELOCITY_INLET) apply_guo<STORE, BUF, true>(p, src, g, x, y, z);
+            else if (fl == LBM_GUO_PRESSURE_OUTLET) apply_guo<STORE, BUF, false>(p, src, g, x, y, z);
+            else if (fl == LBM_ZG_OUTFLOW) apply_zg<STORE, BUF>(p, src, g, x, y, z);
+        }
+    }
     float fx, fy, fz;
     node_force(p, x, y, z, node, fx, fy, fz);
     const Macro3 m = macroscopics3(g, fx, fy, fz);

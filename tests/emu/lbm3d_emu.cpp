@@ -149,7 +149,18 @@ int run3(const lbm3d_config& c, const float* f_in, const uint8_t* flags, const f
             for (int it = 0; it < mk.iters; it++) {
                 q.first = it == 0;
                 q.last = it == mk.iters - 1;
-                for (int l = 0; l < mk.n; l++) ibm3_interpolate(q, l);
+                for (int l = 0; l < mk.n; l++) {  // the wavefront's lanes and butterfly, lane by lane
+                    float sums[64][4], next[64][4];
+                    for (int lane = 0; lane < 64; lane++) ibm3_interp_partial(q, l, lane, 64, sums[lane]);
+                    for (int off = 32; off > 0; off >>= 1) {
+                        for (int lane = 0; lane < 64; lane++) {
+                            for (int c = 0; c < 4; c++) next[lane][c] = sums[lane][c];
+                            ibm3_combine(next[lane], sums[lane ^ off]);
+                        }
+                        std::memcpy(sums, next, sizeof sums);
+                    }
+                    ibm3_interp_finish(q, l, sums[0]);
+                }
                 for (int b = 0; b < q.n_band; b++) ibm3_spread(q, b);
             }
             p.force = total.data();

@@ -457,3 +457,48 @@ def test_gpu_ibm3_markers_removed_and_errors(gpu_available):
         with pytest.raises(LbmError) as e:
             s.ibm_set_markers(xyz + 100.0, ds)
         assert e.value.code == A.LBM_E_INVALID
+
+
+def test_gpu_sphere_body_from_a_config_file_in_the_wind_tunnel(gpu_available, tmp_path):
+    """The reference's wind-tunnel body path at reduced size: a UV-sphere OBJ scaled to D = 8
+    and placed at (16, 12, 12) by a body file with 96 target points (Morton order), in a CM<3>
+    channel with Guo inlet/outlet and walls. The GPU matches the oracle over 6 steps; after 300
+    steps the force the markers apply to the fluid points upstream (drag)."""
+    from lbm_amd import bodies as Bd
+
+    v, f = Bd.uv_sphere(24, 24)
+    Bd.Body.from_mesh(v, f).collect_file(tmp_path / "sphere.obj")
+    (tmp_path / "sphere.ini").write_text(
+        "[General]\nobj_filepath = sphere.obj\n[Transformations]\nscale_overall_size = 8.0\n"
+        "position_anchor_x = 16.0\nposition_anchor_y = 12.0\nposition_anchor_z = 12.0\n"
+        "[Sampling]\ntype = target_points\ntarget_points = 96\n")
+    nx, ny, nz = 48, 24, 24
+    with Bd.Body.from_config(tmp_path / "sphere.ini", (nx, ny, nz), base_dir=tmp_path) as b:
+        xyz = b.markers
+    assert xyz.shape == (96, 3)
+    fl = np.zeros((nz, ny, nx), np.uint8)
+    fl[:, 0, :] = fl[:, -1, :] = A.BOUNCE_BACK
+    fl[0], fl[-1] = A.BOUNCE_BACK, A.BOUNCE_BACK
+    fl[:, :, 0] = A.GUO_VELOCITY_INLET
+    fl[:, :, -1] = A.GUO_PRESSURE_OUTLET
+    c = make_cfg((nx, ny, nz), (0, 0, 0), 1.0 / (3 * 0.05 * ny / 100.0 + 0.5), collision=A.LBM_CM)
+    c.u_inlet = 0.05
+    f0 = np.empty((27, nz, ny, nx), np.float32)
+    f0[:] = O.equilibrium3d(1.0, 0.05, 0.0, 0.0)[:, None, None, None]
+    ds = np.full(96, np.pi * 8.0 ** 2 / 96, np.float32)
+    o = oracle_run_ibm(c, f0, fl, None, xyz, ds, 3, 6)
+    with Solver3D(gpu_config(c)) as s:
+        s.set_pops(f0)
+        s.set_flags(fl)
+        s.ibm_set_markers(xyz, ds, 3)
+        s.step(6)
+        assert np.abs(s.get_pops() - o.get_pops()).max() < 1e-5
+        lag, tot = s.ibm_get_forces()
+        assert np.abs(lag - o.ibm_get_forces()[0]).max() < 1e-5
+        s.step(294)
+        lag, tot = s.ibm_get_forces()
+        rho, u = s.get_macros()
+    assert np.isfinite(u).all()
+    assert tot[0] < 0 and abs(tot[0]) > 5 * max(abs(tot[1]), abs(tot[2]))
+    # the wake: slower flow just behind the sphere than beside it
+    assert u[12, 12, 22, 0] < u[12, 5, 22, 0]
