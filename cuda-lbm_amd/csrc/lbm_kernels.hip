@@ -99,7 +99,8 @@ void fused_step2s_kernel(StepParams p, long long nb_padded, int remap, int strip
     const int sy = int(lb / strips_x), sx = int(lb - (long long)sy * strips_x);
     const int x0 = sx * kT2W, y0 = sy * H;
     const int hs = p.ny - y0 < H ? p.ny - y0 : H;  // output rows of this strip
-    const int l = threadIdx.x & (kT2L - 1), w = threadIdx.x / kT2L;
+    // the wavefront's row: uniform, so the ring slots and row tests below are scalar work
+    const int l = threadIdx.x & (kT2L - 1), w = __builtin_amdgcn_readfirstlane(int(threadIdx.x / kT2L));
     const int K = (hs + 1) / W + 1;  // chunks: phase-A rows 0 .. hs + 1, phase-B rows 0 .. hs - 1
     const bool interior_x = interior_cols(p, x0);
     int i = w, gx = 0, gy = 0;
@@ -152,7 +153,7 @@ void fused_step_ahead_kernel(StepParams p, long long nb_padded, int remap, int s
     const int sy = int(lb / strips_x), sx = int(lb - (long long)sy * strips_x);
     const int x0 = sx * kT2W, y0 = sy * H;
     const int hs = p.ny - y0 < H ? p.ny - y0 : H;
-    const int l = threadIdx.x & (kT2L - 1), w = threadIdx.x / kT2L;
+    const int l = threadIdx.x & (kT2L - 1), w = __builtin_amdgcn_readfirstlane(int(threadIdx.x / kT2L));
     const int K = (hs + 1) / W + 1;
     const bool interior_x = interior_cols(p, x0);
     int i = w, gx = 0, gy = 0;
