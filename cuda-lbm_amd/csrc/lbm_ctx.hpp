@@ -40,6 +40,13 @@ struct IbmState {
     int n_ent = 0;                         // stencil (= spread) table entries
     size_t shmem = 0;                      // LDS of a pass reading its tables from memory
     size_t shmem_tables = 0;               // LDS of a pass with the tables staged, or 0 if too large
+    int* d_mkf_band = nullptr;             // fixed-width lists, slot-major (lbm_ibm_tables.hpp)
+    float* d_mkf_w = nullptr;
+    int* d_bnf_mk = nullptr;
+    float* d_bnf_w = nullptr;
+    int kspread = 0;                       // spread slots per band entry (8 or 16)
+    bool fast = false;                     // the pass keeps its lists in registers
+    size_t shmem_fast = 0;                 // its LDS
     std::vector<float> h_ds;               // arc lengths (host copy, for the force sum)
 };
 

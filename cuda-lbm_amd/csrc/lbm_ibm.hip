@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <mutex>
 #include <vector>
 
@@ -45,6 +46,10 @@ struct IbmParams {
     const int* box_map;           // band index or -1 per box cell
     float* force_total;           // AoS [N][2]
     float* lag_force;             // AoS [n][2], summed over iterations
+    const int* mkf_band;          // fixed-width lists, slot-major (lbm_ibm_tables.hpp)
+    const float* mkf_w;
+    const int* bnf_mk;
+    const float* bnf_w;
 };
 
 // TABLES_LDS: the interpolation and spreading tables are staged in LDS once per pass, so
@@ -99,7 +104,7 @@ __global__ __launch_bounds__(kIbmBlock) void ibm_kernel(IbmParams p) {
         int node = p.band_node[b];
         s_node[b] = node;
         int y = node / nx, x = node - y * nx;
-        Pop9 g = gather_node<STORE>(p.na, x, y);
+        Pop9 g = gather_node_inl<STORE>(p.na, x, y);
         float r = 0.f, jx = 0.f, jy = 0.f;
 #pragma unroll
         for (int i = 0; i < Q; i++) {
@@ -203,6 +208,177 @@ __global__ __launch_bounds__(kIbmBlock) void ibm_kernel(IbmParams p) {
     }
 }
 
+// The same pass with every list in registers: thread t owns marker t (its kIbmSlots stencil
+// slots) and band entries t + r * kIbmBlock, r < NBR (their K spread slots, velocity, density
+// and accumulated force). The lists are read once per pass, the iterations touch LDS only for
+// what other threads read (band velocities and densities, Lagrangian forces), and every slot
+// loop is unrolled, so its LDS reads are issued together rather than one dependent pair per
+// entry. The sums run over the same entries in the same order as ibm_kernel's (padding slots
+// add +0), so both kernels give the same forces. Needs n_markers <= kIbmBlock and
+// n_band <= NBR * kIbmBlock.
+template <int STORE, int K, int NBR>
+__global__ __launch_bounds__(kIbmBlock) void ibm_fast_kernel(IbmParams p) {
+    // band entry b: (rho, ux, uy, -) at s_mac[b]; marker l: its force of the iteration at s_lag[l]
+    extern __shared__ float4 lds4[];
+    float4* s_mac = lds4;
+    float2* s_lag = reinterpret_cast<float2*>(s_mac + p.n_band);
+    const int nb = p.n_band, nm = p.n_markers, t = threadIdx.x;
+    const int nx = p.na.nx;
+
+    // the lists, as LDS element pointers (one register per slot, nothing left to derive from
+    // them inside the iterations), loaded first: their loads overlap the band gather
+    const bool has_mk = t < nm;
+    const float4* mb[kIbmSlots];
+    float mw[kIbmSlots];
+#pragma unroll
+    for (int k = 0; k < kIbmSlots; k++) {
+        mb[k] = s_mac + (has_mk ? p.mkf_band[k * nm + t] : 0);
+        mw[k] = has_mk ? p.mkf_w[k * nm + t] : 0.0f;
+    }
+    const float2* bm[NBR][K];
+    float bw[NBR][K];
+#pragma unroll
+    for (int r = 0; r < NBR; r++) {
+        const int b = t + r * kIbmBlock;
+        const bool has = b < nb;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            bm[r][k] = s_lag + (has ? p.bnf_mk[k * nb + b] : 0);
+            bw[r][k] = has ? p.bnf_w[k * nb + b] : 0.0f;
+        }
+    }
+    // post-stream macroscopics of the band (uncorrected_macroscopics_kernel<2>,
+    // macroscopics.cu:5-38) and the external force (reset_forces, macroscopics.cuh:13-48)
+    int node[NBR];
+    float rho[NBR], ux[NBR], uy[NBR], fxa[NBR], fya[NBR];
+#pragma unroll
+    for (int r = 0; r < NBR; r++) {
+        const int b = t + r * kIbmBlock;
+        rho[r] = 1.0f;
+        ux[r] = uy[r] = fxa[r] = fya[r] = 0.0f;
+        node[r] = 0;
+        if (b < nb) {
+            node[r] = p.band_node[b];
+            const int y = node[r] / nx, x = node[r] - y * nx;
+            Pop9 g = gather_node_inl<STORE>(p.na, x, y);
+            float rr = 0.f, jx = 0.f, jy = 0.f;
+#pragma unroll
+            for (int i = 0; i < Q; i++) {
+                rr += g.v[i];
+                jx += g.v[i] * float(CX[i]);
+                jy += g.v[i] * float(CY[i]);
+            }
+            const float inv = 1.0f / rr;
+            rho[r] = rr;
+            ux[r] = jx * inv;
+            uy[r] = jy * inv;
+            s_mac[b] = make_float4(rr, ux[r], uy[r], 0.0f);
+            fxa[r] = p.fx;
+            fya[r] = p.fy;
+            if (p.force_field) {
+                fxa[r] = p.force_field[2LL * node[r]];
+                fya[r] = p.force_field[2LL * node[r] + 1];
+            }
+        }
+    }
+    __syncthreads();
+
+    float sx = 0.0f, sy = 0.0f;
+    for (int it = 0; it < p.iters; it++) {
+        // interpolate_velocities_kernel<2> + compute_lagrangian_kernel (IBM_impl.cu:5-51,
+        // IBM_impl.cuh:7-26)
+        if (has_mk) {
+            float r = 0.f, vxs = 0.f, vys = 0.f;
+#pragma unroll
+            for (int k = 0; k < kIbmSlots; k++) {
+                const float4 m = *mb[k];
+                r += mw[k] * m.x;
+                vxs += mw[k] * m.y;
+                vys += mw[k] * m.z;
+            }
+            float lx = 2.0f * r * (0.0f - vxs), ly = 2.0f * r * (0.0f - vys);
+            if (p.clamp) {
+                lx = lx > 1e-8f ? lx : 0.0f;
+                ly = ly > 1e-8f ? ly : 0.0f;
+            }
+            s_lag[t] = make_float2(lx, ly);
+            sx += lx;
+            sy += ly;
+        }
+        __syncthreads();
+        // spread_forces_kernel<2> (IBM_impl.cu:120-154) as a gather, then
+        // correct_velocities_kernel and accumulate_forces_kernel (IBM_impl.cuh:28-68)
+#pragma unroll
+        for (int r = 0; r < NBR; r++) {
+            const int b = t + r * kIbmBlock;
+            if (b < nb) {
+                float fx = 0.f, fy = 0.f;
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const float2 l = *bm[r][k];
+                    fx += bw[r][k] * l.x;
+                    fy += bw[r][k] * l.y;
+                }
+                float cx = ux[r] + fx / (2.0f * rho[r]);
+                float cy = uy[r] + fy / (2.0f * rho[r]);
+                if (p.clamp) {
+                    cx = cx > 1e-8f ? cx : 0.0f;
+                    cy = cy > 1e-8f ? cy : 0.0f;
+                }
+                ux[r] = cx;
+                uy[r] = cy;
+                s_mac[b].y = cx;
+                s_mac[b].z = cy;
+                fxa[r] += fx;
+                fya[r] += fy;
+            }
+        }
+        __syncthreads();
+    }
+
+    if (has_mk) {
+        p.lag_force[2 * t] = sx;
+        p.lag_force[2 * t + 1] = sy;
+    }
+    if (!p.write_box) {
+#pragma unroll
+        for (int r = 0; r < NBR; r++) {
+            if (t + r * kIbmBlock < nb) {
+                p.force_total[2LL * node[r]] = fxa[r];
+                p.force_total[2LL * node[r] + 1] = fya[r];
+            }
+        }
+        return;
+    }
+    // every box cell: band entries from LDS (the iterations are over), the rest the external force
+#pragma unroll
+    for (int r = 0; r < NBR; r++) {
+        const int b = t + r * kIbmBlock;
+        if (b < nb) s_mac[b] = make_float4(fxa[r], fya[r], 0.0f, 0.0f);
+    }
+    __syncthreads();
+    const int ncell = p.bw * p.bh;
+    for (int c = t; c < ncell; c += kIbmBlock) {
+        const int yy = p.y0 + c / p.bw, xx = p.x0 + c % p.bw;
+        const long long nd = (long long)yy * nx + xx;
+        const int b = p.box_map[c];
+        float fx, fy;
+        if (b >= 0) {
+            fx = s_mac[b].x;
+            fy = s_mac[b].y;
+        } else {
+            fx = p.fx;
+            fy = p.fy;
+            if (p.force_field) {
+                fx = p.force_field[2 * nd];
+                fy = p.force_field[2 * nd + 1];
+            }
+        }
+        p.force_total[2 * nd] = fx;
+        p.force_total[2 * nd + 1] = fy;
+    }
+}
+
 template <class T>
 int put(lbm_ctx* c, T** dst, const std::vector<T>& v, const char* what) {
     size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
@@ -239,7 +415,8 @@ bool allow_dynamic_lds(int max_bytes) {
 void lbm_ctx::ibm_free() {
     ibm_graph_free();
     void* ptrs[] = {ibm.d_force_total, ibm.d_xy, ibm.d_ds, ibm.d_band_node, ibm.d_mk_ptr, ibm.d_mk_band, ibm.d_mk_w,
-                    ibm.d_bn_ptr, ibm.d_bn_mk, ibm.d_bn_w, ibm.d_lag_force, ibm.d_force_sum, ibm.d_box_map};
+                    ibm.d_bn_ptr, ibm.d_bn_mk, ibm.d_bn_w, ibm.d_lag_force, ibm.d_force_sum, ibm.d_box_map,
+                    ibm.d_mkf_band, ibm.d_mkf_w, ibm.d_bnf_mk, ibm.d_bnf_w};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     ibm = IbmState();
@@ -273,6 +450,34 @@ int lbm_ctx::ibm_pass(int src_idx) {
     p.lag_force = ibm.d_lag_force;
     p.write_box = ibm.box_valid ? 0 : 1;
     p.n_ent = ibm.n_ent;
+    p.mkf_band = ibm.d_mkf_band;
+    p.mkf_w = ibm.d_mkf_w;
+    p.bnf_mk = ibm.d_bnf_mk;
+    p.bnf_w = ibm.d_bnf_w;
+    if (ibm.fast) {
+        if (ibm.shmem_fast != size_t(ibm.n_band) * sizeof(float4) + size_t(ibm.n_markers) * sizeof(float2))
+            return fail(LBM_E_STATE, "IBM pass LDS size inconsistent with its band");
+        const void* k = nullptr;
+        const int st = cfg.storage == LBM_F16_SHIFTED ? 1 : (cfg.storage == LBM_BF16_SHIFTED ? 2 : 0);
+        const int v = ibm.kspread == 16 ? 2 : (ibm.n_band > kIbmBlock ? 1 : 0);
+        static const void* const fast[3][3] = {
+            {reinterpret_cast<const void*>(&ibm_fast_kernel<LBM_F32, 8, 1>),
+             reinterpret_cast<const void*>(&ibm_fast_kernel<LBM_F32, 8, 2>),
+             reinterpret_cast<const void*>(&ibm_fast_kernel<LBM_F32, 16, 1>)},
+            {reinterpret_cast<const void*>(&ibm_fast_kernel<LBM_F16_SHIFTED, 8, 1>),
+             reinterpret_cast<const void*>(&ibm_fast_kernel<LBM_F16_SHIFTED, 8, 2>),
+             reinterpret_cast<const void*>(&ibm_fast_kernel<LBM_F16_SHIFTED, 16, 1>)},
+            {reinterpret_cast<const void*>(&ibm_fast_kernel<LBM_BF16_SHIFTED, 8, 1>),
+             reinterpret_cast<const void*>(&ibm_fast_kernel<LBM_BF16_SHIFTED, 8, 2>),
+             reinterpret_cast<const void*>(&ibm_fast_kernel<LBM_BF16_SHIFTED, 16, 1>)}};
+        k = fast[st][v];
+        void* args[] = {&p};
+        hipError_t e = hipLaunchKernel(k, dim3(1), dim3(kIbmBlock), args, ibm.shmem_fast, stream);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "IBM launch");
+        ibm.box_valid = true;
+        return LBM_OK;
+    }
     const bool tl = ibm.shmem_tables > 0;
     size_t sh = tl ? ibm.shmem_tables : ibm.shmem;
     if (cfg.storage == LBM_F16_SHIFTED) {
@@ -326,6 +531,15 @@ int lbm_ibm_set_markers(lbm_ctx* ctx, const float* xy, const float* ds, int32_t 
     rc = rc ? rc : put(ctx, &I.d_bn_mk, T.bn_mk, "IBM spread lists");
     rc = rc ? rc : put(ctx, &I.d_bn_w, T.bn_w, "IBM spread weights");
     rc = rc ? rc : put(ctx, &I.d_box_map, T.box_map, "IBM box map");
+    // the register-resident pass: one marker per thread, at most two band entries per thread
+    // (eight spread slots each) or one (sixteen)
+    const bool fast = T.kspread > 0 && n <= kIbmBlock && nb <= (T.kspread == 8 ? 2 : 1) * kIbmBlock;
+    if (fast) {
+        rc = rc ? rc : put(ctx, &I.d_mkf_band, T.mkf_band, "IBM stencils");
+        rc = rc ? rc : put(ctx, &I.d_mkf_w, T.mkf_w, "IBM weights");
+        rc = rc ? rc : put(ctx, &I.d_bnf_mk, T.bnf_mk, "IBM spread lists");
+        rc = rc ? rc : put(ctx, &I.d_bnf_w, T.bnf_w, "IBM spread weights");
+    }
     rc = rc ? rc : ctx->dev_alloc(reinterpret_cast<void**>(&I.d_force_total), ctx->N * 8, "IBM force field");
     rc = rc ? rc : ctx->dev_alloc(reinterpret_cast<void**>(&I.d_lag_force), size_t(n) * 8, "IBM Lagrangian forces");
     if (rc) {
@@ -347,6 +561,9 @@ int lbm_ibm_set_markers(lbm_ctx* ctx, const float* xy, const float* ds, int32_t 
     // the tables beside the band data: mk_ptr, bn_ptr, mk_band, bn_mk (ints), mk_w, bn_w
     const size_t tables = (size_t(n) + 1 + size_t(nb) + 1 + 4 * T.mk_band.size()) * 4;
     I.shmem_tables = shmem + tables <= size_t(maxsh) && allow_dynamic_lds(maxsh) ? shmem + tables : 0;
+    I.fast = fast && std::getenv("LBM_IBM_CSR") == nullptr;
+    I.kspread = T.kspread;
+    I.shmem_fast = size_t(nb) * sizeof(float4) + size_t(n) * sizeof(float2);
     I.h_ds = dsv;
     I.active = true;
     return LBM_OK;

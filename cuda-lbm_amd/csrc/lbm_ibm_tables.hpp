@@ -11,7 +11,12 @@
 //   - bn_ptr / bn_mk / bn_w: CSR band entry -> (marker, phi phi ds) in marker order, the
 //     spreading gather whose sum order equals the reference's scatter order;
 //   - box_map: band entry or -1 per cell of the band's bounding box, whose x range is widened
-//     to whole 4-node segments (so one-step launches can split at it).
+//     to whole 4-node segments (so one-step launches can split at it);
+//   - mkf_band / mkf_w and bnf_mk / bnf_w: the same two lists at a fixed width, for a pass that
+//     keeps them in registers: kIbmSlots slots per marker and `kspread` (8 or 16) per band entry,
+//     stored slot-major ([slot][marker], [slot][band entry]); each list's entries in the CSR
+//     order, then padding slots (index 0, weight 0) that add nothing to the sums. kspread is 0
+//     when a band entry is touched by more than 16 markers (then only the CSR lists exist).
 // Every index is checked before the function returns success; the device code forms no
 // index that does not come from these tables.
 #pragma once
@@ -31,10 +36,15 @@ inline float delta4f(float r) {
     return 0.0f;
 }
 
+constexpr int kIbmSlots = 16;  // stencil slots per marker in the fixed-width lists (4x4)
+
 struct IbmTables {
     std::vector<int> band_node, mk_ptr, mk_band, bn_ptr, bn_mk, box_map;
     std::vector<float> mk_w, bn_w;
     int x0 = 0, x1 = 0, y0 = 0, y1 = 0;  // bounding box of the band, exclusive ends
+    int kspread = 0;                      // spread slots per band entry in the fixed-width lists
+    std::vector<int> mkf_band, bnf_mk;
+    std::vector<float> mkf_w, bnf_w;
 };
 
 // Builds the tables for n markers at xy (AoS [n][2]) with arc lengths ds (or null: 1) on an
@@ -102,6 +112,30 @@ inline const char* ibm_build_tables(int nx, int ny, bool two_by_two, bool no_ds,
         }
         T.bn_ptr[size_t(b) + 1] = int(T.bn_mk.size());
     }
+    // fixed-width copies of both lists
+    int longest = 0;
+    for (int b = 0; b < nb; b++) longest = std::max(longest, T.bn_ptr[size_t(b) + 1] - T.bn_ptr[size_t(b)]);
+    T.kspread = longest <= 8 ? 8 : (longest <= 16 ? 16 : 0);
+    T.mkf_band.assign(size_t(kIbmSlots) * size_t(n), 0);
+    T.mkf_w.assign(size_t(kIbmSlots) * size_t(n), 0.0f);
+    for (int l = 0; l < n; l++) {
+        if (T.mk_ptr[size_t(l) + 1] - T.mk_ptr[size_t(l)] > kIbmSlots) return "IBM stencil longer than its slots";
+        for (int e = T.mk_ptr[size_t(l)]; e < T.mk_ptr[size_t(l) + 1]; e++) {
+            const size_t slot = size_t(e - T.mk_ptr[size_t(l)]) * size_t(n) + size_t(l);
+            T.mkf_band[slot] = T.mk_band[size_t(e)];
+            T.mkf_w[slot] = T.mk_w[size_t(e)];
+        }
+    }
+    if (T.kspread) {
+        T.bnf_mk.assign(size_t(T.kspread) * size_t(nb), 0);
+        T.bnf_w.assign(size_t(T.kspread) * size_t(nb), 0.0f);
+        for (int b = 0; b < nb; b++)
+            for (int e = T.bn_ptr[size_t(b)]; e < T.bn_ptr[size_t(b) + 1]; e++) {
+                const size_t slot = size_t(e - T.bn_ptr[size_t(b)]) * size_t(nb) + size_t(b);
+                T.bnf_mk[slot] = T.bn_mk[size_t(e)];
+                T.bnf_w[slot] = T.bn_w[size_t(e)];
+            }
+    }
     xmin = xmin / 4 * 4;
     xmax = std::min(nx, (xmax + 1 + 3) / 4 * 4) - 1;
     const int bw = xmax - xmin + 1, bh = ymax - ymin + 1;
@@ -127,6 +161,13 @@ inline const char* ibm_build_tables(int nx, int ny, bool two_by_two, bool no_ds,
         return "IBM table sizes inconsistent";
     for (int v : T.box_map)
         if (v < -1 || v >= nb) return "IBM box map out of range";
+    for (int v : T.mkf_band)
+        if (v < 0 || v >= nb) return "IBM fixed-width stencil entry out of range";
+    for (int v : T.bnf_mk)
+        if (v < 0 || v >= n) return "IBM fixed-width spread entry out of range";
+    if (T.mkf_band.size() != size_t(kIbmSlots) * size_t(n) || T.mkf_w.size() != T.mkf_band.size() ||
+        T.bnf_mk.size() != size_t(T.kspread) * size_t(nb) || T.bnf_w.size() != T.bnf_mk.size())
+        return "IBM fixed-width table sizes inconsistent";
     if (T.x0 < 0 || T.y0 < 0 || T.x1 > nx || T.y1 > ny || T.x0 >= T.x1 || T.y0 >= T.y1)
         return "IBM box outside the grid";
     return nullptr;

@@ -7,7 +7,6 @@ namespace lbm {
 
 constexpr int kBlock = 256;      // workgroup of the auxiliary kernels (4 wave64)
 constexpr int kMaxStepBlock = 512;
-constexpr int kXcds = 8;         // MI355X: 8 XCDs, workgroups dealt round-robin
 
 // Workgroup i runs on XCD i % 8. With remap, each XCD gets a contiguous run of logical
 // workgroups, so the cache lines that neighbouring row segments share stay in one XCD's L2

@@ -7,6 +7,8 @@
 
 namespace lbm {
 
+constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups dealt round-robin
+
 struct InitParams {
     const float* rho;  // [N]
     const float* u;    // AoS [N][2]

@@ -487,9 +487,12 @@ LBM_HD bool apply_local_bc(float* g, uint8_t fl, int x, int y, const NodeArgs& p
     }
 }
 
-// General path for one node: gather, then the boundary functor of its flag.
+// General path for one node: gather, then the boundary functor of its flag. gather_node is
+// the out-of-line form the step kernels call on their rare general path; gather_node_inl is
+// the same body inlined, for kernels where it is the main path (the IBM band gather), whose
+// call would pass the arguments and the result through scratch memory.
 template <int STORE>
-__host__ __device__ __noinline__ Pop9 gather_node(NodeArgs p, int x, int y) {
+LBM_HD Pop9 gather_node_inl(const NodeArgs& p, int x, int y) {
     Pop9 r;
     float* g = r.v;
 #pragma unroll
@@ -533,6 +536,11 @@ __host__ __device__ __noinline__ Pop9 gather_node(NodeArgs p, int x, int y) {
         default: break;
     }
     return r;
+}
+
+template <int STORE>
+__host__ __device__ __noinline__ Pop9 gather_node(NodeArgs p, int x, int y) {
+    return gather_node_inl<STORE>(p, x, y);
 }
 
 // Node force: uniform, else the force field, else (inside the IBM box) the IBM total.

@@ -5,7 +5,8 @@
 // in:  int32 nx, ny, two_by_two, no_ds, n, has_ds; float32 xy[2n]; float32 ds[n] if has_ds
 // out: int32 status (0 ok, 1 rejected), then on success int32 nb, n_ent, x0, x1, y0, y1,
 //      band_node[nb], mk_ptr[n+1], mk_band[n_ent], float mk_w[n_ent], bn_ptr[nb+1],
-//      bn_mk[n_ent], float bn_w[n_ent], box_map[(x1-x0)*(y1-y0)]
+//      bn_mk[n_ent], float bn_w[n_ent], box_map[(x1-x0)*(y1-y0)], kspread, mkf_band[16 n],
+//      float mkf_w[16 n], bnf_mk[kspread nb], float bnf_w[kspread nb]
 #include <cstdint>
 #include <cstdio>
 #include <vector>
@@ -49,6 +50,12 @@ int main(int argc, char** argv) {
         put(out, T.bn_mk);
         put(out, T.bn_w);
         put(out, T.box_map);
+        int32_t ks = T.kspread;
+        std::fwrite(&ks, 4, 1, out);
+        put(out, T.mkf_band);
+        put(out, T.mkf_w);
+        put(out, T.bnf_mk);
+        put(out, T.bnf_w);
     } else {
         std::fprintf(stderr, "rejected: %s\n", why);
     }

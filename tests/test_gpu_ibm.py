@@ -50,6 +50,59 @@ def test_gpu_ibm_matches_oracle(gpu_available, collision, quirks):
     s.close()
 
 
+# Marker sets that select each IBM pass: the register-resident pass with 8 spread slots and one
+# band entry per thread (small), 16 slots (dense: up to 14 markers per band node), two band
+# entries per thread (wide: 1500 band nodes), and the CSR pass (crowded: 2096 band nodes).
+PASS_CASES = {
+    "slots8": dict(nx=96, ny=48, D=12.0, n_markers=38, Re=20.0, cx=30.0),
+    "slots16": dict(nx=96, ny=48, D=12.0, n_markers=100, Re=20.0, cx=30.0),
+    "two_per_thread": dict(nx=256, ny=160, D=100.0, n_markers=314, Re=100.0, cx=80.0),
+    "csr": dict(nx=256, ny=160, D=140.0, n_markers=440, Re=100.0, cx=80.0),
+}
+
+
+def pass_case(name, storage=A.LBM_F32):
+    return sc.ibm_cylinder(u_in=0.05, collision=A.LBM_CM, storage=storage, **PASS_CASES[name])
+
+
+@pytest.mark.parametrize("name", list(PASS_CASES))
+def test_gpu_ibm_passes_match_oracle(gpu_available, name):
+    print(f"case ibm pass {name}", flush=True)
+    case = pass_case(name)
+    fg, rg, ug, s = run_gpu(case, 12)
+    fo, ro, uo, o = run_oracle(case, 12, 32)
+    assert max_rel(fg, fo) <= 1e-5
+    lg, tg = s.ibm_get_forces()
+    lo, to = o.ibm_get_forces()
+    scale = float(np.max(np.abs(lo)))
+    assert float(np.max(np.abs(lg - lo))) <= 1e-4 * scale + 1e-7
+    assert float(np.max(np.abs(tg - to))) <= 1e-4 * max(abs(to[0]), 1e-6) + 1e-7, (tg, to)
+    s.close()
+
+
+@pytest.mark.parametrize("name", ["slots8", "slots16", "two_per_thread"])
+@pytest.mark.parametrize("storage", [A.LBM_F32, A.LBM_F16_SHIFTED])
+def test_gpu_ibm_register_pass_equals_csr_pass(gpu_available, name, storage):
+    """The register-resident pass sums the same entries in the same order as the CSR pass
+    (LBM_IBM_CSR=1 selects it when the markers are set): equal populations and forces."""
+    import os
+
+    print(f"case ibm register pass vs csr {name} storage={storage}", flush=True)
+    out = {}
+    for mode in ("fast", "csr"):
+        if mode == "csr":
+            os.environ["LBM_IBM_CSR"] = "1"
+        try:
+            fg, rg, ug, s = run_gpu(pass_case(name, storage), 9)
+            lag, tot = s.ibm_get_forces()
+            s.close()
+        finally:
+            os.environ.pop("LBM_IBM_CSR", None)
+        out[mode] = (fg, ug, lag)
+    for a, b in zip(out["fast"], out["csr"]):
+        assert np.array_equal(a, b)
+
+
 def test_gpu_ibm_drag_is_positive_and_steady_sign(gpu_available):
     case = small_cylinder()
     fg, rg, ug, s = run_gpu(case, 400)

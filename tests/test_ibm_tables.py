@@ -56,6 +56,12 @@ def run_check(tmp_path, nx, ny, xy, ds, two, no_ds):
     t["bn_mk"], o = i32(o, ne)
     t["bn_w"], o = f32(o, ne)
     t["box_map"], o = i32(o, (x1 - x0) * (y1 - y0))
+    (ks,), o = i32(o, 1)
+    t["kspread"] = ks
+    t["mkf_band"], o = i32(o, 16 * n)
+    t["mkf_w"], o = f32(o, 16 * n)
+    t["bnf_mk"], o = i32(o, ks * nb)
+    t["bnf_w"], o = f32(o, ks * nb)
     assert o == len(b)
     return t
 
@@ -104,6 +110,21 @@ def verify(t, nx, ny, xy, ds, two, no_ds):
     for b, node in enumerate(band):
         assert bm[node // nx - y0, node % nx - x0] == b
     assert int(np.sum(bm >= 0)) == len(band)
+    # fixed-width lists: slot-major, each list's CSR entries in order, then (0, 0.0) padding
+    longest = int(np.max(np.diff(t["bn_ptr"])))
+    assert t["kspread"] == (8 if longest <= 8 else 16 if longest <= 16 else 0)
+    mb, mw = t["mkf_band"].reshape(16, n), t["mkf_w"].reshape(16, n)
+    for l in range(n):
+        a, e = t["mk_ptr"][l], t["mk_ptr"][l + 1]
+        assert list(mb[: e - a, l]) == list(t["mk_band"][a:e]) and not mb[e - a:, l].any()
+        assert np.array_equal(mw[: e - a, l], t["mk_w"][a:e]) and not mw[e - a:, l].any()
+    ks, nb = t["kspread"], len(band)
+    if ks:
+        km, kw = t["bnf_mk"].reshape(ks, nb), t["bnf_w"].reshape(ks, nb)
+        for b in range(nb):
+            a, e = t["bn_ptr"][b], t["bn_ptr"][b + 1]
+            assert list(km[: e - a, b]) == list(t["bn_mk"][a:e]) and not km[e - a:, b].any()
+            assert np.array_equal(kw[: e - a, b], t["bn_w"][a:e]) and not kw[e - a:, b].any()
 
 
 def test_tables_of_the_bgk_ibm_quirk_case(tmp_path):
@@ -135,6 +156,20 @@ def test_tables_of_random_marker_sets(tmp_path, seed):
         assert not any(expected(nx, ny, xy, ds, two, no_ds))
         return
     verify(t, nx, ny, xy, ds, two, no_ds)
+
+
+def test_crowded_band_has_no_fixed_spread_lists(tmp_path):
+    # 40 markers within a quarter cell: each band node of their common 4x4 stencil is touched
+    # by all of them (more than 16), so only the CSR spread lists exist
+    rng = np.random.default_rng(5)
+    xy = (np.array([6.3, 7.6]) + rng.uniform(0, 0.25, (40, 2))).astype(np.float32)
+    t = run_check(tmp_path, 16, 16, xy, None, False, False)
+    verify(t, 16, 16, xy, None, False, False)
+    assert t["kspread"] == 0 and len(t["bnf_mk"]) == 0
+    xy12 = xy[:12]
+    t = run_check(tmp_path, 16, 16, xy12, None, False, False)
+    verify(t, 16, 16, xy12, None, False, False)
+    assert t["kspread"] == 16
 
 
 def test_far_away_and_invalid_markers(tmp_path):
