@@ -306,7 +306,7 @@ bool lbm_ctx::ibm_graph_ok() const {
     const bool enabled = !(v && v[0] == '0');
     const bool adapt = cfg.collision == LBM_CM && cfg.adapter == LBM_OPTIMAL_ADAPTER;
     return enabled && ibm.active && ibm.split_ok && !adapt && ibm.box_valid && aux_stream && stream &&
-           !edge_in_store;
+           !edge_in_store && !ibm_fused_ok();
 }
 
 // Two split IBM steps through the graph, capturing it first when there is none for this
@@ -351,7 +351,7 @@ int lbm_ctx::one_step(bool write_macros) {
     if (ahead_ok()) return ahead_step(write_macros);
     ahead_valid = ahead_edges = false;
     bool adapt = cfg.collision == LBM_CM && cfg.adapter == LBM_OPTIMAL_ADAPTER;
-    if (ibm.active && ibm.split_ok && !adapt) return split_ibm_step(write_macros);
+    if (ibm.active && ibm.split_ok && !adapt) return ibm_fused_ok() ? fused_ibm_step(write_macros) : split_ibm_step(write_macros);
     if (ibm.active) {
         rc = ibm_pass(cur);
         if (rc) return rc;

@@ -47,6 +47,10 @@ struct IbmState {
     int kspread = 0;                       // spread slots per band entry (8 or 16)
     bool fast = false;                     // the pass keeps its lists in registers
     size_t shmem_fast = 0;                 // its LDS
+    bool fused = false;                    // one launch per step can run the pass (lbm_ibm.hip)
+    size_t shmem_fused = 0;                // the LDS of that launch
+    int fused_wgs = 0;                     // its IBM workgroups (each with whole box rows)
+    void* d_fused_args = nullptr;          // its constant IBM arguments
     std::vector<float> h_ds;               // arc lengths (host copy, for the force sum)
 };
 
@@ -127,6 +131,8 @@ struct lbm_ctx {
     int ahead_step(bool write_macros);
     int one_step(bool write_macros);
     int split_ibm_step(bool write_macros);
+    bool ibm_fused_ok() const;
+    int fused_ibm_step(bool write_macros);
     bool ibm_graph_ok() const;
     int ibm_graph_pair();
     void ibm_graph_free();

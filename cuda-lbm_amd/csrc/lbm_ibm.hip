@@ -379,6 +379,237 @@ __global__ __launch_bounds__(kIbmBlock) void ibm_fast_kernel(IbmParams p) {
     }
 }
 
+// ---- one launch per IBM step ---------------------------------------------------------------
+//
+// The first n_ibm workgroups of the launch each run the whole IBM pass (the sums of
+// ibm_fast_kernel and ibm_kernel: same entries, same order) and then step a share of the IBM
+// box, whole rows of it with at most one segment per thread, reading the forces they wrote for
+// those rows; the other workgroups step the nodes outside the box. The box's step needs
+// nothing from other workgroups and the outside step reads no IBM force, so no workgroup waits
+// for another: the pass runs beside the outside step, on one stream, with no cross-queue
+// dependency per step. Workgroups are 512 threads (two band entries and one marker per
+// thread). The launch's dynamic LDS is allocated to every workgroup, and above ~24 KB it
+// costs the outside step occupancy, so the pass keeps only what threads share there: band
+// state (12 B per entry), Lagrangian forces (8 B per marker) and the spread lists' 16-bit
+// marker indices (CSR); the marker lists sit in registers and the spread weights are read from
+// memory (cached) in each iteration. The pass's
+// constant arguments sit in device memory, leaving the launch's scalar registers to the step.
+constexpr int kIbmStepBlock = 512;
+constexpr int kIbmStepRounds = 2;  // band entries per thread
+constexpr size_t kIbmStepLdsMax = 24 * 1024;
+
+// Waves per SIMD requested for the launch: the outside step's register budget (80 VGPRs for
+// FP32 populations; the 16-bit step needs about 108).
+template <int STORE>
+constexpr int kIbmStepWaves = STORE == LBM_F32 ? 6 : 4;
+
+struct IbmStepArgs {
+    const int* band_node;
+    const int* mkf_band;
+    const float* mkf_w;
+    const int* bn_ptr;
+    const int* bn_mk;
+    const float* bn_w;
+    float* force_total;
+    float* lag_force;
+    int n_band, n_markers, n_ent, iters;
+    int clamp, y0, bh, rows_per;
+};
+
+// LDS of the pass, in bytes: band state (rho, ux, uy), Lagrangian forces (8-byte aligned),
+// spread marker indices.
+inline size_t ibm_step_lds(int nb, int nm, int ne) {
+    return (size_t(nb) * 12 + 7) / 8 * 8 + size_t(nm) * 8 + size_t(ne) * 2;
+}
+
+template <int STORE>
+__device__ __forceinline__ void ibm_step_pass(const StepParams& sp, const IbmStepArgs* __restrict__ ia, int row0,
+                                              int row1) {
+    constexpr int K = 8;  // longest spread list
+    extern __shared__ float2 lds2[];
+    const int nb = ia->n_band, nm = ia->n_markers, ne = ia->n_ent, t = threadIdx.x;
+    float* s_mac = reinterpret_cast<float*>(lds2);
+    float2* s_lag = lds2 + (size_t(nb) * 3 + 1) / 2;
+    unsigned short* s_bmk = reinterpret_cast<unsigned short*>(s_lag + nm);
+    const int nx = sp.nx;
+    const NodeArgs na = node_args(sp);
+    // band nodes, their spread entries [e0, e1) and weights
+    int node[kIbmStepRounds], e0[kIbmStepRounds], e1[kIbmStepRounds];
+#pragma unroll
+    for (int r = 0; r < kIbmStepRounds; r++) {
+        const int b = t + r * kIbmStepBlock;
+        node[r] = b < nb ? ia->band_node[b] : 0;
+        e0[r] = b < nb ? ia->bn_ptr[b] : 0;
+        e1[r] = b < nb ? ia->bn_ptr[b + 1] : 0;
+    }
+    // the spread marker indices into LDS, eight per thread in flight
+    constexpr int SU = 8;
+    for (int base = 0; base < ne; base += SU * kIbmStepBlock) {
+        int m[SU];
+#pragma unroll
+        for (int u = 0; u < SU; u++) {
+            const int e = base + u * kIbmStepBlock + t;
+            m[u] = e < ne ? ia->bn_mk[e] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < SU; u++) {
+            const int e = base + u * kIbmStepBlock + t;
+            if (e < ne) s_bmk[e] = static_cast<unsigned short>(m[u]);
+        }
+    }
+    // post-stream macroscopics of the band and the external force (as in ibm_kernel): the
+    // pulls of both entries issued together, the boundary functor only where a band node is
+    // not plain fluid
+    Pop9 g[kIbmStepRounds];
+    uint8_t fl[kIbmStepRounds];
+#pragma unroll
+    for (int r = 0; r < kIbmStepRounds; r++) {
+        const int y = node[r] / nx, x = node[r] - y * nx;
+#pragma unroll
+        for (int q = 0; q < Q; q++) g[r].v[q] = gather<STORE>(na, q, x, y);
+        fl[r] = na.flags ? na.flags[node[r]] : uint8_t(LBM_FLUID);
+    }
+    float fxa[kIbmStepRounds], fya[kIbmStepRounds];
+#pragma unroll
+    for (int r = 0; r < kIbmStepRounds; r++) {
+        const int b = t + r * kIbmStepBlock;
+        fxa[r] = fya[r] = 0.0f;
+        if (b < nb) {
+            if (fl[r] != LBM_FLUID) {  // rare: the out-of-line general path
+                const int y = node[r] / nx, x = node[r] - y * nx;
+                g[r] = gather_node<STORE>(na, x, y);
+            }
+            float rr = 0.f, jx = 0.f, jy = 0.f;
+#pragma unroll
+            for (int i = 0; i < Q; i++) {
+                rr += g[r].v[i];
+                jx += g[r].v[i] * float(CX[i]);
+                jy += g[r].v[i] * float(CY[i]);
+            }
+            const float inv = 1.0f / rr;
+            s_mac[3 * b] = rr;
+            s_mac[3 * b + 1] = jx * inv;
+            s_mac[3 * b + 2] = jy * inv;
+            fxa[r] = sp.fx;
+            fya[r] = sp.fy;
+            if (sp.force_field) {
+                fxa[r] = sp.force_field[2LL * node[r]];
+                fya[r] = sp.force_field[2LL * node[r] + 1];
+            }
+        }
+    }
+    // the marker lists, as LDS element pointers
+    const bool has_mk = t < nm;
+    const float* mb[kIbmSlots];
+    float mw[kIbmSlots];
+#pragma unroll
+    for (int k = 0; k < kIbmSlots; k++) {
+        mb[k] = s_mac + 3 * (has_mk ? ia->mkf_band[k * nm + t] : 0);
+        mw[k] = has_mk ? ia->mkf_w[k * nm + t] : 0.0f;
+    }
+    __syncthreads();
+    const int clamp = ia->clamp;
+    float sx = 0.0f, sy = 0.0f;
+    for (int it = 0; it < ia->iters; it++) {
+        // the spread weights, read from memory in every iteration (registers and LDS hold the
+        // rest); issued first, so that they arrive during the interpolation
+        float wsp[kIbmStepRounds][K];
+#pragma unroll
+        for (int r = 0; r < kIbmStepRounds; r++)
+#pragma unroll
+            for (int k = 0; k < K; k++) wsp[r][k] = e0[r] + k < e1[r] ? ia->bn_w[e0[r] + k] : 0.0f;
+        if (has_mk) {
+            float r = 0.f, vxs = 0.f, vys = 0.f;
+#pragma unroll
+            for (int k = 0; k < kIbmSlots; k++) {
+                r += mw[k] * mb[k][0];
+                vxs += mw[k] * mb[k][1];
+                vys += mw[k] * mb[k][2];
+            }
+            float lx = 2.0f * r * (0.0f - vxs), ly = 2.0f * r * (0.0f - vys);
+            if (clamp) {
+                lx = lx > 1e-8f ? lx : 0.0f;
+                ly = ly > 1e-8f ? ly : 0.0f;
+            }
+            s_lag[t] = make_float2(lx, ly);
+            sx += lx;
+            sy += ly;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kIbmStepRounds; r++) {
+            const int b = t + r * kIbmStepBlock;
+            if (b < nb) {
+                float fx = 0.f, fy = 0.f;
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    if (e0[r] + k < e1[r]) {
+                        const float2 lg = s_lag[s_bmk[e0[r] + k]];
+                        fx += wsp[r][k] * lg.x;
+                        fy += wsp[r][k] * lg.y;
+                    }
+                }
+                const float rho = s_mac[3 * b];
+                float cx = s_mac[3 * b + 1] + fx / (2.0f * rho);
+                float cy = s_mac[3 * b + 2] + fy / (2.0f * rho);
+                if (clamp) {
+                    cx = cx > 1e-8f ? cx : 0.0f;
+                    cy = cy > 1e-8f ? cy : 0.0f;
+                }
+                s_mac[3 * b + 1] = cx;
+                s_mac[3 * b + 2] = cy;
+                fxa[r] += fx;
+                fya[r] += fy;
+            }
+        }
+        __syncthreads();
+    }
+    // the forces of this workgroup's box rows; the Lagrangian sums from workgroup 0
+    const int y0 = ia->y0;
+    float* ft = ia->force_total;
+#pragma unroll
+    for (int r = 0; r < kIbmStepRounds; r++) {
+        const int b = t + r * kIbmStepBlock;
+        if (b < nb) {
+            const int row = node[r] / nx - y0;
+            if (row >= row0 && row < row1) {
+                ft[2LL * node[r]] = fxa[r];
+                ft[2LL * node[r] + 1] = fya[r];
+            }
+        }
+    }
+    if (blockIdx.x == 0 && has_mk) {
+        ia->lag_force[2 * t] = sx;
+        ia->lag_force[2 * t + 1] = sy;
+    }
+}
+
+// sp.region is 1 (the step outside the box); the IBM workgroups step their box rows as
+// region 2, one segment per thread.
+template <int COLL, int STORE>
+__global__ __launch_bounds__(kIbmStepBlock) __attribute__((amdgpu_waves_per_eu(kIbmStepWaves<STORE>))) void ibm_step_kernel(
+    StepParams sp, const IbmStepArgs* __restrict__ ia, int n_ibm, long long nb_padded, int remap) {
+    constexpr int SEG = kSegNarrow<STORE>;
+    MomAcc acc{0.0, 0.0, 0.0};
+    if (int(blockIdx.x) < n_ibm) {
+        const int rows_per = ia->rows_per, bh = ia->bh;
+        const int row0 = int(blockIdx.x) * rows_per;
+        const int row1 = row0 + rows_per < bh ? row0 + rows_per : bh;
+        ibm_step_pass<STORE>(sp, ia, row0, row1);
+        __syncthreads();  // this workgroup's box forces are written
+        const int bs = (sp.ibm_x1 - sp.ibm_x0 + SEG - 1) / SEG;
+        const int seg = row0 * bs + int(threadIdx.x);
+        if (seg < row1 * bs) {
+            sp.region = 2;
+            step_segment<COLL, STORE, 0, SEG>(sp, seg, acc);
+        }
+        return;
+    }
+    const long long i = (long long)blockIdx.x - n_ibm;
+    const long long lb = remap ? (i % kXcds) * (nb_padded / kXcds) + i / kXcds : i;
+    step_segment<COLL, STORE, 0, SEG>(sp, lb * kIbmStepBlock + threadIdx.x, acc);
+}
+
 template <class T>
 int put(lbm_ctx* c, T** dst, const std::vector<T>& v, const char* what) {
     size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
@@ -412,11 +643,59 @@ bool allow_dynamic_lds(int max_bytes) {
 
 }  // namespace
 
+// The one-launch IBM step applies to cache-resident grids (narrow segments) once the box's
+// external force is in place; LBM_IBM_FUSED=0 in the environment selects the two-stream
+// split step instead.
+bool lbm_ctx::ibm_fused_ok() const {
+    const char* v = std::getenv("LBM_IBM_FUSED");
+    if (v && v[0] == '0') return false;
+    const bool adapt = cfg.collision == LBM_CM && cfg.adapter == LBM_OPTIMAL_ADAPTER;
+    const int narrow = cfg.storage == LBM_F32 ? kSegNarrow<LBM_F32> : kSegNarrow<LBM_F16_SHIFTED>;
+    return ibm.active && ibm.fused && ibm.split_ok && ibm.box_valid && !adapt &&
+           step_policy(plane, cfg.storage).seg == narrow;
+}
+
+int lbm_ctx::fused_ibm_step(bool write_macros) {
+    StepParams sp = step_params(cur, write_macros);
+    sp.region = 1;
+    const int seg = cfg.storage == LBM_F32 ? kSegNarrow<LBM_F32> : kSegNarrow<LBM_F16_SHIFTED>;
+    sp.ntx = (sp.nx + seg - 1) / seg;
+    sp.nt = 0;
+    const long long nseg = (long long)sp.ntx * sp.ny;
+    long long nb = (nseg + kIbmStepBlock - 1) / kIbmStepBlock;
+    nb = (nb + kXcds - 1) / kXcds * kXcds;
+    const long long grid = nb + ibm.fused_wgs;
+    if (grid > 0x7fffffffLL) return fail(LBM_E_UNSUPPORTED, "grid too large for one IBM step launch");
+    const int st = cfg.storage == LBM_F16_SHIFTED ? 1 : (cfg.storage == LBM_BF16_SHIFTED ? 2 : 0);
+    const int co = cfg.collision == LBM_BGK ? 0 : (cfg.collision == LBM_MRT ? 1 : 2);
+    static const void* const kern[3][3] = {
+        {reinterpret_cast<const void*>(&ibm_step_kernel<LBM_BGK, LBM_F32>),
+         reinterpret_cast<const void*>(&ibm_step_kernel<LBM_MRT, LBM_F32>),
+         reinterpret_cast<const void*>(&ibm_step_kernel<LBM_CM, LBM_F32>)},
+        {reinterpret_cast<const void*>(&ibm_step_kernel<LBM_BGK, LBM_F16_SHIFTED>),
+         reinterpret_cast<const void*>(&ibm_step_kernel<LBM_MRT, LBM_F16_SHIFTED>),
+         reinterpret_cast<const void*>(&ibm_step_kernel<LBM_CM, LBM_F16_SHIFTED>)},
+        {reinterpret_cast<const void*>(&ibm_step_kernel<LBM_BGK, LBM_BF16_SHIFTED>),
+         reinterpret_cast<const void*>(&ibm_step_kernel<LBM_MRT, LBM_BF16_SHIFTED>),
+         reinterpret_cast<const void*>(&ibm_step_kernel<LBM_CM, LBM_BF16_SHIFTED>)}};
+    const IbmStepArgs* ia = static_cast<const IbmStepArgs*>(ibm.d_fused_args);
+    int n_ibm = ibm.fused_wgs, remap = 1;
+    long long nb_arg = nb;
+    void* args[] = {&sp, &ia, &n_ibm, &nb_arg, &remap};
+    hipError_t e = hipLaunchKernel(kern[st][co], dim3(unsigned(grid)), dim3(kIbmStepBlock), args, ibm.shmem_fused, stream);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "IBM step launch");
+    cur = 1 - cur;
+    timestep++;
+    if (write_macros) macros_valid = true;
+    return LBM_OK;
+}
+
 void lbm_ctx::ibm_free() {
     ibm_graph_free();
     void* ptrs[] = {ibm.d_force_total, ibm.d_xy, ibm.d_ds, ibm.d_band_node, ibm.d_mk_ptr, ibm.d_mk_band, ibm.d_mk_w,
                     ibm.d_bn_ptr, ibm.d_bn_mk, ibm.d_bn_w, ibm.d_lag_force, ibm.d_force_sum, ibm.d_box_map,
-                    ibm.d_mkf_band, ibm.d_mkf_w, ibm.d_bnf_mk, ibm.d_bnf_w};
+                    ibm.d_mkf_band, ibm.d_mkf_w, ibm.d_bnf_mk, ibm.d_bnf_w, ibm.d_fused_args};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     ibm = IbmState();
@@ -564,6 +843,41 @@ int lbm_ibm_set_markers(lbm_ctx* ctx, const float* xy, const float* ds, int32_t 
     I.fast = fast && std::getenv("LBM_IBM_CSR") == nullptr;
     I.kspread = T.kspread;
     I.shmem_fast = size_t(nb) * sizeof(float4) + size_t(n) * sizeof(float2);
+    // the one-launch step: at most eight spread entries per band node, two band entries and one
+    // marker per thread, LDS within the budget that keeps the outside step's occupancy, box
+    // rows that fit one workgroup's segments
+    const int seg = ctx->cfg.storage == LBM_F32 ? kSegNarrow<LBM_F32> : kSegNarrow<LBM_F16_SHIFTED>;
+    const int bs = (T.x1 - T.x0 + seg - 1) / seg, bh = T.y1 - T.y0;
+    I.shmem_fused = ibm_step_lds(nb, n, int(T.bn_mk.size()));
+    I.fused = fast && T.kspread == 8 && nb <= kIbmStepRounds * kIbmStepBlock && n <= kIbmStepBlock &&
+              T.bn_mk.size() < 65536 && I.shmem_fused <= kIbmStepLdsMax && bs <= kIbmStepBlock;
+    if (I.fused) {
+        const int rows_per = kIbmStepBlock / bs;
+        I.fused_wgs = (bh + rows_per - 1) / rows_per;
+        IbmStepArgs a{};
+        a.band_node = I.d_band_node;
+        a.mkf_band = I.d_mkf_band;
+        a.mkf_w = I.d_mkf_w;
+        a.bn_ptr = I.d_bn_ptr;
+        a.bn_mk = I.d_bn_mk;
+        a.bn_w = I.d_bn_w;
+        a.force_total = I.d_force_total;
+        a.lag_force = I.d_lag_force;
+        a.n_band = nb;
+        a.n_markers = n;
+        a.n_ent = int(T.bn_mk.size());
+        a.iters = iters;
+        a.clamp = (ctx->cfg.quirks & LBM_QUIRK_IBM_CLAMP) ? 1 : 0;
+        a.y0 = T.y0;
+        a.bh = bh;
+        a.rows_per = rows_per;
+        rc = ctx->dev_alloc(&I.d_fused_args, sizeof(a), "IBM step arguments");
+        rc = rc ? rc : ctx->upload(I.d_fused_args, &a, sizeof(a), 0);
+        if (rc) {
+            ctx->ibm_free();
+            return rc;
+        }
+    }
     I.h_ds = dsv;
     I.active = true;
     return LBM_OK;

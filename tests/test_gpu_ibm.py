@@ -103,6 +103,47 @@ def test_gpu_ibm_register_pass_equals_csr_pass(gpu_available, name, storage):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("storage", [A.LBM_F32, A.LBM_F16_SHIFTED])
+@pytest.mark.parametrize("collision", [A.LBM_BGK, A.LBM_MRT, A.LBM_CM])
+def test_gpu_ibm_one_launch_step_matches_split_step(gpu_available, storage, collision):
+    """The one-launch IBM step (the IBM pass in the launch's first workgroups, beside the step
+    outside the box) against the two-stream split step (LBM_IBM_FUSED=0), also across a
+    force-field change (whose first step rewrites the box's external force through the split
+    path) and with the clamp quirk: the same populations, fields and Lagrangian forces. Both
+    sum the same entries in the same order; the bound allows for the compiler contracting a
+    product into a sum differently in the two kernels (one rounding of the last place)."""
+    import os
+
+    from lbm_amd import Solver
+
+    print(f"case ibm one-launch vs split storage={storage} collision={collision}", flush=True)
+    out = {}
+    for mode in ("fused", "split"):
+        if mode == "split":
+            os.environ["LBM_IBM_FUSED"] = "0"
+        try:
+            snaps = []
+            for quirks in (0, A.LBM_QUIRK_IBM_CLAMP):
+                case = small_cylinder(quirks, collision, storage)
+                s = Solver(case.config)
+                case.apply(s)
+                s.step(3)
+                F = np.zeros((case.config.ny, case.config.nx, 2), np.float32)
+                F[..., 0] = 2e-6
+                F[10:20, :, 1] = -1e-6
+                s.set_force_field(F)
+                for n in (1, 4, 3):
+                    s.step(n)
+                    r, u = s.get_macros()
+                    snaps += [s.get_pops(), r, u, s.ibm_get_forces()[0]]
+                s.close()
+        finally:
+            os.environ.pop("LBM_IBM_FUSED", None)
+        out[mode] = snaps
+    for a, b in zip(out["fused"], out["split"]):
+        assert max_rel(a, b, floor=1e-2) <= 1e-5
+
+
 def test_gpu_ibm_drag_is_positive_and_steady_sign(gpu_available):
     case = small_cylinder()
     fg, rg, ug, s = run_gpu(case, 400)
@@ -122,6 +163,7 @@ def test_gpu_ibm_graph_replay_equals_direct_steps(gpu_available):
     runs = {}
     for mode in ("1", "0"):
         os.environ["LBM_HIP_GRAPHS"] = mode
+        os.environ["LBM_IBM_FUSED"] = "0"
         try:
             case = small_cylinder()
             from lbm_amd import Solver
@@ -140,6 +182,7 @@ def test_gpu_ibm_graph_replay_equals_direct_steps(gpu_available):
             s.close()
         finally:
             os.environ.pop("LBM_HIP_GRAPHS", None)
-    for a, b in zip(runs["1"], runs["0"]):
+            os.environ.pop("LBM_IBM_FUSED", None)
+    for a, b in zip(runs["1"], runs["0"]):  # split steps (the one-launch step off)
         for x, y in zip(a, b):
             assert np.array_equal(x, y)
