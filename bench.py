@@ -285,13 +285,14 @@ def run_poiseuille(torch, steps=400, warmup=20, adapter_steps=100):
 
 def run_ibm_cylinder(torch, steps=200, warmup=20):
     """The IBM cylinder channel (2048x512 CM, 201 markers, 3 multi-direct forcing iterations
-    per step): the IBM kernel and a one-step fused launch per step."""
+    per step): one launch per step, the IBM pass in its first workgroups beside the step
+    outside the IBM box."""
     from lbm_amd import scenarios as sc
 
     s, r = run_case(sc.ibm_cylinder(), steps, warmup, torch)
     s.close()
     return dict(r, workload="IBM cylinder 2048x512 CM, 201 markers, 3 IBM iterations per step "
-                            "(IBM kernel + fused step per step)")
+                            "(one launch per step: the IBM pass beside the step outside its box)")
 
 
 def run_d3q27(torch, steps=200, warmup=20):
