@@ -380,9 +380,14 @@ int lbm3d_ctx::one_step(bool write_macros) {
     p.corner_uz = (cfg.quirks & LBM_QUIRK_CORNER3D_UZ) ? 1 : 0;
     p.stale_rest = (cfg.quirks & LBM_QUIRK_STALE_REST) ? 1 : 0;
     // Non-temporal stores keep the written populations from evicting the pulls' shared lines
-    // once the two buffers exceed the 256 MB Infinity Cache (+2% FP32, +4% FP16 at 256^3);
-    // on resident grids they push every write to HBM (-20% FP32 at 64^3 and 96^3).
-    p.nt = 2.0 * Q * double(plane) * double(elem_bytes(cfg.storage)) > 192.0e6 ? 1 : 0;
+    // once the two buffers no longer stay in the 256 MB Infinity Cache (+2% FP32, +4% FP16 at
+    // 256^3); on resident grids they push every write to HBM (-20% FP32 at 64^3 and 96^3).
+    // The switch sits at kNtBufferBytes, three quarters of the cache (the 2-D kernels' cache
+    // threshold too), leaving room for the flags and the other data a step touches; grids
+    // between it and the full cache (about 97^3-105^3 FP32, 122^3-133^3 FP16) were not timed
+    // separately.
+    constexpr double kNtBufferBytes = 192.0e6;
+    p.nt = 2.0 * Q * double(plane) * double(elem_bytes(cfg.storage)) > kNtBufferBytes ? 1 : 0;
     p.wx = wave_cols(cfg.nx);
     p.nbx = (cfg.nx + 64 * p.wx - 1) / (64 * p.wx);
     p.nby = (cfg.ny + 4 / p.wx - 1) / (4 / p.wx);
