@@ -392,8 +392,8 @@ __global__ __launch_bounds__(kIbmBlock) void ibm_fast_kernel(IbmParams p) {
 // costs the outside step occupancy, so the pass keeps only what threads share there: band
 // state (12 B per entry), Lagrangian forces (8 B per marker) and the spread lists' 16-bit
 // marker indices (CSR); the marker lists sit in registers and the spread weights are read from
-// memory (cached) in each iteration. The pass's
-// constant arguments sit in device memory, leaving the launch's scalar registers to the step.
+// memory (cached) in each iteration. The pass's constant arguments sit in device memory,
+// leaving the launch's scalar registers to the step.
 constexpr int kIbmStepBlock = 512;
 constexpr int kIbmStepRounds = 2;  // band entries per thread
 constexpr size_t kIbmStepLdsMax = 24 * 1024;
