@@ -184,7 +184,11 @@ int lbm_reduce_noalloc(lbm_ctx* ctx, int32_t kind, const float* ref_device, doub
 
 /* ---- immersed boundary (src/IBM, multi-direct forcing IBMManager.cuh:222-252) ------
  * Static markers in lattice units, AoS [n][2]; ds: per-marker arc length (NULL = 1).
- * iters = ITER_MAX (IBMManager.cuh:8, 3 in the reference). n = 0 removes the body. */
+ * iters = ITER_MAX (IBMManager.cuh:8, 3 in the reference). n = 0 removes the body.
+ * Each later lbm_step runs the IBM pass per step on the context's stream: on grids that stay
+ * in the Infinity Cache as one launch that also steps the grid; otherwise the pass and the
+ * markers' box are stepped there while the rest of the grid is stepped on an internal stream
+ * that the context's stream joins before lbm_step returns. */
 int lbm_ibm_set_markers(lbm_ctx* ctx, const float* xy, const float* ds, int32_t n, int32_t iters);
 /* Lagrangian forces of the last step, AoS [n][2] (may be NULL), and the total force the
  * markers apply to the fluid, summed in FP64 (force_sum[2], may be NULL). Synchronises. */
