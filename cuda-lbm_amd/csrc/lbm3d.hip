@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -37,13 +38,19 @@ size_t elem_bytes(int storage) { return storage == LBM_F32 ? 4 : 2; }
 // VGPRs, no scratch) measured 1-2% faster than the compiler's own 70-VGPR allocation.
 template <int COLL, int STORE, bool BUF, int GUO>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void step3_kernel(Step3Params p) {
-    const long long nb = (long long)p.nbx * p.nby * p.nz;
+    const long long nb = (long long)p.nbx * p.nby * p.nzl;
     const long long b = blockIdx.x;
     const long long per = nb / 8;
     const long long L = b < per * 8 ? (b % 8) * per + b / 8 : b;
     int x, y, z;
     thread_node(p, L, int(threadIdx.x), x, y, z);
-    if (x >= p.nx || y >= p.ny) return;
+    x += p.ox;
+    y += p.oy;
+    z += p.oz;
+    if (x >= p.xe || y >= p.ye) return;
+    if (p.region == 1 && x >= p.rbox[0] && x < p.rbox[1] && y >= p.rbox[2] && y < p.rbox[3] && z >= p.rbox[4] &&
+        z < p.rbox[5])
+        return;
     if (GUO && p.flags && (p.flags[((long long)z * p.ny + y) * p.nx + x] & kPreLaunch)) return;
     step3_node<COLL, STORE, BUF, GUO>(p, x, y, z);
 }
@@ -187,6 +194,11 @@ struct lbm3d_ctx {
     lbm3d_config cfg{};
     int device = 0;
     hipStream_t own_stream = nullptr, stream = nullptr;
+    // IBM split steps: the step outside the band's box on aux_stream, beside the IBM pass and
+    // the box's step on `stream` (lbm3d_ctx::one_step)
+    hipStream_t aux_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    std::vector<uint8_t> h_flags;      // host copy of the flags (empty: all FLUID)
     long long N = 0, plane = 0;
     void* buf[2] = {nullptr, nullptr};
     int cur = 0;                       // buffer holding the latest post-collision state
@@ -202,6 +214,7 @@ struct lbm3d_ctx {
         bool force_valid = false;      // d_total holds the external force outside the band
         int n_markers = 0, n_band = 0, iters = 3;
         int box[6] = {0, 0, 0, 0, 0, 0};  // the band's bounding box [x0, x1) x [y0, y1) x [z0, z1)
+        bool split_ok = false;         // the box may be stepped apart from the rest (ibm_split_ok)
         int *band_node = nullptr, *mk_ptr = nullptr, *mk_band = nullptr, *bn_ptr = nullptr, *bn_mk = nullptr;
         float *mk_w = nullptr, *bn_w = nullptr;
         float *brho = nullptr, *bu = nullptr, *bf = nullptr, *lag = nullptr, *lag_sum = nullptr;
@@ -219,7 +232,11 @@ struct lbm3d_ctx {
 
     ~lbm3d_ctx() {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (aux_stream) (void)hipStreamSynchronize(aux_stream);
         ibm_free();
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (aux_stream) (void)hipStreamDestroy(aux_stream);
         for (void* p : {buf[0], buf[1], (void*)d_flags, (void*)d_force, (void*)d_rho, (void*)d_u, (void*)d_partials, tmp,
                         (void*)d_pre})
             if (p) (void)hipFree(p);
@@ -289,8 +306,43 @@ struct lbm3d_ctx {
         ibm = Ibm3();
     }
     int ibm_pass(const Step3Params& sp, bool fits);
+    void ibm_update_split();
     int one_step(bool write_macros);
 };
+
+// Whether an IBM step may step the band's box apart from the rest of the grid, concurrently
+// with the IBM pass: the two launches then write disjoint nodes and read only the step's
+// source, which holds when (1) no band node sits on a non-periodic face (whose gather would
+// read stale slots the outside launch rewrites) and (2) the box grown by two nodes holds only
+// FLUID and bounce-back nodes (no functor that reads a neighbour's populations, no
+// pre-launched node, on either side of the box's faces).
+void lbm3d_ctx::ibm_update_split() {
+    ibm.split_ok = false;
+    if (!ibm.active) return;
+    const int n[3] = {cfg.nx, cfg.ny, cfg.nz};
+    const int per[3] = {cfg.periodic_x, cfg.periodic_y, cfg.periodic_z};
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; a++) {
+        if (!per[a] && (ibm.box[2 * a] < 1 || ibm.box[2 * a + 1] > n[a] - 1)) return;
+        lo[a] = std::max(0, ibm.box[2 * a] - 2);
+        hi[a] = std::min(n[a], ibm.box[2 * a + 1] + 2);
+    }
+    if (!h_flags.empty())
+        for (int z = lo[2]; z < hi[2]; z++)
+            for (int y = lo[1]; y < hi[1]; y++)
+                for (int x = lo[0]; x < hi[0]; x++) {
+                    const uint8_t f = h_flags[(size_t(z) * size_t(cfg.ny) + size_t(y)) * size_t(cfg.nx) + size_t(x)];
+                    if (f != LBM_FLUID && f != LBM_BOUNCE_BACK && f != LBM_CORNER_EDGE_BOUNCE_BACK) return;
+                }
+    // periodic axes: a box grown past an end wraps; check the wrapped slabs too
+    for (int a = 0; a < 3; a++) {
+        if (!per[a] || h_flags.empty()) continue;
+        if (ibm.box[2 * a] - 2 >= 0 && ibm.box[2 * a + 1] + 2 <= n[a]) continue;
+        return;  // not worth the special case: the split needs a box clear of the wrap
+    }
+    const char* v = std::getenv("LBM_IBM3_SPLIT");
+    ibm.split_ok = !(v && v[0] == '0');
+}
 
 // The IBM pass of one step, ahead of the step launch on the same stream: it reads the step's
 // source buffer and the stale slots of its destination before the step rewrites them.
@@ -391,15 +443,50 @@ int lbm3d_ctx::one_step(bool write_macros) {
     p.wx = wave_cols(cfg.nx);
     p.nbx = (cfg.nx + 64 * p.wx - 1) / (64 * p.wx);
     p.nby = (cfg.ny + 4 / p.wx - 1) / (4 / p.wx);
+    p.xe = cfg.nx;
+    p.ye = cfg.ny;
+    p.nzl = cfg.nz;
     const long long nblk = (long long)p.nbx * p.nby * cfg.nz;
     if (nblk > 0x7fffffffLL) return fail(LBM_E_INVALID, "grid needs too many workgroups");
     const bool fits = cfg.storage == LBM_F32 ? lbm::buf_fits<float>(Q, plane) : lbm::buf_fits<uint16_t>(Q, plane);
-    if (ibm.active) {
+    // IBM split step: the nodes outside the band's box on aux_stream (they read neither the IBM
+    // force nor anything the pass writes), beside the pass and then the box's nodes on `stream`
+    const bool split = ibm.active && ibm.split_ok;
+    hipStream_t outside_stream = stream;
+    if (split) {
+        if (!aux_stream) {
+            hipError_t e = hipStreamCreateWithFlags(&aux_stream, hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_join, hipEventDisableTiming);
+            if (e != hipSuccess) return hip_fail(e, "IBM stream set-up");
+        }
+        if (!ibm.force_valid) {
+            hipLaunchKernelGGL(force3_fill_kernel, dim3(grid1(N)), dim3(256), 0, stream, ibm.total, d_force, N, plane,
+                               cfg.force[0], cfg.force[1], cfg.force[2]);
+            ibm.force_valid = true;
+        }
+        hipError_t e = hipEventRecord(ev_fork, stream);
+        if (e != hipSuccess) return hip_fail(e, "IBM fork");
+        // the pass is submitted first, so that its small launches start before the outside
+        // step's workgroups fill the device
         int rc = ibm_pass(p, fits);
         if (rc) return rc;
+        e = hipStreamWaitEvent(aux_stream, ev_fork, 0);
+        if (e != hipSuccess) return hip_fail(e, "IBM fork");
+        outside_stream = aux_stream;
+    }
+    if (ibm.active) {
         p.force = ibm.total;
         if (!d_force)  // the uniform force holds outside the band's box
             for (int k = 0; k < 6; k++) p.fbox[k] = ibm.box[k];
+    }
+    if (ibm.active && !split) {
+        int rc = ibm_pass(p, fits);
+        if (rc) return rc;
+    }
+    if (split) {
+        p.region = 1;
+        for (int k = 0; k < 6; k++) p.rbox[k] = ibm.box[k];
     }
     const dim3 grid{unsigned(nblk)}, block{256};
 #define LBM3_PRE(C, S, B, G)                                                                          \
@@ -407,15 +494,15 @@ int lbm3d_ctx::one_step(bool write_macros) {
         const int cnt = pre_level_end[k] - int(beg);                                                  \
         if (cnt > 0)                                                                                  \
             hipLaunchKernelGGL((step3_pre_kernel<C, S, B, G>), dim3(unsigned((cnt + 255) / 256)), block, 0, \
-                               stream, p, d_pre + beg, cnt);                                          \
+                               launch_stream, p, d_pre + beg, cnt);                                   \
     }
 #define LBM3_LAUNCH_G(C, S, G)                                                                        \
     if (fits) {                                                                                       \
         LBM3_PRE(C, S, true, G)                                                                       \
-        hipLaunchKernelGGL((step3_kernel<C, S, true, G>), grid, block, 0, stream, p);                 \
+        hipLaunchKernelGGL((step3_kernel<C, S, true, G>), grid_l, block, 0, launch_stream, p);                 \
     } else {                                                                                          \
         LBM3_PRE(C, S, false, G)                                                                      \
-        hipLaunchKernelGGL((step3_kernel<C, S, false, G>), grid, block, 0, stream, p);                \
+        hipLaunchKernelGGL((step3_kernel<C, S, false, G>), grid_l, block, 0, launch_stream, p);                \
     }
 #define LBM3_LAUNCH(C, S)                                                                       \
     if (has_reg) {                                                                              \
@@ -423,9 +510,9 @@ int lbm3d_ctx::one_step(bool write_macros) {
     } else if (has_guo) {                                                                       \
         LBM3_LAUNCH_G(C, S, 1)                                                                  \
     } else if (fits) {                                                                          \
-        hipLaunchKernelGGL((step3_kernel<C, S, true, 0>), grid, block, 0, stream, p);       \
+        hipLaunchKernelGGL((step3_kernel<C, S, true, 0>), grid_l, block, 0, launch_stream, p);       \
     } else {                                                                                    \
-        hipLaunchKernelGGL((step3_kernel<C, S, false, 0>), grid, block, 0, stream, p);      \
+        hipLaunchKernelGGL((step3_kernel<C, S, false, 0>), grid_l, block, 0, launch_stream, p);      \
     }
 #define LBM3_LAUNCH_S(C)                                                                        \
     switch (cfg.storage) {                                                                      \
@@ -433,12 +520,36 @@ int lbm3d_ctx::one_step(bool write_macros) {
         case LBM_F16_SHIFTED: LBM3_LAUNCH(C, LBM_F16_SHIFTED) break;                            \
         default: LBM3_LAUNCH(C, LBM_BF16_SHIFTED) break;                                        \
     }
-    if (cfg.collision == LBM_MRT) {
-        LBM3_LAUNCH_S(LBM_MRT)
-    } else if (cfg.collision == LBM_CM) {
-        LBM3_LAUNCH_S(LBM_CM)
-    } else {
-        LBM3_LAUNCH_S(LBM_BGK)
+    auto launch_all = [&](hipStream_t launch_stream, dim3 grid_l) {
+        if (cfg.collision == LBM_MRT) {
+            LBM3_LAUNCH_S(LBM_MRT)
+        } else if (cfg.collision == LBM_CM) {
+            LBM3_LAUNCH_S(LBM_CM)
+        } else {
+            LBM3_LAUNCH_S(LBM_BGK)
+        }
+    };
+    launch_all(outside_stream, grid);
+    if (split) {
+        hipError_t e = hipEventRecord(ev_join, aux_stream);
+        if (e != hipSuccess) return hip_fail(e, "IBM join");
+        // the box's nodes (no pre-launched node is among them), after the pass: the same
+        // workgroup shape over the box's rows
+        const auto saved_pre = std::move(pre_level_end);
+        pre_level_end.clear();
+        p.region = 2;
+        p.ox = ibm.box[0];
+        p.oy = ibm.box[2];
+        p.oz = ibm.box[4];
+        p.xe = ibm.box[1];
+        p.ye = ibm.box[3];
+        p.nzl = ibm.box[5] - ibm.box[4];
+        p.nbx = (ibm.box[1] - ibm.box[0] + 64 * p.wx - 1) / (64 * p.wx);
+        p.nby = (ibm.box[3] - ibm.box[2] + 4 / p.wx - 1) / (4 / p.wx);
+        launch_all(stream, dim3(unsigned((long long)p.nbx * p.nby * p.nzl)));
+        pre_level_end = saved_pre;
+        e = hipStreamWaitEvent(stream, ev_join, 0);
+        if (e != hipSuccess) return hip_fail(e, "IBM join");
     }
 #undef LBM3_LAUNCH_S
 #undef LBM3_LAUNCH
@@ -565,6 +676,8 @@ int lbm3d_set_flags(lbm3d_ctx* ctx, const uint8_t* flags, int on_device) {
         }
         ctx->n_pre = 0;
         ctx->pre_level_end.clear();
+        ctx->h_flags.clear();
+        ctx->ibm_update_split();
         return LBM_OK;
     }
     std::vector<uint8_t> h(size_t(ctx->N));
@@ -645,7 +758,11 @@ int lbm3d_set_flags(lbm3d_ctx* ctx, const uint8_t* flags, int on_device) {
         rc = ctx->dev_alloc(reinterpret_cast<void**>(&ctx->d_flags), size_t(ctx->plane), "flags");
         if (rc) return rc;
     }
-    return ctx->upload(ctx->d_flags, h.data(), h.size(), 0);
+    rc = ctx->upload(ctx->d_flags, h.data(), h.size(), 0);
+    if (rc) return rc;
+    ctx->h_flags = std::move(h);  // with the pre-launch bits: such nodes never qualify for a split
+    ctx->ibm_update_split();
+    return LBM_OK;
 }
 
 int lbm3d_init_from_macros(lbm3d_ctx* ctx, const float* rho, const float* u, int on_device) {
@@ -907,6 +1024,7 @@ int lbm3d_ibm_set_markers(lbm3d_ctx* ctx, const float* xyz, const float* ds, int
     if (ds) std::copy(ds, ds + n, I.h_ds.begin());
     I.force_valid = false;
     I.active = true;
+    ctx->ibm_update_split();
     return LBM_OK;
 }
 

@@ -47,10 +47,16 @@ struct Step3Params {
     int stale_rest;        // LBM_QUIRK_STALE_REST
     int nt;                // store with the non-temporal cache policy (buffers beyond the
                            // Infinity Cache)
+    int region;            // 0 every node; 1 the nodes outside rbox; 2 the nodes of rbox only
+                           // (the IBM split step)
+    int rbox[6];           // [x0, x1) x [y0, y1) x [z0, z1)
+    // The launch's node range: workgroups cover nbx x nby x nzl blocks from (ox, oy, oz), nodes
+    // at x >= xe or y >= ye idle. Whole grid: 0, 0, 0, nx, ny, nz; region 2: the box.
     // Workgroup shape: 256 threads = wx wavefronts along x (64*wx consecutive nodes of a row)
     // times 4/wx rows; nbx, nby workgroups along x and y, nz along z.
     int wx;
     int nbx, nby;
+    int ox, oy, oz, xe, ye, nzl;
 };
 
 // Workgroup of 256 threads: wx = min(4, ceil(nx/64)) wavefronts along x, so a workgroup holds

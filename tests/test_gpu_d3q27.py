@@ -459,6 +459,71 @@ def test_gpu_ibm3_markers_removed_and_errors(gpu_available):
         assert e.value.code == A.LBM_E_INVALID
 
 
+@pytest.mark.parametrize("case", range(len(IBM3_CASES)))
+def test_gpu_ibm3_split_step_equals_serial_step(gpu_available, case):
+    """An IBM step whose band box is clear of non-periodic faces and neighbour-reading flags
+    steps the nodes outside the box on a second stream beside the IBM pass, then the box: the
+    same node updates as the serial step (LBM_IBM3_SPLIT=0), bit for bit, in every case (the
+    cases whose box touches a face or a Guo flag run serially either way)."""
+    import os
+
+    c, f0, fl, force, xyz, ds, iters = ibm3_case(case)
+    print(f"case d3q27 ibm split vs serial {case} {IBM3_CASES[case][0]}", flush=True)
+    out = {}
+    for mode in ("split", "serial"):
+        if mode == "serial":
+            os.environ["LBM_IBM3_SPLIT"] = "0"
+        try:
+            out[mode] = run_ibm3_gpu(c, f0, fl, force, xyz, ds, iters, 7)
+        finally:
+            os.environ.pop("LBM_IBM3_SPLIT", None)
+    for a, b in zip(out["split"], out["serial"]):
+        assert np.array_equal(a, b)
+
+
+def test_gpu_ibm3_split_step_in_the_wind_tunnel(gpu_available):
+    """A sphere of markers in a CM<3> channel with Guo inlet/outlet and bounce-back walls (the
+    band box clear of them, so the split applies): split and serial steps agree bit for bit
+    over 40 steps, with the fields stored midway."""
+    import os
+
+    nx, ny, nz = 48, 24, 24
+    k = np.arange(120)
+    th, ph = np.arccos(1 - 2 * (k + 0.5) / 120), np.pi * (1 + 5 ** 0.5) * k
+    xyz = np.stack([16 + 4 * np.sin(th) * np.cos(ph), 12 + 4 * np.sin(th) * np.sin(ph), 12 + 4 * np.cos(th)],
+                   -1).astype(np.float32)
+    fl = np.zeros((nz, ny, nx), np.uint8)
+    fl[:, 0, :] = fl[:, -1, :] = A.BOUNCE_BACK
+    fl[0], fl[-1] = A.BOUNCE_BACK, A.BOUNCE_BACK
+    fl[:, :, 0] = A.GUO_VELOCITY_INLET
+    fl[:, :, -1] = A.GUO_PRESSURE_OUTLET
+    c = make_cfg((nx, ny, nz), (0, 0, 0), 1.0 / (3 * 0.05 * ny / 100.0 + 0.5), collision=A.LBM_CM)
+    c.u_inlet = 0.05
+    f0 = np.empty((27, nz, ny, nx), np.float32)
+    f0[:] = O.equilibrium3d(1.0, 0.05, 0.0, 0.0)[:, None, None, None]
+    ds = np.full(120, 4 * np.pi * 16.0 / 120, np.float32)
+    print("case d3q27 ibm split vs serial wind tunnel 48x24x24", flush=True)
+    out = {}
+    for mode in ("split", "serial"):
+        if mode == "serial":
+            os.environ["LBM_IBM3_SPLIT"] = "0"
+        try:
+            with Solver3D(gpu_config(c)) as s:
+                s.set_pops(f0)
+                s.set_flags(fl)
+                s.ibm_set_markers(xyz, ds, 3)
+                snaps = []
+                for n in (17, 23):
+                    s.step(n)
+                    rho, u = s.get_macros()
+                    snaps += [s.get_pops(), rho, u, s.ibm_get_forces()[0]]
+            out[mode] = snaps
+        finally:
+            os.environ.pop("LBM_IBM3_SPLIT", None)
+    for a, b in zip(out["split"], out["serial"]):
+        assert np.array_equal(a, b)
+
+
 def test_gpu_sphere_body_from_a_config_file_in_the_wind_tunnel(gpu_available, tmp_path):
     """The reference's wind-tunnel body path at reduced size: a UV-sphere OBJ scaled to D = 8
     and placed at (16, 12, 12) by a body file with 96 target points (Morton order), in a CM<3>
