@@ -159,7 +159,9 @@ int lbm3d_reduce(lbm3d_ctx* ctx, int32_t kind, double* out);
  * (IBMManager.cuh:8, 3 in the reference); LBM_QUIRK_IBM_CLAMP applies as in 2-D. Each step
  * interpolates the post-stream rho and u over the 4x4x4 Peskin stencils, iterates the
  * forcing on the band of stencil nodes and adds the spread force to the external force there.
- * n = 0 removes the body. */
+ * n = 0 removes the body. When the band's box is clear of non-periodic faces and of every flag
+ * but bounce-back, the nodes outside the box are stepped on an internal stream beside the IBM
+ * pass, joined to the context's stream before lbm3d_step returns. */
 int lbm3d_ibm_set_markers(lbm3d_ctx* ctx, const float* xyz, const float* ds, int32_t n, int32_t iters);
 /* Lagrangian forces summed over the iterations of the last step, AoS [n][3] (may be NULL), and
  * the total force the markers apply to the fluid, sum_l F_l ds_l in FP64 (force_sum[3], may be
